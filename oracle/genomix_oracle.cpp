@@ -1,0 +1,1137 @@
+// CPU oracle for the Genomix graph build and unitig compression.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, on the CPU and in plain
+// C++17, the algorithm of the Java reference (uci-cbcl/genomix) so that the
+// GPU implementation can be checked against it.  Only tests/, the smoke
+// check and the CPU-baseline leg of bench.py may load it; the product
+// library never links or calls it.
+//
+// The restatement follows the reference's own dataflow at P partitions:
+//   parse (one record per k-mer occurrence, serialised like
+//   Node.marshalToByteArray) -> stable sort by KmerPointable order ->
+//   preclustered group with AggregateKmerAggregateFactory (local) ->
+//   hash partition -> merge -> group again (cross) -> serialised Node
+//   records, then SequenceFile framing.
+// Each function cites the reference file:line it follows (paths relative to
+// the reference root).  The JDK-7 java.util.HashSet iteration order that
+// VKmerList.unionUpdate exposes is modelled explicitly (bucket array,
+// head-inserted chains, resize only on collision).
+//
+// Parity status of each part (what pins it) is listed in the header of
+// oracle/oracle.py.
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace ora {
+
+struct OraError : std::runtime_error {
+  int code;
+  OraError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum { E_INVAL = -22, E_INPUT = -5 };
+
+// ---------------------------------------------------------------------------
+// Java primitives
+// ---------------------------------------------------------------------------
+
+// Marshal.hashBytes (genomix-data/.../utils/Marshal.java:65-70) and the
+// identical KmerPartitionComputerFactory.hashBytes (:31-36): h = 31*h + (int)b
+// over signed bytes, int32 wrap-around.
+static int32_t java_hash_bytes(const uint8_t* b, size_t n) {
+  uint32_t h = 1;
+  for (size_t i = 0; i < n; i++) h = h * 31u + (uint32_t)(int32_t)(int8_t)b[i];
+  return (int32_t)h;
+}
+
+static void put_be32(std::vector<uint8_t>& o, uint32_t v) {
+  o.push_back(v >> 24); o.push_back(v >> 16); o.push_back(v >> 8); o.push_back(v);
+}
+static void put_be64(std::vector<uint8_t>& o, uint64_t v) {
+  for (int s = 56; s >= 0; s -= 8) o.push_back((uint8_t)(v >> s));
+}
+static uint32_t get_be32(const uint8_t* p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+static uint64_t get_be64(const uint8_t* p) {
+  uint64_t v = 0;
+  for (int i = 0; i < 8; i++) v = (v << 8) | p[i];
+  return v;
+}
+
+// GeneCode.getCodeFromSymbol (GeneCode.java:29-50): A/a=0 C/c=1 G/g=2 T/t=3,
+// anything else 0.
+static inline uint8_t gene_code(uint8_t ch) {
+  switch (ch) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+  }
+  return 0;
+}
+static const char kSym[4] = {'A', 'C', 'G', 'T'};
+
+// KmerUtil.getByteNumFromK (KmerUtil.java:21-27).
+static inline int bytes_for_k(int k) { return k / 4 + (k % 4 != 0 ? 1 : 0); }
+
+// ---------------------------------------------------------------------------
+// Kmer / VKmer byte packing
+// ---------------------------------------------------------------------------
+
+// Kmer.setFromStringBytes (Kmer.java:225-242); identical loop in
+// VKmer.setFromStringBytes (VKmer.java:462-479).
+static void pack_forward(const uint8_t* s, size_t slen, size_t start, int k, uint8_t* out) {
+  int nb = bytes_for_k(k);
+  uint8_t l = 0;
+  int bytecount = 0, bcount = nb - 1;
+  for (size_t i = start; i < start + (size_t)k && i < slen; i++) {
+    uint8_t code = gene_code(s[i]);
+    l |= (uint8_t)(code << bytecount);
+    bytecount += 2;
+    if (bytecount == 8) { out[bcount--] = l; l = 0; bytecount = 0; }
+  }
+  if (bcount >= 0) out[0] = l;
+}
+
+// Kmer.setReversedFromStringBytes (Kmer.java:253-272): complement codes taken
+// from the last letter backwards.
+static void pack_reverse(const uint8_t* s, size_t slen, size_t start, int k, uint8_t* out) {
+  int nb = bytes_for_k(k);
+  uint8_t l = 0;
+  int bytecount = 0, bcount = nb - 1;
+  for (long i = (long)start + k - 1; i >= (long)start && (size_t)i < slen; i--) {
+    uint8_t code = 3 - gene_code(s[i]);
+    l |= (uint8_t)(code << bytecount);
+    bytecount += 2;
+    if (bytecount == 8) { out[bcount--] = l; l = 0; bytecount = 0; }
+  }
+  if (bcount >= 0) out[0] = l;
+}
+
+// Kmer.shiftKmerWithNextCode (Kmer.java:292-303) incl. clearLeadBit (:332-336).
+static void shift_with_next(uint8_t* b, int k, uint8_t c) {
+  int nb = bytes_for_k(k);
+  for (int i = nb - 1; i > 0; i--) {
+    uint8_t in = b[i - 1] & 0x03;
+    b[i] = (uint8_t)(((b[i] >> 2) & 0x3f) | (in << 6));
+  }
+  int pos = ((k - 1) % 4) << 1;
+  b[0] = (uint8_t)(((b[0] >> 2) & 0x3f) | (uint8_t)(c << pos));
+  if (k % 4 != 0) b[0] &= (uint8_t)((1 << ((k % 4) << 1)) - 1);
+}
+
+// WritableComparator.compareBytes (Hadoop 0.20.2, un-vendored): unsigned
+// lexicographic compare, then length difference.  Used by
+// BinaryComparable.compareTo in the canonical choice
+// (ReadsKeyValueParserFactory.java:163,181).
+static int compare_bytes(const uint8_t* a, int la, const uint8_t* b, int lb) {
+  int n = std::min(la, lb);
+  for (int i = 0; i < n; i++) {
+    if (a[i] != b[i]) return (int)a[i] - (int)b[i];
+  }
+  return la - lb;
+}
+
+// KmerUtil.recoverKmerFrom (KmerUtil.java:34-48).
+static std::string kmer_to_string(int k, const uint8_t* b, int nb) {
+  std::string s;
+  if (nb - 1 < 0 || k < 1) return s;
+  int byteId = nb - 1;
+  uint8_t cur = b[byteId];
+  for (int g = 0; g < k; g++) {
+    if (g % 4 == 0 && g > 0) cur = b[--byteId];
+    s.push_back(kSym[(cur >> ((g % 4) * 2)) & 3]);
+  }
+  return s;
+}
+
+// VKmer: BE int32 letter count followed by the packed bytes
+// (VKmer.java:389-391 write, :394-396 hashCode over header+bytes).
+struct VKmer {
+  int32_t k = 0;
+  std::vector<uint8_t> b;
+  bool operator==(const VKmer& o) const { return k == o.k && b == o.b; }
+  void write(std::vector<uint8_t>& o) const {
+    put_be32(o, (uint32_t)k);
+    o.insert(o.end(), b.begin(), b.end());
+  }
+  int32_t hash_code() const {
+    std::vector<uint8_t> t;
+    write(t);
+    return java_hash_bytes(t.data(), t.size());
+  }
+  std::string str() const { return kmer_to_string(k, b.data(), (int)b.size()); }
+  static VKmer from_string(const uint8_t* s, size_t len) {  // VKmer.setAsCopy(String) :140-142
+    VKmer v;
+    v.k = (int32_t)len;
+    v.b.assign(bytes_for_k((int)len), 0);
+    if (len) pack_forward(s, len, 0, (int)len, v.b.data());
+    return v;
+  }
+};
+
+static size_t read_vkmer(const uint8_t* p, size_t avail, VKmer& v) {  // VKmer.setAsCopy(byte[],int) :151-156
+  if (avail < 4) throw OraError(E_INPUT, "truncated VKmer");
+  v.k = (int32_t)get_be32(p);
+  int nb = v.k > 0 ? bytes_for_k(v.k) : 0;
+  if (avail < 4 + (size_t)nb) throw OraError(E_INPUT, "truncated VKmer body");
+  v.b.assign(p + 4, p + 4 + nb);
+  return 4 + nb;
+}
+
+// ---------------------------------------------------------------------------
+// JDK-7 java.util.HashSet<VKmer> (HashMap, hashSeed = 0, no alternative
+// hashing).  VKmerList.unionUpdate (VKmerList.java:117-133) builds one with
+// initial capacity |L|+|M|, adds L then M in order and rewrites the list in
+// the set's iteration order.
+// ---------------------------------------------------------------------------
+
+static inline int32_t jdk7_spread(int32_t h) {  // HashMap.hash(int) in JDK 7
+  uint32_t x = (uint32_t)h;
+  x ^= (x >> 20) ^ (x >> 12);
+  return (int32_t)(x ^ (x >> 7) ^ (x >> 4));
+}
+
+struct Jdk7HashSet {
+  struct Entry { int32_t hash; const VKmer* key; };
+  // chains[i] stores the chain with its HEAD at the back (push_back == insert
+  // at head, as HashMap.createEntry does).
+  std::vector<std::vector<Entry>> chains;
+  int cap = 1, thr = 0, size = 0;
+
+  explicit Jdk7HashSet(int n) {
+    // HashMap(int): capacity = roundUpToPowerOf2(n) (n<=1 -> 1), threshold =
+    // (int)(capacity * 0.75f).
+    cap = 1;
+    while (cap < n) cap <<= 1;
+    thr = (int)((float)cap * 0.75f);
+    chains.assign(cap, {});
+  }
+  void add(const VKmer* x) {
+    int32_t h = jdk7_spread(x->hash_code());
+    int bi = h & (cap - 1);
+    for (auto& e : chains[bi])
+      if (e.hash == h && *e.key == *x) return;  // present: HashMap.put replaces value only
+    // HashMap.addEntry: resize only when size >= threshold AND the target
+    // bucket is non-empty.
+    if (size >= thr && !chains[bi].empty()) {
+      int ncap = cap * 2;
+      std::vector<std::vector<Entry>> nt(ncap);
+      // HashMap.transfer: old buckets in order, each chain head->tail, every
+      // entry pushed to the head of its new chain.
+      for (int j = 0; j < cap; j++) {
+        auto& ch = chains[j];
+        for (long t = (long)ch.size() - 1; t >= 0; t--) {
+          const Entry& e = ch[t];
+          nt[e.hash & (ncap - 1)].push_back(e);
+        }
+      }
+      chains.swap(nt);
+      cap = ncap;
+      thr = (int)((float)cap * 0.75f);
+      bi = h & (cap - 1);
+    }
+    chains[bi].push_back({h, x});
+    size++;
+  }
+  std::vector<VKmer> iterate() const {  // HashIterator: buckets ascending, chain head->tail
+    std::vector<VKmer> out;
+    for (int j = 0; j < cap; j++)
+      for (long t = (long)chains[j].size() - 1; t >= 0; t--) out.push_back(*chains[j][t].key);
+    return out;
+  }
+};
+
+// Deliberately different HashMap generations, used only by the tests as
+// negative controls (a wrong model must fail the fixture edge-order replay):
+//   1 = JDK 8 (hash h ^ h>>>16, tail insertion, resize after ++size >
+//       threshold with order-preserving lo/hi split),
+//   2 = JDK 6 (JDK-7 hash and head insertion, but resize after every
+//       size++ >= threshold regardless of collisions).
+static int g_hash_model = 0;
+
+struct OtherHashSet {
+  struct Entry { int32_t hash; const VKmer* key; };
+  std::vector<std::vector<Entry>> chains;  // head at FRONT here
+  int cap = 1, thr = 0, size = 0, model;
+  OtherHashSet(int n, int m) : model(m) {
+    cap = 1;
+    while (cap < n) cap <<= 1;
+    thr = (int)((float)cap * 0.75f);
+    chains.assign(cap, {});
+  }
+  int32_t spread(int32_t h) const {
+    if (model == 1) return h ^ (int32_t)((uint32_t)h >> 16);
+    return jdk7_spread(h);
+  }
+  void rehash(int ncap) {
+    std::vector<std::vector<Entry>> nt(ncap);
+    for (int j = 0; j < cap; j++) {
+      if (model == 1) {
+        for (auto& e : chains[j]) nt[e.hash & (ncap - 1)].push_back(e);  // order kept
+      } else {
+        for (auto& e : chains[j]) {
+          auto& d = nt[e.hash & (ncap - 1)];
+          d.insert(d.begin(), e);
+        }
+      }
+    }
+    chains.swap(nt);
+    cap = ncap;
+    thr = (int)((float)cap * 0.75f);
+  }
+  void add(const VKmer* x) {
+    int32_t h = spread(x->hash_code());
+    int bi = h & (cap - 1);
+    for (auto& e : chains[bi])
+      if (e.hash == h && *e.key == *x) return;
+    if (model == 1) {
+      chains[bi].push_back({h, x});
+      if (++size > thr) rehash(cap * 2);
+    } else {
+      chains[bi].insert(chains[bi].begin(), {h, x});
+      if (size++ >= thr) rehash(cap * 2);
+    }
+  }
+  std::vector<VKmer> iterate() const {
+    std::vector<VKmer> out;
+    for (int j = 0; j < cap; j++)
+      for (auto& e : chains[j]) out.push_back(*e.key);
+    return out;
+  }
+};
+
+// VKmerList.unionUpdate (VKmerList.java:117-133).
+static std::vector<VKmer> union_update(const std::vector<VKmer>& L, const std::vector<VKmer>& M) {
+  if (g_hash_model != 0) {
+    OtherHashSet s((int)(L.size() + M.size()), g_hash_model);
+    for (auto& x : L) s.add(&x);
+    for (auto& x : M) s.add(&x);
+    return s.iterate();
+  }
+  Jdk7HashSet s((int)(L.size() + M.size()));
+  for (auto& x : L) s.add(&x);
+  for (auto& x : M) s.add(&x);
+  return s.iterate();
+}
+
+// ---------------------------------------------------------------------------
+// ReadHeadInfo / ReadHeadSet
+// ---------------------------------------------------------------------------
+
+struct ReadHead {
+  int64_t uuid = 0;
+  VKmer seq;
+  bool has_mate = false;  // mateReadSequence != null && length > 0
+  VKmer mate;
+
+  // ReadHeadInfo bit layout (ReadHeadInfo.java:17-34): offset<<40 | lib<<36 |
+  // mate<<35 | readId.
+  int offset() const {  // getOffset :168-176
+    int off = (int)((uint64_t)uuid >> 40);
+    if (off & (1 << 23)) off = -(off & ((1 << 23) - 1));
+    return off;
+  }
+  int lib() const { return (int)(((uint64_t)uuid >> 36) & 0xF); }
+  int mate_id() const { return (int)(((uint64_t)uuid >> 35) & 1); }
+  int64_t read_id() const { return (int64_t)((uint64_t)uuid & ((1ull << 35) - 1)); }
+};
+
+// ReadHeadInfo.makeUUID (ReadHeadInfo.java:100-127), including its range
+// checks.  Note the readId check masks with ~(-1L << 29): readIds must fit 29
+// bits even though 35 are reserved.
+static int64_t make_uuid(int mate, int lib, int64_t readId, int offset) {
+  if (readId != (readId & (int64_t)~(~0ull << 29)))
+    throw OraError(E_INPUT, "readId " + std::to_string(readId) + " does not fit the ReadHeadInfo uuid");
+  const int maxOff = (1 << 23) - 1;
+  if (std::abs(offset) > maxOff) throw OraError(E_INPUT, "read head offset out of range");
+  if (offset < 0) offset = -offset | (maxOff + 1);
+  return (int64_t)(((uint64_t)(uint32_t)offset << 40) + ((uint64_t)lib << 36) + ((uint64_t)mate << 35) +
+                   (uint64_t)readId);
+}
+
+// ReadHeadInfo.compareTo (:247-264): offset, library, mate, readId.
+static int head_cmp(const ReadHead& a, const ReadHead& b) {
+  if (a.offset() != b.offset()) return a.offset() < b.offset() ? -1 : 1;
+  if (a.lib() != b.lib()) return a.lib() < b.lib() ? -1 : 1;
+  if (a.mate_id() != b.mate_id()) return a.mate_id() < b.mate_id() ? -1 : 1;
+  if (a.read_id() != b.read_id()) return a.read_id() < b.read_id() ? -1 : 1;
+  return 0;
+}
+
+// java.util.TreeSet semantics used by ExternalableTreeSet.add/union
+// (ExternalableTreeSet.java:85-92,138-145): insertion keeps an existing equal
+// element.
+static void treeset_add(std::vector<ReadHead>& s, const ReadHead& h) {
+  auto it = std::lower_bound(s.begin(), s.end(), h,
+                             [](const ReadHead& a, const ReadHead& b) { return head_cmp(a, b) < 0; });
+  if (it != s.end() && head_cmp(*it, h) == 0) return;
+  s.insert(it, h);
+}
+
+// ReadHeadInfo.write (:205-212).
+static void write_head(std::vector<uint8_t>& o, const ReadHead& h) {
+  o.push_back(h.has_mate ? 1 : 0);
+  put_be64(o, (uint64_t)h.uuid);
+  h.seq.write(o);
+  if (h.has_mate) h.mate.write(o);
+}
+
+// ExternalableTreeSet.write with writeEntireBody forced true (:236-267):
+// boolean true, int size, elements in TreeSet order.
+static void write_headset(std::vector<uint8_t>& o, const std::vector<ReadHead>& s) {
+  o.push_back(1);
+  put_be32(o, (uint32_t)s.size());
+  for (auto& h : s) write_head(o, h);
+}
+
+static size_t read_headset(const uint8_t* p, size_t avail, std::vector<ReadHead>& s) {
+  size_t off = 0;
+  if (avail < 1) throw OraError(E_INPUT, "truncated read-head set");
+  bool whole = p[off++] != 0;
+  if (!whole) throw OraError(E_INPUT, "read-head set stored by path is not supported");
+  if (avail < off + 4) throw OraError(E_INPUT, "truncated read-head set");
+  uint32_t n = get_be32(p + off);
+  off += 4;
+  s.clear();
+  for (uint32_t i = 0; i < n; i++) {
+    ReadHead h;
+    if (avail < off + 9) throw OraError(E_INPUT, "truncated read head");
+    uint8_t fields = p[off++];
+    h.uuid = (int64_t)get_be64(p + off);
+    off += 8;
+    off += read_vkmer(p + off, avail - off, h.seq);
+    if (fields & 1) {
+      off += read_vkmer(p + off, avail - off, h.mate);
+      h.has_mate = true;  // readFields creates the mate object when present
+    }
+    treeset_add(s, h);
+  }
+  return off;
+}
+
+// ---------------------------------------------------------------------------
+// Node (Node.java:123-128, write :408-427, active fields :466-487)
+// ---------------------------------------------------------------------------
+
+enum { FF = 0, FR = 1, RF = 2, RR = 3 };
+
+struct Node {
+  std::vector<VKmer> edges[4];
+  bool edges_present[4] = {false, false, false, false};  // null vs non-null list
+  std::vector<ReadHead> unflipped, flipped;
+  bool has_unflipped = false, has_flipped = false;
+  VKmer internal;  // k == 0 -> absent
+  bool has_internal = false;
+  bool has_cov = false;
+  float cov = 0.f;
+
+  void write(std::vector<uint8_t>& o) const {
+    uint8_t f = 0;
+    for (int e = 0; e < 4; e++)
+      if (edges_present[e] && !edges[e].empty()) f |= (uint8_t)(1 << e);
+    if (has_unflipped && !unflipped.empty()) f |= 1 << 4;
+    if (has_flipped && !flipped.empty()) f |= 1 << 5;
+    if (has_internal && internal.k > 0) f |= 1 << 6;
+    if (has_cov) f |= 1 << 7;
+    o.push_back(f);
+    for (int e = 0; e < 4; e++) {
+      if (f & (1 << e)) {  // VKmerList.write (VKmerList.java:314-316)
+        put_be32(o, (uint32_t)edges[e].size());
+        for (auto& v : edges[e]) v.write(o);
+      }
+    }
+    if (f & (1 << 4)) write_headset(o, unflipped);
+    if (f & (1 << 5)) write_headset(o, flipped);
+    if (f & (1 << 6)) internal.write(o);
+    if (f & (1 << 7)) {
+      uint32_t u;
+      std::memcpy(&u, &cov, 4);
+      put_be32(o, u);
+    }
+  }
+
+  // Node.readFields / setAsCopy (:336-360, :435-456).
+  size_t read(const uint8_t* p, size_t avail) {
+    *this = Node();
+    if (avail < 1) throw OraError(E_INPUT, "truncated Node");
+    size_t off = 0;
+    uint8_t f = p[off++];
+    for (int e = 0; e < 4; e++) {
+      if (f & (1 << e)) {
+        if (avail < off + 4) throw OraError(E_INPUT, "truncated edge list");
+        uint32_t n = get_be32(p + off);
+        off += 4;
+        edges_present[e] = true;
+        for (uint32_t i = 0; i < n; i++) {
+          VKmer v;
+          off += read_vkmer(p + off, avail - off, v);
+          edges[e].push_back(v);
+        }
+      }
+    }
+    if (f & (1 << 4)) { has_unflipped = true; off += read_headset(p + off, avail - off, unflipped); }
+    if (f & (1 << 5)) { has_flipped = true; off += read_headset(p + off, avail - off, flipped); }
+    if (f & (1 << 6)) { has_internal = true; off += read_vkmer(p + off, avail - off, internal); }
+    if (f & (1 << 7)) {
+      if (avail < off + 4) throw OraError(E_INPUT, "truncated coverage");
+      uint32_t u = get_be32(p + off);
+      std::memcpy(&cov, &u, 4);
+      has_cov = true;
+      off += 4;
+    }
+    return off;
+  }
+};
+
+// Java Float.toString for the values the text dumps hold: shortest decimal
+// that reads back as the same float, plain notation in [1e-3, 1e7),
+// computerised scientific notation otherwise.
+static std::string java_float_str(float f) {
+  if (f != f) return "NaN";
+  if (f == 0.f) return std::signbit(f) ? "-0.0" : "0.0";
+  if (f > 3.4e38f * 10) return "Infinity";
+  char buf[64];
+  int prec = 1;
+  for (; prec <= 9; prec++) {
+    snprintf(buf, sizeof buf, "%.*e", prec - 1, (double)f);
+    if (strtof(buf, nullptr) == f) break;
+  }
+  // buf = d.ddde[+-]XX
+  std::string m(buf);
+  size_t epos = m.find('e');
+  int ex = atoi(m.c_str() + epos + 1);
+  std::string digits;
+  bool neg = m[0] == '-';
+  for (size_t i = neg ? 1 : 0; i < epos; i++)
+    if (m[i] != '.') digits.push_back(m[i]);
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out = neg ? "-" : "";
+  double a = std::fabs((double)f);
+  if (a >= 1e-3 && a < 1e7) {
+    if (ex >= 0) {
+      std::string ip = digits.substr(0, std::min((size_t)ex + 1, digits.size()));
+      while ((int)ip.size() < ex + 1) ip.push_back('0');
+      std::string fp = (int)digits.size() > ex + 1 ? digits.substr(ex + 1) : "0";
+      out += ip + "." + fp;
+    } else {
+      out += "0.";
+      for (int i = 0; i < -ex - 1; i++) out.push_back('0');
+      out += digits;
+    }
+  } else {
+    out += digits.substr(0, 1) + "." + (digits.size() > 1 ? digits.substr(1) : "0") + "E" + std::to_string(ex);
+  }
+  return out;
+}
+
+// ReadHeadInfo.toString (:236-240); ReadHeadSet.toString (ReadHeadSet.java:46-58).
+static std::string head_str(const ReadHead& h) {
+  return std::to_string(h.read_id()) + "-" + std::to_string(h.offset()) + "_" + std::to_string(h.mate_id()) + "-" +
+         std::to_string(h.lib()) + " readSeq: " + h.seq.str() + " mateReadSeq: " + (h.has_mate ? h.mate.str() : "null");
+}
+static std::string headset_str(const std::vector<ReadHead>& s) {
+  std::string o = "[";
+  for (size_t i = 0; i < s.size(); i++) {
+    if (i) o += ",";
+    o += head_str(s[i]);
+  }
+  return o + "]";
+}
+
+// Node.toString (Node.java:525-537) + VKmerList.toString (VKmerList.java:349-362).
+static std::string node_str(const Node& n) {
+  static const char* et[4] = {"FF", "FR", "RF", "RR"};
+  std::string o = "{";
+  for (int e = 0; e < 4; e++) {
+    o += et[e];
+    o += ":";
+    if (!n.edges_present[e]) {
+      o += "null";
+    } else {
+      o += "[";
+      for (size_t i = 0; i < n.edges[e].size(); i++) {
+        if (i) o += ",";
+        o += n.edges[e][i].str();
+      }
+      o += "]";
+    }
+    o += "\t";
+  }
+  o += "5':" + (n.has_unflipped ? headset_str(n.unflipped) : std::string("null"));
+  o += ", ~5':" + (n.has_flipped ? headset_str(n.flipped) : std::string("null")) + "\t";
+  o += "kmer:" + (n.has_internal ? n.internal.str() : std::string("null")) + "\t";
+  o += "cov:" + (n.has_cov ? java_float_str(n.cov) + "x" : std::string("null")) + "}";
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// Parser: ReadsKeyValueParserFactory.parse / SplitReads
+// ---------------------------------------------------------------------------
+
+struct Record {       // one Hyracks tuple: (Kmer bytes, Node bytes)
+  std::vector<uint8_t> key;   // nb bytes
+  std::vector<uint8_t> node;  // Node.marshalToByteArray
+};
+
+struct ParseCtx {
+  int k;
+  std::vector<Record>* out;
+};
+
+static bool valid_acgt(const uint8_t* s, size_t n) {  // "[ACGTacgt]+".matches
+  if (n == 0) return false;
+  for (size_t i = 0; i < n; i++) {
+    switch (s[i]) {
+      case 'A': case 'C': case 'G': case 'T': case 'a': case 'c': case 'g': case 't': break;
+      default: return false;
+    }
+  }
+  return true;
+}
+
+// Long.parseLong: optional sign then decimal digits, must fit int64.
+static int64_t java_parse_long(const uint8_t* s, size_t n, bool* ok) {
+  *ok = false;
+  if (n == 0) return 0;
+  size_t i = 0;
+  bool neg = false;
+  if (s[0] == '-' || s[0] == '+') { neg = s[0] == '-'; i = 1; if (n == 1) return 0; }
+  unsigned __int128 v = 0;
+  for (; i < n; i++) {
+    if (s[i] < '0' || s[i] > '9') return 0;
+    v = v * 10 + (s[i] - '0');
+    if (v > ((unsigned __int128)1 << 63)) return 0;
+  }
+  if (!neg && v == ((unsigned __int128)1 << 63)) return 0;
+  *ok = true;
+  return neg ? -(int64_t)(v - 1) - 1 : (int64_t)v;
+}
+
+// SplitReads (ReadsKeyValueParserFactory.java:150-196) with
+// setEdgesForCurAndNext (:209-233) and writeToFrame/InsertToFrame (:198-254).
+static void split_reads(ParseCtx& c, const ReadHead& head_template, const uint8_t* r, size_t L) {
+  const int k = c.k, nb = bytes_for_k(k);
+  if ((size_t)k >= L)
+    throw OraError(E_INPUT, "kmersize (k=" + std::to_string(k) + ") is larger than the read length (" +
+                                std::to_string(L) + ")");
+  std::vector<uint8_t> curF(nb), curR(nb), nxtF(nb), nxtR(nb);
+  Node cur, nxt;
+  cur.has_cov = true; cur.cov = 1.f;
+  pack_forward(r, L, 0, k, curF.data());
+  pack_reverse(r, L, 0, k, curR.data());
+  bool curFwd = compare_bytes(curF.data(), nb, curR.data(), nb) <= 0;
+  ReadHead h = head_template;
+  if (curFwd) {
+    cur.has_unflipped = true;
+    treeset_add(cur.unflipped, h);
+  } else {
+    h.uuid = make_uuid(h.mate_id(), h.lib(), h.read_id(), k - 1);  // resetOffset(K-1)
+    cur.has_flipped = true;
+    treeset_add(cur.flipped, h);
+  }
+  nxt = Node(); nxt.has_cov = true; nxt.cov = 1.f;
+  nxtF = curF;
+  auto vk = [&](const std::vector<uint8_t>& b) { VKmer v; v.k = k; v.b = b; return v; };
+  auto emit = [&](const std::vector<uint8_t>& F, const std::vector<uint8_t>& R, bool fwd, const Node& n) {
+    Record rec;
+    rec.key = fwd ? F : R;
+    n.write(rec.node);
+    c.out->push_back(std::move(rec));
+  };
+  for (size_t i = k; i < L; i++) {
+    shift_with_next(nxtF.data(), k, gene_code(r[i]));
+    pack_reverse(r, L, i - k + 1, k, nxtR.data());
+    bool nxtFwd = compare_bytes(nxtF.data(), nb, nxtR.data(), nb) <= 0;
+    if (curFwd && nxtFwd) {
+      cur.edges_present[FF] = true; cur.edges[FF].push_back(vk(nxtF));
+      nxt.edges_present[RR] = true; nxt.edges[RR].push_back(vk(curF));
+    } else if (curFwd && !nxtFwd) {
+      cur.edges_present[FR] = true; cur.edges[FR].push_back(vk(nxtR));
+      nxt.edges_present[FR] = true; nxt.edges[FR].push_back(vk(curF));
+    } else if (!curFwd && nxtFwd) {
+      cur.edges_present[RF] = true; cur.edges[RF].push_back(vk(nxtF));
+      nxt.edges_present[RF] = true; nxt.edges[RF].push_back(vk(curR));
+    } else {
+      cur.edges_present[RR] = true; cur.edges[RR].push_back(vk(nxtR));
+      nxt.edges_present[FF] = true; nxt.edges[FF].push_back(vk(curR));
+    }
+    emit(curF, curR, curFwd, cur);
+    curF = nxtF; curR = nxtR; cur = nxt; curFwd = nxtFwd;
+    nxt = Node(); nxt.has_cov = true; nxt.cov = 1.f;
+  }
+  emit(curF, curR, curFwd, cur);
+}
+
+// ReadsKeyValueParserFactory.parse (:95-148).  libraryId is always 0: the
+// pattern's group(0) is read without matches() and throws, which is caught.
+static void parse_line(ParseCtx& c, const uint8_t* line, size_t n) {
+  // String.split("\\t"): trailing empty strings removed.
+  std::vector<std::pair<size_t, size_t>> f;
+  size_t st = 0;
+  for (size_t i = 0; i <= n; i++) {
+    if (i == n || line[i] == '\t') { f.push_back({st, i - st}); st = i + 1; }
+  }
+  while (!f.empty() && f.back().second == 0) f.pop_back();
+  if (f.empty()) f.push_back({0, 0});  // "".split gives [""]
+  if (f.size() != 2 && f.size() != 3)
+    throw OraError(E_INPUT, "input format is not correct! only support id'\\t'readSeq'\\t'mateReadSeq or id'\\t'readSeq");
+  bool ok;
+  int64_t readId = java_parse_long(line + f[0].first, f[0].second, &ok);
+  if (!ok) throw OraError(E_INPUT, "bad read id");
+  const uint8_t* m0 = line + f[1].first;
+  size_t l0 = f[1].second;
+  const uint8_t* m1 = f.size() == 3 ? line + f[2].first : nullptr;
+  size_t l1 = f.size() == 3 ? f[2].second : 0;
+  if (valid_acgt(m0, l0)) {
+    ReadHead h;
+    h.uuid = make_uuid(0, 0, readId, 0);
+    h.seq = VKmer::from_string(m0, l0);
+    if (m1) { h.mate = VKmer::from_string(m1, l1); h.has_mate = l1 > 0; }
+    split_reads(c, h, m0, l0);
+  }
+  if (m1 && valid_acgt(m1, l1)) {
+    ReadHead h;
+    h.uuid = make_uuid(1, 0, readId, 0);
+    h.seq = VKmer::from_string(m1, l1);
+    h.mate = VKmer::from_string(m0, l0);
+    h.has_mate = l0 > 0;
+    split_reads(c, h, m1, l1);
+  }
+}
+
+// Hadoop LineReader: lines end at '\n', '\r' or "\r\n"; no record after the
+// final terminator.
+static void for_each_line(const uint8_t* t, size_t len,
+                          const std::function<void(size_t, const uint8_t*, size_t)>& fn) {
+  size_t i = 0;
+  while (i < len) {
+    size_t j = i;
+    while (j < len && t[j] != '\n' && t[j] != '\r') j++;
+    fn(i, t + i, j - i);
+    if (j < len && t[j] == '\r' && j + 1 < len && t[j + 1] == '\n') j++;
+    i = j + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow (JobGenBuildBrujinGraph.java:92-117)
+// ---------------------------------------------------------------------------
+
+// KmerPointable.compareTo (KmerPointable.java:94-107): length, then bytes
+// from the LAST to the first, unsigned.
+static int kmer_pointable_cmp(const std::vector<uint8_t>& a, const std::vector<uint8_t>& b) {
+  if (a.size() != b.size()) return (int)a.size() - (int)b.size();
+  for (long i = (long)a.size() - 1; i >= 0; i--)
+    if (a[i] != b[i]) return (int)a[i] - (int)b[i];
+  return 0;
+}
+
+// AggregateKmerAggregateFactory init/aggregate (:93-144).
+struct AggState {
+  Node n;
+};
+static void agg_init(AggState& s, const Node& rec) {
+  s.n = Node();
+  for (int e = 0; e < 4; e++) {
+    s.n.edges[e] = union_update({}, rec.edges[e]);
+    s.n.edges_present[e] = true;
+  }
+  s.n.has_unflipped = s.n.has_flipped = true;
+  for (auto& h : rec.unflipped) treeset_add(s.n.unflipped, h);
+  for (auto& h : rec.flipped) treeset_add(s.n.flipped, h);
+  s.n.has_cov = true;
+  s.n.cov = rec.cov;
+}
+static void agg_step(AggState& s, const Node& rec) {
+  for (int e = 0; e < 4; e++) s.n.edges[e] = union_update(s.n.edges[e], rec.edges[e]);
+  for (auto& h : rec.unflipped) treeset_add(s.n.unflipped, h);
+  for (auto& h : rec.flipped) treeset_add(s.n.flipped, h);
+  s.n.cov = s.n.cov + rec.cov;
+}
+
+// ExternalSort (stable, FrameSorterMergeSort.java:153-185) +
+// PreclusteredGroup (PreclusteredGroupWriter.java:86-124) with the
+// aggregator.  Records within a group are visited in input order.
+static std::vector<Record> sort_and_group(std::vector<Record>& in) {
+  std::vector<size_t> idx(in.size());
+  for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](size_t a, size_t b) { return kmer_pointable_cmp(in[a].key, in[b].key) < 0; });
+  std::vector<Record> out;
+  size_t i = 0;
+  Node tmp;
+  while (i < idx.size()) {
+    size_t j = i;
+    AggState st;
+    tmp.read(in[idx[i]].node.data(), in[idx[i]].node.size());
+    agg_init(st, tmp);
+    for (j = i + 1; j < idx.size() && kmer_pointable_cmp(in[idx[j]].key, in[idx[i]].key) == 0; j++) {
+      tmp.read(in[idx[j]].node.data(), in[idx[j]].node.size());
+      agg_step(st, tmp);
+    }
+    Record r;
+    r.key = in[idx[i]].key;
+    st.n.write(r.node);  // outputFinalResult (:185-214)
+    out.push_back(std::move(r));
+    i = j;
+  }
+  return out;
+}
+
+// KmerPartitionComputerFactory.partition (:41-53).
+static int partition_of(const std::vector<uint8_t>& key, int P) {
+  int32_t h = java_hash_bytes(key.data(), key.size());
+  if (h < 0) h = -(h + 1);
+  return h % P;
+}
+
+struct BuildResult {
+  int k = 0;
+  std::vector<std::vector<Record>> parts;  // records in output order
+};
+
+// The whole graph build.  Producers: the input lines are split into
+// `nsplits` producers by the byte offsets in `split_starts` (a line belongs to
+// the split holding its first byte), each producer sorts and groups locally;
+// every local record goes to partition hash%P; the consumer merges producer
+// streams in key order with ties taken in ascending producer order and
+// groups again (cross aggregation).  local_only skips the cross step and
+// returns producer outputs as "parts" (used to check fixture edge orders).
+static BuildResult build(const uint8_t* text, size_t len, int k, int P, const int64_t* split_starts, int nsplits,
+                         bool local_only) {
+  if (k < 3) throw OraError(E_INVAL, "kmerLength must be at least 3!");  // GenomixJobConf.java:422-423
+  if (k > 63) throw OraError(E_INVAL, "k > 63 is not supported");
+  if (P < 1) throw OraError(E_INVAL, "nparts must be >= 1");
+  if (nsplits < 1) nsplits = 1;
+  std::vector<std::vector<Record>> prod(nsplits);
+  for_each_line(text, len, [&](size_t start, const uint8_t* line, size_t n) {
+    int p = 0;
+    if (split_starts)
+      while (p + 1 < nsplits && (size_t)split_starts[p + 1] <= start) p++;
+    ParseCtx c{k, &prod[p]};
+    parse_line(c, line, n);
+  });
+  std::vector<std::vector<Record>> local(nsplits);
+  for (int p = 0; p < nsplits; p++) local[p] = sort_and_group(prod[p]);
+  BuildResult res;
+  res.k = k;
+  if (local_only) {
+    res.parts = std::move(local);
+    return res;
+  }
+  res.parts.resize(P);
+  for (int part = 0; part < P; part++) {
+    // Gather this partition's records from each producer (each already in
+    // key order) and merge them; stable_sort over the concatenation in
+    // producer order gives "ties in ascending producer order".
+    std::vector<Record> merged;
+    for (int p = 0; p < nsplits; p++)
+      for (auto& r : local[p])
+        if (partition_of(r.key, P) == part) merged.push_back(r);
+    res.parts[part] = sort_and_group(merged);
+  }
+  return res;
+}
+
+// Record stream of one part as SequenceFile records without header or syncs:
+// i32 recLen, i32 keyLen, VKmer key, Node value
+// (KmerNodePairSequenceWriterFactory.java:79-94; SequenceFile.Writer.append).
+static std::vector<uint8_t> part_records(const BuildResult& r, int part) {
+  std::vector<uint8_t> o;
+  for (auto& rec : r.parts[part]) {
+    VKmer key;
+    key.k = r.k;
+    key.b = rec.key;
+    std::vector<uint8_t> kb;
+    key.write(kb);
+    put_be32(o, (uint32_t)(kb.size() + rec.node.size()));
+    put_be32(o, (uint32_t)kb.size());
+    o.insert(o.end(), kb.begin(), kb.end());
+    o.insert(o.end(), rec.node.begin(), rec.node.end());
+  }
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// SequenceFile v6 container (Hadoop 0.20.2 SequenceFile.Writer) and the
+// local-filesystem .crc sidecar (ChecksumFileSystem).
+// ---------------------------------------------------------------------------
+
+static void write_vint_text(std::vector<uint8_t>& o, const std::string& s) {
+  // WritableUtils.writeVInt for 0 <= n <= 127 is one byte; class names are short.
+  if (s.size() > 127) throw OraError(E_INVAL, "class name too long");
+  o.push_back((uint8_t)s.size());
+  o.insert(o.end(), s.begin(), s.end());
+}
+
+static std::vector<uint8_t> seqfile_wrap(const uint8_t* recs, size_t n, const uint8_t sync[16],
+                                         const char* keycls, const char* valcls) {
+  std::vector<uint8_t> o = {'S', 'E', 'Q', 6};
+  write_vint_text(o, keycls);
+  write_vint_text(o, valcls);
+  o.push_back(0);  // compressed
+  o.push_back(0);  // block compressed
+  put_be32(o, 0);  // metadata count
+  o.insert(o.end(), sync, sync + 16);
+  size_t lastSync = 0, off = 0;
+  while (off < n) {
+    if (off + 8 > n) throw OraError(E_INPUT, "truncated record stream");
+    uint32_t recLen = get_be32(recs + off);
+    size_t sz = 8 + (size_t)recLen;
+    if (off + sz > n) throw OraError(E_INPUT, "truncated record");
+    // checkAndWriteSync: pos >= lastSyncPos + SYNC_INTERVAL (100 * 20)
+    if (o.size() >= lastSync + 2000) {
+      put_be32(o, 0xffffffffu);
+      o.insert(o.end(), sync, sync + 16);
+      lastSync = o.size();
+    }
+    o.insert(o.end(), recs + off, recs + off + sz);
+    off += sz;
+  }
+  return o;
+}
+
+static uint32_t crc32_ieee(const uint8_t* p, size_t n) {
+  static uint32_t tab[256];
+  static bool init = false;
+  if (!init) {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int j = 0; j < 8; j++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+      tab[i] = c;
+    }
+    init = true;
+  }
+  uint32_t c = 0xffffffffu;
+  for (size_t i = 0; i < n; i++) c = tab[(c ^ p[i]) & 0xff] ^ (c >> 8);
+  return c ^ 0xffffffffu;
+}
+
+static std::vector<uint8_t> crc_sidecar(const uint8_t* p, size_t n) {
+  std::vector<uint8_t> o = {'c', 'r', 'c', 0};
+  put_be32(o, 512);
+  for (size_t off = 0; off < n; off += 512) put_be32(o, crc32_ieee(p + off, std::min((size_t)512, n - off)));
+  return o;
+}
+
+// Text dump of a record stream: key.toString() + "\t" + value.toString()
+// (GenomixClusterManager.copyBinAndTextToLocal, GenomixClusterManager.java:371).
+static std::string dump_text(const uint8_t* recs, size_t n) {
+  std::string o;
+  size_t off = 0;
+  while (off < n) {
+    if (off + 8 > n) throw OraError(E_INPUT, "truncated record stream");
+    uint32_t recLen = get_be32(recs + off);
+    uint32_t keyLen = get_be32(recs + off + 4);
+    const uint8_t* kp = recs + off + 8;
+    if (off + 8 + recLen > n || keyLen > recLen) throw OraError(E_INPUT, "bad record");
+    VKmer key;
+    read_vkmer(kp, keyLen, key);
+    Node node;
+    node.read(kp + keyLen, recLen - keyLen);
+    o += key.str() + "\t" + node_str(node) + "\n";
+    off += 8 + recLen;
+  }
+  return o;
+}
+
+// GenomixDriver.convertAndUploadFastqToHDFS (GenomixDriver.java:665-714):
+// readId = line number (1-based) of the sequence line = 4i+2; lines trimmed
+// (String.trim removes chars <= ' ').
+static std::string fastq_to_readids(const uint8_t* m1, size_t n1, const uint8_t* m2, size_t n2, bool paired) {
+  auto lines = [](const uint8_t* t, size_t n) {
+    std::vector<std::pair<const uint8_t*, size_t>> v;
+    for_each_line(t, n, [&](size_t, const uint8_t* l, size_t ln) { v.push_back({l, ln}); });
+    return v;
+  };
+  auto trim = [](std::pair<const uint8_t*, size_t> l) {
+    const uint8_t* p = l.first;
+    size_t n = l.second;
+    while (n && *p <= ' ') { p++; n--; }
+    while (n && p[n - 1] <= ' ') n--;
+    return std::string((const char*)p, n);
+  };
+  auto a = lines(m1, n1);
+  std::string o;
+  if (paired) {
+    auto b = lines(m2, n2);
+    if (a.size() != b.size()) throw OraError(E_INPUT, "Fastq files didn't have the same number of lines!");
+    for (size_t i = 0; i < a.size(); i++)
+      if (i % 4 == 1) o += std::to_string(i + 1) + "\t" + trim(a[i]) + "\t" + trim(b[i]) + "\n";
+  } else {
+    for (size_t i = 0; i < a.size(); i++)
+      if (i % 4 == 1) o += std::to_string(i + 1) + "\t" + trim(a[i]) + "\n";
+  }
+  return o;
+}
+
+}  // namespace ora
+
+// ---------------------------------------------------------------------------
+// C ABI for ctypes (oracle/oracle.py)
+// ---------------------------------------------------------------------------
+
+using namespace ora;
+
+static thread_local std::string g_err;
+
+struct ora_result {
+  BuildResult r;
+  std::vector<std::vector<uint8_t>> recs;
+};
+
+static int guard(const std::function<void()>& f) {
+  try {
+    f();
+    return 0;
+  } catch (const OraError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+static uint8_t* dup_bytes(const void* p, size_t n) {
+  uint8_t* o = (uint8_t*)malloc(n ? n : 1);
+  if (n) memcpy(o, p, n);
+  return o;
+}
+
+extern "C" {
+
+const char* ora_last_error() { return g_err.c_str(); }
+// Test hook: 0 = JDK-7 HashSet (the model), 1/2 = planted wrong models.
+void ora_set_hash_model(int m) { g_hash_model = m; }
+void ora_free(void* p) { free(p); }
+
+int ora_build(const char* text, size_t len, int k, int nparts, const int64_t* split_starts, int nsplits,
+              int local_only, ora_result** out) {
+  return guard([&] {
+    auto* r = new ora_result;
+    try {
+      r->r = build((const uint8_t*)text, len, k, nparts, split_starts, nsplits, local_only != 0);
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    for (size_t p = 0; p < r->r.parts.size(); p++) r->recs.push_back(part_records(r->r, (int)p));
+    *out = r;
+  });
+}
+
+int ora_result_nparts(const ora_result* r) { return (int)r->recs.size(); }
+int64_t ora_result_nrecords(const ora_result* r, int part) { return (int64_t)r->r.parts[part].size(); }
+size_t ora_result_part(const ora_result* r, int part, const uint8_t** data) {
+  *data = r->recs[part].data();
+  return r->recs[part].size();
+}
+void ora_result_free(ora_result* r) { delete r; }
+
+int ora_seqfile(const uint8_t* recs, size_t n, const uint8_t* sync16, const char* keycls, const char* valcls,
+                uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = seqfile_wrap(recs, n, sync16, keycls, valcls);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_crc_sidecar(const uint8_t* p, size_t n, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = crc_sidecar(p, n);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_dump_text(const uint8_t* recs, size_t n, char** out, size_t* outlen) {
+  return guard([&] {
+    auto s = dump_text(recs, n);
+    *outlen = s.size();
+    *out = (char*)dup_bytes(s.data(), s.size());
+  });
+}
+
+int ora_fastq_to_readids(const char* m1, size_t n1, const char* m2, size_t n2, int paired, char** out,
+                         size_t* outlen) {
+  return guard([&] {
+    auto s = fastq_to_readids((const uint8_t*)m1, n1, (const uint8_t*)m2, n2, paired != 0);
+    *outlen = s.size();
+    *out = (char*)dup_bytes(s.data(), s.size());
+  });
+}
+
+// Primitive hooks for the tests (string-level in, string-level out).
+int ora_pack(const char* s, int k, int reverse, uint8_t* out) {
+  return guard([&] {
+    size_t n = strlen(s);
+    if (reverse) pack_reverse((const uint8_t*)s, n, 0, k, out);
+    else pack_forward((const uint8_t*)s, n, 0, k, out);
+  });
+}
+int ora_shift(uint8_t* b, int k, char c) {
+  return guard([&] { shift_with_next(b, k, gene_code((uint8_t)c)); });
+}
+int ora_kmer_str(const uint8_t* b, int k, char* out) {
+  return guard([&] {
+    auto s = kmer_to_string(k, b, bytes_for_k(k));
+    memcpy(out, s.c_str(), s.size() + 1);
+  });
+}
+int32_t ora_vkmer_hash(const char* s) {
+  auto v = VKmer::from_string((const uint8_t*)s, strlen(s));
+  return v.hash_code();
+}
+// Union-update of two comma-separated k-mer lists (JDK-7 HashSet order);
+// writes the comma-separated result.
+int ora_union_update(const char* L, const char* M, char* out, size_t cap) {
+  return guard([&] {
+    auto split = [](const char* s) {
+      std::vector<VKmer> v;
+      std::string cur;
+      for (const char* p = s;; p++) {
+        if (*p == ',' || *p == 0) {
+          if (!cur.empty()) v.push_back(VKmer::from_string((const uint8_t*)cur.data(), cur.size()));
+          cur.clear();
+          if (!*p) break;
+        } else {
+          cur.push_back(*p);
+        }
+      }
+      return v;
+    };
+    auto r = union_update(split(L), split(M));
+    std::string o;
+    for (size_t i = 0; i < r.size(); i++) o += (i ? "," : "") + r[i].str();
+    if (o.size() + 1 > cap) throw OraError(E_INVAL, "buffer too small");
+    memcpy(out, o.c_str(), o.size() + 1);
+  });
+}
+int64_t ora_make_uuid(int mate, int lib, int64_t readId, int offset) {
+  int64_t v = 0;
+  if (guard([&] { v = make_uuid(mate, lib, readId, offset); }) != 0) return INT64_MIN;
+  return v;
+}
+const char* ora_float_str(float f) {
+  static thread_local std::string s;
+  s = java_float_str(f);
+  return s.c_str();
+}
+
+}  // extern "C"

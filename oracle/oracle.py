@@ -1,0 +1,219 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+The oracle restates the Java reference's graph build on the CPU
+(genomix_oracle.cpp).  It is imported only by tests/, by
+``__graft_entry__.smoke()`` as the checker, and by bench.py's
+``cpu_baseline`` leg.  The product library (genomix_amd) never loads it.
+
+What pins each part of the oracle (no JVM exists here, so the reference
+cannot be run; nothing was denied, it is simply unbuildable):
+
+* 2-bit packing, rolling shift, reverse complement, toString:
+  KmerFixedTest.java:31-100 vectors (tests/test_oracle_primitives.py) plus
+  brute force against string reverse complement for k = 3..63.
+* Canonical choice, coverage, read heads and their order, edge SETS:
+  the nine current-format text fixtures of genomix-hyracks
+  (tests/golden/hyracks/**, k = 3), compared line by line after sorting,
+  edge lists as sets (as BatchTestFilesTest does) and read-head sets exactly.
+* Edge-list ORDER (JDK-7 HashSet replay): every multi-element edge list in
+  those fixtures is reproduced exactly by replaying the fixture input through
+  the reference's split/partition structure (four Hadoop input splits, local
+  aggregation per split, cross aggregation in some producer order); see
+  tests/test_oracle_fixtures.py::test_fixture_edge_order_replay.  Spilled
+  external sorts are outside the model (the oracle never spills).
+* Record order within a part and partition assignment: closed forms of
+  KmerPointable.java:94-107 and KmerPartitionComputerFactory.java:28-53,
+  checked against hand-computed Java int arithmetic; parity unpinned by any
+  reference test (the reference's tests sort lines).
+* Node / ReadHeadSet byte layout: source only (no current-format binary
+  fixture exists); round-trip tested.
+* SequenceFile container and .crc sidecar: pinned by the reference's binary
+  fixtures (tests/golden/seqfile/**): header fields, sync cadence and CRC
+  chunks are recomputed from their bytes.
+* FASTQ -> readids (readId = 4i+2): source only (GenomixDriver.java:665-714).
+* Unitig walker: VKmerFixedTest merge vectors for the overlap concatenation,
+  the RandMergeOffsetTest invariant (one genome -> one unitig equal to it or
+  its reverse complement, read-head offsets at read positions), and brute
+  force on small graphs.
+"""
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ only)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp, sz, i32, i64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int64
+        L.ora_last_error.restype = ctypes.c_char_p
+        L.ora_build.argtypes = [ctypes.c_char_p, sz, i32, i32, ctypes.POINTER(i64), i32, i32,
+                                ctypes.POINTER(vp)]
+        L.ora_result_nparts.argtypes = [vp]
+        L.ora_result_nrecords.argtypes = [vp, i32]
+        L.ora_result_nrecords.restype = i64
+        L.ora_result_part.argtypes = [vp, i32, ctypes.POINTER(vp)]
+        L.ora_result_part.restype = sz
+        L.ora_result_free.argtypes = [vp]
+        L.ora_free.argtypes = [vp]
+        L.ora_seqfile.argtypes = [vp, sz, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                  ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_crc_sidecar.argtypes = [vp, sz, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_dump_text.argtypes = [vp, sz, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_fastq_to_readids.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, sz, i32,
+                                           ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_pack.argtypes = [ctypes.c_char_p, i32, i32, ctypes.c_char_p]
+        L.ora_shift.argtypes = [ctypes.c_char_p, i32, ctypes.c_char]
+        L.ora_kmer_str.argtypes = [ctypes.c_char_p, i32, ctypes.c_char_p]
+        L.ora_vkmer_hash.argtypes = [ctypes.c_char_p]
+        L.ora_vkmer_hash.restype = ctypes.c_int32
+        L.ora_union_update.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
+        L.ora_make_uuid.argtypes = [i32, i32, i64, i32]
+        L.ora_make_uuid.restype = i64
+        L.ora_float_str.argtypes = [ctypes.c_float]
+        L.ora_float_str.restype = ctypes.c_char_p
+        L.ora_set_hash_model.argtypes = [i32]
+        if hasattr(L, "ora_unitigs"):
+            L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"oracle error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(rc, lib().ora_last_error().decode(errors="replace"))
+
+
+def _take(p, n):
+    out = ctypes.string_at(p, n.value if hasattr(n, "value") else n)
+    lib().ora_free(p)
+    return out
+
+
+def build_graph(text: bytes, k: int, nparts: int = 1, split_starts=None, local_only=False):
+    """Run the restated Hyracks graph build; returns a list of per-part record
+    streams (i32 recLen, i32 keyLen, VKmer key, Node value; no header)."""
+    L = lib()
+    res = ctypes.c_void_p()
+    if split_starts is None:
+        sp, ns = None, 0
+    else:
+        arr = (ctypes.c_int64 * len(split_starts))(*split_starts)
+        sp, ns = arr, len(split_starts)
+    _check(L.ora_build(text, len(text), k, nparts, sp, ns, 1 if local_only else 0, ctypes.byref(res)))
+    try:
+        parts = []
+        for p in range(L.ora_result_nparts(res)):
+            d = ctypes.c_void_p()
+            n = L.ora_result_part(res, p, ctypes.byref(d))
+            parts.append(ctypes.string_at(d, n))
+        return parts
+    finally:
+        L.ora_result_free(res)
+
+
+KEY_CLASS = "edu.uci.ics.genomix.data.types.VKmer"
+VALUE_CLASS = "edu.uci.ics.genomix.data.types.Node"
+
+
+def seqfile(records: bytes, sync: bytes, key_class=KEY_CLASS, value_class=VALUE_CLASS) -> bytes:
+    """Wrap a record stream in a SequenceFile v6 container (header + syncs)."""
+    assert len(sync) == 16
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_seqfile(records, len(records), sync, key_class.encode(), value_class.encode(),
+                             ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def crc_sidecar(data: bytes) -> bytes:
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_crc_sidecar(data, len(data), ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def dump_text(records: bytes) -> str:
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_dump_text(records, len(records), ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n).decode()
+
+
+def fastq_to_readids(mate1: bytes, mate2: bytes = None) -> bytes:
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    paired = mate2 is not None
+    _check(lib().ora_fastq_to_readids(mate1, len(mate1), mate2 or b"", len(mate2 or b""), int(paired),
+                                      ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def pack(s: str, k: int, reverse=False) -> bytes:
+    nb = (k + 3) // 4
+    out = ctypes.create_string_buffer(nb)
+    _check(lib().ora_pack(s.encode(), k, int(reverse), out))
+    return out.raw[:nb]
+
+
+def shift(b: bytes, k: int, c: str) -> bytes:
+    buf = ctypes.create_string_buffer(bytes(b), len(b))
+    _check(lib().ora_shift(buf, k, c.encode()))
+    return buf.raw[:len(b)]
+
+
+def kmer_str(b: bytes, k: int) -> str:
+    out = ctypes.create_string_buffer(k + 1)
+    _check(lib().ora_kmer_str(bytes(b), k, out))
+    return out.value.decode()
+
+
+def vkmer_hash(s: str) -> int:
+    return lib().ora_vkmer_hash(s.encode())
+
+
+def union_update(L, M):
+    out = ctypes.create_string_buffer(4096)
+    _check(lib().ora_union_update(",".join(L).encode(), ",".join(M).encode(), out, 4096))
+    v = out.value.decode()
+    return v.split(",") if v else []
+
+
+def set_hash_model(model: int):
+    """Test hook: 0 = JDK-7 HashSet (the model), 1 = JDK-8, 2 = JDK-6 (planted
+    wrong models, negative controls only)."""
+    lib().ora_set_hash_model(model)
+
+
+def make_uuid(mate, lib_id, read_id, offset):
+    v = lib().ora_make_uuid(mate, lib_id, read_id, offset)
+    if v == -(1 << 63):
+        raise OracleError(-5, lib().ora_last_error().decode())
+    return v
+
+
+def float_str(f: float) -> str:
+    return lib().ora_float_str(f).decode()
+
+
+def unitigs(records: bytes, k: int) -> bytes:
+    """Sequential maximal-unitig walker over a graph record stream; returns one
+    merged record per maximal unitig (key = its smallest canonical k-mer,
+    sequence oriented so that k-mer reads forward), sorted by key."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_unitigs(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
