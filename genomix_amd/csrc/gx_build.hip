@@ -1,0 +1,1392 @@
+// Graph build on one MI355X: readids text in HBM -> Node records in HBM.
+//
+// Pipeline (one HIP stream per handle, kernels in this order):
+//   1. k_line_count / k_line_write   line terminators (Hadoop LineReader rules)
+//   2. k_parse_lines                 one wave per line: tab fields, readId
+//   3. k_pack_mates                  one wave per mate: validity + 2-bit pack
+//   4. k_insert<W>                   one wave per mate: canonical k-mers into an
+//                                    open-addressing table (count + neighbour
+//                                    mask with atomics), one record per
+//                                    occurrence in emission order
+//   5. k_compact / radix sort        dense node array in KmerPointable order
+//   6. replay (k_sel_*, radix sort, k_replay)
+//                                    JDK-7 HashSet edge order for nodes whose
+//                                    edge lists hold >= 2 k-mers
+//   7. heads (k_heads, radix sort)   TreeSet order of 5' read heads
+//   8. k_ser_size / k_ser_write      Node bytes, Java layout, big endian
+// Each kernel cites the reference code whose semantics it reproduces.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/genomix_gx.h"
+#include "gx_kmer.h"
+#include "gx_prims.h"
+
+namespace gx {
+
+struct GxError : std::runtime_error {
+  int code;
+  GxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum : uint32_t { ERR_FIELDS = 1, ERR_READID = 2, ERR_READID_RANGE = 3, ERR_SHORT = 4, ERR_TABLE = 6 };
+
+// Device error word: (line << 8) | code; the smallest line wins, as the
+// reference aborts on the first bad line.
+__device__ inline void report(unsigned long long* err, uint64_t line, uint32_t code) {
+  atomicMin(err, (unsigned long long)((line << 8) | code));
+}
+
+// ---------------------------------------------------------------------------
+// 1. line terminators: '\n', '\r' not followed by '\n', and the end of the
+//    buffer when the last line has no terminator (Hadoop 0.20.2 LineReader).
+// ---------------------------------------------------------------------------
+
+constexpr int kLB = 16;                 // bytes per thread
+constexpr int kLTile = kBlock * kLB;    // bytes per block
+
+__device__ inline uint32_t count_terms(const uint8_t* __restrict__ t, size_t n, size_t i0, uint32_t* bits,
+                                       unsigned long long* nonascii) {
+  uint32_t c = 0, m = 0;
+#pragma unroll
+  for (int j = 0; j < kLB; j++) {
+    size_t i = i0 + j;
+    bool term = false;
+    if (i < n) {
+      uint8_t ch = t[i];
+      if (ch & 0x80) atomicMin(nonascii, (unsigned long long)i);
+      term = ch == '\n' || (ch == '\r' && (i + 1 == n || t[i + 1] != '\n'));
+    } else if (i == n) {
+      term = n > 0 && t[n - 1] != '\n' && t[n - 1] != '\r';
+    }
+    if (term) { c++; m |= 1u << j; }
+  }
+  *bits = m;
+  return c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_line_count(const uint8_t* __restrict__ t, size_t n,
+                                                       uint64_t* __restrict__ blockcnt,
+                                                       unsigned long long* nonascii) {
+  size_t i0 = (size_t)blockIdx.x * kLTile + (size_t)threadIdx.x * kLB;
+  uint32_t bits;
+  uint64_t c = count_terms(t, n, i0, &bits, nonascii);
+  uint64_t tot;
+  (void)block_exclusive_scan<uint64_t>(c, &tot);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kBlock) k_line_write(const uint8_t* __restrict__ t, size_t n,
+                                                       const uint64_t* __restrict__ blockbase,
+                                                       uint64_t* __restrict__ term,
+                                                       unsigned long long* nonascii) {
+  size_t i0 = (size_t)blockIdx.x * kLTile + (size_t)threadIdx.x * kLB;
+  uint32_t bits;
+  uint64_t c = count_terms(t, n, i0, &bits, nonascii);
+  uint64_t tot;
+  uint64_t pos = block_exclusive_scan<uint64_t>(c, &tot) + blockbase[blockIdx.x];
+  while (bits) {
+    int j = __builtin_ctz(bits);
+    bits &= bits - 1;
+    term[pos++] = i0 + j;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. fields of each line: ReadsKeyValueParserFactory.parse (:95-148).
+//    value.toString().split("\\t") drops trailing empty strings; 2 or 3
+//    fields are required, the first must parse as a Java long.
+// ---------------------------------------------------------------------------
+
+struct LineInfo {
+  int64_t rid;
+};
+
+__global__ void __launch_bounds__(kBlock) k_parse_lines(const uint8_t* __restrict__ t, size_t n,
+                                                        const uint64_t* __restrict__ term, uint64_t nlines,
+                                                        int64_t* __restrict__ rid, uint64_t* __restrict__ mstart,
+                                                        uint32_t* __restrict__ mlen, unsigned long long* err) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t line = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); line < nlines; line += nw) {
+    uint64_t s = line ? term[line - 1] + 1 : 0;
+    uint64_t e = term[line];
+    if (e < n && t[e] == '\n' && e > s && t[e - 1] == '\r') e--;
+    uint64_t ntabs = 0;
+    int64_t t1 = -1, t2 = -1, lastnt = (int64_t)s - 1;
+    for (uint64_t b = s; b < e; b += 64) {
+      uint64_t pos = b + lane;
+      bool in = pos < e;
+      uint8_t ch = in ? t[pos] : 0;
+      uint64_t mt = __ballot(in && ch == '\t');
+      uint64_t mn = __ballot(in && ch != '\t');
+      ntabs += __popcll(mt);
+      if (t1 < 0 && mt) { t1 = (int64_t)(b + __builtin_ctzll(mt)); mt &= mt - 1; }
+      if (t2 < 0 && mt) { t2 = (int64_t)(b + __builtin_ctzll(mt)); }
+      if (mn) lastnt = (int64_t)(b + 63 - __builtin_clzll(mn));
+    }
+    uint64_t e2 = (uint64_t)(lastnt + 1);           // trailing tabs stripped
+    uint64_t nt = ntabs - (e - e2);                  // tabs inside [s, e2)
+    uint64_t f1s = 0, f1e = 0, f2s = 0, f2e = 0;
+    bool ok = e2 > s && (nt == 1 || nt == 2);
+    int64_t id = 0;
+    if (ok) {
+      f1s = (uint64_t)t1 + 1;
+      f1e = nt == 2 ? (uint64_t)t2 : e2;
+      if (nt == 2) { f2s = (uint64_t)t2 + 1; f2e = e2; }
+      // Long.parseLong of field 0 (lane-uniform loop; the field is short).
+      uint64_t fs = s, fe = (uint64_t)t1;
+      bool neg = false, good = fe > fs;
+      uint64_t i = fs;
+      if (good && (t[i] == '-' || t[i] == '+')) { neg = t[i] == '-'; i++; good = i < fe; }
+      unsigned __int128 v = 0;
+      for (; good && i < fe; i++) {
+        uint8_t ch = t[i];
+        if (ch < '0' || ch > '9') { good = false; break; }
+        v = v * 10 + (ch - '0');
+        if (v > ((unsigned __int128)1 << 63)) good = false;
+      }
+      if (good && !neg && v == ((unsigned __int128)1 << 63)) good = false;
+      if (!good) {
+        if (lane == 0) report(err, line, ERR_READID);
+        ok = false;
+      } else {
+        id = neg ? (int64_t)(0 - (uint64_t)v) : (int64_t)(uint64_t)v;
+      }
+    } else if (lane == 0) {
+      report(err, line, ERR_FIELDS);
+    }
+    if (lane == 0) {
+      rid[line] = id;
+      mstart[2 * line] = f1s;
+      mlen[2 * line] = ok ? (uint32_t)(f1e - f1s) : 0;
+      mstart[2 * line + 1] = f2s;
+      mlen[2 * line + 1] = ok ? (uint32_t)(f2e - f2s) : 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. per mate: "[ACGTacgt]+" validity (:125-128,139-141) and 2-bit packing
+//    with GeneCode.getCodeFromSymbol (GeneCode.java:29-50; other bytes -> A),
+//    base i at bits 2i of 64-bit word i/32, zero padded.
+// ---------------------------------------------------------------------------
+
+__device__ inline uint64_t spread32(uint32_t x32) {
+  uint64_t x = x32;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+__device__ inline uint32_t code_of(uint8_t ch) {
+  switch (ch) {
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+  }
+  return 0;
+}
+__device__ inline bool is_acgt(uint8_t ch) {
+  switch (ch) {
+    case 'A': case 'C': case 'G': case 'T': case 'a': case 'c': case 'g': case 't': return true;
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_mates(const uint8_t* __restrict__ t, uint64_t nmates,
+                                                       const uint64_t* __restrict__ mstart,
+                                                       const uint32_t* __restrict__ mlen,
+                                                       const uint64_t* __restrict__ wbase,
+                                                       const int64_t* __restrict__ rid, int k,
+                                                       uint64_t* __restrict__ words, uint8_t* __restrict__ valid,
+                                                       uint64_t* __restrict__ nocc, unsigned long long* err) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t m = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); m < nmates; m += nw) {
+    uint32_t L = mlen[m];
+    uint64_t fs = mstart[m];
+    uint64_t* w = words + wbase[m];
+    bool bad = false;
+    for (uint32_t b = 0; b < L; b += 64) {
+      uint32_t i = b + lane;
+      uint8_t ch = i < L ? t[fs + i] : 'A';
+      uint32_t c = i < L ? code_of(ch) : 0;
+      bad |= __ballot(i < L && !is_acgt(ch)) != 0;
+      uint64_t lo = __ballot(c & 1), hi = __ballot(c >> 1);
+      uint64_t w0 = spread32((uint32_t)lo) | (spread32((uint32_t)hi) << 1);
+      uint64_t w1 = spread32((uint32_t)(lo >> 32)) | (spread32((uint32_t)(hi >> 32)) << 1);
+      uint32_t wi = b / 32;
+      if (lane == 0) w[wi] = w0;
+      if (lane == 1 && b + 32 < L) w[wi + 1] = w1;
+    }
+    bool v = L > 0 && !bad;
+    uint64_t occ = 0;
+    if (v) {
+      // ReadHeadInfo.makeUUID readId check (ReadHeadInfo.java:110-113, 29
+      // bits) and SplitReads' length check (ReadsKeyValueParserFactory.java:152-155).
+      int64_t id = rid[m >> 1];
+      if (id < 0 || id >= (1ll << 29)) {
+        if (lane == 0) report(err, m >> 1, ERR_READID_RANGE);
+      } else if ((uint32_t)k >= L) {
+        if (lane == 0) report(err, m >> 1, ERR_SHORT);
+      } else {
+        occ = L - k + 1;
+      }
+    }
+    if (lane == 0) {
+      valid[m] = v ? 1 : 0;
+      nocc[m] = occ;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. canonical k-mers into the table.  Per occurrence (SplitReads,
+//    ReadsKeyValueParserFactory.java:150-196): key = min(V, R), dir F iff
+//    V <= R; neighbours become an 8-bit mask relative to the canonical
+//    node (bit c: successor X[1:]+c, bit 4+c: predecessor c+X[:-1]),
+//    equivalent to setEdgesForCurAndNext (:209-233) over both the cur and
+//    the mirrored next side.  Coverage = number of occurrences
+//    (AggregateKmerAggregateFactory.java:124,144).
+//    The occurrence record code byte: bits 0-1 successor letter, bit 2
+//    successor present, bits 3-4 predecessor letter, bit 5 predecessor
+//    present, bit 6 occurrence read in reverse (flipped).
+// ---------------------------------------------------------------------------
+
+struct Slot1 {
+  uint64_t key;
+  uint32_t count;
+  uint32_t mask;
+};
+struct Slot2 {
+  uint64_t hi;
+  uint64_t lo;
+  uint32_t count;
+  uint32_t mask;
+  uint64_t pad;
+};
+constexpr uint64_t kEmpty = ~0ull;
+constexpr uint64_t kPend = 1ull << 63;
+
+__device__ inline uint64_t ld_relaxed(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_acquire(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t ld_relaxed32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns the slot index, or ~0 when the probe limit was hit (table too full).
+template <int W>
+__device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, uint64_t limit) {
+  uint64_t s = khash<W>(key) & capmask;
+  if (W == 1) {
+    Slot1* T = (Slot1*)table;
+    for (uint64_t probe = 0; probe < limit; probe++) {
+      uint64_t cur = ld_relaxed(&T[s].key);
+      if (cur == key.lo) return s;
+      if (cur == kEmpty) {
+        uint64_t prev = atomicCAS((unsigned long long*)&T[s].key, (unsigned long long)kEmpty,
+                                  (unsigned long long)key.lo);
+        if (prev == kEmpty || prev == key.lo) return s;
+      }
+      s = (s + 1) & capmask;
+    }
+    return ~0ull;
+  } else {
+    // Two-word keys without a 128-bit CAS: claim the slot by CAS-ing the
+    // high word from EMPTY to (hi | PENDING), store the low word, then
+    // publish hi with a release store.  Readers that see PENDING retry the
+    // same slot; the claiming lane publishes within the same loop
+    // iteration, so lanes of one wave never wait on each other.
+    Slot2* T = (Slot2*)table;
+    uint64_t probe = 0;
+    while (probe < limit) {
+      uint64_t cur = ld_acquire(&T[s].hi);
+      if (cur == kEmpty) {
+        uint64_t prev = atomicCAS((unsigned long long*)&T[s].hi, (unsigned long long)kEmpty,
+                                  (unsigned long long)(key.hi | kPend));
+        if (prev == kEmpty) {
+          __hip_atomic_store(&T[s].lo, key.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&T[s].hi, key.hi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          return s;
+        }
+        cur = prev;
+      }
+      if (cur & kPend) {
+        probe++;  // bounded wait on a slot being published
+        continue;
+      }
+      if (cur == key.hi && ld_relaxed(&T[s].lo) == key.lo) return s;
+      s = (s + 1) & capmask;
+      probe++;
+    }
+    return ~0ull;
+  }
+}
+
+template <int W>
+__device__ inline KV load_kmer(const uint64_t* __restrict__ w, uint32_t p, int k) {
+  // bits [2p, 2p+2k) of the packed stream
+  uint32_t bit = 2 * p;
+  uint32_t wi = bit >> 6, sh = bit & 63;
+  KV v;
+  uint64_t a = w[wi];
+  uint64_t b = (2 * k + sh > 64) ? w[wi + 1] : 0;
+  v.lo = sh ? (a >> sh) | (b << (64 - sh)) : a;
+  if (W == 2) {
+    uint64_t c = (2 * k + sh > 128) ? w[wi + 2] : 0;
+    v.hi = sh ? (b >> sh) | (c << (64 - sh)) : b;
+  } else {
+    v.hi = 0;
+  }
+  KV m = kmask<W>(k);
+  v.lo &= m.lo;
+  v.hi &= m.hi;
+  return v;
+}
+
+__device__ inline uint32_t base_at(const uint64_t* __restrict__ w, uint32_t i) {
+  return (uint32_t)(w[i >> 5] >> (2 * (i & 31))) & 3;
+}
+
+// Edge types in EDGETYPE order (EDGETYPE.java:6-9).  The table's neighbour
+// mask has bit 4t+x set when list t holds the neighbour with letter x:
+// successor X[1:]+x for FF (stored as is) and FR (stored reverse-
+// complemented), predecessor x+X[:-1] for RR (as is) and RF (reverse-
+// complemented).  For odd k the type follows from the neighbour's
+// canonicality; for even k a palindromic neighbour is stored under either
+// type depending on the strand of the read, so the type is recorded per
+// occurrence.
+enum : uint32_t { ET_FF = 0, ET_FR = 1, ET_RF = 2, ET_RR = 3 };
+
+// Occurrence code: low nibble = successor-side entry, high nibble =
+// predecessor-side entry; per nibble bit 3 present, bit 2 FR (resp. RF),
+// bits 0-1 letter.
+__device__ inline uint32_t nibble(uint32_t t, uint32_t x) {
+  return 8u | ((t == ET_FR || t == ET_RF) ? 4u : 0u) | x;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32_t* __restrict__ mlen,
+                                                   const uint64_t* __restrict__ wbase,
+                                                   const uint64_t* __restrict__ words,
+                                                   const uint64_t* __restrict__ occbase,
+                                                   const uint64_t* __restrict__ nocc, int k, void* table,
+                                                   uint64_t capmask, uint64_t limit,
+                                                   uint64_t* __restrict__ occ_slot, uint8_t* __restrict__ occ_code,
+                                                   unsigned long long* err) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t m = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); m < nmates; m += nw) {
+    uint64_t nk = nocc[m];
+    if (nk == 0) continue;
+    uint32_t L = mlen[m];
+    const uint64_t* w = words + wbase[m];
+    uint64_t ob = occbase[m];
+    for (uint32_t p = lane; p < nk; p += 64) {
+      KV v = load_kmer<W>(w, p, k);
+      bool fwd;
+      KV key = canon<W>(v, k, &fwd);
+      uint32_t code = 0, bits = 0;
+      if (p + k < L) {  // the read's next k-mer w' = w[1:]+c
+        uint32_t c = base_at(w, p + k);
+        KV nx = succ_of<W>(v, k, c);
+        bool nfwd = kle<W>(nx, revcomp<W>(nx, k));
+        // (cur F, next F) -> FF w'; (F, R) -> FR rc(w'); (R, F) -> RF w';
+        // (R, R) -> RR rc(w')   (setEdgesForCurAndNext, cur side)
+        uint32_t t = fwd ? (nfwd ? ET_FF : ET_FR) : (nfwd ? ET_RF : ET_RR);
+        uint32_t x = fwd ? c : 3 - c;  // letter relative to the canonical node
+        code |= nibble(t, x) << (fwd ? 0 : 4);
+        bits |= 1u << (4 * t + x);
+      }
+      if (p > 0) {  // the read's previous k-mer w'' = c+w[:-1]
+        uint32_t c = base_at(w, p - 1);
+        KV pv = pred_of<W>(v, k, c);
+        bool pfwd = kle<W>(pv, revcomp<W>(pv, k));
+        // (prev F, cur F) -> RR w''; (F, R) -> FR w''; (R, F) -> RF rc(w'');
+        // (R, R) -> FF rc(w'')   (setEdgesForCurAndNext, next side)
+        uint32_t t = fwd ? (pfwd ? ET_RR : ET_RF) : (pfwd ? ET_FR : ET_FF);
+        uint32_t x = fwd ? c : 3 - c;
+        code |= nibble(t, x) << (fwd ? 4 : 0);
+        bits |= 1u << (4 * t + x);
+      }
+      uint64_t s = table_insert<W>(table, capmask, key, limit);
+      if (s == ~0ull) {
+        report(err, ~0ull >> 8, ERR_TABLE);
+        occ_slot[ob + p] = 0;
+      } else {
+        if (W == 1) {
+          Slot1* T = (Slot1*)table;
+          atomicAdd(&T[s].count, 1u);
+          if ((ld_relaxed32(&T[s].mask) & bits) != bits) atomicOr(&T[s].mask, bits);
+        } else {
+          Slot2* T = (Slot2*)table;
+          atomicAdd(&T[s].count, 1u);
+          if ((ld_relaxed32(&T[s].mask) & bits) != bits) atomicOr(&T[s].mask, bits);
+        }
+        occ_slot[ob + p] = s;
+      }
+      occ_code[ob + p] = (uint8_t)code;
+    }
+  }
+}
+
+template <int W>
+__global__ void k_table_clear(void* table, uint64_t cap) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (W == 1) {
+      Slot1* T = (Slot1*)table;
+      T[i].key = kEmpty;
+      T[i].count = 0;
+      T[i].mask = 0;
+    } else {
+      Slot2* T = (Slot2*)table;
+      T[i].hi = kEmpty;
+      T[i].lo = 0;
+      T[i].count = 0;
+      T[i].mask = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. compaction to a dense node array and the KmerPointable sort key:
+//    bytes compared from the last to the first (KmerPointable.java:94-107),
+//    i.e. the integer whose byte i is big-endian key byte i.
+// ---------------------------------------------------------------------------
+
+constexpr int kCItems = 16;
+constexpr int kCTile = kBlock * kCItems;
+
+template <int W>
+__device__ inline bool slot_used(const void* table, uint64_t i) {
+  return W == 1 ? ((const Slot1*)table)[i].key != kEmpty : ((const Slot2*)table)[i].hi != kEmpty;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_compact_count(const void* table, uint64_t cap, uint64_t* blockcnt) {
+  uint64_t c = 0;
+  uint64_t base = (uint64_t)blockIdx.x * kCTile;
+  for (int i = 0; i < kCItems; i++) {
+    uint64_t j = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (j < cap && slot_used<W>(table, j)) c++;
+  }
+  uint64_t tot;
+  (void)block_exclusive_scan<uint64_t>(c, &tot);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+template <int W>
+__device__ inline KV slot_key(const void* table, uint64_t i) {
+  KV v;
+  if (W == 1) {
+    v.lo = ((const Slot1*)table)[i].key;
+    v.hi = 0;
+  } else {
+    v.lo = ((const Slot2*)table)[i].lo;
+    v.hi = ((const Slot2*)table)[i].hi;
+  }
+  return v;
+}
+
+__device__ inline uint64_t sortkey64(uint64_t v, int nb) { return __builtin_bswap64(v) >> (64 - 8 * nb); }
+__device__ inline Key128 sortkey128(KV v, int nb) {
+  // bswap over 128 bits, then shift right by 128 - 8nb
+  uint64_t lo = __builtin_bswap64(v.hi), hi = __builtin_bswap64(v.lo);
+  int sh = 128 - 8 * nb;
+  Key128 r;
+  if (sh >= 64) {
+    r.lo = hi >> (sh - 64);
+    r.hi = 0;
+  } else if (sh == 0) {
+    r.lo = lo;
+    r.hi = hi;
+  } else {
+    r.lo = (lo >> sh) | (hi << (64 - sh));
+    r.hi = hi >> sh;
+  }
+  return r;
+}
+
+template <int W, class SK>
+__global__ void __launch_bounds__(kBlock) k_compact_write(const void* table, uint64_t cap,
+                                                          const uint64_t* __restrict__ blockbase, int nb,
+                                                          SK* __restrict__ skey, uint32_t* __restrict__ sval,
+                                                          uint64_t* __restrict__ nslot) {
+  uint64_t base = (uint64_t)blockIdx.x * kCTile + (uint64_t)threadIdx.x * kCItems;
+  uint64_t c = 0;
+  for (int i = 0; i < kCItems; i++) {
+    uint64_t j = base + i;
+    if (j < cap && slot_used<W>(table, j)) c++;
+  }
+  uint64_t tot;
+  uint64_t pos = block_exclusive_scan<uint64_t>(c, &tot) + blockbase[blockIdx.x];
+  for (int i = 0; i < kCItems; i++) {
+    uint64_t j = base + i;
+    if (j < cap && slot_used<W>(table, j)) {
+      KV v = slot_key<W>(table, j);
+      if constexpr (W == 1) skey[pos] = sortkey64(v.lo, nb);
+      else skey[pos] = sortkey128(v, nb);
+      sval[pos] = (uint32_t)pos;
+      nslot[pos] = j;
+      pos++;
+    }
+  }
+}
+
+// Node arrays in output order; slot2node maps table slots to output ranks.
+template <int W>
+__global__ void k_gather_nodes(uint64_t nn, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ nslot,
+                               const void* table, uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
+                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ msk,
+                               uint32_t* __restrict__ slot2node) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t s = nslot[perm[i]];
+    KV v = slot_key<W>(table, s);
+    klo[i] = v.lo;
+    if (W == 2) khi[i] = v.hi;
+    if (W == 1) {
+      cnt[i] = ((const Slot1*)table)[s].count;
+      msk[i] = ((const Slot1*)table)[s].mask;
+    } else {
+      cnt[i] = ((const Slot2*)table)[s].count;
+      msk[i] = ((const Slot2*)table)[s].mask;
+    }
+    slot2node[s] = (uint32_t)i;
+  }
+}
+
+template <int W>
+__device__ inline KV node_key(const uint64_t* klo, const uint64_t* khi, uint64_t i) {
+  KV v;
+  v.lo = klo[i];
+  v.hi = W == 2 ? khi[i] : 0;
+  return v;
+}
+
+__device__ inline void type_sizes(uint32_t mask, int n[4]) {
+  for (int t = 0; t < 4; t++) n[t] = __popc((mask >> (4 * t)) & 0xF);
+}
+
+__global__ void k_node_info(uint64_t nn, const uint32_t* __restrict__ msk, const uint32_t* __restrict__ cnt, int jdk7,
+                            uint32_t* __restrict__ needs, uint64_t* __restrict__ selcnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    int n[4];
+    type_sizes(msk[i], n);
+    bool need = jdk7 && (n[0] > 1 || n[1] > 1 || n[2] > 1 || n[3] > 1);
+    needs[i] = need ? 1 : 0;
+    selcnt[i] = need ? cnt[i] : 0;
+  }
+}
+
+// Stored neighbour k-mer of list t, letter x.
+template <int W>
+__device__ inline KV neighbour(KV v, int k, uint32_t t, uint32_t x) {
+  switch (t) {
+    case ET_FF: return succ_of<W>(v, k, x);
+    case ET_FR: return revcomp<W>(succ_of<W>(v, k, x), k);
+    case ET_RF: return revcomp<W>(pred_of<W>(v, k, x), k);
+    default: return pred_of<W>(v, k, x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. JDK-7 edge order.  Every record of a node passes through
+//    VKmerList.unionUpdate (VKmerList.java:117-133) per edge type: a new
+//    java.util.HashSet(|L| + |rec|) receives L then the record's k-mer and
+//    the list becomes its iteration order; the cross aggregator repeats it
+//    once more with an empty record (JobGenBuildBrujinGraph.java:119-143,
+//    P = 1).  Records are visited in emission order, which the stable sort of
+//    the reference keeps (FrameSorterMergeSort.java:153-185).
+// ---------------------------------------------------------------------------
+
+__global__ void k_sel_flags(uint64_t nocc, const uint64_t* __restrict__ occ_slot,
+                            const uint32_t* __restrict__ slot2node, const uint32_t* __restrict__ needs,
+                            uint32_t* __restrict__ flag) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nocc; j += (uint64_t)gridDim.x * blockDim.x)
+    flag[j] = needs[slot2node[occ_slot[j]]];
+}
+
+__global__ void k_sel_write(uint64_t nocc, const uint64_t* __restrict__ occ_slot, const uint8_t* __restrict__ occ_code,
+                            const uint32_t* __restrict__ slot2node, const uint32_t* __restrict__ flag,
+                            const uint64_t* __restrict__ pos, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nocc; j += (uint64_t)gridDim.x * blockDim.x) {
+    if (flag[j]) {
+      uint64_t p = pos[j];
+      keys[p] = slot2node[occ_slot[j]];
+      vals[p] = occ_code[j];
+    }
+  }
+}
+
+// Small ordered list of 2-bit letters (iteration order of the HashSet).
+struct LList {
+  uint32_t bits;  // letter j at bits 2j
+  int n;
+};
+
+__device__ inline uint32_t ll_get(const LList& l, int j) { return (l.bits >> (2 * j)) & 3; }
+
+// One VKmerList.unionUpdate with at most one new element.  h[c] is the
+// JDK-7-spread hash of the neighbour k-mer with letter c.
+__device__ inline LList j7_union(LList L, int sym, const int32_t h[4]) {
+  int n = L.n + (sym >= 0 ? 1 : 0);
+  int cap = 1;
+  while (cap < n) cap <<= 1;
+  int thr = (cap * 3) >> 2;  // (int)(cap * 0.75f)
+  LList o{0, 0};
+  for (int idx = 0; idx < n; idx++) {
+    int x = idx < L.n ? (int)ll_get(L, idx) : sym;
+    bool present = false;
+    for (int j = 0; j < o.n; j++) present |= (int)ll_get(o, j) == x;
+    if (present) continue;
+    int b = h[x] & (cap - 1);
+    bool nonempty = false;
+    for (int j = 0; j < o.n; j++) nonempty |= (h[ll_get(o, j)] & (cap - 1)) == b;
+    if (o.n >= thr && nonempty) {
+      // HashMap.resize/transfer: new order = by new bucket, ties in reverse
+      // of the old iteration order.
+      int ncap = cap * 2;
+      LList r{0, 0};
+      for (int bk = 0; bk < ncap; bk++)
+        for (int j = o.n - 1; j >= 0; j--) {
+          uint32_t y = ll_get(o, j);
+          if ((h[y] & (ncap - 1)) == bk) { r.bits |= y << (2 * r.n); r.n++; }
+        }
+      o = r;
+      cap = ncap;
+      thr = (cap * 3) >> 2;
+      b = h[x] & (cap - 1);
+    }
+    // head of bucket b: after every element of a lower bucket
+    int pos = 0;
+    for (int j = 0; j < o.n; j++) pos += (h[ll_get(o, j)] & (cap - 1)) < b ? 1 : 0;
+    uint32_t low = o.bits & ((1u << (2 * pos)) - 1);
+    uint32_t high = pos >= 16 ? 0 : (o.bits >> (2 * pos));
+    o.bits = low | ((uint32_t)x << (2 * pos)) | (high << (2 * pos + 2));
+    o.n++;
+  }
+  return o;
+}
+
+template <int W>
+__global__ void k_replay(uint64_t nn, const uint32_t* __restrict__ needs, const uint64_t* __restrict__ seg,
+                         const uint32_t* __restrict__ codes, const uint64_t* __restrict__ klo,
+                         const uint64_t* __restrict__ khi, int k, int nb, uint64_t* __restrict__ lists) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!needs[i]) continue;
+    KV v = node_key<W>(klo, khi, i);
+    // hash of each list element: JDK-7-spread VKmer.hashCode of the stored
+    // neighbour (FF/FR store the same canonical k-mer for a letter, as do
+    // RF/RR)
+    int32_t h[4][4];
+    for (uint32_t t = 0; t < 4; t++)
+      for (uint32_t x = 0; x < 4; x++) h[t][x] = jdk7_spread(java_hash_vkmer<W>(neighbour<W>(v, k, t, x), k, nb));
+    LList L[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    uint64_t a = seg[i], e = seg[i + 1];
+    for (uint64_t r = a; r < e; r++) {
+      uint32_t code = codes[r];
+      int sym[4] = {-1, -1, -1, -1};
+      uint32_t sn = code & 0xF, pn = (code >> 4) & 0xF;
+      if (sn & 8) sym[(sn & 4) ? ET_FR : ET_FF] = (int)(sn & 3);
+      if (pn & 8) sym[(pn & 4) ? ET_RF : ET_RR] = (int)(pn & 3);
+      for (int t = 0; t < 4; t++) {
+        if (r == a) L[t] = j7_union(LList{0, 0}, sym[t], h[t]);  // init: union into an empty list
+        else L[t] = j7_union(L[t], sym[t], h[t]);
+      }
+    }
+    uint64_t out = 0;
+    for (int t = 0; t < 4; t++) {
+      LList f = j7_union(L[t], -1, h[t]);  // cross aggregator init
+      out |= ((uint64_t)f.n | ((uint64_t)f.bits << 3)) << (16 * t);
+    }
+    lists[i] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 7. 5' read heads (ReadHeadInfo.java:100-136; ReadsKeyValueParserFactory.java
+//    :129-135,142-144,165-170): one per valid mate on its first k-mer,
+//    unflipped at offset 0 or flipped at offset k-1.  Sort key =
+//    node<<31 | flipped<<30 | mate<<29 | readId: within a set the TreeSet
+//    order (offset, lib = 0, mate, readId) of ReadHeadInfo.compareTo
+//    (:247-264); equal keys keep the first in emission order (TreeSet.add).
+// ---------------------------------------------------------------------------
+
+template <int W>
+__global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, const uint64_t* __restrict__ nocc,
+                        const uint64_t* __restrict__ hidx, const uint64_t* __restrict__ occbase,
+                        const uint64_t* __restrict__ occ_slot, const uint32_t* __restrict__ slot2node,
+                        const int64_t* __restrict__ rid, const uint64_t* __restrict__ wbase,
+                        const uint64_t* __restrict__ words, int k, uint64_t* __restrict__ hkey,
+                        uint32_t* __restrict__ hval) {
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x) {
+    if (!valid[m] || nocc[m] == 0) continue;
+    uint64_t o = occbase[m];
+    uint64_t node = slot2node[occ_slot[o]];
+    bool fwd;
+    (void)canon<W>(load_kmer<W>(words + wbase[m], 0, k), k, &fwd);
+    uint64_t flipped = fwd ? 0 : 1;
+    uint64_t key = (node << 31) | (flipped << 30) | ((m & 1) << 29) | (uint64_t)rid[m >> 1];
+    uint64_t j = hidx[m];
+    hkey[j] = key;
+    hval[j] = (uint32_t)m;
+  }
+}
+
+__device__ inline uint64_t lower_bound64(const uint64_t* a, uint64_t n, uint64_t x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// 8. Node serialisation (Node.write, Node.java:408-487; VKmerList.write
+//    :314-316; ExternalableTreeSet.write with writeEntireBody :236-267;
+//    ReadHeadInfo.write :205-212; VKmer.write :389-391) inside a SequenceFile
+//    record (i32 recLen, i32 keyLen, VKmer key, Node).
+// ---------------------------------------------------------------------------
+
+struct SerCtx {
+  int k, nb, W, jdk7;
+  uint64_t nn, nheads;
+  const uint64_t *klo, *khi;
+  const uint32_t *cnt, *msk;
+  const uint32_t* needs;
+  const uint64_t* lists;
+  const uint64_t* hkey;
+  const uint32_t* hval;
+  const uint32_t* mlen;
+  const uint64_t* wbase;
+  const uint64_t* words;
+  unsigned long long* bad;  // internal consistency flag (first bad node)
+};
+
+__device__ inline uint32_t nbytes_of(uint32_t L) { return (L + 3) / 4; }
+
+__device__ inline uint64_t head_bytes(const SerCtx& c, uint32_t m) {
+  uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
+  return 1 + 8 + 4 + nbytes_of(L) + (Lm > 0 ? 4 + nbytes_of(Lm) : 0);
+}
+
+// Letters of edge type t in output order.
+template <int W>
+__device__ inline LList edge_list(const SerCtx& c, uint64_t i, int t, KV v) {
+  uint32_t sel = (c.msk[i] >> (4 * t)) & 0xF;
+  LList l{0, 0};
+  if (c.jdk7 && c.needs[i]) {
+    uint64_t w = c.lists[i] >> (16 * t);
+    l.n = (int)(w & 7);
+    l.bits = (uint32_t)(w >> 3) & 0xFF;
+    return l;
+  }
+  // ascending neighbour k-mer (numeric V = VKmer byte order)
+  KV nk[4];
+  int idx[4], n = 0;
+  for (uint32_t x = 0; x < 4; x++) {
+    if (!((sel >> x) & 1)) continue;
+    nk[n] = neighbour<W>(v, c.k, (uint32_t)t, x);
+    idx[n] = (int)x;
+    n++;
+  }
+  for (int a = 1; a < n; a++)
+    for (int b = a; b > 0 && !kle<W>(nk[b - 1], nk[b]); b--) {
+      KV tk = nk[b]; nk[b] = nk[b - 1]; nk[b - 1] = tk;
+      int ti = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = ti;
+    }
+  for (int a = 0; a < n; a++) l.bits |= (uint32_t)idx[a] << (2 * a);
+  l.n = n;
+  return l;
+}
+
+struct HeadRange {
+  uint64_t a, f, e;
+};
+
+__device__ inline HeadRange head_range(const SerCtx& c, uint64_t i) {
+  HeadRange r;
+  r.a = lower_bound64(c.hkey, c.nheads, i << 31);
+  r.f = lower_bound64(c.hkey, c.nheads, (i << 31) | (1ull << 30));
+  r.e = lower_bound64(c.hkey, c.nheads, (i + 1) << 31);
+  return r;
+}
+
+template <int W>
+__global__ void k_ser_size(SerCtx c, uint64_t* __restrict__ rsize) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    int n[4];
+    type_sizes(c.msk[i], n);
+    uint64_t val = 1 + 4;
+    for (int t = 0; t < 4; t++)
+      if (n[t]) val += 4 + (uint64_t)n[t] * (4 + c.nb);
+    HeadRange hr = head_range(c, i);
+    uint64_t hb[2] = {0, 0};
+    uint64_t cntset[2] = {0, 0};
+    for (uint64_t j = hr.a; j < hr.e; j++) {
+      if (j > 0 && c.hkey[j] == c.hkey[j - 1]) continue;  // duplicate uuid: TreeSet keeps the first
+      int f = j >= hr.f;
+      hb[f] += head_bytes(c, c.hval[j]);
+      cntset[f]++;
+    }
+    for (int f = 0; f < 2; f++)
+      if (cntset[f]) val += 1 + 4 + hb[f];
+    rsize[i] = 8 + 4 + c.nb + val;
+  }
+}
+
+__device__ inline void put32(uint8_t*& p, uint32_t v) {
+  p[0] = v >> 24; p[1] = v >> 16; p[2] = v >> 8; p[3] = v;
+  p += 4;
+}
+
+template <int W>
+__device__ inline void put_key(uint8_t*& p, KV v, int k, int nb) {
+  put32(p, (uint32_t)k);
+  for (int j = 0; j < nb; j++) *p++ = be_byte<W>(v, nb, j);
+}
+
+// A read as VKmer bytes: BE32(L) then the packed stream's bytes reversed.
+__device__ inline void put_read(uint8_t*& p, const uint64_t* w, uint32_t L) {
+  put32(p, L);
+  uint32_t nbl = nbytes_of(L);
+  for (uint32_t j = 0; j < nbl; j++) {
+    uint32_t q = nbl - 1 - j;
+    *p++ = (uint8_t)(w[q >> 3] >> (8 * (q & 7)));
+  }
+}
+
+template <int W>
+__global__ void k_ser_write(SerCtx c, const uint64_t* __restrict__ roff, const int64_t* __restrict__ rid,
+                            uint8_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint8_t* p = out + roff[i];
+    uint64_t rec = roff[i + 1] - roff[i];
+    KV v = node_key<W>(c.klo, c.khi, i);
+    uint32_t keyLen = 4 + c.nb;
+    put32(p, (uint32_t)(rec - 8));
+    put32(p, keyLen);
+    put_key<W>(p, v, c.k, c.nb);
+    int n[4];
+    type_sizes(c.msk[i], n);
+    HeadRange hr = head_range(c, i);
+    bool hu = false, hf = false;
+    for (uint64_t j = hr.a; j < hr.e; j++) {
+      if (j >= hr.f) hf = true;
+      else hu = true;
+    }
+    uint8_t flags = 0x80;
+    for (int t = 0; t < 4; t++)
+      if (n[t]) flags |= 1 << t;
+    if (hu) flags |= 1 << 4;
+    if (hf) flags |= 1 << 5;
+    *p++ = flags;
+    for (int t = 0; t < 4; t++) {
+      if (!n[t]) continue;
+      LList l = edge_list<W>(c, i, t, v);
+      if (l.n != n[t]) {  // cannot happen: the replayed set holds every neighbour once
+        if (c.bad) atomicMin(c.bad, (unsigned long long)i);
+        l.n = n[t];
+      }
+      put32(p, (uint32_t)n[t]);
+      for (int a = 0; a < n[t]; a++) {
+        uint32_t x = ll_get(l, a);
+        put_key<W>(p, neighbour<W>(v, c.k, (uint32_t)t, x), c.k, c.nb);
+      }
+    }
+    for (int f = 0; f < 2; f++) {
+      uint64_t a = f ? hr.f : hr.a, e = f ? hr.e : hr.f;
+      if (a == e) continue;
+      uint32_t cntu = 0;
+      for (uint64_t j = a; j < e; j++)
+        if (!(j > 0 && c.hkey[j] == c.hkey[j - 1])) cntu++;
+      *p++ = 1;  // whole body in stream
+      put32(p, cntu);
+      for (uint64_t j = a; j < e; j++) {
+        if (j > 0 && c.hkey[j] == c.hkey[j - 1]) continue;
+        uint32_t m = c.hval[j];
+        uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
+        *p++ = Lm > 0 ? 1 : 0;
+        uint64_t off = f ? (uint64_t)(c.k - 1) : 0;
+        uint64_t uuid = (off << 40) | ((uint64_t)(m & 1) << 35) | (uint64_t)rid[m >> 1];
+        for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
+        put_read(p, c.words + c.wbase[m], L);
+        if (Lm > 0) put_read(p, c.words + c.wbase[m ^ 1], Lm);
+      }
+    }
+    // coverage: Java float accumulation of 1.0f saturates at 2^24
+    uint32_t cv = c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i];
+    float fc = (float)cv;
+    uint32_t u;
+    memcpy(&u, &fc, 4);
+    put32(p, u);
+  }
+}
+
+// words per mate for the packed store
+__global__ void k_word_counts(uint64_t nmates, const uint32_t* __restrict__ mlen, uint64_t* __restrict__ wc) {
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x)
+    wc[m] = (mlen[m] + 31) / 32 + 1;  // +1: k-mer loads may touch one word past the end
+}
+
+__global__ void k_valid_u64(uint64_t nmates, const uint8_t* __restrict__ valid, const uint64_t* __restrict__ nocc,
+                            uint64_t* __restrict__ out) {
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x)
+    out[m] = (valid[m] && nocc[m]) ? 1 : 0;
+}
+
+}  // namespace gx
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+using namespace gx;
+
+struct gx_build {
+  gx_build_cfg cfg{};
+  int W = 1, nb = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  uint64_t text_len = 0;
+  bool finished = false;
+  // device buffers (grown on demand, reused across builds)
+  DevBuf text, scratch, blk, blk2, term, rid, mstart, mlen, wcnt, wbase, words, valid, nocc, occbase, hmask, hidx;
+  DevBuf table, occ_slot, occ_code;
+  DevBuf ckey, cval, ckey2, cval2, nslot, klo, khi, cnt, msk, slot2node, needs, selcnt, seg;
+  DevBuf selflag, selpos, selk, selv, selk2, selv2, lists;
+  DevBuf hkey, hval, hkey2, hval2, rsize, roff, out;
+  DevBuf errw;
+  uint64_t stats[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t out_bytes = 0, nnodes = 0;
+};
+
+static std::string g_create_err;
+
+static char* scratch_for(gx_build* b, size_t bytes) { return b->scratch.ensure(bytes) ? b->scratch.as<char>() : nullptr; }
+
+template <class T>
+static T* buf(DevBuf& d, size_t n) {
+  return static_cast<T*>(d.ensure(n * sizeof(T)));
+}
+
+template <class T>
+static T read1(hipStream_t st, const T* dptr) {
+  T v;
+  GX_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return v;
+}
+
+static const char* errname(uint32_t code) {
+  switch (code) {
+    case ERR_FIELDS:
+      return "input format is not correct! only support id'\\t'readSeq'\\t'mateReadSeq or id'\\t'readSeq";
+    case ERR_READID: return "read id is not a valid long";
+    case ERR_READID_RANGE: return "readId will lose some of its bits when saved (ReadHeadInfo allows 29 bits)";
+    case ERR_SHORT: return "kmersize is larger than the read length";
+    case ERR_TABLE: return "hash table overflow";
+  }
+  return "input error";
+}
+
+static void check_err(gx_build* b) {
+  unsigned long long e = read1(b->st, b->errw.as<unsigned long long>());
+  if (e != ~0ull) {
+    uint32_t code = (uint32_t)(e & 0xff);
+    uint64_t line = e >> 8;
+    if (code == ERR_TABLE) throw GxError(GX_ENOMEM, "hash table overflow");
+    throw GxError(GX_EINPUT, std::string(errname(code)) + " (line " + std::to_string(line + 1) + ")");
+  }
+}
+
+template <int W>
+static void run_build(gx_build* b) {
+  hipStream_t st = b->st;
+  const int k = b->cfg.k, nb = b->nb;
+  const uint64_t n = b->text_len;
+  const uint8_t* t = b->text.as<uint8_t>();
+  const unsigned G = 2048;  // grid for grid-stride kernels (8 blocks per CU)
+  unsigned long long* errw = buf<unsigned long long>(b->errw, 2);
+  GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
+
+  // 1. lines
+  unsigned lblocks = grid_for(n + 1, kLTile, 0xffffffffu);
+  uint64_t* blk = buf<uint64_t>(b->blk, (size_t)lblocks + 1);
+  uint64_t* blkb = buf<uint64_t>(b->blk2, (size_t)lblocks + 1);
+  hipLaunchKernelGGL(k_line_count, dim3(lblocks), dim3(kBlock), 0, st, t, n, blk, errw + 1);
+  GX_HIP(hipGetLastError());
+  exclusive_scan<uint64_t, uint64_t>(blk, blkb, lblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(lblocks)), st);
+  uint64_t nlines = read1(st, blkb + lblocks);
+  unsigned long long na = read1(st, errw + 1);
+  if (na != ~0ull) throw GxError(GX_EINPUT, "non-ASCII byte at offset " + std::to_string(na) + " (input must be ASCII)");
+  uint64_t* term = buf<uint64_t>(b->term, nlines + 1);
+  hipLaunchKernelGGL(k_line_write, dim3(lblocks), dim3(kBlock), 0, st, t, n, blkb, term, errw + 1);
+  GX_HIP(hipGetLastError());
+
+  // 2. fields
+  uint64_t nmates = 2 * nlines;
+  int64_t* rid = buf<int64_t>(b->rid, nlines + 1);
+  uint64_t* mstart = buf<uint64_t>(b->mstart, nmates + 1);
+  uint32_t* mlen = buf<uint32_t>(b->mlen, nmates + 1);
+  if (nlines) {
+    unsigned g = grid_for(nlines, kBlock / 64, 1u << 16);
+    hipLaunchKernelGGL(k_parse_lines, dim3(g), dim3(kBlock), 0, st, t, n, term, nlines, rid, mstart, mlen, errw);
+    GX_HIP(hipGetLastError());
+  }
+  // 3. pack
+  uint64_t* wcnt = buf<uint64_t>(b->wcnt, nmates + 1);
+  uint64_t* wbase = buf<uint64_t>(b->wbase, nmates + 1);
+  if (nmates) {
+    hipLaunchKernelGGL(k_word_counts, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, mlen, wcnt);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(wcnt, wbase, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
+  uint64_t nwords = read1(st, wbase + nmates);
+  uint64_t* words = buf<uint64_t>(b->words, nwords + 4);
+  uint8_t* valid = buf<uint8_t>(b->valid, nmates + 1);
+  uint64_t* nocc = buf<uint64_t>(b->nocc, nmates + 1);
+  if (nmates) {
+    unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
+    hipLaunchKernelGGL(k_pack_mates, dim3(g), dim3(kBlock), 0, st, t, nmates, mstart, mlen, wbase, rid, k, words,
+                       valid, nocc, errw);
+    GX_HIP(hipGetLastError());
+  }
+  check_err(b);
+  uint64_t* occbase = buf<uint64_t>(b->occbase, nmates + 1);
+  exclusive_scan<uint64_t, uint64_t>(nocc, occbase, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
+  uint64_t N = read1(st, occbase + nmates);
+  uint64_t* hm = buf<uint64_t>(b->hmask, nmates + 1);
+  uint64_t* hidx = buf<uint64_t>(b->hidx, nmates + 1);
+  if (nmates) {
+    hipLaunchKernelGGL(k_valid_u64, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, valid, nocc, hm);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(hm, hidx, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
+  uint64_t nheads = read1(st, hidx + nmates);
+
+  // 4. table insert (grow-and-rebuild on overflow)
+  uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint : N;
+  uint64_t cap = 1024;
+  while (cap < 2 * expect) cap <<= 1;
+  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
+  uint64_t* occ_slot = buf<uint64_t>(b->occ_slot, N + 1);
+  uint8_t* occ_code = buf<uint8_t>(b->occ_code, N + 1);
+  void* table = nullptr;
+  for (int attempt = 0;; attempt++) {
+    table = b->table.ensure(cap * slotsz);
+    hipLaunchKernelGGL(k_table_clear<W>, dim3(grid_for(cap, kBlock, 65536)), dim3(kBlock), 0, st, table, cap);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
+    uint64_t limit = std::min<uint64_t>(cap, 1u << 16);
+    if (nmates) {
+      unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
+      hipLaunchKernelGGL(k_insert<W>, dim3(g), dim3(kBlock), 0, st, nmates, mlen, wbase, words, occbase, nocc, k,
+                         table, cap - 1, limit, occ_slot, occ_code, errw);
+      GX_HIP(hipGetLastError());
+    }
+    unsigned long long e = read1(st, errw);
+    if (e == ~0ull) break;
+    if ((e & 0xff) != ERR_TABLE || attempt > 8) check_err(b);
+    cap <<= 1;  // fallback: rebuild in a table twice as large
+  }
+
+  // 5. compaction + KmerPointable order
+  unsigned cblocks = grid_for(cap, kCTile, 0xffffffffu);
+  uint64_t* cb = buf<uint64_t>(b->blk, (size_t)cblocks + 1);
+  uint64_t* cbb = buf<uint64_t>(b->blk2, (size_t)cblocks + 1);
+  hipLaunchKernelGGL(k_compact_count<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cb);
+  GX_HIP(hipGetLastError());
+  exclusive_scan<uint64_t, uint64_t>(cb, cbb, cblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(cblocks)), st);
+  uint64_t nn = read1(st, cbb + cblocks);
+  if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
+  using SK = typename std::conditional<W == 1, uint64_t, Key128>::type;
+  SK* ck = buf<SK>(b->ckey, nn + 1);
+  SK* ck2 = buf<SK>(b->ckey2, nn + 1);
+  uint32_t* cv = buf<uint32_t>(b->cval, nn + 1);
+  uint32_t* cv2 = buf<uint32_t>(b->cval2, nn + 1);
+  uint64_t* nslot = buf<uint64_t>(b->nslot, nn + 1);
+  hipLaunchKernelGGL((k_compact_write<W, SK>), dim3(cblocks), dim3(kBlock), 0, st, table, cap, cbb, nb, ck, cv, nslot);
+  GX_HIP(hipGetLastError());
+  bool fl = radix_sort<SK>(ck, cv, ck2, cv2, nn, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nn)), st);
+  uint32_t* perm = fl ? cv2 : cv;
+  uint64_t* klo = buf<uint64_t>(b->klo, nn + 1);
+  uint64_t* khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
+  uint32_t* cnt = buf<uint32_t>(b->cnt, nn + 1);
+  uint32_t* msk = buf<uint32_t>(b->msk, nn + 1);
+  uint32_t* slot2node = buf<uint32_t>(b->slot2node, cap);
+  hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, nslot, table,
+                     klo, khi, cnt, msk, slot2node);
+  GX_HIP(hipGetLastError());
+
+  // 6. replay
+  const int jdk7 = b->cfg.edge_order == GX_EDGE_ORDER_JDK7;
+  uint32_t* needs = buf<uint32_t>(b->needs, nn + 1);
+  uint64_t* selcnt = buf<uint64_t>(b->selcnt, nn + 1);
+  uint64_t* seg = buf<uint64_t>(b->seg, nn + 1);
+  hipLaunchKernelGGL(k_node_info, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, msk, cnt, jdk7, needs,
+                     selcnt);
+  GX_HIP(hipGetLastError());
+  exclusive_scan<uint64_t, uint64_t>(selcnt, seg, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+  uint64_t nsel = read1(st, seg + nn);
+  uint64_t* lists = buf<uint64_t>(b->lists, nn + 1);
+  uint64_t nrepl = 0;
+  if (nsel) {
+    uint32_t* flag = buf<uint32_t>(b->selflag, N + 1);
+    uint64_t* pos = buf<uint64_t>(b->selpos, N + 1);
+    hipLaunchKernelGGL(k_sel_flags, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, slot2node, needs,
+                       flag);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(flag, pos, N, scratch_for(b, scan_scratch_bytes<uint64_t>(N)), st);
+    uint32_t* sk = buf<uint32_t>(b->selk, nsel + 1);
+    uint32_t* sv = buf<uint32_t>(b->selv, nsel + 1);
+    uint32_t* sk2 = buf<uint32_t>(b->selk2, nsel + 1);
+    uint32_t* sv2 = buf<uint32_t>(b->selv2, nsel + 1);
+    hipLaunchKernelGGL(k_sel_write, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, occ_code,
+                       slot2node, flag, pos, sk, sv);
+    GX_HIP(hipGetLastError());
+    int nbytes_id = 1;
+    while (nbytes_id < 4 && (nn >> (8 * nbytes_id)) > 0) nbytes_id++;
+    bool f2 = radix_sort<uint32_t>(sk, sv, sk2, sv2, nsel, 0, nbytes_id,
+                                   scratch_for(b, radix_scratch_bytes<uint32_t>(nsel)), st);
+    uint32_t* codes = f2 ? sv2 : sv;
+    hipLaunchKernelGGL(k_replay<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, needs, seg, codes, klo,
+                       khi, k, nb, lists);
+    GX_HIP(hipGetLastError());
+  }
+
+  // 7. heads
+  uint64_t* hk = buf<uint64_t>(b->hkey, nheads + 1);
+  uint32_t* hv = buf<uint32_t>(b->hval, nheads + 1);
+  uint64_t* hk2 = buf<uint64_t>(b->hkey2, nheads + 1);
+  uint32_t* hv2 = buf<uint32_t>(b->hval2, nheads + 1);
+  if (nmates) {
+    hipLaunchKernelGGL(k_heads<W>, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, valid, nocc, hidx,
+                       occbase, occ_slot, slot2node, rid, wbase, words, k, hk, hv);
+    GX_HIP(hipGetLastError());
+  }
+  int hbytes = 1;
+  while (hbytes < 8 && (((nn << 31) | ((1ull << 31) - 1)) >> (8 * hbytes)) > 0) hbytes++;
+  bool f3 = radix_sort<uint64_t>(hk, hv, hk2, hv2, nheads, 0, hbytes,
+                                 scratch_for(b, radix_scratch_bytes<uint64_t>(nheads)), st);
+  if (f3) { std::swap(hk, hk2); std::swap(hv, hv2); }
+
+  // 8. serialisation
+  SerCtx c;
+  c.k = k; c.nb = nb; c.W = W; c.jdk7 = jdk7; c.nn = nn; c.nheads = nheads;
+  c.klo = klo; c.khi = khi; c.cnt = cnt; c.msk = msk; c.needs = needs; c.lists = lists;
+  c.hkey = hk; c.hval = hv; c.mlen = mlen; c.wbase = wbase; c.words = words;
+  c.bad = errw + 1;
+  GX_HIP(hipMemsetAsync(errw + 1, 0xff, 8, st));
+  uint64_t* rsize = buf<uint64_t>(b->rsize, nn + 1);
+  uint64_t* roff = buf<uint64_t>(b->roff, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_ser_size<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, rsize);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(rsize, roff, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+  uint64_t total = read1(st, roff + nn);
+  uint8_t* out = buf<uint8_t>(b->out, total + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_ser_write<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, roff, rid, out);
+    GX_HIP(hipGetLastError());
+  }
+  GX_HIP(hipStreamSynchronize(st));
+  unsigned long long badnode = read1(st, errw + 1);
+  if (badnode != ~0ull)
+    throw GxError(GX_EDEVICE, "internal error: replayed edge list inconsistent at node " + std::to_string(badnode));
+  b->out_bytes = total;
+  b->nnodes = nn;
+  b->stats[0] = N;
+  b->stats[1] = nn;
+  b->stats[2] = nheads;
+  b->stats[3] = cap;
+  b->stats[4] = 0;
+  b->stats[5] = nsel;
+  (void)nrepl;
+}
+
+template <class F>
+static int guarded(gx_build* b, F&& f) {
+  try {
+    f();
+    return GX_OK;
+  } catch (const GxError& e) {
+    if (b) b->err = e.what();
+    return e.code;
+  } catch (const HipError& e) {
+    if (b) b->err = e.what();
+    return e.code == hipErrorOutOfMemory ? GX_ENOMEM : GX_EDEVICE;
+  } catch (const std::bad_alloc&) {
+    if (b) b->err = "host allocation failed";
+    return GX_ENOMEM;
+  } catch (const std::exception& e) {
+    if (b) b->err = e.what();
+    return GX_EDEVICE;
+  }
+}
+
+static void append_text(gx_build* b, const void* src, size_t len, hipMemcpyKind kind) {
+  if (b->finished) throw GxError(GX_ESTATE, "add_reads after finish; call gx_build_reset first");
+  size_t need = b->text_len + len + 64;
+  if (need > b->text.cap) {
+    size_t ncap = std::max(need, b->text.cap * 3 / 2);
+    void* np = nullptr;
+    GX_HIP(hipMalloc(&np, ncap));
+    if (b->text_len) GX_HIP(hipMemcpyAsync(np, b->text.p, b->text_len, hipMemcpyDeviceToDevice, b->st));
+    GX_HIP(hipStreamSynchronize(b->st));
+    b->text.release();
+    b->text.p = np;
+    b->text.cap = ncap;
+  }
+  if (len) GX_HIP(hipMemcpyAsync(b->text.as<char>() + b->text_len, src, len, kind, b->st));
+  GX_HIP(hipStreamSynchronize(b->st));
+  b->text_len += len;
+}
+
+extern "C" {
+
+int gx_build_create(const gx_build_cfg* cfg, gx_build** out) {
+  g_create_err.clear();
+  if (!cfg || !out) { g_create_err = "null argument"; return GX_EINVAL; }
+  if (cfg->k < 3) { g_create_err = "kmerLength must be at least 3!"; return GX_EINVAL; }
+  if (cfg->k > 63) { g_create_err = "k > 63 is not supported"; return GX_EINVAL; }
+  if (cfg->max_readids_per_edge < 0) { g_create_err = "maxReadIDsPerEdge must be non-negative!"; return GX_EINVAL; }
+  if (cfg->nparts < 1) { g_create_err = "nparts must be >= 1"; return GX_EINVAL; }
+  if (cfg->nparts != 1) { g_create_err = "nparts > 1 is not implemented on the device yet"; return GX_ENOTSUP; }
+  if (cfg->edge_order != GX_EDGE_ORDER_CANONICAL && cfg->edge_order != GX_EDGE_ORDER_JDK7) {
+    g_create_err = "unknown edge_order";
+    return GX_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_err = "no HIP device available: the graph build runs on the GPU only";
+    return GX_EDEVICE;
+  }
+  if (cfg->device < 0 || cfg->device >= ndev) { g_create_err = "bad device ordinal"; return GX_EINVAL; }
+  gx_build* b = new (std::nothrow) gx_build;
+  if (!b) return GX_ENOMEM;
+  b->cfg = *cfg;
+  b->W = cfg->k <= 31 ? 1 : 2;
+  b->nb = (cfg->k + 3) / 4;
+  int rc = guarded(b, [&] {
+    GX_HIP(hipSetDevice(cfg->device));
+    GX_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+  });
+  if (rc) {
+    g_create_err = b->err;
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return GX_OK;
+}
+
+int gx_build_add_reads(gx_build* b, const char* buf_, size_t len, int fmt) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (fmt != GX_FMT_READIDS) throw GxError(GX_ENOTSUP, "only GX_FMT_READIDS is supported");
+    if (len && !buf_) throw GxError(GX_EINVAL, "null buffer");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    append_text(b, buf_, len, hipMemcpyHostToDevice);
+  });
+}
+
+int gx_build_add_reads_device(gx_build* b, const void* dev_buf, size_t len, int fmt) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (fmt != GX_FMT_READIDS) throw GxError(GX_ENOTSUP, "only GX_FMT_READIDS is supported");
+    if (len && !dev_buf) throw GxError(GX_EINVAL, "null buffer");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    append_text(b, dev_buf, len, hipMemcpyDeviceToDevice);
+  });
+}
+
+int gx_build_finish(gx_build* b) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (b->finished) throw GxError(GX_ESTATE, "finish called twice; call gx_build_reset first");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    if (b->text.cap == 0) b->text.ensure(64);
+    if (b->W == 1) run_build<1>(b);
+    else run_build<2>(b);
+    b->finished = true;
+  });
+}
+
+int gx_build_part_info(gx_build* b, int part, uint64_t* nrec, uint64_t* nbytes) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "part_info before finish");
+    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
+    if (nrec) *nrec = b->nnodes;
+    if (nbytes) *nbytes = b->out_bytes;
+  });
+}
+
+int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* written) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "emit before finish");
+    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
+    if (cap < b->out_bytes) throw GxError(GX_EINVAL, "destination too small");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    if (b->out_bytes) GX_HIP(hipMemcpyAsync(dst, b->out.p, b->out_bytes, hipMemcpyDeviceToHost, b->st));
+    GX_HIP(hipStreamSynchronize(b->st));
+    if (written) *written = b->out_bytes;
+  });
+}
+
+int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* nbytes) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "part_device before finish");
+    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
+    *dev_ptr = b->out.p;
+    *nbytes = b->out_bytes;
+  });
+}
+
+int gx_build_stats(gx_build* b, uint64_t* out, int n) {
+  if (!b || !out) return GX_EINVAL;
+  for (int i = 0; i < n && i < 6; i++) out[i] = b->stats[i];
+  return GX_OK;
+}
+
+int gx_build_reset(gx_build* b) {
+  if (!b) return GX_EINVAL;
+  b->text_len = 0;
+  b->finished = false;
+  b->out_bytes = 0;
+  b->nnodes = 0;
+  return GX_OK;
+}
+
+const char* gx_last_error(const gx_build* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+void gx_build_destroy(gx_build* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->cfg.device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+// Device-wide primitives for the graph build, written for gfx950 (wave64):
+// an exclusive prefix sum and a stable LSD radix sort.  Both are plain
+// integer, HBM-bound kernels; tiles are sized for 256-thread workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gx {
+
+#define GX_HIP(call)                                                                               \
+  do {                                                                                             \
+    hipError_t _e = (call);                                                                        \
+    if (_e != hipSuccess)                                                                          \
+      throw ::gx::HipError(std::string(#call) + ": " + hipGetErrorString(_e), (int)_e);            \
+  } while (0)
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+// Growable device allocation, reused across builds on one handle.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  void* ensure(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (bytes > cap) {
+      release();
+      GX_HIP(hipMalloc(&p, bytes));
+      cap = bytes;
+    }
+    return p;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 1u << 20) {
+  size_t g = (n + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  return (unsigned)(g > cap ? cap : g);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan: out[i] = sum(in[0..i)), out[n] = total.  out may not alias in.
+// ---------------------------------------------------------------------------
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T* total) {
+  __shared__ T wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  T before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; w++) {
+    if (w < wid) before += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+template <class Tin, class T>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const Tin* __restrict__ in, size_t n, T* __restrict__ part) {
+  size_t base = (size_t)blockIdx.x * kScanTile;
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + (size_t)i * kBlock + threadIdx.x;
+    if (j < n) s += (T)in[j];
+  }
+  T tot;
+  (void)block_exclusive_scan<T>(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+template <class Tin, class T>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const Tin* __restrict__ in, size_t n, const T* __restrict__ part,
+                                                       T* __restrict__ out) {
+  size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  T v[kScanItems];
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + i;
+    v[i] = j < n ? (T)in[j] : (T)0;
+    s += v[i];
+  }
+  T tot;
+  T ex = block_exclusive_scan<T>(s, &tot) + (part ? part[blockIdx.x] : (T)0);
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    size_t j = base + i;
+    if (j < n) out[j] = ex;
+    ex += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
+}
+
+// Scratch needed by exclusive_scan for n elements (bytes), all levels.
+template <class T>
+static size_t scan_scratch_bytes(size_t n) {
+  size_t b = 0;
+  while (n > (size_t)kScanTile) {
+    n = (n + kScanTile - 1) / kScanTile;
+    b += (n + 1) * sizeof(T) * 2 + 256;
+  }
+  return b + 256;
+}
+
+template <class Tin, class T>
+static void exclusive_scan(const Tin* in, T* out, size_t n, char* scratch, hipStream_t st) {
+  size_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 0) tiles = 1;
+  if (tiles == 1) {
+    hipLaunchKernelGGL((k_scan_apply<Tin, T>), dim3(1), dim3(kBlock), 0, st, in, n, (const T*)nullptr, out);
+    GX_HIP(hipGetLastError());
+    return;
+  }
+  T* part = reinterpret_cast<T*>(scratch);
+  T* partscan = part + (tiles + 1);
+  char* next = reinterpret_cast<char*>(partscan + tiles + 1);
+  next = reinterpret_cast<char*>(((uintptr_t)next + 255) & ~(uintptr_t)255);
+  hipLaunchKernelGGL((k_scan_reduce<Tin, T>), dim3((unsigned)tiles), dim3(kBlock), 0, st, in, n, part);
+  GX_HIP(hipGetLastError());
+  exclusive_scan<T, T>(part, partscan, tiles, next, st);
+  hipLaunchKernelGGL((k_scan_apply<Tin, T>), dim3((unsigned)tiles), dim3(kBlock), 0, st, in, n,
+                     (const T*)partscan, out);
+  GX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  Keys are up to 128 bits (KeyT =
+// uint32_t, uint64_t or Key128); the digits sorted are bytes
+// [byte_lo, byte_hi) of the key's integer value.  Values are uint32_t.
+// ---------------------------------------------------------------------------
+
+struct Key128 {
+  uint64_t lo, hi;
+};
+
+__device__ __host__ inline uint32_t key_byte(uint32_t k, int b) { return (k >> (8 * b)) & 0xff; }
+__device__ __host__ inline uint32_t key_byte(uint64_t k, int b) { return (uint32_t)(k >> (8 * b)) & 0xff; }
+__device__ __host__ inline uint32_t key_byte(const Key128& k, int b) {
+  return b < 8 ? (uint32_t)(k.lo >> (8 * b)) & 0xff : (uint32_t)(k.hi >> (8 * (b - 8))) & 0xff;
+}
+
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;
+
+template <class KeyT>
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const KeyT* __restrict__ keys, size_t n, int byte,
+                                                       uint32_t* __restrict__ hist, unsigned ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int i = 0; i < kSortItems; i++) {
+    size_t j = base + (size_t)i * kBlock + threadIdx.x;
+    if (j < n) atomicAdd(&h[key_byte(keys[j], byte)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter with stable ranks: the tile is processed in kSortItems rounds of
+// 256 consecutive elements; inside a round the rank of an element among equal
+// digits comes from a wave-level match (8 ballots) plus the counts of the
+// lower waves, so ties keep their input order.
+template <class KeyT, bool kVals>
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                          size_t n, int byte, const uint64_t* __restrict__ offs,
+                                                          unsigned ntiles, KeyT* __restrict__ okeys,
+                                                          uint32_t* __restrict__ ovals) {
+  __shared__ uint64_t run[256];
+  __shared__ uint16_t wcnt[kBlock / 64][256];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  size_t base = (size_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; r++) {
+    for (int w = 0; w < kBlock / 64; w++) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    size_t j = base + (size_t)r * kBlock + threadIdx.x;
+    bool valid = j < n;
+    KeyT key{};
+    uint32_t val = 0, d = 0;
+    if (valid) {
+      key = keys[j];
+      if (kVals) val = vals[j];
+      d = key_byte(key, byte);
+    }
+    uint64_t act = __ballot(valid);
+    uint64_t match = act;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot((d >> b) & 1);
+      match &= ((d >> b) & 1) ? m : ~m;
+    }
+    uint32_t rank = __popcll(match & lt);
+    if (valid && rank == 0) wcnt[wid][d] = (uint16_t)__popcll(match);
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = run[d] + rank;
+      for (int w = 0; w < wid; w++) pos += wcnt[w][d];
+      okeys[pos] = key;
+      if (kVals) ovals[pos] = val;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; w++) tot += wcnt[w][threadIdx.x];
+    run[threadIdx.x] += tot;
+    __syncthreads();
+  }
+}
+
+template <class KeyT>
+static size_t radix_scratch_bytes(size_t n) {
+  size_t ntiles = (n + kSortTile - 1) / kSortTile;
+  if (ntiles == 0) ntiles = 1;
+  size_t h = 256 * ntiles;
+  return h * 4 + (h + 1) * 8 + scan_scratch_bytes<uint64_t>(h) + 1024;
+}
+
+// Sorts (keys, vals) by bytes [byte_lo, byte_hi).  Uses (k2, v2) as the
+// ping-pong buffers; returns true when the result ended in (k2, v2).
+template <class KeyT>
+static bool radix_sort(KeyT* k1, uint32_t* v1, KeyT* k2, uint32_t* v2, size_t n, int byte_lo, int byte_hi,
+                       char* scratch, hipStream_t st) {
+  if (n <= 1) return false;
+  unsigned ntiles = (unsigned)((n + kSortTile - 1) / kSortTile);
+  size_t h = (size_t)256 * ntiles;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
+  uint64_t* offs = reinterpret_cast<uint64_t*>(((uintptr_t)(hist + h) + 255) & ~(uintptr_t)255);
+  char* sc = reinterpret_cast<char*>(((uintptr_t)(offs + h + 1) + 255) & ~(uintptr_t)255);
+  bool flip = false;
+  for (int b = byte_lo; b < byte_hi; b++) {
+    KeyT* ik = flip ? k2 : k1;
+    KeyT* ok = flip ? k1 : k2;
+    uint32_t* iv = flip ? v2 : v1;
+    uint32_t* ov = flip ? v1 : v2;
+    hipLaunchKernelGGL((k_radix_hist<KeyT>), dim3(ntiles), dim3(kBlock), 0, st, ik, n, b, hist, ntiles);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(hist, offs, h, sc, st);
+    if (v1)
+      hipLaunchKernelGGL((k_radix_scatter<KeyT, true>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b, offs,
+                         ntiles, ok, ov);
+    else
+      hipLaunchKernelGGL((k_radix_scatter<KeyT, false>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b, offs,
+                         ntiles, ok, ov);
+    GX_HIP(hipGetLastError());
+    flip = !flip;
+  }
+  return flip;
+}
+
+}  // namespace gx

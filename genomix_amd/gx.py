@@ -1,0 +1,139 @@
+"""ctypes binding of libgenomix_gx.so (include/genomix_gx.h).
+
+``build_graph`` plays the part of GenomixHyracksDriver.runJob(conf,
+Plan.BUILD_DEBRUIJN_GRAPH) (GenomixHyracksDriver.java:84-135) for one task:
+readids text in, ``part-<p>`` record streams out.  There is no CPU path: if
+the HIP library or a HIP device is missing, calls fail with GxError.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgenomix_gx.so")
+
+GX_OK = 0
+GX_EINVAL = -22
+GX_EINPUT = -5
+GX_ENOMEM = -12
+GX_EDEVICE = -19
+GX_ESTATE = -16
+GX_ENOTSUP = -95
+GX_FMT_READIDS = 0
+EDGE_ORDER = {"canonical": 0, "jdk7": 1}
+
+EXPORTED = [
+    "gx_build_create", "gx_build_add_reads", "gx_build_add_reads_device", "gx_build_finish",
+    "gx_build_part_info", "gx_build_emit_part", "gx_build_part_device", "gx_build_stats",
+    "gx_build_reset", "gx_last_error", "gx_build_destroy",
+]
+
+
+class GxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"genomix_gx error {code}: {msg}")
+        self.code = code
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("nparts", ctypes.c_int), ("max_readids_per_edge", ctypes.c_int),
+                ("edge_order", ctypes.c_int), ("device", ctypes.c_int), ("table_hint", ctypes.c_size_t)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree HIP library (never a fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GxError(GX_EDEVICE, f"{LIB_PATH} is not built; run genomix_amd/build.py (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i32, u64p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)
+        L.gx_build_create.argtypes = [ctypes.POINTER(_Cfg), ctypes.POINTER(vp)]
+        L.gx_build_add_reads.argtypes = [vp, ctypes.c_char_p, sz, i32]
+        L.gx_build_add_reads_device.argtypes = [vp, vp, sz, i32]
+        L.gx_build_finish.argtypes = [vp]
+        L.gx_build_part_info.argtypes = [vp, i32, u64p, u64p]
+        L.gx_build_emit_part.argtypes = [vp, i32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        L.gx_build_part_device.argtypes = [vp, i32, ctypes.POINTER(vp), u64p]
+        L.gx_build_stats.argtypes = [vp, u64p, i32]
+        L.gx_build_reset.argtypes = [vp]
+        L.gx_last_error.argtypes = [vp]
+        L.gx_last_error.restype = ctypes.c_char_p
+        L.gx_build_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+class GraphBuilder:
+    """One graph-build task on one GPU (a gx_build handle)."""
+
+    def __init__(self, k, nparts=1, edge_order="jdk7", device=0, table_hint=0, max_readids_per_edge=250):
+        self._h = ctypes.c_void_p()
+        cfg = _Cfg(k, nparts, max_readids_per_edge, EDGE_ORDER[edge_order], device, table_hint)
+        rc = lib().gx_build_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != 0:
+            raise GxError(rc, lib().gx_last_error(None).decode())
+        self.k = k
+        self.nparts = nparts
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GxError(rc, lib().gx_last_error(self._h).decode(errors="replace"))
+
+    def add_reads(self, text: bytes):
+        self._check(lib().gx_build_add_reads(self._h, text, len(text), GX_FMT_READIDS))
+
+    def add_reads_device(self, ptr: int, nbytes: int):
+        self._check(lib().gx_build_add_reads_device(self._h, ctypes.c_void_p(ptr), nbytes, GX_FMT_READIDS))
+
+    def finish(self):
+        self._check(lib().gx_build_finish(self._h))
+
+    def part_info(self, part=0):
+        n, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().gx_build_part_info(self._h, part, ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
+    def part_bytes(self, part=0) -> bytes:
+        _, nb = self.part_info(part)
+        buf = ctypes.create_string_buffer(max(nb, 1))
+        w = ctypes.c_size_t()
+        self._check(lib().gx_build_emit_part(self._h, part, buf, nb, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
+    def stats(self):
+        a = (ctypes.c_uint64 * 6)()
+        self._check(lib().gx_build_stats(self._h, a, 6))
+        keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences"]
+        return dict(zip(keys, list(a)))
+
+    def reset(self):
+        self._check(lib().gx_build_reset(self._h))
+
+    def close(self):
+        if self._h:
+            lib().gx_build_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", device: int = 0):
+    """Graph build of one readids buffer; returns the per-part record streams."""
+    with GraphBuilder(k, nparts=nparts, edge_order=edge_order, device=device) as b:
+        b.add_reads(text)
+        b.finish()
+        return [b.part_bytes(p) for p in range(nparts)]

@@ -1,0 +1,126 @@
+"""Seeded synthetic genomes and reads (readids text), vectorised with numpy.
+
+Genome: iid uniform ACGT; optionally repeat families: copies of random
+templates, each copy with iid substitutions at the stated divergence, placed
+until the stated fraction of the genome is covered.  Reads: start uniform,
+strand uniform, iid substitution errors (one of the other three bases).
+Paired-end: fragment length ~ round(N(mean, sd)) clipped to [L, G]; mate 1 is
+the fragment's first L bases, mate 2 the reverse complement of its last L;
+on the reverse strand the fragment is reverse-complemented first (the same
+pairing as genomix-driver/src/test/python/generate_graph.py).  Lines follow
+the FASTQ conversion of GenomixDriver.convertAndUploadFastqToHDFS
+(GenomixDriver.java:665-714): readId = 4i + 2.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+ASCII = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+@dataclass
+class ReadSetSpec:
+    genome_len: int
+    read_len: int
+    coverage: float
+    error_rate: float
+    paired: bool = False
+    insert_mean: int = 500
+    insert_sd: int = 50
+    repeat_frac: float = 0.0
+    repeat_len: int = 2000          # or (lo, hi) for a family of lengths
+    repeat_div: float = 0.0
+    seed: int = 20261202
+
+
+def genome(spec: ReadSetSpec, rng) -> np.ndarray:
+    g = rng.integers(0, 4, spec.genome_len, dtype=np.uint8)
+    if spec.repeat_frac > 0:
+        covered = 0
+        target = int(spec.repeat_frac * spec.genome_len)
+        fam = None
+        while covered < target:
+            if isinstance(spec.repeat_len, tuple):
+                ln = int(rng.integers(spec.repeat_len[0], spec.repeat_len[1] + 1))
+                fam = rng.integers(0, 4, ln, dtype=np.uint8)
+            elif fam is None:
+                fam = rng.integers(0, 4, spec.repeat_len, dtype=np.uint8)
+            ln = min(len(fam), spec.genome_len)
+            copy = fam[:ln].copy()
+            mut = rng.random(ln) < spec.repeat_div
+            copy[mut] = (copy[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+            pos = int(rng.integers(0, spec.genome_len - ln + 1))
+            g[pos:pos + ln] = copy
+            covered += ln
+    return g
+
+
+def _add_errors(reads, rate, rng):
+    if rate <= 0:
+        return reads
+    mut = rng.random(reads.shape) < rate
+    reads[mut] = (reads[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+    return reads
+
+
+def _rc(codes):
+    return (3 - codes[..., ::-1]).astype(np.uint8)
+
+
+def reads(spec: ReadSetSpec, g=None, rng=None):
+    """Returns (mate1, mate2 or None) code matrices of shape (n, L)."""
+    rng = rng if rng is not None else np.random.default_rng(spec.seed)
+    if g is None:
+        g = genome(spec, rng)
+    G, L = len(g), spec.read_len
+    if spec.paired:
+        n = int(round(spec.coverage * G / (2 * L)))
+        frag = np.rint(rng.normal(spec.insert_mean, spec.insert_sd, n)).astype(np.int64)
+        frag = np.clip(frag, L, G)
+        start = (rng.random(n) * (G - frag + 1)).astype(np.int64)
+        rev = rng.random(n) < 0.5
+        idx = np.arange(L, dtype=np.int64)
+        a = g[start[:, None] + idx[None, :]]
+        b = g[(start + frag - L)[:, None] + idx[None, :]]
+        # forward: (start, rc(end)); reverse strand = the reverse-complemented
+        # fragment: (rc(end), rc(rc(start)) = start)
+        m1 = np.where(rev[:, None], _rc(b), a)
+        m2 = np.where(rev[:, None], a, _rc(b))
+        return _add_errors(m1.astype(np.uint8), spec.error_rate, rng), _add_errors(
+            m2.astype(np.uint8), spec.error_rate, rng)
+    n = int(round(spec.coverage * G / L))
+    start = (rng.random(n) * (G - L + 1)).astype(np.int64)
+    rev = rng.random(n) < 0.5
+    idx = np.arange(L, dtype=np.int64)
+    r = g[start[:, None] + idx[None, :]]
+    r = np.where(rev[:, None], _rc(r), r)
+    return _add_errors(r.astype(np.uint8), spec.error_rate, rng), None
+
+
+def readids_text(m1, m2=None, first_read=0) -> bytes:
+    """Format code matrices as readids lines (readId = 4i + 2)."""
+    n, L = m1.shape
+    ids = [b"%d\t" % (4 * (first_read + i) + 2) for i in range(n)]
+    s1 = ASCII[m1]
+    if m2 is None:
+        body = np.concatenate([s1, np.full((n, 1), ord("\n"), np.uint8)], axis=1)
+    else:
+        s2 = ASCII[m2]
+        body = np.concatenate([s1, np.full((n, 1), ord("\t"), np.uint8), s2,
+                               np.full((n, 1), ord("\n"), np.uint8)], axis=1)
+    rows = body.tobytes()
+    w = body.shape[1]
+    return b"".join(ids[i] + rows[i * w:(i + 1) * w] for i in range(n))
+
+
+def simulate(spec: ReadSetSpec) -> bytes:
+    m1, m2 = reads(spec)
+    return readids_text(m1, m2)
+
+
+# Workloads named by what they are (sizes as in the benchmark plan).
+PLAIN_CPU_50KBP = ReadSetSpec(genome_len=50_000, read_len=100, coverage=30, error_rate=0.005, seed=20261203)
+PLAIN_GPU_5MBP = ReadSetSpec(genome_len=5_000_000, read_len=150, coverage=50, error_rate=0.01, paired=True,
+                             insert_mean=500, insert_sd=50, repeat_frac=0.05, repeat_len=2000, repeat_div=0.01,
+                             seed=20261204)

@@ -1,0 +1,108 @@
+/* genomix_gx.h — C ABI of the MI355X graph build and unitig compression.
+ *
+ * Drop-in boundary for the Genomix Hyracks graph-build path.  In the
+ * reference, GenomixHyracksDriver.runJob (genomix-hyracks/.../graph/driver/
+ * GenomixHyracksDriver.java:84-135) builds a Hyracks job whose operators are
+ *   HDFSReadOperatorDescriptor -> ReadsKeyValueParserFactory (IKeyValueParser)
+ *   -> ExternalSort(KmerPointable) -> PreclusteredGroup(AggregateKmerAggregate)
+ *   -> MToN hash partition -> PreclusteredGroup(AggregateKmerAggregate)
+ *   -> HDFSWriteOperatorDescriptor(KmerNodePairSequenceWriterFactory)
+ * (JobGenBuildBrujinGraph.java:79-154).  A gx_build handle replaces the
+ * parser -> sort -> local group -> MToN -> cross group chain of ONE Hyracks
+ * task: reads go in as the same readids text lines the parser consumes, and
+ * Node records come out in the exact byte form the SequenceFile writer
+ * appends.  All entry points return 0 on success and a negative GX_E* code
+ * on failure (gx_last_error gives the message); nothing throws across the
+ * ABI.  A handle is not thread-safe; distinct handles may run concurrently,
+ * each on its own HIP stream.  The library owns all device memory it
+ * allocates; callers own the buffers they pass in.
+ */
+#ifndef GENOMIX_GX_H
+#define GENOMIX_GX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define GX_OK 0
+#define GX_EINVAL -22   /* bad argument or configuration (HyracksDataException / IllegalArgumentException) */
+#define GX_EINPUT -5    /* malformed input line; the reference job fails on the same line */
+#define GX_ENOMEM -12   /* device allocation failed */
+#define GX_EDEVICE -19  /* no usable HIP device / HIP runtime error */
+#define GX_ESTATE -16   /* call out of order (e.g. emit before finish) */
+#define GX_ENOTSUP -95  /* option not implemented in this build */
+
+/* input formats */
+#define GX_FMT_READIDS 0 /* "readId\tseq[\tmateSeq]" lines (library-N.readids) */
+
+/* edge-list order inside each FF/FR/RF/RR list */
+#define GX_EDGE_ORDER_CANONICAL 0 /* ascending neighbour k-mer (the reference's tests compare lists as sets) */
+#define GX_EDGE_ORDER_JDK7 1      /* byte-identical to the reference at P=1: JDK-7 HashSet replay */
+
+typedef struct gx_build gx_build;
+
+typedef struct {
+  int k;                    /* GenomixJobConf KMER_LENGTH (GenomixJobConf.java:303); 3 <= k <= 63 */
+  int nparts;               /* output partitions P (Hyracks partitions); 1 = CLI default */
+  int max_readids_per_edge; /* GenomixJobConf MAX_READIDS_PER_EDGE (:333); validated >= 0, no effect (as in the reference) */
+  int edge_order;           /* GX_EDGE_ORDER_* */
+  int device;               /* HIP device ordinal */
+  size_t table_hint;        /* expected distinct k-mers (0 = size from the input) */
+} gx_build_cfg;
+
+/* Replaces the per-task set-up of ReadsKeyValueParserFactory.createKeyValueParser
+ * (ReadsKeyValueParserFactory.java:64-80), AggregateKmerAggregateFactory.createAggregator
+ * (AggregateKmerAggregateFactory.java:54-66) and GenomixJobConf.setGlobalStaticConstants
+ * (GenomixJobConf.java:689-694): validates k (GenomixJobConf.java:417-423) and binds a device. */
+int gx_build_create(const gx_build_cfg* cfg, gx_build** out);
+
+/* Replaces IKeyValueParser.parse (hyracks-hdfs/.../api/IKeyValueParser.java:29-57;
+ * ReadsKeyValueParserFactory.java:95-148) for a whole buffer of lines: the
+ * text is copied (host -> HBM) before return.  Buffers concatenate, so a line
+ * may span two calls. */
+int gx_build_add_reads(gx_build* b, const char* buf, size_t len, int fmt);
+
+/* Same, for text already resident in device memory (no host copy; the
+ * buffer is copied device -> device into the handle before return). */
+int gx_build_add_reads_device(gx_build* b, const void* dev_buf, size_t len, int fmt);
+
+/* Runs the whole build on the device (split reads, canonical k-mers, table
+ * insert, aggregation, read heads, record serialisation) and waits for it.
+ * Replaces ExternalSort + both PreclusteredGroup operators + the MToN
+ * connector (JobGenBuildBrujinGraph.java:92-117). */
+int gx_build_finish(gx_build* b);
+
+/* Number of records and bytes of one output partition (part-<p>). */
+int gx_build_part_info(gx_build* b, int part, uint64_t* nrec, uint64_t* nbytes);
+
+/* Copies partition `part` into dst: SequenceFile records
+ * (i32 recLen, i32 keyLen, VKmer key, Node value) in KmerPointable order,
+ * without the file header and sync markers, i.e. the bytes
+ * KmerNodePairSequenceWriterFactory.write (KmerNodePairSequenceWriterFactory.java:79-94)
+ * hands to SequenceFile.Writer.append. */
+int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* written);
+
+/* Device pointer to partition `part`'s record bytes (valid until the next
+ * finish/reset/destroy on this handle). */
+int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* nbytes);
+
+/* Statistics of the last finish: [0] occurrences, [1] distinct k-mers,
+ * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed occurrences. */
+int gx_build_stats(gx_build* b, uint64_t* out, int n);
+
+/* Drops the reads and results but keeps device allocations for reuse. */
+int gx_build_reset(gx_build* b);
+
+/* Message of the last failing call on this handle (or of gx_build_create when b is NULL). */
+const char* gx_last_error(const gx_build* b);
+
+void gx_build_destroy(gx_build* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENOMIX_GX_H */

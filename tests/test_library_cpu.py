@@ -1,0 +1,49 @@
+"""CPU-side checks of the product library: it builds for gfx950, loads, exports
+every symbol the header declares, and refuses to run without a GPU."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from .conftest import ROOT
+
+
+def _declared():
+    h = open(os.path.join(ROOT, "include", "genomix_gx.h")).read()
+    return sorted(set(re.findall(r"\b(gx_[a-z_]+)\s*\(", h)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "gx_build_create" in names and "gx_build_finish" in names and "gx_build_emit_part" in names
+
+
+def test_library_exports_all_declared_symbols():
+    from genomix_amd import gx
+    if not os.path.exists(gx.LIB_PATH):
+        from genomix_amd import build
+        build.build()
+    L = ctypes.CDLL(gx.LIB_PATH)
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert set(_declared()) <= set(gx.EXPORTED) | {"gx_unitig_run", "gx_unitig_info", "gx_unitig_emit"}
+
+
+def test_no_cpu_fallback_without_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from genomix_amd import gx
+    with pytest.raises(gx.GxError) as e:
+        gx.GraphBuilder(21)
+    assert e.value.code in (gx.GX_EDEVICE,)
+
+
+def test_create_validates_config():
+    from genomix_amd import gx
+    for kw in ({"k": 2}, {"k": 64}, {"k": 21, "max_readids_per_edge": -1}):
+        with pytest.raises(gx.GxError) as e:
+            gx.GraphBuilder(**kw)
+        assert e.value.code == gx.GX_EINVAL
