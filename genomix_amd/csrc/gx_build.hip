@@ -975,10 +975,42 @@ struct gx_build {
   DevBuf hkey, hval, hkey2, hval2, rsize, roff, out;
   DevBuf errw;
   uint64_t stats[6] = {0, 0, 0, 0, 0, 0};
+  // per-phase device timings (hipEvents on the handle's stream)
+  bool prof = false;
+  hipEvent_t ev[2 * 8] = {};
+  bool ev_used[8] = {};
+  double phase_ms[8] = {};
   uint64_t out_bytes = 0, nnodes = 0;
 };
 
 static std::string g_create_err;
+
+enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, NPH };
+static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert"};
+
+static void ph_begin(gx_build* b, int ph) {
+  if (!b->prof) return;
+  if (!b->ev[2 * ph]) {
+    GX_HIP(hipEventCreate(&b->ev[2 * ph]));
+    GX_HIP(hipEventCreate(&b->ev[2 * ph + 1]));
+  }
+  GX_HIP(hipEventRecord(b->ev[2 * ph], b->st));
+  b->ev_used[ph] = true;
+}
+static void ph_end(gx_build* b, int ph) {
+  if (b->prof) GX_HIP(hipEventRecord(b->ev[2 * ph + 1], b->st));
+}
+static void ph_collect(gx_build* b) {
+  for (int i = 0; i < NPH; i++) {
+    b->phase_ms[i] = 0;
+    if (b->prof && b->ev_used[i]) {
+      float ms = 0;
+      GX_HIP(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+      b->phase_ms[i] = ms;
+    }
+    b->ev_used[i] = false;
+  }
+}
 
 static char* scratch_for(gx_build* b, size_t bytes) { return b->scratch.ensure(bytes) ? b->scratch.as<char>() : nullptr; }
 
@@ -1028,6 +1060,7 @@ static void run_build(gx_build* b) {
   GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
 
   // 1. lines
+  ph_begin(b, PH_PARSE);
   unsigned lblocks = grid_for(n + 1, kLTile, 0xffffffffu);
   uint64_t* blk = buf<uint64_t>(b->blk, (size_t)lblocks + 1);
   uint64_t* blkb = buf<uint64_t>(b->blk2, (size_t)lblocks + 1);
@@ -1069,6 +1102,7 @@ static void run_build(gx_build* b) {
                        valid, nocc, errw);
     GX_HIP(hipGetLastError());
   }
+  ph_end(b, PH_PARSE);
   check_err(b);
   uint64_t* occbase = buf<uint64_t>(b->occbase, nmates + 1);
   exclusive_scan<uint64_t, uint64_t>(nocc, occbase, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
@@ -1083,6 +1117,7 @@ static void run_build(gx_build* b) {
   uint64_t nheads = read1(st, hidx + nmates);
 
   // 4. table insert (grow-and-rebuild on overflow)
+  ph_begin(b, PH_INSERT);
   uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint : N;
   uint64_t cap = 1024;
   while (cap < 2 * expect) cap <<= 1;
@@ -1098,9 +1133,11 @@ static void run_build(gx_build* b) {
     uint64_t limit = std::min<uint64_t>(cap, 1u << 16);
     if (nmates) {
       unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
+      ph_begin(b, PH_KINSERT);
       hipLaunchKernelGGL(k_insert<W>, dim3(g), dim3(kBlock), 0, st, nmates, mlen, wbase, words, occbase, nocc, k,
                          table, cap - 1, limit, occ_slot, occ_code, errw);
       GX_HIP(hipGetLastError());
+      ph_end(b, PH_KINSERT);
     }
     unsigned long long e = read1(st, errw);
     if (e == ~0ull) break;
@@ -1108,7 +1145,9 @@ static void run_build(gx_build* b) {
     cap <<= 1;  // fallback: rebuild in a table twice as large
   }
 
+  ph_end(b, PH_INSERT);
   // 5. compaction + KmerPointable order
+  ph_begin(b, PH_NODES);
   unsigned cblocks = grid_for(cap, kCTile, 0xffffffffu);
   uint64_t* cb = buf<uint64_t>(b->blk, (size_t)cblocks + 1);
   uint64_t* cbb = buf<uint64_t>(b->blk2, (size_t)cblocks + 1);
@@ -1136,7 +1175,9 @@ static void run_build(gx_build* b) {
                      klo, khi, cnt, msk, slot2node);
   GX_HIP(hipGetLastError());
 
+  ph_end(b, PH_NODES);
   // 6. replay
+  ph_begin(b, PH_REPLAY);
   const int jdk7 = b->cfg.edge_order == GX_EDGE_ORDER_JDK7;
   uint32_t* needs = buf<uint32_t>(b->needs, nn + 1);
   uint64_t* selcnt = buf<uint64_t>(b->selcnt, nn + 1);
@@ -1172,7 +1213,9 @@ static void run_build(gx_build* b) {
     GX_HIP(hipGetLastError());
   }
 
+  ph_end(b, PH_REPLAY);
   // 7. heads
+  ph_begin(b, PH_HEADS);
   uint64_t* hk = buf<uint64_t>(b->hkey, nheads + 1);
   uint32_t* hv = buf<uint32_t>(b->hval, nheads + 1);
   uint64_t* hk2 = buf<uint64_t>(b->hkey2, nheads + 1);
@@ -1188,7 +1231,9 @@ static void run_build(gx_build* b) {
                                  scratch_for(b, radix_scratch_bytes<uint64_t>(nheads)), st);
   if (f3) { std::swap(hk, hk2); std::swap(hv, hv2); }
 
+  ph_end(b, PH_HEADS);
   // 8. serialisation
+  ph_begin(b, PH_SER);
   SerCtx c;
   c.k = k; c.nb = nb; c.W = W; c.jdk7 = jdk7; c.nn = nn; c.nheads = nheads;
   c.klo = klo; c.khi = khi; c.cnt = cnt; c.msk = msk; c.needs = needs; c.lists = lists;
@@ -1208,7 +1253,9 @@ static void run_build(gx_build* b) {
     hipLaunchKernelGGL(k_ser_write<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, roff, rid, out);
     GX_HIP(hipGetLastError());
   }
+  ph_end(b, PH_SER);
   GX_HIP(hipStreamSynchronize(st));
+  ph_collect(b);
   unsigned long long badnode = read1(st, errw + 1);
   if (badnode != ~0ull)
     throw GxError(GX_EDEVICE, "internal error: replayed edge list inconsistent at node " + std::to_string(badnode));
@@ -1370,6 +1417,20 @@ int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   return GX_OK;
 }
 
+int gx_build_set_profiling(gx_build* b, int on) {
+  if (!b) return GX_EINVAL;
+  b->prof = on != 0;
+  return GX_OK;
+}
+
+int gx_build_phase_times(gx_build* b, double* ms, int n) {
+  if (!b || !ms) return GX_EINVAL;
+  for (int i = 0; i < n && i < NPH; i++) ms[i] = b->phase_ms[i];
+  return NPH;
+}
+
+const char* gx_build_phase_name(int i) { return (i >= 0 && i < NPH) ? kPhaseNames[i] : ""; }
+
 int gx_build_reset(gx_build* b) {
   if (!b) return GX_EINVAL;
   b->text_len = 0;
@@ -1386,6 +1447,8 @@ void gx_build_destroy(gx_build* b) {
   (void)hipSetDevice(b->cfg.device);
   if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->st) (void)hipStreamDestroy(b->st);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
   delete b;
 }
 

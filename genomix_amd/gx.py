@@ -25,7 +25,8 @@ EDGE_ORDER = {"canonical": 0, "jdk7": 1}
 EXPORTED = [
     "gx_build_create", "gx_build_add_reads", "gx_build_add_reads_device", "gx_build_finish",
     "gx_build_part_info", "gx_build_emit_part", "gx_build_part_device", "gx_build_stats",
-    "gx_build_reset", "gx_last_error", "gx_build_destroy",
+    "gx_build_reset", "gx_last_error", "gx_build_destroy", "gx_build_set_profiling",
+    "gx_build_phase_times", "gx_build_phase_name",
 ]
 
 
@@ -63,6 +64,10 @@ def lib():
         L.gx_last_error.argtypes = [vp]
         L.gx_last_error.restype = ctypes.c_char_p
         L.gx_build_destroy.argtypes = [vp]
+        L.gx_build_set_profiling.argtypes = [vp, i32]
+        L.gx_build_phase_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32]
+        L.gx_build_phase_name.argtypes = [i32]
+        L.gx_build_phase_name.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -109,6 +114,15 @@ class GraphBuilder:
         self._check(lib().gx_build_stats(self._h, a, 6))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences"]
         return dict(zip(keys, list(a)))
+
+    def set_profiling(self, on=True):
+        self._check(lib().gx_build_set_profiling(self._h, int(on)))
+
+    def phase_times(self):
+        """Device milliseconds per phase of the last finish (hipEvents)."""
+        a = (ctypes.c_double * 16)()
+        n = lib().gx_build_phase_times(self._h, a, 16)
+        return {lib().gx_build_phase_name(i).decode(): a[i] for i in range(n)}
 
     def reset(self):
         self._check(lib().gx_build_reset(self._h))

@@ -94,6 +94,16 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
  * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed occurrences. */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
+/* Per-phase device timings of the last finish, measured with hipEvents on
+ * the handle's stream (the C-ABI counterpart of GenomixJobConf.tick/tock,
+ * GenomixJobConf.java:669-687).  set_profiling(1) enables recording;
+ * phase_times fills ms[0..n) and returns the number of phases;
+ * phase_name(i) names phase i (the last, "k_insert", is the insert kernel
+ * alone). */
+int gx_build_set_profiling(gx_build* b, int on);
+int gx_build_phase_times(gx_build* b, double* ms, int n);
+const char* gx_build_phase_name(int i);
+
 /* Drops the reads and results but keeps device allocations for reuse. */
 int gx_build_reset(gx_build* b);
 
