@@ -147,7 +147,9 @@ def main():
                                "divergence), 150 bp PE reads, insert 500/50, 50x, 1% error, k=31, "
                                "graph build to Node records (unitig merge timed separately)",
                    "k": k, "read_len": L, "read_pairs": text.count(b"\n"), "kmer_occurrences": N,
-                   "distinct_kmers": st["distinct"], "edge_order": "jdk7", "parallelism": "single GPU"},
+                   "distinct_kmers": st["distinct"], "replayed_nodes": st["replayed_nodes"],
+                   "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"], "max_coverage": st["max_coverage"],
+                   "edge_order": "jdk7", "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -288,10 +288,12 @@ __device__ inline uint32_t ld_relaxed32(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Returns the slot index, or ~0 when the probe limit was hit (table too full).
+// Inserts key; returns the slot index (or ~0 when the probe limit was hit,
+// i.e. the table is too full) and whether this call claimed the slot.
 template <int W>
-__device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, uint64_t limit) {
+__device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, uint64_t limit, bool* claimed) {
   uint64_t s = khash<W>(key) & capmask;
+  *claimed = false;
   if (W == 1) {
     Slot1* T = (Slot1*)table;
     for (uint64_t probe = 0; probe < limit; probe++) {
@@ -300,7 +302,8 @@ __device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, u
       if (cur == kEmpty) {
         uint64_t prev = atomicCAS((unsigned long long*)&T[s].key, (unsigned long long)kEmpty,
                                   (unsigned long long)key.lo);
-        if (prev == kEmpty || prev == key.lo) return s;
+        if (prev == kEmpty) { *claimed = true; return s; }
+        if (prev == key.lo) return s;
       }
       s = (s + 1) & capmask;
     }
@@ -321,6 +324,7 @@ __device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, u
         if (prev == kEmpty) {
           __hip_atomic_store(&T[s].lo, key.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __hip_atomic_store(&T[s].hi, key.hi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          *claimed = true;
           return s;
         }
         cur = prev;
@@ -369,8 +373,10 @@ __device__ inline uint32_t base_at(const uint64_t* __restrict__ w, uint32_t i) {
 // complemented).  For odd k the type follows from the neighbour's
 // canonicality; for even k a palindromic neighbour is stored under either
 // type depending on the strand of the read, so the type is recorded per
-// occurrence.
+// occurrence.  Bit 31 of the mask marks nodes whose lists need the JDK-7
+// replay (set after the insert).
 enum : uint32_t { ET_FF = 0, ET_FR = 1, ET_RF = 2, ET_RR = 3 };
+constexpr uint32_t kNeedsBit = 1u << 31;
 
 // Occurrence code: low nibble = successor-side entry, high nibble =
 // predecessor-side entry; per nibble bit 3 present, bit 2 FR (resp. RF),
@@ -386,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
                                                    const uint64_t* __restrict__ occbase,
                                                    const uint64_t* __restrict__ nocc, int k, void* table,
                                                    uint64_t capmask, uint64_t limit,
-                                                   uint64_t* __restrict__ occ_slot, uint8_t* __restrict__ occ_code,
+                                                   uint32_t* __restrict__ occ_slot, uint8_t* __restrict__ occ_code,
                                                    unsigned long long* err) {
   const int lane = threadIdx.x & 63;
   const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
@@ -423,22 +429,19 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
         code |= nibble(t, x) << (fwd ? 4 : 0);
         bits |= 1u << (4 * t + x);
       }
-      uint64_t s = table_insert<W>(table, capmask, key, limit);
+      bool claimed;
+      uint64_t s = table_insert<W>(table, capmask, key, limit, &claimed);
+      (void)claimed;
       if (s == ~0ull) {
         report(err, ~0ull >> 8, ERR_TABLE);
-        occ_slot[ob + p] = 0;
+        s = 0;
       } else {
-        if (W == 1) {
-          Slot1* T = (Slot1*)table;
-          atomicAdd(&T[s].count, 1u);
-          if ((ld_relaxed32(&T[s].mask) & bits) != bits) atomicOr(&T[s].mask, bits);
-        } else {
-          Slot2* T = (Slot2*)table;
-          atomicAdd(&T[s].count, 1u);
-          if ((ld_relaxed32(&T[s].mask) & bits) != bits) atomicOr(&T[s].mask, bits);
-        }
-        occ_slot[ob + p] = s;
+        uint32_t* cnt = W == 1 ? &((Slot1*)table)[s].count : &((Slot2*)table)[s].count;
+        uint32_t* msk = W == 1 ? &((Slot1*)table)[s].mask : &((Slot2*)table)[s].mask;
+        atomicAdd(cnt, 1u);
+        if ((ld_relaxed32(msk) & bits) != bits) atomicOr(msk, bits);
       }
+      occ_slot[ob + p] = (uint32_t)s;
       occ_code[ob + p] = (uint8_t)code;
     }
   }
@@ -463,31 +466,11 @@ __global__ void k_table_clear(void* table, uint64_t cap) {
 }
 
 // ---------------------------------------------------------------------------
-// 5. compaction to a dense node array and the KmerPointable sort key:
-//    bytes compared from the last to the first (KmerPointable.java:94-107),
-//    i.e. the integer whose byte i is big-endian key byte i.
+// 5. dense node array in KmerPointable order: bytes compared from the last
+//    to the first (KmerPointable.java:94-107), i.e. the integer whose byte i
+//    is big-endian key byte i.  The claimed-slot list gives the nodes
+//    without scanning the table; the radix sort fixes their order.
 // ---------------------------------------------------------------------------
-
-constexpr int kCItems = 16;
-constexpr int kCTile = kBlock * kCItems;
-
-template <int W>
-__device__ inline bool slot_used(const void* table, uint64_t i) {
-  return W == 1 ? ((const Slot1*)table)[i].key != kEmpty : ((const Slot2*)table)[i].hi != kEmpty;
-}
-
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_compact_count(const void* table, uint64_t cap, uint64_t* blockcnt) {
-  uint64_t c = 0;
-  uint64_t base = (uint64_t)blockIdx.x * kCTile;
-  for (int i = 0; i < kCItems; i++) {
-    uint64_t j = base + (uint64_t)i * kBlock + threadIdx.x;
-    if (j < cap && slot_used<W>(table, j)) c++;
-  }
-  uint64_t tot;
-  (void)block_exclusive_scan<uint64_t>(c, &tot);
-  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
-}
 
 template <int W>
 __device__ inline KV slot_key(const void* table, uint64_t i) {
@@ -521,51 +504,76 @@ __device__ inline Key128 sortkey128(KV v, int nb) {
   return r;
 }
 
-template <int W, class SK>
+template <int W>
+__device__ inline bool slot_used(const void* table, uint64_t i) {
+  return W == 1 ? ((const Slot1*)table)[i].key != kEmpty : ((const Slot2*)table)[i].hi != kEmpty;
+}
+
+constexpr int kCItems = 16;
+constexpr int kCTile = kBlock * kCItems;
+
+// Occupied slots, striped over the block so every load is coalesced; the
+// order of the list is irrelevant (the radix sort fixes the node order).
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_compact_count(const void* table, uint64_t cap, uint64_t* blockcnt) {
+  uint64_t c = 0;
+  uint64_t base = (uint64_t)blockIdx.x * kCTile;
+  for (int i = 0; i < kCItems; i++) {
+    uint64_t j = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (j < cap && slot_used<W>(table, j)) c++;
+  }
+  uint64_t tot;
+  (void)block_exclusive_scan<uint64_t>(c, &tot);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+template <int W>
 __global__ void __launch_bounds__(kBlock) k_compact_write(const void* table, uint64_t cap,
-                                                          const uint64_t* __restrict__ blockbase, int nb,
-                                                          SK* __restrict__ skey, uint32_t* __restrict__ sval,
-                                                          uint64_t* __restrict__ nslot) {
-  uint64_t base = (uint64_t)blockIdx.x * kCTile + (uint64_t)threadIdx.x * kCItems;
+                                                          const uint64_t* __restrict__ blockbase,
+                                                          uint32_t* __restrict__ claimed) {
+  uint64_t base = (uint64_t)blockIdx.x * kCTile;
   uint64_t c = 0;
   for (int i = 0; i < kCItems; i++) {
-    uint64_t j = base + i;
+    uint64_t j = base + (uint64_t)i * kBlock + threadIdx.x;
     if (j < cap && slot_used<W>(table, j)) c++;
   }
   uint64_t tot;
   uint64_t pos = block_exclusive_scan<uint64_t>(c, &tot) + blockbase[blockIdx.x];
   for (int i = 0; i < kCItems; i++) {
-    uint64_t j = base + i;
-    if (j < cap && slot_used<W>(table, j)) {
-      KV v = slot_key<W>(table, j);
-      if constexpr (W == 1) skey[pos] = sortkey64(v.lo, nb);
-      else skey[pos] = sortkey128(v, nb);
-      sval[pos] = (uint32_t)pos;
-      nslot[pos] = j;
-      pos++;
-    }
+    uint64_t j = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (j < cap && slot_used<W>(table, j)) claimed[pos++] = (uint32_t)j;
   }
 }
 
-// Node arrays in output order; slot2node maps table slots to output ranks.
-template <int W>
-__global__ void k_gather_nodes(uint64_t nn, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ nslot,
-                               const void* table, uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
-                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ msk,
-                               uint32_t* __restrict__ slot2node) {
+template <int W, class SK>
+__global__ void k_compact_list(const void* table, const uint32_t* __restrict__ claimed, uint64_t nn, int nb,
+                               SK* __restrict__ skey, uint32_t* __restrict__ sval) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t s = nslot[perm[i]];
+    KV v = slot_key<W>(table, claimed[i]);
+    if constexpr (W == 1) skey[i] = sortkey64(v.lo, nb);
+    else skey[i] = sortkey128(v, nb);
+    sval[i] = (uint32_t)i;
+  }
+}
+
+// Node arrays in output order.  The slot's count field is replaced by the
+// node's output rank once its count has been copied out.
+template <int W>
+__global__ void k_gather_nodes(uint64_t nn, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ claimed,
+                               void* table, uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
+                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ msk,
+                               uint32_t* __restrict__ nslot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t s = claimed[perm[i]];
     KV v = slot_key<W>(table, s);
     klo[i] = v.lo;
     if (W == 2) khi[i] = v.hi;
-    if (W == 1) {
-      cnt[i] = ((const Slot1*)table)[s].count;
-      msk[i] = ((const Slot1*)table)[s].mask;
-    } else {
-      cnt[i] = ((const Slot2*)table)[s].count;
-      msk[i] = ((const Slot2*)table)[s].mask;
-    }
-    slot2node[s] = (uint32_t)i;
+    uint32_t* c = W == 1 ? &((Slot1*)table)[s].count : &((Slot2*)table)[s].count;
+    uint32_t* m = W == 1 ? &((Slot1*)table)[s].mask : &((Slot2*)table)[s].mask;
+    cnt[i] = *c;
+    msk[i] = *m;
+    *c = (uint32_t)i;
+    nslot[i] = s;
   }
 }
 
@@ -581,14 +589,26 @@ __device__ inline void type_sizes(uint32_t mask, int n[4]) {
   for (int t = 0; t < 4; t++) n[t] = __popc((mask >> (4 * t)) & 0xF);
 }
 
-__global__ void k_node_info(uint64_t nn, const uint32_t* __restrict__ msk, const uint32_t* __restrict__ cnt, int jdk7,
-                            uint32_t* __restrict__ needs, uint64_t* __restrict__ selcnt) {
+// Nodes whose lists need the replay: some list holds >= 2 k-mers.  Marks
+// the slot (kNeedsBit), counts the records to replay and emits the replay
+// list key (count, clamped) for length-balanced scheduling.
+template <int W>
+__global__ void k_node_info(uint64_t nn, const uint32_t* __restrict__ msk, const uint32_t* __restrict__ cnt,
+                            const uint32_t* __restrict__ nslot, void* table, int jdk7,
+                            uint32_t* __restrict__ needs, uint64_t* __restrict__ selcnt,
+                            unsigned long long* maxcnt) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (cnt[i] > 255) atomicMax(maxcnt, (unsigned long long)cnt[i]);
     int n[4];
     type_sizes(msk[i], n);
     bool need = jdk7 && (n[0] > 1 || n[1] > 1 || n[2] > 1 || n[3] > 1);
     needs[i] = need ? 1 : 0;
     selcnt[i] = need ? cnt[i] : 0;
+    if (need) {
+      uint32_t s = nslot[i];
+      uint32_t* m = W == 1 ? &((Slot1*)table)[s].mask : &((Slot2*)table)[s].mask;
+      *m |= kNeedsBit;
+    }
   }
 }
 
@@ -613,22 +633,41 @@ __device__ inline KV neighbour(KV v, int k, uint32_t t, uint32_t x) {
 //    the reference keeps (FrameSorterMergeSort.java:153-185).
 // ---------------------------------------------------------------------------
 
-__global__ void k_sel_flags(uint64_t nocc, const uint64_t* __restrict__ occ_slot,
-                            const uint32_t* __restrict__ slot2node, const uint32_t* __restrict__ needs,
+template <int W>
+__global__ void k_sel_flags(uint64_t nocc, const uint32_t* __restrict__ occ_slot, const void* table,
                             uint32_t* __restrict__ flag) {
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nocc; j += (uint64_t)gridDim.x * blockDim.x)
-    flag[j] = needs[slot2node[occ_slot[j]]];
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nocc; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t s = occ_slot[j];
+    uint32_t m = W == 1 ? ((const Slot1*)table)[s].mask : ((const Slot2*)table)[s].mask;
+    flag[j] = m >> 31;
+  }
 }
 
-__global__ void k_sel_write(uint64_t nocc, const uint64_t* __restrict__ occ_slot, const uint8_t* __restrict__ occ_code,
-                            const uint32_t* __restrict__ slot2node, const uint32_t* __restrict__ flag,
-                            const uint64_t* __restrict__ pos, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals) {
+template <int W>
+__global__ void k_sel_write(uint64_t nocc, const uint32_t* __restrict__ occ_slot, const uint8_t* __restrict__ occ_code,
+                            const void* table, const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nocc; j += (uint64_t)gridDim.x * blockDim.x) {
     if (flag[j]) {
       uint64_t p = pos[j];
-      keys[p] = slot2node[occ_slot[j]];
+      uint32_t s = occ_slot[j];
+      keys[p] = W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
       vals[p] = occ_code[j];
+    }
+  }
+}
+
+// Replay list: nodes needing the replay, keyed by descending record count
+// so that one wave gets nodes of similar length.
+__global__ void k_replay_list(uint64_t nn, const uint32_t* __restrict__ needs, const uint64_t* __restrict__ rpos,
+                              const uint32_t* __restrict__ cnt, uint32_t* __restrict__ rkey,
+                              uint32_t* __restrict__ rval) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (needs[i]) {
+      uint64_t p = rpos[i];
+      uint32_t c = cnt[i] > 0xffffu ? 0xffffu : cnt[i];
+      rkey[p] = 0xffffu - c;
+      rval[p] = (uint32_t)i;
     }
   }
 }
@@ -641,22 +680,30 @@ struct LList {
 
 __device__ inline uint32_t ll_get(const LList& l, int j) { return (l.bits >> (2 * j)) & 3; }
 
-// One VKmerList.unionUpdate with at most one new element.  h[c] is the
-// JDK-7-spread hash of the neighbour k-mer with letter c.
-__device__ inline LList j7_union(LList L, int sym, const int32_t h[4]) {
+// One VKmerList.unionUpdate with at most one new element.  The lists hold
+// at most 4 k-mers, so the HashSet capacity never exceeds 8 (capacity
+// nextPow2(<= 5); a resize needs size >= 0.75 * capacity with a collision,
+// which with <= 4 elements only happens at capacity 4) and only the low 3
+// bits of each element's JDK-7-spread hash matter: hb packs them, letter x
+// at bits 3x..3x+2.
+__device__ inline uint32_t bucket_of(uint32_t hb, uint32_t x, int cap) { return (hb >> (3 * x)) & (uint32_t)(cap - 1); }
+
+__device__ inline LList j7_union(LList L, int sym, uint32_t hb) {
   int n = L.n + (sym >= 0 ? 1 : 0);
-  int cap = 1;
-  while (cap < n) cap <<= 1;
+  int cap = n <= 1 ? 1 : (n <= 2 ? 2 : (n <= 4 ? 4 : 8));
   int thr = (cap * 3) >> 2;  // (int)(cap * 0.75f)
   LList o{0, 0};
   for (int idx = 0; idx < n; idx++) {
-    int x = idx < L.n ? (int)ll_get(L, idx) : sym;
+    uint32_t x = idx < L.n ? ll_get(L, idx) : (uint32_t)sym;
     bool present = false;
-    for (int j = 0; j < o.n; j++) present |= (int)ll_get(o, j) == x;
-    if (present) continue;
-    int b = h[x] & (cap - 1);
+    uint32_t b = bucket_of(hb, x, cap);
     bool nonempty = false;
-    for (int j = 0; j < o.n; j++) nonempty |= (h[ll_get(o, j)] & (cap - 1)) == b;
+    for (int j = 0; j < o.n; j++) {
+      uint32_t y = ll_get(o, j);
+      present |= y == x;
+      nonempty |= bucket_of(hb, y, cap) == b;
+    }
+    if (present) continue;
     if (o.n >= thr && nonempty) {
       // HashMap.resize/transfer: new order = by new bucket, ties in reverse
       // of the old iteration order.
@@ -665,53 +712,98 @@ __device__ inline LList j7_union(LList L, int sym, const int32_t h[4]) {
       for (int bk = 0; bk < ncap; bk++)
         for (int j = o.n - 1; j >= 0; j--) {
           uint32_t y = ll_get(o, j);
-          if ((h[y] & (ncap - 1)) == bk) { r.bits |= y << (2 * r.n); r.n++; }
+          if (bucket_of(hb, y, ncap) == (uint32_t)bk) { r.bits |= y << (2 * r.n); r.n++; }
         }
       o = r;
       cap = ncap;
       thr = (cap * 3) >> 2;
-      b = h[x] & (cap - 1);
+      b = bucket_of(hb, x, cap);
     }
     // head of bucket b: after every element of a lower bucket
     int pos = 0;
-    for (int j = 0; j < o.n; j++) pos += (h[ll_get(o, j)] & (cap - 1)) < b ? 1 : 0;
+    for (int j = 0; j < o.n; j++) pos += bucket_of(hb, ll_get(o, j), cap) < b ? 1 : 0;
     uint32_t low = o.bits & ((1u << (2 * pos)) - 1);
-    uint32_t high = pos >= 16 ? 0 : (o.bits >> (2 * pos));
-    o.bits = low | ((uint32_t)x << (2 * pos)) | (high << (2 * pos + 2));
+    uint32_t high = o.bits >> (2 * pos);
+    o.bits = low | (x << (2 * pos)) | (high << (2 * pos + 2));
     o.n++;
   }
   return o;
 }
 
+// One thread per node of the replay list.  Only lists that end with >= 2
+// k-mers are simulated; a list with <= 1 k-mer has one order.  Fast paths:
+// unions that cannot reorder (empty or one-element list, no new element)
+// are skipped, and a union's result depends only on (list size, order,
+// symbol class) -- a new letter, a letter already present, or none -- so
+// the last two transitions of each list are cached.  Codes are loaded four
+// at a time.
 template <int W>
-__global__ void k_replay(uint64_t nn, const uint32_t* __restrict__ needs, const uint64_t* __restrict__ seg,
+__global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, const uint64_t* __restrict__ seg,
                          const uint32_t* __restrict__ codes, const uint64_t* __restrict__ klo,
-                         const uint64_t* __restrict__ khi, int k, int nb, uint64_t* __restrict__ lists) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    if (!needs[i]) continue;
+                         const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk, int k, int nb,
+                         uint64_t* __restrict__ lists) {
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrep; q += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t i = rlist[q];
     KV v = node_key<W>(klo, khi, i);
-    // hash of each list element: JDK-7-spread VKmer.hashCode of the stored
-    // neighbour (FF/FR store the same canonical k-mer for a letter, as do
-    // RF/RR)
-    int32_t h[4][4];
+    uint32_t mask = msk[i];
+    uint32_t rel = 0;
+    int nfin[4];
+    for (int t = 0; t < 4; t++) {
+      nfin[t] = __popc((mask >> (4 * t)) & 0xF);
+      if (nfin[t] > 1) rel |= 1u << t;
+    }
+    // low 3 bits of the JDK-7-spread VKmer.hashCode of each stored neighbour
+    uint32_t hb[4] = {0, 0, 0, 0};
     for (uint32_t t = 0; t < 4; t++)
-      for (uint32_t x = 0; x < 4; x++) h[t][x] = jdk7_spread(java_hash_vkmer<W>(neighbour<W>(v, k, t, x), k, nb));
+      if ((rel >> t) & 1)
+        for (uint32_t x = 0; x < 4; x++)
+          hb[t] |= ((uint32_t)jdk7_spread(java_hash_vkmer<W>(neighbour<W>(v, k, t, x), k, nb)) & 7u) << (3 * x);
     LList L[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    // two-entry transition cache per list: (n, order, symbol class) ->
+    // (n', order'); an all-ones word is empty (no valid key has n = 7)
+    uint32_t cache[4][2];
+    for (int t = 0; t < 4; t++) cache[t][0] = cache[t][1] = 0xffffffffu;
     uint64_t a = seg[i], e = seg[i + 1];
-    for (uint64_t r = a; r < e; r++) {
-      uint32_t code = codes[r];
-      int sym[4] = {-1, -1, -1, -1};
-      uint32_t sn = code & 0xF, pn = (code >> 4) & 0xF;
-      if (sn & 8) sym[(sn & 4) ? ET_FR : ET_FF] = (int)(sn & 3);
-      if (pn & 8) sym[(pn & 4) ? ET_RF : ET_RR] = (int)(pn & 3);
-      for (int t = 0; t < 4; t++) {
-        if (r == a) L[t] = j7_union(LList{0, 0}, sym[t], h[t]);  // init: union into an empty list
-        else L[t] = j7_union(L[t], sym[t], h[t]);
+    for (uint64_t r0 = a; r0 < e; r0 += 4) {
+      uint32_t c4[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) c4[j] = (r0 + j < e) ? codes[r0 + j] : 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (r0 + j >= e) break;
+        uint32_t code = c4[j];
+        int sym[4] = {-1, -1, -1, -1};
+        uint32_t sn = code & 0xF, pn = (code >> 4) & 0xF;
+        if (sn & 8) sym[(sn & 4) ? ET_FR : ET_FF] = (int)(sn & 3);
+        if (pn & 8) sym[(pn & 4) ? ET_RF : ET_RR] = (int)(pn & 3);
+        for (int t = 0; t < 4; t++) {
+          if (!((rel >> t) & 1)) continue;
+          int sy = sym[t];
+          LList& l = L[t];
+          if (l.n <= 1 && (sy < 0 || (l.n == 1 && (int)ll_get(l, 0) == sy))) continue;
+          if (l.n == 0) { l.bits = (uint32_t)sy; l.n = 1; continue; }
+          // symbol class: the new letter, 4 = a letter already present, 5 = none
+          uint32_t cls = 5;
+          if (sy >= 0) {
+            cls = (uint32_t)sy;
+            for (int j = 0; j < l.n; j++)
+              if ((int)ll_get(l, j) == sy) cls = 4;
+          }
+          uint32_t key = ((uint32_t)l.n << 11) | (l.bits << 3) | cls;  // 14 bits
+          uint32_t c0 = cache[t][0], c1 = cache[t][1];
+          if ((c0 >> 11) == key) { l.n = (c0 >> 8) & 7; l.bits = c0 & 0xff; continue; }
+          if ((c1 >> 11) == key) { l.n = (c1 >> 8) & 7; l.bits = c1 & 0xff; continue; }
+          LList o = j7_union(l, sy, hb[t]);
+          cache[t][1] = c0;
+          cache[t][0] = (key << 11) | ((uint32_t)o.n << 8) | o.bits;
+          l = o;
+        }
       }
     }
     uint64_t out = 0;
     for (int t = 0; t < 4; t++) {
-      LList f = j7_union(L[t], -1, h[t]);  // cross aggregator init
+      if (!((rel >> t) & 1)) continue;
+      LList f = j7_union(L[t], -1, hb[t]);  // cross aggregator init
       out |= ((uint64_t)f.n | ((uint64_t)f.bits << 3)) << (16 * t);
     }
     lists[i] = out;
@@ -730,14 +822,14 @@ __global__ void k_replay(uint64_t nn, const uint32_t* __restrict__ needs, const 
 template <int W>
 __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, const uint64_t* __restrict__ nocc,
                         const uint64_t* __restrict__ hidx, const uint64_t* __restrict__ occbase,
-                        const uint64_t* __restrict__ occ_slot, const uint32_t* __restrict__ slot2node,
+                        const uint32_t* __restrict__ occ_slot, const void* table,
                         const int64_t* __restrict__ rid, const uint64_t* __restrict__ wbase,
                         const uint64_t* __restrict__ words, int k, uint64_t* __restrict__ hkey,
                         uint32_t* __restrict__ hval) {
   for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x) {
     if (!valid[m] || nocc[m] == 0) continue;
-    uint64_t o = occbase[m];
-    uint64_t node = slot2node[occ_slot[o]];
+    uint32_t s = occ_slot[occbase[m]];
+    uint64_t node = W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
     bool fwd;
     (void)canon<W>(load_kmer<W>(words + wbase[m], 0, k), k, &fwd);
     uint64_t flipped = fwd ? 0 : 1;
@@ -748,14 +840,29 @@ __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, cons
   }
 }
 
-__device__ inline uint64_t lower_bound64(const uint64_t* a, uint64_t n, uint64_t x) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1;
-    else hi = mid;
+// Unique heads (TreeSet keeps the first of equal keys) and per-node counts
+// of unflipped / flipped heads.
+__global__ void k_head_unique(uint64_t nh, const uint64_t* __restrict__ hkey, uint32_t* __restrict__ uflag) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x)
+    uflag[j] = (j == 0 || hkey[j] != hkey[j - 1]) ? 1 : 0;
+}
+
+__global__ void k_head_compact(uint64_t nh, const uint64_t* __restrict__ hkey, const uint32_t* __restrict__ hval,
+                               const uint32_t* __restrict__ uflag, const uint64_t* __restrict__ upos,
+                               uint32_t* __restrict__ uval, uint32_t* __restrict__ hcu, uint32_t* __restrict__ hcf) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x) {
+    if (!uflag[j]) continue;
+    uint64_t key = hkey[j];
+    uval[upos[j]] = hval[j] | ((uint32_t)((key >> 30) & 1) << 31);
+    uint64_t node = key >> 31;
+    atomicAdd(((key >> 30) & 1) ? &hcf[node] : &hcu[node], 1u);
   }
-  return lo;
+}
+
+__global__ void k_head_total(uint64_t nn, const uint32_t* __restrict__ hcu, const uint32_t* __restrict__ hcf,
+                             uint64_t* __restrict__ tot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x)
+    tot[i] = (uint64_t)hcu[i] + hcf[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -766,17 +873,19 @@ __device__ inline uint64_t lower_bound64(const uint64_t* a, uint64_t n, uint64_t
 // ---------------------------------------------------------------------------
 
 struct SerCtx {
-  int k, nb, W, jdk7;
-  uint64_t nn, nheads;
+  int k, nb, jdk7;
+  uint64_t nn;
   const uint64_t *klo, *khi;
   const uint32_t *cnt, *msk;
   const uint32_t* needs;
   const uint64_t* lists;
-  const uint64_t* hkey;
-  const uint32_t* hval;
+  const uint64_t* hstart;  // first unique head of each node
+  const uint32_t *hcu, *hcf;
+  const uint32_t* uval;     // unique heads: mate index | flipped << 31
   const uint32_t* mlen;
   const uint64_t* wbase;
   const uint64_t* words;
+  const int64_t* rid;
   unsigned long long* bad;  // internal consistency flag (first bad node)
 };
 
@@ -792,7 +901,7 @@ template <int W>
 __device__ inline LList edge_list(const SerCtx& c, uint64_t i, int t, KV v) {
   uint32_t sel = (c.msk[i] >> (4 * t)) & 0xF;
   LList l{0, 0};
-  if (c.jdk7 && c.needs[i]) {
+  if (c.jdk7 && c.needs[i] && __popc(sel) > 1) {
     uint64_t w = c.lists[i] >> (16 * t);
     l.n = (int)(w & 7);
     l.bits = (uint32_t)(w >> 3) & 0xFF;
@@ -817,39 +926,28 @@ __device__ inline LList edge_list(const SerCtx& c, uint64_t i, int t, KV v) {
   return l;
 }
 
-struct HeadRange {
-  uint64_t a, f, e;
-};
-
-__device__ inline HeadRange head_range(const SerCtx& c, uint64_t i) {
-  HeadRange r;
-  r.a = lower_bound64(c.hkey, c.nheads, i << 31);
-  r.f = lower_bound64(c.hkey, c.nheads, (i << 31) | (1ull << 30));
-  r.e = lower_bound64(c.hkey, c.nheads, (i + 1) << 31);
-  return r;
+__device__ inline uint64_t record_size(const SerCtx& c, uint64_t i) {
+  int n[4];
+  type_sizes(c.msk[i], n);
+  uint64_t val = 1 + 4;
+  for (int t = 0; t < 4; t++)
+    if (n[t]) val += 4 + (uint64_t)n[t] * (4 + c.nb);
+  uint64_t h0 = c.hstart[i];
+  uint32_t nu = c.hcu[i], nf = c.hcf[i];
+  if (nu) {
+    val += 1 + 4;
+    for (uint32_t j = 0; j < nu; j++) val += head_bytes(c, c.uval[h0 + j] & 0x7fffffffu);
+  }
+  if (nf) {
+    val += 1 + 4;
+    for (uint32_t j = 0; j < nf; j++) val += head_bytes(c, c.uval[h0 + nu + j] & 0x7fffffffu);
+  }
+  return 8 + 4 + c.nb + val;
 }
 
-template <int W>
 __global__ void k_ser_size(SerCtx c, uint64_t* __restrict__ rsize) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    int n[4];
-    type_sizes(c.msk[i], n);
-    uint64_t val = 1 + 4;
-    for (int t = 0; t < 4; t++)
-      if (n[t]) val += 4 + (uint64_t)n[t] * (4 + c.nb);
-    HeadRange hr = head_range(c, i);
-    uint64_t hb[2] = {0, 0};
-    uint64_t cntset[2] = {0, 0};
-    for (uint64_t j = hr.a; j < hr.e; j++) {
-      if (j > 0 && c.hkey[j] == c.hkey[j - 1]) continue;  // duplicate uuid: TreeSet keeps the first
-      int f = j >= hr.f;
-      hb[f] += head_bytes(c, c.hval[j]);
-      cntset[f]++;
-    }
-    for (int f = 0; f < 2; f++)
-      if (cntset[f]) val += 1 + 4 + hb[f];
-    rsize[i] = 8 + 4 + c.nb + val;
-  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x)
+    rsize[i] = record_size(c, i);
 }
 
 __device__ inline void put32(uint8_t*& p, uint32_t v) {
@@ -874,70 +972,88 @@ __device__ inline void put_read(uint8_t*& p, const uint64_t* w, uint32_t L) {
 }
 
 template <int W>
-__global__ void k_ser_write(SerCtx c, const uint64_t* __restrict__ roff, const int64_t* __restrict__ rid,
-                            uint8_t* __restrict__ out) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint8_t* p = out + roff[i];
-    uint64_t rec = roff[i + 1] - roff[i];
-    KV v = node_key<W>(c.klo, c.khi, i);
-    uint32_t keyLen = 4 + c.nb;
-    put32(p, (uint32_t)(rec - 8));
-    put32(p, keyLen);
-    put_key<W>(p, v, c.k, c.nb);
-    int n[4];
-    type_sizes(c.msk[i], n);
-    HeadRange hr = head_range(c, i);
-    bool hu = false, hf = false;
-    for (uint64_t j = hr.a; j < hr.e; j++) {
-      if (j >= hr.f) hf = true;
-      else hu = true;
+__device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uint64_t rec) {
+  KV v = node_key<W>(c.klo, c.khi, i);
+  put32(p, (uint32_t)(rec - 8));
+  put32(p, 4 + c.nb);
+  put_key<W>(p, v, c.k, c.nb);
+  int n[4];
+  type_sizes(c.msk[i], n);
+  uint32_t nu = c.hcu[i], nf = c.hcf[i];
+  uint64_t h0 = c.hstart[i];
+  uint8_t flags = 0x80;
+  for (int t = 0; t < 4; t++)
+    if (n[t]) flags |= 1 << t;
+  if (nu) flags |= 1 << 4;
+  if (nf) flags |= 1 << 5;
+  *p++ = flags;
+  for (int t = 0; t < 4; t++) {
+    if (!n[t]) continue;
+    LList l = edge_list<W>(c, i, t, v);
+    if (l.n != n[t]) {  // cannot happen: the replayed set holds every neighbour once
+      if (c.bad) atomicMin(c.bad, (unsigned long long)i);
+      l.n = n[t];
     }
-    uint8_t flags = 0x80;
-    for (int t = 0; t < 4; t++)
-      if (n[t]) flags |= 1 << t;
-    if (hu) flags |= 1 << 4;
-    if (hf) flags |= 1 << 5;
-    *p++ = flags;
-    for (int t = 0; t < 4; t++) {
-      if (!n[t]) continue;
-      LList l = edge_list<W>(c, i, t, v);
-      if (l.n != n[t]) {  // cannot happen: the replayed set holds every neighbour once
-        if (c.bad) atomicMin(c.bad, (unsigned long long)i);
-        l.n = n[t];
-      }
-      put32(p, (uint32_t)n[t]);
-      for (int a = 0; a < n[t]; a++) {
-        uint32_t x = ll_get(l, a);
-        put_key<W>(p, neighbour<W>(v, c.k, (uint32_t)t, x), c.k, c.nb);
-      }
-    }
-    for (int f = 0; f < 2; f++) {
-      uint64_t a = f ? hr.f : hr.a, e = f ? hr.e : hr.f;
-      if (a == e) continue;
-      uint32_t cntu = 0;
-      for (uint64_t j = a; j < e; j++)
-        if (!(j > 0 && c.hkey[j] == c.hkey[j - 1])) cntu++;
-      *p++ = 1;  // whole body in stream
-      put32(p, cntu);
-      for (uint64_t j = a; j < e; j++) {
-        if (j > 0 && c.hkey[j] == c.hkey[j - 1]) continue;
-        uint32_t m = c.hval[j];
-        uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
-        *p++ = Lm > 0 ? 1 : 0;
-        uint64_t off = f ? (uint64_t)(c.k - 1) : 0;
-        uint64_t uuid = (off << 40) | ((uint64_t)(m & 1) << 35) | (uint64_t)rid[m >> 1];
-        for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
-        put_read(p, c.words + c.wbase[m], L);
-        if (Lm > 0) put_read(p, c.words + c.wbase[m ^ 1], Lm);
-      }
-    }
-    // coverage: Java float accumulation of 1.0f saturates at 2^24
-    uint32_t cv = c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i];
-    float fc = (float)cv;
-    uint32_t u;
-    memcpy(&u, &fc, 4);
-    put32(p, u);
+    put32(p, (uint32_t)n[t]);
+    for (int a = 0; a < n[t]; a++) put_key<W>(p, neighbour<W>(v, c.k, (uint32_t)t, ll_get(l, a)), c.k, c.nb);
   }
+  for (int f = 0; f < 2; f++) {
+    uint32_t cntf = f ? nf : nu;
+    if (!cntf) continue;
+    uint64_t a = h0 + (f ? nu : 0);
+    *p++ = 1;  // whole body in stream
+    put32(p, cntf);
+    for (uint64_t j = a; j < a + cntf; j++) {
+      uint32_t m = c.uval[j] & 0x7fffffffu;
+      uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
+      *p++ = Lm > 0 ? 1 : 0;
+      uint64_t off = f ? (uint64_t)(c.k - 1) : 0;
+      uint64_t uuid = (off << 40) | ((uint64_t)(m & 1) << 35) | (uint64_t)c.rid[m >> 1];
+      for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
+      put_read(p, c.words + c.wbase[m], L);
+      if (Lm > 0) put_read(p, c.words + c.wbase[m ^ 1], Lm);
+    }
+  }
+  // coverage: Java float accumulation of 1.0f saturates at 2^24
+  uint32_t cv = c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i];
+  float fc = (float)cv;
+  uint32_t u;
+  memcpy(&u, &fc, 4);
+  put32(p, u);
+}
+
+// Records of 256 consecutive nodes are contiguous in the output: each
+// thread writes its record into LDS, then the block stores the whole span
+// with coalesced 4-byte writes.  Spans larger than the staging buffer are
+// written directly.
+constexpr int kSerStage = 40960;
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* __restrict__ roff,
+                                                      uint8_t* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[kSerStage + 8];
+  uint64_t first = (uint64_t)blockIdx.x * kBlock;
+  if (first >= c.nn) return;
+  uint64_t last = first + kBlock < c.nn ? first + kBlock : c.nn;
+  uint64_t s0 = roff[first], s1 = roff[last];
+  uint64_t i = first + threadIdx.x;
+  // the stage starts at the 4-byte boundary at or below s0
+  uint64_t base = s0 & ~3ull;
+  bool staged = (s1 - base) <= kSerStage;
+  if (i < last) {
+    uint64_t rec = roff[i + 1] - roff[i];
+    write_record<W>(c, i, staged ? stage + (roff[i] - base) : out + roff[i], rec);
+  }
+  if (!staged) return;
+  __syncthreads();
+  // head and tail bytes that are not 4-byte aligned
+  uint64_t a4 = (s0 + 3) & ~3ull, e4 = s1 & ~3ull;
+  if (a4 > e4) { a4 = s1; e4 = s1; }
+  for (uint64_t x = s0 + threadIdx.x; x < a4; x += kBlock) out[x] = stage[x - base];
+  for (uint64_t x = e4 + threadIdx.x; x < s1; x += kBlock) out[x] = stage[x - base];
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
+  uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
+  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
 }
 
 // words per mate for the packed store
@@ -969,12 +1085,12 @@ struct gx_build {
   bool finished = false;
   // device buffers (grown on demand, reused across builds)
   DevBuf text, scratch, blk, blk2, term, rid, mstart, mlen, wcnt, wbase, words, valid, nocc, occbase, hmask, hidx;
-  DevBuf table, occ_slot, occ_code;
-  DevBuf ckey, cval, ckey2, cval2, nslot, klo, khi, cnt, msk, slot2node, needs, selcnt, seg;
-  DevBuf selflag, selpos, selk, selv, selk2, selv2, lists;
-  DevBuf hkey, hval, hkey2, hval2, rsize, roff, out;
+  DevBuf table, occ_slot, occ_code, claimed;
+  DevBuf ckey, cval, ckey2, cval2, nslot, klo, khi, cnt, msk, needs, selcnt, seg;
+  DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
+  DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t ev[2 * 8] = {};
@@ -985,8 +1101,9 @@ struct gx_build {
 
 static std::string g_create_err;
 
-enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, NPH };
-static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert"};
+enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, NPH };
+static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
+                                       "k_replay"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -1056,8 +1173,9 @@ static void run_build(gx_build* b) {
   const uint64_t n = b->text_len;
   const uint8_t* t = b->text.as<uint8_t>();
   const unsigned G = 2048;  // grid for grid-stride kernels (8 blocks per CU)
-  unsigned long long* errw = buf<unsigned long long>(b->errw, 2);
+  unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
   GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
+  GX_HIP(hipMemsetAsync(errw + 2, 0, 16, st));
 
   // 1. lines
   ph_begin(b, PH_PARSE);
@@ -1116,16 +1234,19 @@ static void run_build(gx_build* b) {
   exclusive_scan<uint64_t, uint64_t>(hm, hidx, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
   uint64_t nheads = read1(st, hidx + nmates);
 
-  // 4. table insert (grow-and-rebuild on overflow)
+  // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
+  //    of two >= the expected distinct count (table_hint) or the occurrence
+  //    count, so the load stays below 1 and is typically well below 1/2.
   ph_begin(b, PH_INSERT);
-  uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint : N;
+  uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint * 2 : N;
   uint64_t cap = 1024;
-  while (cap < 2 * expect) cap <<= 1;
+  while (cap < expect) cap <<= 1;
   const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
-  uint64_t* occ_slot = buf<uint64_t>(b->occ_slot, N + 1);
+  uint32_t* occ_slot = buf<uint32_t>(b->occ_slot, N + 1);
   uint8_t* occ_code = buf<uint8_t>(b->occ_code, N + 1);
   void* table = nullptr;
   for (int attempt = 0;; attempt++) {
+    if (cap > (1ull << 32)) throw GxError(GX_ENOTSUP, "hash table beyond 2^32 slots");
     table = b->table.ensure(cap * slotsz);
     hipLaunchKernelGGL(k_table_clear<W>, dim3(grid_for(cap, kBlock, 65536)), dim3(kBlock), 0, st, table, cap);
     GX_HIP(hipGetLastError());
@@ -1144,9 +1265,9 @@ static void run_build(gx_build* b) {
     if ((e & 0xff) != ERR_TABLE || attempt > 8) check_err(b);
     cap <<= 1;  // fallback: rebuild in a table twice as large
   }
-
   ph_end(b, PH_INSERT);
-  // 5. compaction + KmerPointable order
+
+  // 5. KmerPointable order
   ph_begin(b, PH_NODES);
   unsigned cblocks = grid_for(cap, kCTile, 0xffffffffu);
   uint64_t* cb = buf<uint64_t>(b->blk, (size_t)cblocks + 1);
@@ -1156,64 +1277,87 @@ static void run_build(gx_build* b) {
   exclusive_scan<uint64_t, uint64_t>(cb, cbb, cblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(cblocks)), st);
   uint64_t nn = read1(st, cbb + cblocks);
   if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
+  uint32_t* claimed = buf<uint32_t>(b->claimed, nn + 1);
+  hipLaunchKernelGGL(k_compact_write<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cbb, claimed);
+  GX_HIP(hipGetLastError());
   using SK = typename std::conditional<W == 1, uint64_t, Key128>::type;
   SK* ck = buf<SK>(b->ckey, nn + 1);
   SK* ck2 = buf<SK>(b->ckey2, nn + 1);
   uint32_t* cv = buf<uint32_t>(b->cval, nn + 1);
   uint32_t* cv2 = buf<uint32_t>(b->cval2, nn + 1);
-  uint64_t* nslot = buf<uint64_t>(b->nslot, nn + 1);
-  hipLaunchKernelGGL((k_compact_write<W, SK>), dim3(cblocks), dim3(kBlock), 0, st, table, cap, cbb, nb, ck, cv, nslot);
-  GX_HIP(hipGetLastError());
+  if (nn) {
+    hipLaunchKernelGGL((k_compact_list<W, SK>), dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, table, claimed,
+                       nn, nb, ck, cv);
+    GX_HIP(hipGetLastError());
+  }
   bool fl = radix_sort<SK>(ck, cv, ck2, cv2, nn, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nn)), st);
   uint32_t* perm = fl ? cv2 : cv;
   uint64_t* klo = buf<uint64_t>(b->klo, nn + 1);
   uint64_t* khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
   uint32_t* cnt = buf<uint32_t>(b->cnt, nn + 1);
   uint32_t* msk = buf<uint32_t>(b->msk, nn + 1);
-  uint32_t* slot2node = buf<uint32_t>(b->slot2node, cap);
-  hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, nslot, table,
-                     klo, khi, cnt, msk, slot2node);
-  GX_HIP(hipGetLastError());
-
+  uint32_t* nslot = buf<uint32_t>(b->nslot, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, claimed, table,
+                       klo, khi, cnt, msk, nslot);
+    GX_HIP(hipGetLastError());
+  }
   ph_end(b, PH_NODES);
+
   // 6. replay
   ph_begin(b, PH_REPLAY);
   const int jdk7 = b->cfg.edge_order == GX_EDGE_ORDER_JDK7;
   uint32_t* needs = buf<uint32_t>(b->needs, nn + 1);
   uint64_t* selcnt = buf<uint64_t>(b->selcnt, nn + 1);
   uint64_t* seg = buf<uint64_t>(b->seg, nn + 1);
-  hipLaunchKernelGGL(k_node_info, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, msk, cnt, jdk7, needs,
-                     selcnt);
-  GX_HIP(hipGetLastError());
+  if (nn) {
+    hipLaunchKernelGGL(k_node_info<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, msk, cnt, nslot, table,
+                       jdk7, needs, selcnt, errw + 3);
+    GX_HIP(hipGetLastError());
+  }
   exclusive_scan<uint64_t, uint64_t>(selcnt, seg, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   uint64_t nsel = read1(st, seg + nn);
   uint64_t* lists = buf<uint64_t>(b->lists, nn + 1);
-  uint64_t nrepl = 0;
+  uint64_t nrep = 0;
   if (nsel) {
     uint32_t* flag = buf<uint32_t>(b->selflag, N + 1);
     uint64_t* pos = buf<uint64_t>(b->selpos, N + 1);
-    hipLaunchKernelGGL(k_sel_flags, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, slot2node, needs,
-                       flag);
+    hipLaunchKernelGGL(k_sel_flags<W>, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, table, flag);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(flag, pos, N, scratch_for(b, scan_scratch_bytes<uint64_t>(N)), st);
     uint32_t* sk = buf<uint32_t>(b->selk, nsel + 1);
     uint32_t* sv = buf<uint32_t>(b->selv, nsel + 1);
     uint32_t* sk2 = buf<uint32_t>(b->selk2, nsel + 1);
     uint32_t* sv2 = buf<uint32_t>(b->selv2, nsel + 1);
-    hipLaunchKernelGGL(k_sel_write, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, occ_code,
-                       slot2node, flag, pos, sk, sv);
+    hipLaunchKernelGGL(k_sel_write<W>, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, occ_code, table,
+                       flag, pos, sk, sv);
     GX_HIP(hipGetLastError());
     int nbytes_id = 1;
     while (nbytes_id < 4 && (nn >> (8 * nbytes_id)) > 0) nbytes_id++;
     bool f2 = radix_sort<uint32_t>(sk, sv, sk2, sv2, nsel, 0, nbytes_id,
                                    scratch_for(b, radix_scratch_bytes<uint32_t>(nsel)), st);
     uint32_t* codes = f2 ? sv2 : sv;
-    hipLaunchKernelGGL(k_replay<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, needs, seg, codes, klo,
-                       khi, k, nb, lists);
+    // replay list ordered by descending record count
+    uint64_t* rpos = buf<uint64_t>(b->rpos, nn + 1);
+    exclusive_scan<uint32_t, uint64_t>(needs, rpos, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+    nrep = read1(st, rpos + nn);
+    uint32_t* rk = buf<uint32_t>(b->rkey, nrep + 1);
+    uint32_t* rv = buf<uint32_t>(b->rval, nrep + 1);
+    uint32_t* rk2 = buf<uint32_t>(b->rkey2, nrep + 1);
+    uint32_t* rv2 = buf<uint32_t>(b->rval2, nrep + 1);
+    hipLaunchKernelGGL(k_replay_list, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, needs, rpos, cnt, rk, rv);
     GX_HIP(hipGetLastError());
+    bool f4 = radix_sort<uint32_t>(rk, rv, rk2, rv2, nrep, 0, 2, scratch_for(b, radix_scratch_bytes<uint32_t>(nrep)),
+                                   st);
+    uint32_t* rlist = f4 ? rv2 : rv;
+    ph_begin(b, PH_KREPLAY);
+    hipLaunchKernelGGL(k_replay<W>, dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep, rlist, seg,
+                       codes, klo, khi, msk, k, nb, lists);
+    GX_HIP(hipGetLastError());
+    ph_end(b, PH_KREPLAY);
   }
-
   ph_end(b, PH_REPLAY);
+
   // 7. heads
   ph_begin(b, PH_HEADS);
   uint64_t* hk = buf<uint64_t>(b->hkey, nheads + 1);
@@ -1222,7 +1366,7 @@ static void run_build(gx_build* b) {
   uint32_t* hv2 = buf<uint32_t>(b->hval2, nheads + 1);
   if (nmates) {
     hipLaunchKernelGGL(k_heads<W>, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, valid, nocc, hidx,
-                       occbase, occ_slot, slot2node, rid, wbase, words, k, hk, hv);
+                       occbase, occ_slot, table, rid, wbase, words, k, hk, hv);
     GX_HIP(hipGetLastError());
   }
   int hbytes = 1;
@@ -1230,27 +1374,52 @@ static void run_build(gx_build* b) {
   bool f3 = radix_sort<uint64_t>(hk, hv, hk2, hv2, nheads, 0, hbytes,
                                  scratch_for(b, radix_scratch_bytes<uint64_t>(nheads)), st);
   if (f3) { std::swap(hk, hk2); std::swap(hv, hv2); }
-
+  uint32_t* uflag = buf<uint32_t>(b->uflag, nheads + 1);
+  uint64_t* upos = buf<uint64_t>(b->upos, nheads + 1);
+  uint32_t* uval = buf<uint32_t>(b->uval, nheads + 1);
+  uint32_t* hcu = buf<uint32_t>(b->hcu, nn + 1);
+  uint32_t* hcf = buf<uint32_t>(b->hcf, nn + 1);
+  uint64_t* htot = buf<uint64_t>(b->htot, nn + 1);
+  uint64_t* hstart = buf<uint64_t>(b->hstart, nn + 1);
+  GX_HIP(hipMemsetAsync(hcu, 0, (nn + 1) * 4, st));
+  GX_HIP(hipMemsetAsync(hcf, 0, (nn + 1) * 4, st));
+  if (nheads) {
+    hipLaunchKernelGGL(k_head_unique, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, uflag);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(uflag, upos, nheads, scratch_for(b, scan_scratch_bytes<uint64_t>(nheads)), st);
+  if (nheads) {
+    hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, hv, uflag,
+                       upos, uval, hcu, hcf);
+    GX_HIP(hipGetLastError());
+  }
+  if (nn) {
+    hipLaunchKernelGGL(k_head_total, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, hcu, hcf, htot);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   ph_end(b, PH_HEADS);
+
   // 8. serialisation
   ph_begin(b, PH_SER);
   SerCtx c;
-  c.k = k; c.nb = nb; c.W = W; c.jdk7 = jdk7; c.nn = nn; c.nheads = nheads;
+  c.k = k; c.nb = nb; c.jdk7 = jdk7; c.nn = nn;
   c.klo = klo; c.khi = khi; c.cnt = cnt; c.msk = msk; c.needs = needs; c.lists = lists;
-  c.hkey = hk; c.hval = hv; c.mlen = mlen; c.wbase = wbase; c.words = words;
+  c.hstart = hstart; c.hcu = hcu; c.hcf = hcf; c.uval = uval;
+  c.mlen = mlen; c.wbase = wbase; c.words = words; c.rid = rid;
   c.bad = errw + 1;
   GX_HIP(hipMemsetAsync(errw + 1, 0xff, 8, st));
   uint64_t* rsize = buf<uint64_t>(b->rsize, nn + 1);
   uint64_t* roff = buf<uint64_t>(b->roff, nn + 1);
   if (nn) {
-    hipLaunchKernelGGL(k_ser_size<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, rsize);
+    hipLaunchKernelGGL(k_ser_size, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, rsize);
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint64_t, uint64_t>(rsize, roff, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   uint64_t total = read1(st, roff + nn);
   uint8_t* out = buf<uint8_t>(b->out, total + 1);
   if (nn) {
-    hipLaunchKernelGGL(k_ser_write<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, c, roff, rid, out);
+    hipLaunchKernelGGL(k_ser_write<W>, dim3(grid_for(nn, kBlock, 0xffffffffu)), dim3(kBlock), 0, st, c, roff, out);
     GX_HIP(hipGetLastError());
   }
   ph_end(b, PH_SER);
@@ -1265,9 +1434,9 @@ static void run_build(gx_build* b) {
   b->stats[1] = nn;
   b->stats[2] = nheads;
   b->stats[3] = cap;
-  b->stats[4] = 0;
+  b->stats[4] = nrep;
   b->stats[5] = nsel;
-  (void)nrepl;
+  b->stats[6] = read1(st, errw + 3);
 }
 
 template <class F>
@@ -1413,7 +1582,7 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
 
 int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
-  for (int i = 0; i < n && i < 6; i++) out[i] = b->stats[i];
+  for (int i = 0; i < n && i < 7; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 

@@ -110,9 +110,10 @@ class GraphBuilder:
         return buf.raw[:w.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 6)()
-        self._check(lib().gx_build_stats(self._h, a, 6))
-        keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences"]
+        a = (ctypes.c_uint64 * 7)()
+        self._check(lib().gx_build_stats(self._h, a, 7))
+        keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
+                "max_coverage"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

@@ -91,7 +91,8 @@ int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* 
 int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* nbytes);
 
 /* Statistics of the last finish: [0] occurrences, [1] distinct k-mers,
- * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed occurrences. */
+ * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed
+ * occurrences, [6] largest coverage above 255 (0 if none). */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on
