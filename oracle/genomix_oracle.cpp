@@ -979,6 +979,223 @@ static std::string fastq_to_readids(const uint8_t* m1, size_t n1, const uint8_t*
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Unitig compression, sequential restatement of the fixed point of P4 path
+// merging (P4ForPathMergeVertex.java:21-266, BasicPathMergeVertex.java:33-219,
+// DeBruijnGraphCleanVertex.java:234-245, Node.mergeWithNodeUsingTruncatedKmer
+// Node.java:568-806).  A link X -(t)- Y is merged iff X's side of t has
+// degree 1, Y's side of mirror(t) has degree 1, X != Y and neither node has
+// itself in an edge list (tandem repeat: restrictNeighbors restricts both
+// directions, BasicPathMergeVertex.java:33-73).  The random head/tail choice
+// of P4 only decides which k-mer survives and the orientation; this walker
+// fixes them: the survivor is the unitig's first k-mer in KmerPointable
+// order, the sequence is oriented so that it reads forward, and a cycle is
+// cut at the link entering it (the cycle closes on that node, as the
+// tandem-repeat stop of P4 leaves it).
+// ---------------------------------------------------------------------------
+
+static std::string rc_string(const std::string& s) {
+  std::string r(s.rbegin(), s.rend());
+  for (auto& c : r) c = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+  return r;
+}
+
+struct GNode {
+  VKmer key;
+  Node node;
+};
+
+// EDGETYPE.dir / neighborDir / mirror (EDGETYPE.java:44-188): side 0 = F, 1 = R.
+static inline int et_side(int t) { return (t == FF || t == FR) ? 0 : 1; }
+static inline int et_mirror(int t) { return t == FF ? RR : t == RR ? FF : t; }
+
+static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
+  std::vector<GNode> g;
+  size_t off = 0;
+  while (off < n) {
+    if (off + 8 > n) throw OraError(E_INPUT, "truncated record stream");
+    uint32_t recLen = get_be32(recs + off), keyLen = get_be32(recs + off + 4);
+    if (off + 8 + recLen > n || keyLen > recLen) throw OraError(E_INPUT, "bad record");
+    GNode x;
+    read_vkmer(recs + off + 8, keyLen, x.key);
+    x.node.read(recs + off + 8 + keyLen, recLen - keyLen);
+    if (x.key.k != k) throw OraError(E_INPUT, "key length differs from k");
+    g.push_back(std::move(x));
+    off += 8 + recLen;
+  }
+  const size_t N = g.size();
+  std::map<std::vector<uint8_t>, size_t> idx;
+  for (size_t i = 0; i < N; i++) idx[g[i].key.b] = i;
+  std::vector<int> deg(2 * N, 0);
+  std::vector<char> tandem(N, 0);
+  std::vector<long> nbr(2 * N, -1);
+  std::vector<int> nbr_side(2 * N, -1);
+  for (size_t i = 0; i < N; i++) {
+    for (int t = 0; t < 4; t++) {
+      for (auto& z : g[i].node.edges[t]) {
+        deg[2 * i + et_side(t)]++;
+        if (z == g[i].key) tandem[i] = 1;
+      }
+    }
+    for (int t = 0; t < 4; t++) {
+      if (deg[2 * i + et_side(t)] == 1 && g[i].node.edges[t].size() == 1) {
+        auto it = idx.find(g[i].node.edges[t][0].b);
+        if (it == idx.end()) throw OraError(E_INPUT, "edge to a k-mer that is not in the graph");
+        nbr[2 * i + et_side(t)] = (long)it->second;
+        nbr_side[2 * i + et_side(t)] = et_side(et_mirror(t));
+      }
+    }
+  }
+  auto mergeable = [&](size_t i, int d) {
+    long j = nbr[2 * i + d];
+    if (deg[2 * i + d] != 1 || tandem[i] || j < 0 || (size_t)j == i) return false;
+    return deg[2 * j + nbr_side[2 * i + d]] == 1 && !tandem[j];
+  };
+  struct Elem { size_t node; bool fwd; };
+  std::vector<long> comp_of(N, -1);
+  std::vector<std::vector<Elem>> comps;
+  for (size_t i = 0; i < N; i++) {
+    if (comp_of[i] >= 0) continue;
+    // walk exiting F (i forward) and exiting R (i reversed)
+    std::vector<Elem> fwdpart{{i, true}}, backpart;
+    bool cycle = false;
+    size_t cur = i;
+    int exit = 0;
+    while (mergeable(cur, exit)) {
+      size_t j = (size_t)nbr[2 * cur + exit];
+      int entered = nbr_side[2 * cur + exit];
+      if (j == i) { cycle = true; break; }
+      cur = j;
+      exit = 1 - entered;
+      fwdpart.push_back({cur, exit == 0});
+    }
+    if (!cycle) {
+      cur = i;
+      exit = 1;
+      while (mergeable(cur, exit)) {
+        size_t j = (size_t)nbr[2 * cur + exit];
+        int entered = nbr_side[2 * cur + exit];
+        cur = j;
+        exit = 1 - entered;
+        backpart.push_back({cur, exit == 1});  // orientation relative to i-forward
+      }
+    }
+    std::vector<Elem> path(backpart.rbegin(), backpart.rend());
+    path.insert(path.end(), fwdpart.begin(), fwdpart.end());
+    size_t mn = path[0].node, mpos = 0;
+    for (size_t q = 0; q < path.size(); q++)
+      if (path[q].node < mn) { mn = path[q].node; mpos = q; }
+    if (cycle) {
+      // rotate so that the smallest node comes first, reading forward
+      std::vector<Elem> rot(path.begin() + mpos, path.end());
+      rot.insert(rot.end(), path.begin(), path.begin() + mpos);
+      if (!rot[0].fwd) {
+        std::vector<Elem> rev{rot[0]};
+        for (size_t q = rot.size() - 1; q >= 1; q--) rev.push_back(rot[q]);
+        rot = rev;
+        for (auto& e : rot) e.fwd = !e.fwd;
+      }
+      path = rot;
+    } else if (!path[mpos].fwd) {
+      std::reverse(path.begin(), path.end());
+      for (auto& e : path) e.fwd = !e.fwd;
+    }
+    for (auto& e : path) comp_of[e.node] = (long)comps.size();
+    comps.push_back(path);
+  }
+  // position / orientation of every node in its unitig
+  std::vector<size_t> pos(N);
+  std::vector<char> ori(N);
+  std::vector<size_t> ukey(comps.size());
+  for (size_t c = 0; c < comps.size(); c++) {
+    size_t mn = comps[c][0].node;
+    for (size_t q = 0; q < comps[c].size(); q++) {
+      pos[comps[c][q].node] = q;
+      ori[comps[c][q].node] = comps[c][q].fwd;
+      mn = std::min(mn, comps[c][q].node);
+    }
+    ukey[c] = mn;
+  }
+  std::vector<size_t> order(comps.size());
+  for (size_t c = 0; c < comps.size(); c++) order[c] = c;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ukey[a] < ukey[b]; });
+  std::vector<uint8_t> out;
+  for (size_t c : order) {
+    const auto& path = comps[c];
+    const size_t m = path.size();
+    Node u;
+    // sequence: overlap-(k-1) concatenation of the oriented k-mers
+    // (VKmer.mergeWithFFKmer etc., VKmer.java:525-692)
+    std::string seq;
+    for (size_t q = 0; q < m; q++) {
+      std::string s = g[path[q].node].key.str();
+      if (!path[q].fwd) s = rc_string(s);
+      if (q == 0) seq = s;
+      else seq.push_back(s.back());
+    }
+    if (m > 1) {
+      u.has_internal = true;
+      u.internal = VKmer::from_string((const uint8_t*)seq.data(), seq.size());
+    }
+    // coverage: sum of node coverages / number of k-mers (Node.mergeCoverage
+    // is this weighted average, computed pairwise in float by the reference)
+    double csum = 0;
+    for (auto& e : path) csum += g[e.node].node.cov;
+    u.has_cov = true;
+    u.cov = (float)(csum / (double)m);
+    // read heads: offset = k-mer position in the unitig; flipped heads sit
+    // at position + k - 1; a reversed node swaps the two sets
+    // (Node.mergeUnflippedAndFlippedReadIDs, Node.java:740-806)
+    for (size_t q = 0; q < m; q++) {
+      const Node& x = g[path[q].node].node;
+      for (int f = 0; f < 2; f++) {
+        const auto& set = f ? x.flipped : x.unflipped;
+        for (auto h : set) {
+          bool nf = path[q].fwd ? f : !f;
+          int o = (int)q + (nf ? k - 1 : 0);
+          h.uuid = make_uuid(h.mate_id(), h.lib(), h.read_id(), o);
+          if (nf) { u.has_flipped = true; treeset_add(u.flipped, h); }
+          else { u.has_unflipped = true; treeset_add(u.unflipped, h); }
+        }
+      }
+    }
+    // outer edges of the two ends, re-targeted to neighbour unitigs
+    std::set<std::pair<int, std::vector<uint8_t>>> edges;
+    for (int end = 0; end < 2; end++) {
+      const Elem& e = end == 0 ? path[0] : path[m - 1];
+      int uletter = end == 0 ? 1 : 0;                 // 5' end -> R, 3' end -> F
+      int side = (end == 0) == e.fwd ? 1 : 0;          // node side facing out
+      const Node& x = g[e.node].node;
+      for (int t = 0; t < 4; t++) {
+        if (et_side(t) != side) continue;
+        for (auto& z : x.edges[t]) {
+          size_t zi = idx.at(z.b);
+          bool z5 = (t == FF || t == RF);  // touches the neighbour's 5' end
+          int wletter = (z5 == (bool)ori[zi]) ? 0 : 1;
+          int ut = uletter == 0 ? (wletter == 0 ? FF : FR) : (wletter == 0 ? RF : RR);
+          edges.insert({ut, g[ukey[comp_of[zi]]].key.b});
+        }
+      }
+    }
+    for (auto& pe : edges) {
+      VKmer v;
+      v.k = k;
+      v.b = pe.second;
+      u.edges_present[pe.first] = true;
+      u.edges[pe.first].push_back(v);
+    }
+    VKmer key = g[ukey[c]].key;
+    std::vector<uint8_t> kb, vb;
+    key.write(kb);
+    u.write(vb);
+    put_be32(out, (uint32_t)(kb.size() + vb.size()));
+    put_be32(out, (uint32_t)kb.size());
+    out.insert(out.end(), kb.begin(), kb.end());
+    out.insert(out.end(), vb.begin(), vb.end());
+  }
+  return out;
+}
+
 }  // namespace ora
 
 // ---------------------------------------------------------------------------
@@ -1128,6 +1345,14 @@ int64_t ora_make_uuid(int mate, int lib, int64_t readId, int offset) {
   if (guard([&] { v = make_uuid(mate, lib, readId, offset); }) != 0) return INT64_MIN;
   return v;
 }
+int ora_unitigs(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = unitigs(recs, n, k);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
 const char* ora_float_str(float f) {
   static thread_local std::string s;
   s = java_float_str(f);
