@@ -1068,6 +1068,456 @@ __global__ void k_valid_u64(uint64_t nmates, const uint8_t* __restrict__ valid, 
     out[m] = (valid[m] && nocc[m]) ? 1 : 0;
 }
 
+// ===========================================================================
+// Unitig compression (MERGE / P4 path merge) on the built graph.
+//
+// The fixed point of P4ForPathMergeVertex (P4ForPathMergeVertex.java:21-266;
+// BasicPathMergeVertex.java:33-219) merges exactly the links X -(t)- Y where
+// X's side of t and Y's side of mirror(t) both have degree 1, X != Y and
+// neither node lists itself (a tandem repeat restricts both directions,
+// DeBruijnGraphCleanVertex.java:234-245).  Here the merged paths are found by
+// pointer-jumping list ranking over node ends: element 2i+d traverses node i
+// leaving through side d (0 = F, 1 = R); its successor is the element of
+// the merge partner that leaves through the partner's other side.  The
+// surviving k-mer (random in P4) is the path's first k-mer in KmerPointable
+// order, the path is oriented so that it reads forward, and a cycle is cut
+// at the link entering that k-mer.
+// ===========================================================================
+
+constexpr uint32_t kEnd = 0xffffffffu;
+constexpr int kUEdges = 16;
+
+// Read-only lookup of a key that is known to be in the table.
+template <int W>
+__device__ inline uint64_t table_find(const void* table, uint64_t capmask, KV key) {
+  uint64_t s = khash<W>(key) & capmask;
+  for (uint64_t probe = 0; probe <= capmask; probe++) {
+    if (W == 1) {
+      uint64_t cur = ((const Slot1*)table)[s].key;
+      if (cur == key.lo) return s;
+      if (cur == kEmpty) return ~0ull;
+    } else {
+      uint64_t cur = ((const Slot2*)table)[s].hi;
+      if (cur == key.hi && ((const Slot2*)table)[s].lo == key.lo) return s;
+      if (cur == kEmpty) return ~0ull;
+    }
+    s = (s + 1) & capmask;
+  }
+  return ~0ull;
+}
+
+template <int W>
+__device__ inline uint32_t node_of_key(const void* table, uint64_t capmask, KV key) {
+  uint64_t s = table_find<W>(table, capmask, key);
+  if (s == ~0ull) return kEnd;
+  return W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
+}
+
+__device__ inline int side_deg(uint32_t mask, int d) {
+  return d == 0 ? __popc(mask & 0xFFu) : __popc((mask >> 8) & 0xFFu);
+}
+// EDGETYPE.mirror / dir (EDGETYPE.java:44-60,130-160): the neighbour's side
+// facing back along an edge of type t.
+__device__ inline int back_side(uint32_t t) { return (t == ET_FF || t == ET_RF) ? 1 : 0; }
+
+// Per node: tandem flag, and for each side of degree 1 its partner
+// (node << 1 | partner side) or kEnd.
+template <int W>
+__global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
+                        const uint32_t* __restrict__ msk, const void* table, uint64_t capmask, int k,
+                        uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    KV v = node_key<W>(klo, khi, i);
+    uint32_t mask = msk[i] & 0xFFFFu;
+    bool tnd = false;
+    uint32_t l[2] = {kEnd, kEnd};
+    for (uint32_t t = 0; t < 4; t++) {
+      for (uint32_t x = 0; x < 4; x++) {
+        if (!((mask >> (4 * t + x)) & 1)) continue;
+        KV z = neighbour<W>(v, k, t, x);
+        if (keq<W>(z, v)) tnd = true;
+        int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
+        if (side_deg(mask, d) == 1) {
+          uint32_t j = node_of_key<W>(table, capmask, z);
+          if (j == kEnd) atomicMin(err, (unsigned long long)i);
+          else l[d] = (j << 1) | (uint32_t)back_side(t);
+        }
+      }
+    }
+    tand[i] = tnd ? 1 : 0;
+    lnk[2 * i] = l[0];
+    lnk[2 * i + 1] = l[1];
+  }
+}
+
+__device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const uint32_t* msk, const uint8_t* tand,
+                                 const uint8_t* cut) {
+  uint32_t l = lnk[2 * i + d];
+  if (l == kEnd || tand[i] || cut[2 * i + d]) return false;
+  uint32_t j = l >> 1;
+  int sj = (int)(l & 1);
+  if (j == i || tand[j]) return false;
+  return side_deg(msk[j] & 0xFFFFu, sj) == 1;
+}
+
+__global__ void k_elem_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
+                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut,
+                            uint32_t* __restrict__ ptr, uint32_t* __restrict__ dist, uint32_t* __restrict__ mn,
+                            uint32_t* __restrict__ last) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * nn;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i = e >> 1;
+    int d = (int)(e & 1);
+    uint32_t p = kEnd;
+    if (mergeable(i, d, lnk, msk, tand, cut)) {
+      uint32_t l = lnk[e];
+      p = ((l >> 1) << 1) | (1u - (l & 1));  // leave the partner through its other side
+    }
+    ptr[e] = p;
+    dist[e] = p == kEnd ? 0 : 1;
+    mn[e] = (uint32_t)i;
+    last[e] = (uint32_t)e;
+  }
+}
+
+// One doubling round.  active != 0 afterwards while some chain is still
+// being followed (cycles never finish).
+__global__ void __launch_bounds__(kBlock) k_jump(uint64_t ne, const uint32_t* __restrict__ ptr,
+                                                 const uint32_t* __restrict__ dist, const uint32_t* __restrict__ mn,
+                                                 const uint32_t* __restrict__ last, uint32_t* __restrict__ ptr2,
+                                                 uint32_t* __restrict__ dist2, uint32_t* __restrict__ mn2,
+                                                 uint32_t* __restrict__ last2, uint32_t* active) {
+  int any = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t p = ptr[e];
+    if (p == kEnd) {
+      ptr2[e] = kEnd;
+      dist2[e] = dist[e];
+      mn2[e] = mn[e];
+      last2[e] = last[e];
+    } else {
+      uint32_t q = ptr[p];
+      ptr2[e] = q;
+      dist2[e] = dist[e] + dist[p];
+      mn2[e] = min(mn[e], mn[p]);
+      last2[e] = last[p];
+      any |= q != kEnd;
+    }
+  }
+  if (__syncthreads_or(any) && threadIdx.x == 0) *active = 1;
+}
+
+// Cycles: the smallest node of each cycle gives up the link on its R side.
+__global__ void k_cut_cycles(uint64_t ne, const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ mn,
+                             const uint32_t* __restrict__ lnk, uint8_t* __restrict__ cut) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
+    if (ptr[e] == kEnd || (e & 1) == 0 || mn[e] != (uint32_t)(e >> 1)) continue;
+    cut[e] = 1;
+    uint32_t l = lnk[e];
+    cut[2 * (uint64_t)(l >> 1) + (l & 1)] = 1;
+  }
+}
+
+// Per node: unitig representative (smallest node), orientation and position.
+// C1 = the chain holding (rep, F); a node reads forward in the unitig iff
+// its F element shares C1's tail.
+__global__ void k_node_final(uint64_t nn, const uint32_t* __restrict__ dist, const uint32_t* __restrict__ mn,
+                             const uint32_t* __restrict__ last, uint32_t* __restrict__ urep,
+                             uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
+                             uint64_t* __restrict__ ulen_of_rep) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t r = min(mn[2 * i], mn[2 * i + 1]);
+    uint32_t dF = dist[2 * i], dR = dist[2 * i + 1];
+    uint32_t m = dF + dR + 1;
+    bool fwd = last[2 * i] == last[2 * (uint64_t)r];
+    uint32_t pos = m - 1 - (fwd ? dF : dR);
+    urep[i] = r | (fwd ? 0u : 0x80000000u);
+    upos[i] = pos;
+    isrep[i] = r == (uint32_t)i ? 1 : 0;
+    ulen_of_rep[i] = r == (uint32_t)i ? m : 0;
+  }
+}
+
+// Unitig u = rank of its representative; nodes placed in unitig order.
+__global__ void k_place(uint64_t nn, const uint32_t* __restrict__ urep, const uint32_t* __restrict__ upos,
+                        const uint64_t* __restrict__ uid_of_rep, const uint64_t* __restrict__ ubase,
+                        const uint32_t* __restrict__ cnt, uint32_t* __restrict__ order,
+                        unsigned long long* __restrict__ ucov) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t r = urep[i] & 0x7fffffffu;
+    uint64_t u = uid_of_rep[r];
+    order[ubase[u] + upos[i]] = (uint32_t)i | (urep[i] & 0x80000000u);
+    uint32_t c = cnt[i] > (1u << 24) ? (1u << 24) : cnt[i];
+    atomicAdd(&ucov[u], (unsigned long long)c);
+  }
+}
+
+__global__ void k_rep_list(uint64_t nn, const uint32_t* __restrict__ isrep, const uint64_t* __restrict__ uid_of_rep,
+                           const uint64_t* __restrict__ ulen_of_rep, uint32_t* __restrict__ urep_node,
+                           uint64_t* __restrict__ ulen) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!isrep[i]) continue;
+    uint64_t u = uid_of_rep[i];
+    urep_node[u] = (uint32_t)i;
+    ulen[u] = ulen_of_rep[i];
+  }
+}
+
+// Letter j (0 = first) of node i's k-mer read forward (fwd) or reverse-
+// complemented.
+template <int W>
+__device__ inline uint32_t oriented_letter(const uint64_t* klo, const uint64_t* khi, uint32_t i, bool fwd, int k,
+                                           int j) {
+  int q = fwd ? j : k - 1 - j;
+  uint32_t x = (uint32_t)((q < 32 ? (klo[i] >> (2 * q)) : (W == 2 ? khi[i] >> (2 * (q - 32)) : 0)) & 3);
+  return fwd ? x : 3 - x;
+}
+
+// Outer edges of unitig u: the 5' end node's outward side and the 3' end
+// node's outward side, re-targeted to neighbour unitigs (their representative
+// k-mer) and re-typed by the neighbour's orientation; sorted by type then
+// neighbour k-mer, duplicates removed.  Up to 16 entries (for even k a
+// palindromic neighbour is listed under two types): (type << 30 | rep).
+template <int W>
+__global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, const uint64_t* __restrict__ ulen,
+                               const uint32_t* __restrict__ order, const uint64_t* __restrict__ klo,
+                               const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk,
+                               const void* table, uint64_t capmask, const uint32_t* __restrict__ urep, int k,
+                               uint32_t* __restrict__ uedge, uint8_t* __restrict__ uecnt) {
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t ent[kUEdges];
+    KV entk[kUEdges];
+    int ne = 0;
+    uint64_t m = ulen[u];
+    for (int end = 0; end < 2; end++) {
+      uint32_t o = order[ubase[u] + (end == 0 ? 0 : m - 1)];
+      uint32_t i = o & 0x7fffffffu;
+      bool fwd = !(o & 0x80000000u);
+      uint32_t uletter = end == 0 ? 1 : 0;              // 5' end -> R, 3' end -> F
+      int side = ((end == 0) == fwd) ? 1 : 0;           // node side facing out
+      KV v = node_key<W>(klo, khi, i);
+      uint32_t mask = msk[i];
+      for (uint32_t t = 0; t < 4; t++) {
+        int ts = (t == ET_FF || t == ET_FR) ? 0 : 1;
+        if (ts != side) continue;
+        for (uint32_t x = 0; x < 4; x++) {
+          if (!((mask >> (4 * t + x)) & 1)) continue;
+          KV z = neighbour<W>(v, k, t, x);
+          uint32_t zi = node_of_key<W>(table, capmask, z);
+          if (zi == kEnd) continue;  // cannot happen: the graph is symmetric
+          uint32_t zr = urep[zi];
+          bool z5 = (t == ET_FF || t == ET_RF);           // touches z's 5' end
+          bool zfwd = !(zr & 0x80000000u);
+          uint32_t wletter = (z5 == zfwd) ? 0 : 1;
+          uint32_t ut = uletter == 0 ? (wletter == 0 ? ET_FF : ET_FR) : (wletter == 0 ? ET_RF : ET_RR);
+          uint32_t rep = zr & 0x7fffffffu;
+          KV rk = node_key<W>(klo, khi, rep);
+          bool dup = false;
+          for (int a = 0; a < ne; a++) dup |= (ent[a] >> 30) == ut && (ent[a] & 0x3fffffffu) == rep;
+          if (!dup && ne < kUEdges) {
+            // insertion: by type, then by neighbour k-mer (VKmer byte order)
+            int p = ne;
+            while (p > 0) {
+              uint32_t pt = ent[p - 1] >> 30;
+              if (pt < ut || (pt == ut && kle<W>(entk[p - 1], rk))) break;
+              ent[p] = ent[p - 1];
+              entk[p] = entk[p - 1];
+              p--;
+            }
+            ent[p] = (ut << 30) | rep;
+            entk[p] = rk;
+            ne++;
+          }
+        }
+      }
+    }
+    for (int a = 0; a < kUEdges; a++) uedge[kUEdges * u + a] = a < ne ? ent[a] : kEnd;
+    uecnt[u] = (uint8_t)ne;
+  }
+}
+
+// Read heads of every node carried into its unitig: offset = k-mer position,
+// + k-1 for flipped heads; a reversed node swaps the two sets
+// (Node.mergeUnflippedAndFlippedReadIDs, Node.java:740-806).  Sort key:
+// hi = unitig, lo = flipped << 53 | offset << 30 | mate << 29 | readId.
+__global__ void k_unitig_heads(uint64_t nn, const uint64_t* __restrict__ hstart, const uint32_t* __restrict__ hcu,
+                               const uint32_t* __restrict__ hcf, const uint32_t* __restrict__ uval,
+                               const uint32_t* __restrict__ urep, const uint32_t* __restrict__ upos,
+                               const uint64_t* __restrict__ uid_of_rep, const int64_t* __restrict__ rid, int k,
+                               Key128* __restrict__ key, uint32_t* __restrict__ val) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t nh = hcu[i] + hcf[i];
+    if (!nh) continue;
+    uint64_t h0 = hstart[i];
+    bool fwd = !(urep[i] & 0x80000000u);
+    uint64_t u = uid_of_rep[urep[i] & 0x7fffffffu];
+    for (uint32_t j = 0; j < nh; j++) {
+      uint32_t m = uval[h0 + j] & 0x7fffffffu;
+      bool f = j >= hcu[i];
+      bool nf = fwd ? f : !f;
+      uint64_t off = upos[i] + (nf ? (uint64_t)(k - 1) : 0);
+      Key128 kk;
+      kk.hi = u;
+      kk.lo = ((uint64_t)nf << 53) | (off << 30) | ((uint64_t)(m & 1) << 29) | (uint64_t)rid[m >> 1];
+      key[h0 + j] = kk;
+      val[h0 + j] = m | ((uint32_t)nf << 31);
+    }
+  }
+}
+
+__global__ void k_uhead_unique(uint64_t nh, const Key128* __restrict__ key, uint32_t* __restrict__ uflag,
+                               uint32_t* __restrict__ hu, uint32_t* __restrict__ hf) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x) {
+    bool first = j == 0 || key[j].lo != key[j - 1].lo || key[j].hi != key[j - 1].hi;
+    uflag[j] = first ? 1 : 0;
+    if (first) atomicAdd(((key[j].lo >> 53) & 1) ? &hf[key[j].hi] : &hu[key[j].hi], 1u);
+  }
+}
+
+__global__ void k_uhead_compact(uint64_t nh, const uint32_t* __restrict__ uflag, const uint64_t* __restrict__ upos_,
+                                const Key128* __restrict__ key, const uint32_t* __restrict__ val,
+                                uint64_t* __restrict__ ukey_lo, uint32_t* __restrict__ uval_out) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x) {
+    if (!uflag[j]) continue;
+    ukey_lo[upos_[j]] = key[j].lo;
+    uval_out[upos_[j]] = val[j];
+  }
+}
+
+struct USerCtx {
+  int k, nb;
+  uint64_t nu;
+  const uint64_t *klo, *khi;
+  const uint64_t *ubase, *ulen;
+  const uint32_t* order;
+  const uint32_t* urep_node;
+  const unsigned long long* ucov;
+  const uint32_t* uedge;
+  const uint8_t* uecnt;
+  const uint64_t* hstart;  // first head of each unitig
+  const uint32_t *hu, *hf;
+  const uint64_t* hkey;    // offset etc.
+  const uint32_t* hval;    // mate index | flipped << 31
+  const uint32_t* mlen;
+  const uint64_t* wbase;
+  const uint64_t* words;
+  const int64_t* rid;
+};
+
+__device__ inline uint64_t useq_len(const USerCtx& c, uint64_t u) { return c.ulen[u] + c.k - 1; }
+
+__device__ inline uint64_t unitig_record_size(const USerCtx& c, uint64_t u) {
+  uint64_t val = 1 + 4;  // flags + coverage
+  int ne = c.uecnt[u];
+  int per[4] = {0, 0, 0, 0};
+  for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> 30]++;
+  for (int t = 0; t < 4; t++)
+    if (per[t]) val += 4 + (uint64_t)per[t] * (4 + c.nb);
+  uint64_t h0 = c.hstart[u];
+  uint32_t nu_ = c.hu[u], nf = c.hf[u];
+  for (int f = 0; f < 2; f++) {
+    uint32_t n = f ? nf : nu_;
+    if (!n) continue;
+    val += 1 + 4;
+    uint64_t a = h0 + (f ? nu_ : 0);
+    for (uint64_t j = a; j < a + n; j++) {
+      uint32_t m = c.hval[j] & 0x7fffffffu;
+      uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
+      val += 1 + 8 + 4 + (L + 3) / 4 + (Lm > 0 ? 4 + (Lm + 3) / 4 : 0);
+    }
+  }
+  if (c.ulen[u] > 1) val += 4 + (useq_len(c, u) + 3) / 4;
+  return 8 + 4 + c.nb + val;
+}
+
+__global__ void k_useq_size(USerCtx c, uint64_t* __restrict__ rsize) {
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < c.nu; u += (uint64_t)gridDim.x * blockDim.x)
+    rsize[u] = unitig_record_size(c, u);
+}
+
+template <int W>
+__global__ void k_useq_write(USerCtx c, const uint64_t* __restrict__ roff, uint8_t* __restrict__ out) {
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < c.nu; u += (uint64_t)gridDim.x * blockDim.x) {
+    uint8_t* p = out + roff[u];
+    uint64_t rec = roff[u + 1] - roff[u];
+    uint32_t rep = c.urep_node[u];
+    KV v = node_key<W>(c.klo, c.khi, rep);
+    put32(p, (uint32_t)(rec - 8));
+    put32(p, 4 + c.nb);
+    put_key<W>(p, v, c.k, c.nb);
+    int ne = c.uecnt[u];
+    int per[4] = {0, 0, 0, 0};
+    for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> 30]++;
+    uint32_t nu_ = c.hu[u], nf = c.hf[u];
+    uint64_t m = c.ulen[u];
+    uint8_t flags = 0x80;
+    for (int t = 0; t < 4; t++)
+      if (per[t]) flags |= 1 << t;
+    if (nu_) flags |= 1 << 4;
+    if (nf) flags |= 1 << 5;
+    if (m > 1) flags |= 1 << 6;
+    *p++ = flags;
+    for (int t = 0; t < 4; t++) {
+      if (!per[t]) continue;
+      put32(p, (uint32_t)per[t]);
+      for (int a = 0; a < ne; a++) {
+        uint32_t e = c.uedge[kUEdges * u + a];
+        if ((int)(e >> 30) != t) continue;
+        put_key<W>(p, node_key<W>(c.klo, c.khi, e & 0x3fffffffu), c.k, c.nb);
+      }
+    }
+    uint64_t h0 = c.hstart[u];
+    for (int f = 0; f < 2; f++) {
+      uint32_t n = f ? nf : nu_;
+      if (!n) continue;
+      *p++ = 1;
+      put32(p, n);
+      uint64_t a = h0 + (f ? nu_ : 0);
+      for (uint64_t j = a; j < a + n; j++) {
+        uint32_t mi = c.hval[j] & 0x7fffffffu;
+        uint32_t L = c.mlen[mi], Lm = c.mlen[mi ^ 1];
+        *p++ = Lm > 0 ? 1 : 0;
+        uint64_t off = (c.hkey[j] >> 30) & ((1ull << 23) - 1);
+        uint64_t uuid = (off << 40) | ((uint64_t)(mi & 1) << 35) | (uint64_t)c.rid[mi >> 1];
+        for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
+        put_read(p, c.words + c.wbase[mi], L);
+        if (Lm > 0) put_read(p, c.words + c.wbase[mi ^ 1], Lm);
+      }
+    }
+    if (m > 1) {
+      // internal k-mer: the unitig sequence (overlap-(k-1) concatenation,
+      // VKmer.mergeWith*Kmer, VKmer.java:525-692) as VKmer bytes
+      uint64_t len = useq_len(c, u);
+      put32(p, (uint32_t)len);
+      uint64_t nbl = (len + 3) / 4;
+      const uint32_t* ord = c.order + c.ubase[u];
+      uint32_t o0 = ord[0];
+      for (uint64_t bq = 0; bq < nbl; bq++) {
+        uint64_t q = nbl - 1 - bq;  // byte index from the least significant end
+        uint32_t byte = 0;
+        for (int s2 = 0; s2 < 4; s2++) {
+          uint64_t j = 4 * q + s2;
+          if (j >= len) break;
+          uint32_t x;
+          if (j < (uint64_t)c.k) {
+            x = oriented_letter<W>(c.klo, c.khi, o0 & 0x7fffffffu, !(o0 & 0x80000000u), c.k, (int)j);
+          } else {
+            uint32_t o = ord[j - c.k + 1];
+            x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
+          }
+          byte |= x << (2 * s2);
+        }
+        *p++ = (uint8_t)byte;
+      }
+    }
+    double cov = (double)c.ucov[u] / (double)m;
+    float fc = (float)cov;
+    uint32_t bits;
+    memcpy(&bits, &fc, 4);
+    put32(p, bits);
+  }
+}
+
 }  // namespace gx
 
 // ===========================================================================
@@ -1093,17 +1543,23 @@ struct gx_build {
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
-  hipEvent_t ev[2 * 8] = {};
-  bool ev_used[8] = {};
-  double phase_ms[8] = {};
+  hipEvent_t ev[2 * 16] = {};
+  bool ev_used[16] = {};
+  double phase_ms[16] = {};
+  // unitig pass
+  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_dist[2], u_mn[2], u_last[2], u_active, u_rep, u_pos, u_isrep, u_lenrep;
+  DevBuf u_uid, u_repnode, u_len, u_base, u_order, u_cov, u_edge, u_ecnt, u_hk, u_hk2, u_hv, u_hv2, u_uflag, u_uposh;
+  DevBuf u_hu, u_hf, u_htot, u_hstart, u_hklo, u_hvc, u_rsize, u_roff, u_out;
+  uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0;
+  bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
 };
 
 static std::string g_create_err;
 
-enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, NPH };
+enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, NPH };
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay"};
+                                       "k_replay", "unitig"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -1119,7 +1575,6 @@ static void ph_end(gx_build* b, int ph) {
 }
 static void ph_collect(gx_build* b) {
   for (int i = 0; i < NPH; i++) {
-    b->phase_ms[i] = 0;
     if (b->prof && b->ev_used[i]) {
       float ms = 0;
       GX_HIP(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
@@ -1388,6 +1843,7 @@ static void run_build(gx_build* b) {
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint32_t, uint64_t>(uflag, upos, nheads, scratch_for(b, scan_scratch_bytes<uint64_t>(nheads)), st);
+  b->nuheads = read1(st, upos + nheads);
   if (nheads) {
     hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, hv, uflag,
                        upos, uval, hcu, hcf);
@@ -1437,6 +1893,166 @@ static void run_build(gx_build* b) {
   b->stats[4] = nrep;
   b->stats[5] = nsel;
   b->stats[6] = read1(st, errw + 3);
+}
+
+template <int W>
+static void run_unitigs(gx_build* b) {
+  hipStream_t st = b->st;
+  const int k = b->cfg.k, nb = b->nb;
+  const uint64_t nn = b->nnodes, cap = b->stats[3];
+  const unsigned G = 2048;
+  void* table = b->table.p;
+  const uint64_t* klo = b->klo.as<uint64_t>();
+  const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
+  const uint32_t* cnt = b->cnt.as<uint32_t>();
+  const uint32_t* msk = b->msk.as<uint32_t>();
+  unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
+  GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
+  uint64_t ne = 2 * nn;
+  if (ne >= 0xffffffffull) throw GxError(GX_ENOTSUP, "unitig pass limited to 2^31 nodes");
+  ph_begin(b, PH_UNITIG);
+  uint32_t* lnk = buf<uint32_t>(b->u_lnk, ne + 1);
+  uint8_t* tand = buf<uint8_t>(b->u_tand, nn + 1);
+  uint8_t* cut = buf<uint8_t>(b->u_cut, ne + 1);
+  GX_HIP(hipMemsetAsync(cut, 0, ne + 1, st));
+  if (nn) {
+    hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, table,
+                       cap - 1, k, lnk, tand, errw);
+    GX_HIP(hipGetLastError());
+  }
+  uint32_t* P[2] = {buf<uint32_t>(b->u_ptr[0], ne + 1), buf<uint32_t>(b->u_ptr[1], ne + 1)};
+  uint32_t* D[2] = {buf<uint32_t>(b->u_dist[0], ne + 1), buf<uint32_t>(b->u_dist[1], ne + 1)};
+  uint32_t* M[2] = {buf<uint32_t>(b->u_mn[0], ne + 1), buf<uint32_t>(b->u_mn[1], ne + 1)};
+  uint32_t* Lt[2] = {buf<uint32_t>(b->u_last[0], ne + 1), buf<uint32_t>(b->u_last[1], ne + 1)};
+  uint32_t* active = buf<uint32_t>(b->u_active, 2);
+  int rmax = 2;
+  while ((1ull << rmax) < ne + 1) rmax++;
+  int cur = 0, rounds = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    cur = 0;
+    if (nn) {
+      hipLaunchKernelGGL(k_elem_init, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut,
+                         P[0], D[0], M[0], Lt[0]);
+      GX_HIP(hipGetLastError());
+    }
+    uint32_t act = nn ? 1 : 0;
+    for (int r = 0; r <= rmax && act; r++) {
+      GX_HIP(hipMemsetAsync(active, 0, 4, st));
+      hipLaunchKernelGGL(k_jump, dim3(grid_for(ne, kBlock, 8192)), dim3(kBlock), 0, st, ne, P[cur], D[cur], M[cur],
+                         Lt[cur], P[cur ^ 1], D[cur ^ 1], M[cur ^ 1], Lt[cur ^ 1], active);
+      GX_HIP(hipGetLastError());
+      cur ^= 1;
+      rounds++;
+      act = read1(st, active);
+    }
+    if (!act) break;
+    if (attempt == 1) throw GxError(GX_EDEVICE, "internal error: unitig chains did not terminate after cutting cycles");
+    hipLaunchKernelGGL(k_cut_cycles, dim3(grid_for(ne, kBlock, G)), dim3(kBlock), 0, st, ne, P[cur], M[cur], lnk, cut);
+    GX_HIP(hipGetLastError());
+  }
+  unsigned long long e = read1(st, errw);
+  if (e != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the table");
+
+  uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
+  uint32_t* upos = buf<uint32_t>(b->u_pos, nn + 1);
+  uint32_t* isrep = buf<uint32_t>(b->u_isrep, nn + 1);
+  uint64_t* ulen_of_rep = buf<uint64_t>(b->u_lenrep, nn + 1);
+  uint64_t* uid_of_rep = buf<uint64_t>(b->u_uid, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_node_final, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, D[cur], M[cur], Lt[cur],
+                       urep, upos, isrep, ulen_of_rep);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(isrep, uid_of_rep, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+  uint64_t nu = read1(st, uid_of_rep + nn);
+  uint32_t* urep_node = buf<uint32_t>(b->u_repnode, nu + 1);
+  uint64_t* ulen = buf<uint64_t>(b->u_len, nu + 1);
+  uint64_t* ubase = buf<uint64_t>(b->u_base, nu + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_rep_list, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, isrep, uid_of_rep,
+                       ulen_of_rep, urep_node, ulen);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(ulen, ubase, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
+  uint32_t* order = buf<uint32_t>(b->u_order, nn + 1);
+  unsigned long long* ucov = buf<unsigned long long>(b->u_cov, nu + 1);
+  GX_HIP(hipMemsetAsync(ucov, 0, (nu + 1) * 8, st));
+  if (nn) {
+    hipLaunchKernelGGL(k_place, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, urep, upos, uid_of_rep, ubase,
+                       cnt, order, ucov);
+    GX_HIP(hipGetLastError());
+  }
+  uint32_t* uedge = buf<uint32_t>(b->u_edge, kUEdges * (nu + 1));
+  uint8_t* uecnt = buf<uint8_t>(b->u_ecnt, nu + 1);
+  if (nu) {
+    hipLaunchKernelGGL(k_unitig_edges<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, order,
+                       klo, khi, msk, table, cap - 1, urep, k, uedge, uecnt);
+    GX_HIP(hipGetLastError());
+  }
+  // heads
+  uint64_t nh = b->nuheads;
+  Key128* hk = buf<Key128>(b->u_hk, nh + 1);
+  Key128* hk2 = buf<Key128>(b->u_hk2, nh + 1);
+  uint32_t* hv = buf<uint32_t>(b->u_hv, nh + 1);
+  uint32_t* hv2 = buf<uint32_t>(b->u_hv2, nh + 1);
+  if (nn && nh) {
+    hipLaunchKernelGGL(k_unitig_heads, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
+                       b->hstart.as<uint64_t>(), b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), b->uval.as<uint32_t>(),
+                       urep, upos, uid_of_rep, b->rid.as<int64_t>(), k, hk, hv);
+    GX_HIP(hipGetLastError());
+  }
+  bool fl = radix_sort<Key128>(hk, hv, hk2, hv2, nh, 0, 12, scratch_for(b, radix_scratch_bytes<Key128>(nh)), st);
+  if (fl) { std::swap(hk, hk2); std::swap(hv, hv2); }
+  uint32_t* uflag = buf<uint32_t>(b->u_uflag, nh + 1);
+  uint64_t* uposh = buf<uint64_t>(b->u_uposh, nh + 1);
+  uint32_t* hu = buf<uint32_t>(b->u_hu, nu + 1);
+  uint32_t* hf = buf<uint32_t>(b->u_hf, nu + 1);
+  uint64_t* htot = buf<uint64_t>(b->u_htot, nu + 1);
+  uint64_t* hstart = buf<uint64_t>(b->u_hstart, nu + 1);
+  GX_HIP(hipMemsetAsync(hu, 0, (nu + 1) * 4, st));
+  GX_HIP(hipMemsetAsync(hf, 0, (nu + 1) * 4, st));
+  if (nh) {
+    hipLaunchKernelGGL(k_uhead_unique, dim3(grid_for(nh, kBlock, G)), dim3(kBlock), 0, st, nh, hk, uflag, hu, hf);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(uflag, uposh, nh, scratch_for(b, scan_scratch_bytes<uint64_t>(nh)), st);
+  uint64_t* hkey_lo = buf<uint64_t>(b->u_hklo, nh + 1);
+  uint32_t* hval_c = buf<uint32_t>(b->u_hvc, nh + 1);
+  if (nh) {
+    hipLaunchKernelGGL(k_uhead_compact, dim3(grid_for(nh, kBlock, G)), dim3(kBlock), 0, st, nh, uflag, uposh, hk, hv,
+                       hkey_lo, hval_c);
+    GX_HIP(hipGetLastError());
+  }
+  if (nu) {
+    hipLaunchKernelGGL(k_head_total, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, hu, hf, htot);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(htot, hstart, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
+  // records
+  USerCtx c;
+  c.k = k; c.nb = nb; c.nu = nu; c.klo = klo; c.khi = khi; c.ubase = ubase; c.ulen = ulen; c.order = order;
+  c.urep_node = urep_node; c.ucov = ucov; c.uedge = uedge; c.uecnt = uecnt; c.hstart = hstart; c.hu = hu; c.hf = hf;
+  c.hkey = hkey_lo; c.hval = hval_c; c.mlen = b->mlen.as<uint32_t>(); c.wbase = b->wbase.as<uint64_t>();
+  c.words = b->words.as<uint64_t>(); c.rid = b->rid.as<int64_t>();
+  uint64_t* rsize = buf<uint64_t>(b->u_rsize, nu + 1);
+  uint64_t* roff = buf<uint64_t>(b->u_roff, nu + 1);
+  if (nu) {
+    hipLaunchKernelGGL(k_useq_size, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, c, rsize);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(rsize, roff, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
+  uint64_t total = read1(st, roff + nu);
+  uint8_t* out = buf<uint8_t>(b->u_out, total + 1);
+  if (nu) {
+    hipLaunchKernelGGL(k_useq_write<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, c, roff, out);
+    GX_HIP(hipGetLastError());
+  }
+  ph_end(b, PH_UNITIG);
+  GX_HIP(hipStreamSynchronize(st));
+  ph_collect(b);
+  b->u_bytes = total;
+  b->u_count = nu;
+  b->u_rounds = (uint64_t)rounds;
 }
 
 template <class F>
@@ -1541,6 +2157,7 @@ int gx_build_finish(gx_build* b) {
     if (b->finished) throw GxError(GX_ESTATE, "finish called twice; call gx_build_reset first");
     GX_HIP(hipSetDevice(b->cfg.device));
     if (b->text.cap == 0) b->text.ensure(64);
+    b->unitigs_done = false;
     if (b->W == 1) run_build<1>(b);
     else run_build<2>(b);
     b->finished = true;
@@ -1586,6 +2203,38 @@ int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   return GX_OK;
 }
 
+int gx_unitig_run(gx_build* b) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "unitig_run before finish");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    if (b->W == 1) run_unitigs<1>(b);
+    else run_unitigs<2>(b);
+    b->unitigs_done = true;
+  });
+}
+
+int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->unitigs_done) throw GxError(GX_ESTATE, "unitig_info before unitig_run");
+    if (nrec) *nrec = b->u_count;
+    if (nbytes) *nbytes = b->u_bytes;
+  });
+}
+
+int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->unitigs_done) throw GxError(GX_ESTATE, "unitig_emit before unitig_run");
+    if (cap < b->u_bytes) throw GxError(GX_EINVAL, "destination too small");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    if (b->u_bytes) GX_HIP(hipMemcpyAsync(dst, b->u_out.p, b->u_bytes, hipMemcpyDeviceToHost, b->st));
+    GX_HIP(hipStreamSynchronize(b->st));
+    if (written) *written = b->u_bytes;
+  });
+}
+
 int gx_build_set_profiling(gx_build* b, int on) {
   if (!b) return GX_EINVAL;
   b->prof = on != 0;
@@ -1604,6 +2253,7 @@ int gx_build_reset(gx_build* b) {
   if (!b) return GX_EINVAL;
   b->text_len = 0;
   b->finished = false;
+  b->unitigs_done = false;
   b->out_bytes = 0;
   b->nnodes = 0;
   return GX_OK;

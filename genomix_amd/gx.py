@@ -26,7 +26,7 @@ EXPORTED = [
     "gx_build_create", "gx_build_add_reads", "gx_build_add_reads_device", "gx_build_finish",
     "gx_build_part_info", "gx_build_emit_part", "gx_build_part_device", "gx_build_stats",
     "gx_build_reset", "gx_last_error", "gx_build_destroy", "gx_build_set_profiling",
-    "gx_build_phase_times", "gx_build_phase_name",
+    "gx_build_phase_times", "gx_build_phase_name", "gx_unitig_run", "gx_unitig_info", "gx_unitig_emit",
 ]
 
 
@@ -68,6 +68,9 @@ def lib():
         L.gx_build_phase_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32]
         L.gx_build_phase_name.argtypes = [i32]
         L.gx_build_phase_name.restype = ctypes.c_char_p
+        L.gx_unitig_run.argtypes = [vp]
+        L.gx_unitig_info.argtypes = [vp, u64p, u64p]
+        L.gx_unitig_emit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -109,6 +112,16 @@ class GraphBuilder:
         self._check(lib().gx_build_emit_part(self._h, part, buf, nb, ctypes.byref(w)))
         return buf.raw[:w.value]
 
+    def unitigs(self) -> bytes:
+        """Unitig compression of the built graph; returns the merged records."""
+        self._check(lib().gx_unitig_run(self._h))
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        w = ctypes.c_size_t()
+        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
     def stats(self):
         a = (ctypes.c_uint64 * 7)()
         self._check(lib().gx_build_stats(self._h, a, 7))
@@ -144,6 +157,16 @@ class GraphBuilder:
             self.close()
         except Exception:
             pass
+
+
+def build_and_merge(text: bytes, k: int, edge_order: str = "jdk7", device: int = 0):
+    """Graph build followed by unitig compression; returns (graph records,
+    unitig records)."""
+    with GraphBuilder(k, edge_order=edge_order, device=device) as b:
+        b.add_reads(text)
+        b.finish()
+        g = b.part_bytes(0)
+        return g, b.unitigs()
 
 
 def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", device: int = 0):

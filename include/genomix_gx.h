@@ -90,6 +90,22 @@ int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* 
  * finish/reset/destroy on this handle). */
 int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* nbytes);
 
+/* Unitig compression of the built graph (Genomix MERGE step: P4 path merge,
+ * P4ForPathMergeVertex run by Pregelix through
+ * DeBruijnGraphCleanVertex.getConfiguredJob, DeBruijnGraphCleanVertex.java:188-206,
+ * with NodeToVertexInputFormat / VertexToNodeOutputFormat around it).  Every
+ * maximal path whose links have degree 1 on both sides (no tandem repeats)
+ * becomes one Node record: key = the path's first k-mer in KmerPointable
+ * order (P4 keeps a random one), internal k-mer = the path sequence oriented
+ * so that key reads forward (dropped for one-k-mer paths, as
+ * GenericVertexToNodeOutputFormat.java:44-47 does), coverage = mean k-mer
+ * count, read-head offsets moved to unitig positions, outer edges re-targeted
+ * to neighbour unitigs (each list ascending).  Records in key order.  Run
+ * after gx_build_finish; the build's own records stay available. */
+int gx_unitig_run(gx_build* b);
+int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes);
+int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
+
 /* Statistics of the last finish: [0] occurrences, [1] distinct k-mers,
  * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed
  * occurrences, [6] largest coverage above 255 (0 if none). */

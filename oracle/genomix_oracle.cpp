@@ -1345,6 +1345,37 @@ int64_t ora_make_uuid(int mate, int lib, int64_t readId, int offset) {
   if (guard([&] { v = make_uuid(mate, lib, readId, offset); }) != 0) return INT64_MIN;
   return v;
 }
+// Summary of a merged record stream: records, total k-mers (unitig
+// lengths), sum of coverage x length, and whether keys ascend in
+// KmerPointable order.
+int ora_unitig_summary(const uint8_t* recs, size_t n, int k, double* out4) {
+  return guard([&] {
+    size_t off = 0;
+    double nrec = 0, nk = 0, cv = 0;
+    std::vector<uint8_t> prev;
+    bool asc = true;
+    while (off < n) {
+      if (off + 8 > n) throw OraError(E_INPUT, "truncated record stream");
+      uint32_t recLen = get_be32(recs + off), keyLen = get_be32(recs + off + 4);
+      if (off + 8 + recLen > n || keyLen > recLen) throw OraError(E_INPUT, "bad record");
+      VKmer key;
+      read_vkmer(recs + off + 8, keyLen, key);
+      Node x;
+      size_t used = x.read(recs + off + 8 + keyLen, recLen - keyLen);
+      if (used != recLen - keyLen) throw OraError(E_INPUT, "trailing bytes in a Node");
+      double m = x.has_internal ? (double)(x.internal.k - k + 1) : 1.0;
+      nrec += 1;
+      nk += m;
+      cv += (double)x.cov * m;
+      std::vector<uint8_t> rk(key.b.rbegin(), key.b.rend());
+      if (!prev.empty() && !(prev < rk)) asc = false;
+      prev = rk;
+      off += 8 + recLen;
+    }
+    out4[0] = nrec; out4[1] = nk; out4[2] = cv; out4[3] = asc ? 1 : 0;
+  });
+}
+
 int ora_unitigs(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
   return guard([&] {
     auto o = unitigs(recs, n, k);

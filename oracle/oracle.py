@@ -85,6 +85,7 @@ def lib():
         L.ora_float_str.argtypes = [ctypes.c_float]
         L.ora_float_str.restype = ctypes.c_char_p
         L.ora_set_hash_model.argtypes = [i32]
+        L.ora_unitig_summary.argtypes = [vp, sz, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_unitigs"):
             L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
         _lib = L
@@ -217,3 +218,11 @@ def unitigs(records: bytes, k: int) -> bytes:
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_unitigs(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
+
+
+def unitig_summary(records: bytes, k: int):
+    """(records, total k-mers, sum of coverage x length, keys ascending) of a
+    merged record stream, decoded by the oracle's own Node reader."""
+    a = (ctypes.c_double * 4)()
+    _check(lib().ora_unitig_summary(records, len(records), k, a))
+    return int(a[0]), int(a[1]), a[2], bool(a[3])

@@ -1,0 +1,104 @@
+"""Parity of the HIP unitig compression with the oracle's sequential walker
+(byte for byte: both fix P4's random survivor and orientation the same way),
+plus the RandMergeOffsetTest invariant on the device path."""
+
+import glob
+import os
+import random
+import re
+
+import pytest
+
+from oracle import oracle
+
+from .conftest import GOLDEN
+from .test_oracle_unitigs import _genome_without_repeats, field, lines_of, rc, tiled_reads
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "hyracks", "**", "*.test.txt"), recursive=True))
+
+
+@pytest.fixture(scope="module")
+def gx():
+    import genomix_amd.gx as g
+    return g
+
+
+def check(gx, text, k):
+    g_gpu, u_gpu = gx.build_and_merge(text, k)
+    g_ora = oracle.build_graph(text, k)[0]
+    assert g_gpu == g_ora
+    u_ora = oracle.unitigs(g_ora, k)
+    if u_gpu != u_ora:
+        a, b = lines_of(u_gpu), lines_of(u_ora)
+        for x, y in zip(a, b):
+            if x != y:
+                pytest.fail(f"first differing unitig:\n gpu: {x}\n ora: {y}")
+        pytest.fail(f"unitig counts {len(a)} vs {len(b)}")
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(f) for f in FIXTURES])
+def test_fixture_unitigs(gx, path):
+    k = int(re.search(r"_k(\d+)\.", os.path.basename(path)).group(1))
+    check(gx, open(path, "rb").read(), k)
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 11, 21, 31, 32, 33, 55, 63])
+def test_random_unitigs(gx, k):
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=4000, read_len=k + 60, coverage=15, error_rate=0.01, paired=True,
+                           insert_mean=3 * (k + 60), insert_sd=20, repeat_frac=0.1, repeat_len=300,
+                           repeat_div=0.02, seed=500 + k)
+    check(gx, sim.simulate(spec), k)
+
+
+def test_cycles_and_tandems(gx):
+    rng = random.Random(3)
+    lines = []
+    # circular genomes (reads wrap around) make cycles; short units make tandems
+    for c in range(20):
+        G = rng.randint(12, 40)
+        g = "".join(rng.choice("ACGT") for _ in range(G))
+        gg = g * 4
+        for j in range(8):
+            p = rng.randrange(G)
+            s = gg[p:p + rng.randint(10, 30)]
+            lines.append(s if rng.random() < 0.5 else rc(s))
+    for u in ("A", "AC", "ACG", "AAT"):
+        lines.append(u * 12)
+    text = "".join(f"{4 * i + 2}\t{s}\n" for i, s in enumerate(lines)).encode()
+    for k in (5, 6, 7, 9):
+        check(gx, text, k)
+
+
+@pytest.mark.parametrize("k,G", [(21, 1200), (31, 2000), (55, 2000)])
+def test_single_genome_one_unitig_on_gpu(gx, k, G):
+    rng = random.Random(k)
+    genome = _genome_without_repeats(rng, G, k)
+    reads = tiled_reads(genome, 100, 7, rng)
+    text = "".join(f"{4 * i + 2}\t{s}\n" for i, (_, _, s) in enumerate(reads)).encode()
+    _, u = gx.build_and_merge(text, k)
+    ls = lines_of(u)
+    assert len(ls) == 1
+    assert field(ls[0], "kmer") in (genome, rc(genome))
+
+
+def test_plain_gpu_workload_unitig_invariants(gx):
+    """5 Mbp workload at full size: every node lands in exactly one unitig
+    (sum of unitig lengths = distinct k-mers), the k-mer counts add up
+    (sum of coverage x length = occurrences, within float rounding), and the
+    pass is deterministic."""
+    from genomix_amd import sim
+    text = sim.simulate(sim.PLAIN_GPU_5MBP)
+    with gx.GraphBuilder(31) as b:
+        b.add_reads(text)
+        b.finish()
+        st = b.stats()
+        u = b.unitigs()
+        assert b.unitigs() == u
+    nrec, total_len, total_cov, ascending = oracle.unitig_summary(u, 31)
+    assert ascending
+    assert nrec < st["distinct"]
+    assert total_len == st["distinct"]
+    assert abs(total_cov - st["occurrences"]) < 1e-5 * st["occurrences"]
