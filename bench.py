@@ -4,10 +4,11 @@
 A step is one pass of the hot path over one batch: the readids text of the
 workload, already resident in HBM, goes through the whole device build
 (parse, canonical k-mers, table insert, aggregation, JDK-7 edge-order
-replay, read heads, Node record serialisation) and ends as the part-0
-record bytes in HBM, byte-identical to the reference's graph build at P=1
-(the parity tests check that).  Throughput = k-mer occurrences of valid
-reads per second (= sum of coverage over the output nodes).
+replay, read heads, Node record serialisation; the part-0 record bytes,
+byte-identical to the reference's graph build at P=1) and then through
+unitig compression (merged Node records).  Both outputs stay in HBM.
+Throughput = k-mer occurrences of valid reads per second (= sum of
+coverage over the graph's nodes).
 
 Workload (default): seeded 5 Mbp genome with 5% in 2 kb repeat copies at
 1% divergence; 150 bp paired-end reads, insert 500/50, 50x, 1% substitution
@@ -110,6 +111,7 @@ def main():
         builder.reset()
         builder.add_reads_device(dtext.data_ptr(), len(text))
         builder.finish()
+        builder.run_unitigs()
 
     for i in range(args.warmup):
         step()
@@ -145,11 +147,11 @@ def main():
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "plain GPU build, one launch: seeded 5 Mbp genome (5% in 2 kb repeats at 1% "
                                "divergence), 150 bp PE reads, insert 500/50, 50x, 1% error, k=31, "
-                               "graph build to Node records (unitig merge timed separately)",
+                               "graph build to Node records + unitig path merge",
                    "k": k, "read_len": L, "read_pairs": text.count(b"\n"), "kmer_occurrences": N,
                    "distinct_kmers": st["distinct"], "replayed_nodes": st["replayed_nodes"],
                    "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"], "max_coverage": st["max_coverage"],
-                   "edge_order": "jdk7", "parallelism": "single GPU"},
+                   "unitigs": builder.unitig_count(), "edge_order": "jdk7", "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -112,6 +112,15 @@ class GraphBuilder:
         self._check(lib().gx_build_emit_part(self._h, part, buf, nb, ctypes.byref(w)))
         return buf.raw[:w.value]
 
+    def run_unitigs(self):
+        """Unitig compression of the built graph; records stay on the device."""
+        self._check(lib().gx_unitig_run(self._h))
+
+    def unitig_count(self):
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
+        return n.value
+
     def unitigs(self) -> bytes:
         """Unitig compression of the built graph; returns the merged records."""
         self._check(lib().gx_unitig_run(self._h))
