@@ -1160,10 +1160,13 @@ __device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const u
   return side_deg(msk[j] & 0xFFFFu, sj) == 1;
 }
 
-__global__ void k_elem_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
-                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut,
-                            uint32_t* __restrict__ ptr, uint32_t* __restrict__ dist, uint32_t* __restrict__ mn,
-                            uint32_t* __restrict__ last) {
+// Packed list ranking: one 16-byte record per element (ptr, dist, min node,
+// last element), so a hop is one random access; rounds are double-buffered
+// over all elements (own records coalesced), and each block reports whether
+// any of its elements is still following its chain.
+
+__global__ void k_pj_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
+                          const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut, uint4* __restrict__ R0) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * nn;
        e += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t i = e >> 1;
@@ -1173,64 +1176,54 @@ __global__ void k_elem_init(uint64_t nn, const uint32_t* __restrict__ lnk, const
       uint32_t l = lnk[e];
       p = ((l >> 1) << 1) | (1u - (l & 1));  // leave the partner through its other side
     }
-    ptr[e] = p;
-    dist[e] = p == kEnd ? 0 : 1;
-    mn[e] = (uint32_t)i;
-    last[e] = (uint32_t)e;
+    R0[e] = make_uint4(p, p == kEnd ? 0u : 1u, (uint32_t)i, (uint32_t)e);
   }
 }
 
-// One doubling round.  active != 0 afterwards while some chain is still
-// being followed (cycles never finish).
-__global__ void __launch_bounds__(kBlock) k_jump(uint64_t ne, const uint32_t* __restrict__ ptr,
-                                                 const uint32_t* __restrict__ dist, const uint32_t* __restrict__ mn,
-                                                 const uint32_t* __restrict__ last, uint32_t* __restrict__ ptr2,
-                                                 uint32_t* __restrict__ dist2, uint32_t* __restrict__ mn2,
-                                                 uint32_t* __restrict__ last2, uint32_t* active) {
+__global__ void __launch_bounds__(kBlock) k_pj_round(uint64_t ne, const uint4* __restrict__ Rc,
+                                                     uint4* __restrict__ Rn, uint8_t* __restrict__ bflag) {
   int any = 0;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t p = ptr[e];
-    if (p == kEnd) {
-      ptr2[e] = kEnd;
-      dist2[e] = dist[e];
-      mn2[e] = mn[e];
-      last2[e] = last[e];
-    } else {
-      uint32_t q = ptr[p];
-      ptr2[e] = q;
-      dist2[e] = dist[e] + dist[p];
-      mn2[e] = min(mn[e], mn[p]);
-      last2[e] = last[p];
-      any |= q != kEnd;
+    uint4 r = Rc[e];
+    if (r.x != kEnd) {
+      uint4 q = Rc[r.x];
+      r = make_uint4(q.x, r.y + q.y, min(r.z, q.z), q.w);
+      any |= q.x != kEnd;
     }
+    Rn[e] = r;
   }
-  if (__syncthreads_or(any) && threadIdx.x == 0) *active = 1;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) bflag[blockIdx.x] = any ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_flag_or(const uint8_t* __restrict__ bflag, uint32_t n, uint32_t* out) {
+  int any = 0;
+  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) any |= bflag[j];
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) *out = any ? 1u : 0u;
 }
 
 // Cycles: the smallest node of each cycle gives up the link on its R side.
-__global__ void k_cut_cycles(uint64_t ne, const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ mn,
-                             const uint32_t* __restrict__ lnk, uint8_t* __restrict__ cut) {
+__global__ void k_pj_cut(uint64_t ne, const uint4* __restrict__ R, const uint32_t* __restrict__ lnk,
+                         uint8_t* __restrict__ cut) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
-    if (ptr[e] == kEnd || (e & 1) == 0 || mn[e] != (uint32_t)(e >> 1)) continue;
+    uint4 r = R[e];
+    if (r.x == kEnd || (e & 1) == 0 || r.z != (uint32_t)(e >> 1)) continue;
     cut[e] = 1;
     uint32_t l = lnk[e];
     cut[2 * (uint64_t)(l >> 1) + (l & 1)] = 1;
   }
 }
 
-// Per node: unitig representative (smallest node), orientation and position.
-// C1 = the chain holding (rep, F); a node reads forward in the unitig iff
-// its F element shares C1's tail.
-__global__ void k_node_final(uint64_t nn, const uint32_t* __restrict__ dist, const uint32_t* __restrict__ mn,
-                             const uint32_t* __restrict__ last, uint32_t* __restrict__ urep,
-                             uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
-                             uint64_t* __restrict__ ulen_of_rep) {
+__global__ void k_pj_node_final(uint64_t nn, const uint4* __restrict__ R, uint32_t* __restrict__ urep,
+                                uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
+                                uint64_t* __restrict__ ulen_of_rep) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t r = min(mn[2 * i], mn[2 * i + 1]);
-    uint32_t dF = dist[2 * i], dR = dist[2 * i + 1];
-    uint32_t m = dF + dR + 1;
-    bool fwd = last[2 * i] == last[2 * (uint64_t)r];
-    uint32_t pos = m - 1 - (fwd ? dF : dR);
+    uint4 f = R[2 * i], b = R[2 * i + 1];
+    uint32_t r = min(f.z, b.z);
+    uint32_t m = f.y + b.y + 1;
+    bool fwd = f.w == R[2 * (uint64_t)r].w;
+    uint32_t pos = m - 1 - (fwd ? f.y : b.y);
     urep[i] = r | (fwd ? 0u : 0x80000000u);
     upos[i] = pos;
     isrep[i] = r == (uint32_t)i ? 1 : 0;
@@ -1547,7 +1540,7 @@ struct gx_build {
   bool ev_used[16] = {};
   double phase_ms[16] = {};
   // unitig pass
-  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_dist[2], u_mn[2], u_last[2], u_active, u_rep, u_pos, u_isrep, u_lenrep;
+  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep;
   DevBuf u_uid, u_repnode, u_len, u_base, u_order, u_cov, u_edge, u_ecnt, u_hk, u_hk2, u_hv, u_hv2, u_uflag, u_uposh;
   DevBuf u_hu, u_hf, u_htot, u_hstart, u_hklo, u_hvc, u_rsize, u_roff, u_out;
   uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0;
@@ -1920,34 +1913,34 @@ static void run_unitigs(gx_build* b) {
                        cap - 1, k, lnk, tand, errw);
     GX_HIP(hipGetLastError());
   }
-  uint32_t* P[2] = {buf<uint32_t>(b->u_ptr[0], ne + 1), buf<uint32_t>(b->u_ptr[1], ne + 1)};
-  uint32_t* D[2] = {buf<uint32_t>(b->u_dist[0], ne + 1), buf<uint32_t>(b->u_dist[1], ne + 1)};
-  uint32_t* M[2] = {buf<uint32_t>(b->u_mn[0], ne + 1), buf<uint32_t>(b->u_mn[1], ne + 1)};
-  uint32_t* Lt[2] = {buf<uint32_t>(b->u_last[0], ne + 1), buf<uint32_t>(b->u_last[1], ne + 1)};
-  uint32_t* active = buf<uint32_t>(b->u_active, 2);
+  uint4* R[2] = {buf<uint4>(b->u_ptr[0], ne + 1), buf<uint4>(b->u_ptr[1], ne + 1)};
+  const unsigned jgrid = 8192;
+  uint8_t* bflag = buf<uint8_t>(b->u_flag, jgrid);
+  uint32_t* anyflag = buf<uint32_t>(b->u_any, 4);
   int rmax = 2;
   while ((1ull << rmax) < ne + 1) rmax++;
   int cur = 0, rounds = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     cur = 0;
     if (nn) {
-      hipLaunchKernelGGL(k_elem_init, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut,
-                         P[0], D[0], M[0], Lt[0]);
+      hipLaunchKernelGGL(k_pj_init, dim3(grid_for(ne, kBlock, G * 4)), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut,
+                         R[0]);
       GX_HIP(hipGetLastError());
     }
     uint32_t act = nn ? 1 : 0;
     for (int r = 0; r <= rmax && act; r++) {
-      GX_HIP(hipMemsetAsync(active, 0, 4, st));
-      hipLaunchKernelGGL(k_jump, dim3(grid_for(ne, kBlock, 8192)), dim3(kBlock), 0, st, ne, P[cur], D[cur], M[cur],
-                         Lt[cur], P[cur ^ 1], D[cur ^ 1], M[cur ^ 1], Lt[cur ^ 1], active);
+      unsigned g = grid_for(ne, kBlock, jgrid);
+      hipLaunchKernelGGL(k_pj_round, dim3(g), dim3(kBlock), 0, st, ne, R[cur], R[cur ^ 1], bflag);
+      GX_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_flag_or, dim3(1), dim3(kBlock), 0, st, bflag, g, anyflag);
       GX_HIP(hipGetLastError());
       cur ^= 1;
       rounds++;
-      act = read1(st, active);
+      act = read1(st, anyflag);
     }
     if (!act) break;
     if (attempt == 1) throw GxError(GX_EDEVICE, "internal error: unitig chains did not terminate after cutting cycles");
-    hipLaunchKernelGGL(k_cut_cycles, dim3(grid_for(ne, kBlock, G)), dim3(kBlock), 0, st, ne, P[cur], M[cur], lnk, cut);
+    hipLaunchKernelGGL(k_pj_cut, dim3(grid_for(ne, kBlock, G)), dim3(kBlock), 0, st, ne, R[cur], lnk, cut);
     GX_HIP(hipGetLastError());
   }
   unsigned long long e = read1(st, errw);
@@ -1959,8 +1952,8 @@ static void run_unitigs(gx_build* b) {
   uint64_t* ulen_of_rep = buf<uint64_t>(b->u_lenrep, nn + 1);
   uint64_t* uid_of_rep = buf<uint64_t>(b->u_uid, nn + 1);
   if (nn) {
-    hipLaunchKernelGGL(k_node_final, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, D[cur], M[cur], Lt[cur],
-                       urep, upos, isrep, ulen_of_rep);
+    hipLaunchKernelGGL(k_pj_node_final, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, R[cur], urep, upos,
+                       isrep, ulen_of_rep);
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint32_t, uint64_t>(isrep, uid_of_rep, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
