@@ -60,7 +60,7 @@ __device__ inline uint32_t count_terms(const uint8_t* __restrict__ t, size_t n, 
     bool term = false;
     if (i < n) {
       uint8_t ch = t[i];
-      if (ch & 0x80) atomicMin(nonascii, (unsigned long long)i);
+      if ((ch & 0x80) && nonascii) atomicMin(nonascii, (unsigned long long)i);
       term = ch == '\n' || (ch == '\r' && (i + 1 == n || t[i + 1] != '\n'));
     } else if (i == n) {
       term = n > 0 && t[n - 1] != '\n' && t[n - 1] != '\r';
@@ -95,6 +95,75 @@ __global__ void __launch_bounds__(kBlock) k_line_write(const uint8_t* __restrict
     int j = __builtin_ctz(bits);
     bits &= bits - 1;
     term[pos++] = i0 + j;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 0. FASTQ ingest: GenomixDriver.convertAndUploadFastqToHDFS
+//    (GenomixDriver.java:665-714).  Line i (0-based, BufferedReader.readLine
+//    terminators) with i % 4 == 1 becomes "<i+1>\t<mate1.trim()>\n", or with a
+//    second file "<i+1>\t<mate1.trim()>\t<mate2.trim()>\n"; String.trim drops
+//    leading and trailing bytes <= ' '.  The result is the library-N.readids
+//    text the graph build parses.
+// ---------------------------------------------------------------------------
+
+__device__ inline void fq_line(const uint8_t* __restrict__ t, const uint64_t* __restrict__ term, uint64_t i,
+                               uint64_t& s, uint64_t& e) {
+  s = i ? term[i - 1] + 1 : 0;
+  e = term[i];
+  while (s < e && t[s] <= ' ') s++;
+  while (e > s && t[e - 1] <= ' ') e--;
+}
+
+__device__ inline uint32_t ndigits(uint64_t v) {
+  uint32_t d = 1;
+  while (v >= 10) { v /= 10; d++; }
+  return d;
+}
+
+__global__ void k_fq_size(uint64_t nrec, const uint8_t* __restrict__ t1, const uint64_t* __restrict__ term1,
+                          const uint8_t* __restrict__ t2, const uint64_t* __restrict__ term2,
+                          uint64_t* __restrict__ size) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrec; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i = 4 * r + 1, s, e;
+    fq_line(t1, term1, i, s, e);
+    uint64_t z = ndigits(i + 1) + 1 + (e - s) + 1;
+    if (t2) {
+      fq_line(t2, term2, i, s, e);
+      z += 1 + (e - s);
+    }
+    size[r] = z;
+  }
+}
+
+// One wave per record; lanes copy the sequence bytes.
+__global__ void __launch_bounds__(kBlock) k_fq_write(uint64_t nrec, const uint8_t* __restrict__ t1,
+                                                     const uint64_t* __restrict__ term1,
+                                                     const uint8_t* __restrict__ t2,
+                                                     const uint64_t* __restrict__ term2,
+                                                     const uint64_t* __restrict__ off, uint8_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t r = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < nrec; r += nw) {
+    uint64_t i = 4 * r + 1, id = i + 1;
+    uint8_t* p = out + off[r];
+    uint32_t nd = ndigits(id);
+    if (lane < (int)nd) {
+      uint64_t v = id;
+      for (uint32_t j = 0; j + 1 + lane < nd; j++) v /= 10;
+      p[lane] = (uint8_t)('0' + v % 10);
+    }
+    if (lane == 0) p[nd] = '\t';
+    p += nd + 1;
+    for (int f = 0; f < (t2 ? 2 : 1); f++) {
+      const uint8_t* t = f ? t2 : t1;
+      uint64_t s, e;
+      fq_line(t, f ? term2 : term1, i, s, e);
+      for (uint64_t x = s + lane; x < e; x += 64) p[x - s] = t[x];
+      p += e - s;
+      if (lane == 0) *p = (f == 0 && t2) ? '\t' : '\n';
+      p += 1;
+    }
   }
 }
 
@@ -887,7 +956,10 @@ struct SerCtx {
   const uint64_t* words;
   const int64_t* rid;
   unsigned long long* bad;  // internal consistency flag (first bad node)
+  const uint32_t* perm;     // output position -> node (partitioned output), or null
 };
+
+__device__ inline uint64_t ser_node(const SerCtx& c, uint64_t j) { return c.perm ? c.perm[j] : j; }
 
 __device__ inline uint32_t nbytes_of(uint32_t L) { return (L + 3) / 4; }
 
@@ -947,7 +1019,7 @@ __device__ inline uint64_t record_size(const SerCtx& c, uint64_t i) {
 
 __global__ void k_ser_size(SerCtx c, uint64_t* __restrict__ rsize) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.nn; i += (uint64_t)gridDim.x * blockDim.x)
-    rsize[i] = record_size(c, i);
+    rsize[i] = record_size(c, ser_node(c, i));
 }
 
 __device__ inline void put32(uint8_t*& p, uint32_t v) {
@@ -1042,7 +1114,7 @@ __global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* 
   bool staged = (s1 - base) <= kSerStage;
   if (i < last) {
     uint64_t rec = roff[i + 1] - roff[i];
-    write_record<W>(c, i, staged ? stage + (roff[i] - base) : out + roff[i], rec);
+    write_record<W>(c, ser_node(c, i), staged ? stage + (roff[i] - base) : out + roff[i], rec);
   }
   if (!staged) return;
   __syncthreads();
@@ -1054,6 +1126,36 @@ __global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* 
   const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
   uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
   for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
+}
+
+// Output partition of each node: nonneg(hashBytes(key)) % P, the reference's
+// KmerPartitionComputerFactory (KmerPartitionComputerFactory.java:28-53),
+// where nonneg(h) = h < 0 ? -(h + 1) : h.
+template <int W>
+__global__ void k_part_ids(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi, int nb,
+                           uint32_t P, uint32_t* __restrict__ pid, uint32_t* __restrict__ idx) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t h = java_hash_key<W>(node_key<W>(klo, khi, i), nb);
+    if (h < 0) h = -(h + 1);
+    pid[i] = (uint32_t)h % P;
+    idx[i] = (uint32_t)i;
+  }
+}
+
+__global__ void k_part_bytes(uint32_t P, const uint64_t* __restrict__ start, const uint64_t* __restrict__ roff,
+                             uint64_t* __restrict__ boff) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= P; p += gridDim.x * blockDim.x) boff[p] = roff[start[p]];
+}
+
+// First output position of every partition, from the partition ids sorted
+// stably (start[P] = nn; empty partitions start where the next one does).
+__global__ void k_part_starts(uint64_t nn, const uint32_t* __restrict__ spid, uint32_t P,
+                              uint64_t* __restrict__ start) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= nn; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = j == 0 ? 0 : spid[j - 1] + 1;
+    uint32_t hi = j == nn ? P : spid[j];
+    for (uint32_t p = lo; p <= hi && p <= P; p++) start[p] = j;
+  }
 }
 
 // words per mate for the packed store
@@ -1525,6 +1627,11 @@ struct gx_build {
   hipStream_t st = nullptr;
   std::string err;
   uint64_t text_len = 0;
+  int fmt = -1;                     // input format fixed by the first add call (GX_FMT_*)
+  DevBuf text2, rtext, fterm1, fterm2, fsz, foff;  // second FASTQ file; readids text made from FASTQ
+  uint64_t text2_len = 0, rtext_len = 0;
+  const uint8_t* src = nullptr;     // readids text the build parses
+  uint64_t src_len = 0;
   bool finished = false;
   // device buffers (grown on demand, reused across builds)
   DevBuf text, scratch, blk, blk2, term, rid, mstart, mlen, wcnt, wbase, words, valid, nocc, occbase, hmask, hidx;
@@ -1546,6 +1653,8 @@ struct gx_build {
   uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0;
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
+  DevBuf ppid, pidx, ppid2, pidx2, pbound;
+  std::vector<uint64_t> part_rec, part_off;  // per partition: first record, first byte (P + 1 entries)
 };
 
 static std::string g_create_err;
@@ -1618,8 +1727,8 @@ template <int W>
 static void run_build(gx_build* b) {
   hipStream_t st = b->st;
   const int k = b->cfg.k, nb = b->nb;
-  const uint64_t n = b->text_len;
-  const uint8_t* t = b->text.as<uint8_t>();
+  const uint64_t n = b->src_len;
+  const uint8_t* t = b->src;
   const unsigned G = 2048;  // grid for grid-stride kernels (8 blocks per CU)
   unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
   GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
@@ -1857,7 +1966,31 @@ static void run_build(gx_build* b) {
   c.hstart = hstart; c.hcu = hcu; c.hcf = hcf; c.uval = uval;
   c.mlen = mlen; c.wbase = wbase; c.words = words; c.rid = rid;
   c.bad = errw + 1;
+  c.perm = nullptr;
   GX_HIP(hipMemsetAsync(errw + 1, 0xff, 8, st));
+  // P > 1: records are grouped by output partition, each group in key order
+  const uint32_t P = (uint32_t)b->cfg.nparts;
+  uint64_t* pstart = nullptr;
+  if (P > 1) {
+    uint32_t* pid = buf<uint32_t>(b->ppid, nn + 1);
+    uint32_t* pidx = buf<uint32_t>(b->pidx, nn + 1);
+    uint32_t* pid2 = buf<uint32_t>(b->ppid2, nn + 1);
+    uint32_t* pidx2 = buf<uint32_t>(b->pidx2, nn + 1);
+    pstart = buf<uint64_t>(b->pbound, 2 * ((uint64_t)P + 1));
+    if (nn) {
+      hipLaunchKernelGGL(k_part_ids<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, P, pid,
+                         pidx);
+      GX_HIP(hipGetLastError());
+    }
+    int pbytes = 1;
+    while (pbytes < 4 && ((P - 1) >> (8 * pbytes)) > 0) pbytes++;
+    bool fp = radix_sort<uint32_t>(pid, pidx, pid2, pidx2, nn, 0, pbytes,
+                                   scratch_for(b, radix_scratch_bytes<uint32_t>(nn)), st);
+    if (fp) { std::swap(pid, pid2); std::swap(pidx, pidx2); }
+    hipLaunchKernelGGL(k_part_starts, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, pid, P, pstart);
+    GX_HIP(hipGetLastError());
+    c.perm = pidx;
+  }
   uint64_t* rsize = buf<uint64_t>(b->rsize, nn + 1);
   uint64_t* roff = buf<uint64_t>(b->roff, nn + 1);
   if (nn) {
@@ -1866,6 +1999,20 @@ static void run_build(gx_build* b) {
   }
   exclusive_scan<uint64_t, uint64_t>(rsize, roff, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   uint64_t total = read1(st, roff + nn);
+  b->part_rec.assign(P + 1, 0);
+  b->part_off.assign(P + 1, 0);
+  if (P > 1) {
+    uint64_t* pbyte = pstart + P + 1;
+    hipLaunchKernelGGL(k_part_bytes, dim3(grid_for((uint64_t)P + 1, kBlock, G)), dim3(kBlock), 0, st, P, pstart, roff,
+                       pbyte);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemcpyAsync(b->part_rec.data(), pstart, (P + 1) * 8, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipMemcpyAsync(b->part_off.data(), pbyte, (P + 1) * 8, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+  } else {
+    b->part_rec[1] = nn;
+    b->part_off[1] = total;
+  }
   uint8_t* out = buf<uint8_t>(b->out, total + 1);
   if (nn) {
     hipLaunchKernelGGL(k_ser_write<W>, dim3(grid_for(nn, kBlock, 0xffffffffu)), dim3(kBlock), 0, st, c, roff, out);
@@ -2068,22 +2215,89 @@ static int guarded(gx_build* b, F&& f) {
   }
 }
 
-static void append_text(gx_build* b, const void* src, size_t len, hipMemcpyKind kind) {
-  if (b->finished) throw GxError(GX_ESTATE, "add_reads after finish; call gx_build_reset first");
-  size_t need = b->text_len + len + 64;
-  if (need > b->text.cap) {
-    size_t ncap = std::max(need, b->text.cap * 3 / 2);
+static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind) {
+  size_t need = dlen + len + 64;
+  if (need > dst.cap) {
+    size_t ncap = std::max(need, dst.cap * 3 / 2);
     void* np = nullptr;
     GX_HIP(hipMalloc(&np, ncap));
-    if (b->text_len) GX_HIP(hipMemcpyAsync(np, b->text.p, b->text_len, hipMemcpyDeviceToDevice, b->st));
+    if (dlen) GX_HIP(hipMemcpyAsync(np, dst.p, dlen, hipMemcpyDeviceToDevice, b->st));
     GX_HIP(hipStreamSynchronize(b->st));
-    b->text.release();
-    b->text.p = np;
-    b->text.cap = ncap;
+    dst.release();
+    dst.p = np;
+    dst.cap = ncap;
   }
-  if (len) GX_HIP(hipMemcpyAsync(b->text.as<char>() + b->text_len, src, len, kind, b->st));
+  if (len) GX_HIP(hipMemcpyAsync(dst.as<char>() + dlen, src, len, kind, b->st));
   GX_HIP(hipStreamSynchronize(b->st));
-  b->text_len += len;
+  dlen += len;
+}
+
+constexpr int kFmtFastqPair = 2;  // internal: two FASTQ files given through gx_build_add_fastq_pair
+
+static void set_format(gx_build* b, int fmt) {
+  if (b->finished) throw GxError(GX_ESTATE, "add_reads after finish; call gx_build_reset first");
+  if (fmt != GX_FMT_READIDS && fmt != GX_FMT_FASTQ && fmt != kFmtFastqPair)
+    throw GxError(GX_EINVAL, "unknown input format");
+  if (b->fmt >= 0 && b->fmt != fmt)
+    throw GxError(GX_EINVAL, "input format differs from earlier calls on this handle (one library per handle)");
+  b->fmt = fmt;
+}
+
+static void append_text(gx_build* b, const void* src, size_t len, hipMemcpyKind kind, int fmt) {
+  set_format(b, fmt);
+  append_to(b, b->text, b->text_len, src, len, kind);
+}
+
+// Line ends of a text (same terminators as the readids parser).
+static uint64_t line_ends(gx_build* b, const uint8_t* t, uint64_t n, DevBuf& termbuf, uint64_t** term_out) {
+  hipStream_t st = b->st;
+  unsigned lblocks = grid_for(n + 1, kLTile, 0xffffffffu);
+  uint64_t* blk = buf<uint64_t>(b->blk, (size_t)lblocks + 1);
+  uint64_t* blkb = buf<uint64_t>(b->blk2, (size_t)lblocks + 1);
+  hipLaunchKernelGGL(k_line_count, dim3(lblocks), dim3(kBlock), 0, st, t, n, blk, nullptr);
+  GX_HIP(hipGetLastError());
+  exclusive_scan<uint64_t, uint64_t>(blk, blkb, lblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(lblocks)), st);
+  uint64_t nl = read1(st, blkb + lblocks);
+  uint64_t* term = buf<uint64_t>(termbuf, nl + 1);
+  hipLaunchKernelGGL(k_line_write, dim3(lblocks), dim3(kBlock), 0, st, t, n, blkb, term, nullptr);
+  GX_HIP(hipGetLastError());
+  *term_out = term;
+  return nl;
+}
+
+// FASTQ (one file, or two line-aligned files) -> readids text in b->rtext.
+static void fastq_convert(gx_build* b) {
+  hipStream_t st = b->st;
+  const bool paired = b->fmt == kFmtFastqPair;
+  const uint8_t* t1 = b->text.as<uint8_t>();
+  const uint8_t* t2 = nullptr;
+  uint64_t *term1 = nullptr, *term2 = nullptr;
+  uint64_t nl1 = line_ends(b, t1, b->text_len, b->fterm1, &term1);
+  if (paired) {
+    if (b->text2.cap == 0) b->text2.ensure(64);
+    t2 = b->text2.as<uint8_t>();
+    uint64_t nl2 = line_ends(b, t2, b->text2_len, b->fterm2, &term2);
+    if (nl1 != nl2)
+      throw GxError(GX_EINPUT, "Fastq files didn't have the same number of lines! (" + std::to_string(nl1) +
+                                   " and " + std::to_string(nl2) + " lines)");
+  }
+  const uint64_t nrec = (nl1 + 2) / 4;  // lines 1, 5, 9, ...
+  uint64_t* sz = buf<uint64_t>(b->fsz, nrec + 1);
+  uint64_t* off = buf<uint64_t>(b->foff, nrec + 1);
+  if (nrec) {
+    hipLaunchKernelGGL(k_fq_size, dim3(grid_for(nrec, kBlock, 2048)), dim3(kBlock), 0, st, nrec, t1, term1, t2, term2,
+                       sz);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(sz, off, nrec, scratch_for(b, scan_scratch_bytes<uint64_t>(nrec)), st);
+  uint64_t total = read1(st, off + nrec);
+  uint8_t* out = buf<uint8_t>(b->rtext, total + 64);
+  if (nrec) {
+    hipLaunchKernelGGL(k_fq_write, dim3(grid_for(nrec, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, nrec, t1, term1,
+                       t2, term2, off, out);
+    GX_HIP(hipGetLastError());
+  }
+  b->rtext_len = total;
 }
 
 extern "C" {
@@ -2095,7 +2309,7 @@ int gx_build_create(const gx_build_cfg* cfg, gx_build** out) {
   if (cfg->k > 63) { g_create_err = "k > 63 is not supported"; return GX_EINVAL; }
   if (cfg->max_readids_per_edge < 0) { g_create_err = "maxReadIDsPerEdge must be non-negative!"; return GX_EINVAL; }
   if (cfg->nparts < 1) { g_create_err = "nparts must be >= 1"; return GX_EINVAL; }
-  if (cfg->nparts != 1) { g_create_err = "nparts > 1 is not implemented on the device yet"; return GX_ENOTSUP; }
+  if (cfg->nparts > 65536) { g_create_err = "nparts above 65536 is not supported"; return GX_ENOTSUP; }
   if (cfg->edge_order != GX_EDGE_ORDER_CANONICAL && cfg->edge_order != GX_EDGE_ORDER_JDK7) {
     g_create_err = "unknown edge_order";
     return GX_EINVAL;
@@ -2127,20 +2341,48 @@ int gx_build_create(const gx_build_cfg* cfg, gx_build** out) {
 int gx_build_add_reads(gx_build* b, const char* buf_, size_t len, int fmt) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
-    if (fmt != GX_FMT_READIDS) throw GxError(GX_ENOTSUP, "only GX_FMT_READIDS is supported");
     if (len && !buf_) throw GxError(GX_EINVAL, "null buffer");
     GX_HIP(hipSetDevice(b->cfg.device));
-    append_text(b, buf_, len, hipMemcpyHostToDevice);
+    append_text(b, buf_, len, hipMemcpyHostToDevice, fmt);
   });
 }
 
 int gx_build_add_reads_device(gx_build* b, const void* dev_buf, size_t len, int fmt) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
-    if (fmt != GX_FMT_READIDS) throw GxError(GX_ENOTSUP, "only GX_FMT_READIDS is supported");
     if (len && !dev_buf) throw GxError(GX_EINVAL, "null buffer");
     GX_HIP(hipSetDevice(b->cfg.device));
-    append_text(b, dev_buf, len, hipMemcpyDeviceToDevice);
+    append_text(b, dev_buf, len, hipMemcpyDeviceToDevice, fmt);
+  });
+}
+
+int gx_build_add_fastq_pair(gx_build* b, const char* mate1, size_t n1, const char* mate2, size_t n2) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if ((n1 && !mate1) || (n2 && !mate2)) throw GxError(GX_EINVAL, "null buffer");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    set_format(b, kFmtFastqPair);
+    append_to(b, b->text, b->text_len, mate1, n1, hipMemcpyHostToDevice);
+    append_to(b, b->text2, b->text2_len, mate2, n2, hipMemcpyHostToDevice);
+  });
+}
+
+int gx_build_readids(gx_build* b, char* dst, size_t cap, size_t* written) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "readids before finish");
+    if (b->fmt != GX_FMT_FASTQ && b->fmt != kFmtFastqPair)
+      throw GxError(GX_ESTATE, "the handle was given readids text, not FASTQ");
+    if (!dst) {
+      if (!written) throw GxError(GX_EINVAL, "null argument");
+      *written = b->rtext_len;
+      return;
+    }
+    if (cap < b->rtext_len) throw GxError(GX_EINVAL, "destination too small");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    if (b->rtext_len) GX_HIP(hipMemcpyAsync(dst, b->rtext.p, b->rtext_len, hipMemcpyDeviceToHost, b->st));
+    GX_HIP(hipStreamSynchronize(b->st));
+    if (written) *written = b->rtext_len;
   });
 }
 
@@ -2150,6 +2392,14 @@ int gx_build_finish(gx_build* b) {
     if (b->finished) throw GxError(GX_ESTATE, "finish called twice; call gx_build_reset first");
     GX_HIP(hipSetDevice(b->cfg.device));
     if (b->text.cap == 0) b->text.ensure(64);
+    if (b->fmt == GX_FMT_FASTQ || b->fmt == kFmtFastqPair) {
+      fastq_convert(b);
+      b->src = b->rtext.as<uint8_t>();
+      b->src_len = b->rtext_len;
+    } else {
+      b->src = b->text.as<uint8_t>();
+      b->src_len = b->text_len;
+    }
     b->unitigs_done = false;
     if (b->W == 1) run_build<1>(b);
     else run_build<2>(b);
@@ -2161,9 +2411,9 @@ int gx_build_part_info(gx_build* b, int part, uint64_t* nrec, uint64_t* nbytes) 
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "part_info before finish");
-    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
-    if (nrec) *nrec = b->nnodes;
-    if (nbytes) *nbytes = b->out_bytes;
+    if (part < 0 || part >= b->cfg.nparts) throw GxError(GX_EINVAL, "no such partition");
+    if (nrec) *nrec = b->part_rec[part + 1] - b->part_rec[part];
+    if (nbytes) *nbytes = b->part_off[part + 1] - b->part_off[part];
   });
 }
 
@@ -2171,12 +2421,14 @@ int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* 
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "emit before finish");
-    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
-    if (cap < b->out_bytes) throw GxError(GX_EINVAL, "destination too small");
+    if (part < 0 || part >= b->cfg.nparts) throw GxError(GX_EINVAL, "no such partition");
+    uint64_t off = b->part_off[part], n = b->part_off[part + 1] - off;
+    if (cap < n) throw GxError(GX_EINVAL, "destination too small");
+    if (n && !dst) throw GxError(GX_EINVAL, "null destination");
     GX_HIP(hipSetDevice(b->cfg.device));
-    if (b->out_bytes) GX_HIP(hipMemcpyAsync(dst, b->out.p, b->out_bytes, hipMemcpyDeviceToHost, b->st));
+    if (n) GX_HIP(hipMemcpyAsync(dst, b->out.as<uint8_t>() + off, n, hipMemcpyDeviceToHost, b->st));
     GX_HIP(hipStreamSynchronize(b->st));
-    if (written) *written = b->out_bytes;
+    if (written) *written = n;
   });
 }
 
@@ -2184,9 +2436,10 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "part_device before finish");
-    if (part != 0) throw GxError(GX_EINVAL, "no such partition");
-    *dev_ptr = b->out.p;
-    *nbytes = b->out_bytes;
+    if (part < 0 || part >= b->cfg.nparts) throw GxError(GX_EINVAL, "no such partition");
+    if (!dev_ptr || !nbytes) throw GxError(GX_EINVAL, "null argument");
+    *dev_ptr = b->out.as<uint8_t>() + b->part_off[part];
+    *nbytes = b->part_off[part + 1] - b->part_off[part];
   });
 }
 
@@ -2245,6 +2498,9 @@ const char* gx_build_phase_name(int i) { return (i >= 0 && i < NPH) ? kPhaseName
 int gx_build_reset(gx_build* b) {
   if (!b) return GX_EINVAL;
   b->text_len = 0;
+  b->text2_len = 0;
+  b->rtext_len = 0;
+  b->fmt = -1;
   b->finished = false;
   b->unitigs_done = false;
   b->out_bytes = 0;

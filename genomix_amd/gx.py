@@ -20,6 +20,7 @@ GX_EDEVICE = -19
 GX_ESTATE = -16
 GX_ENOTSUP = -95
 GX_FMT_READIDS = 0
+GX_FMT_FASTQ = 1
 EDGE_ORDER = {"canonical": 0, "jdk7": 1}
 
 EXPORTED = [
@@ -27,6 +28,7 @@ EXPORTED = [
     "gx_build_part_info", "gx_build_emit_part", "gx_build_part_device", "gx_build_stats",
     "gx_build_reset", "gx_last_error", "gx_build_destroy", "gx_build_set_profiling",
     "gx_build_phase_times", "gx_build_phase_name", "gx_unitig_run", "gx_unitig_info", "gx_unitig_emit",
+    "gx_build_add_fastq_pair", "gx_build_readids",
 ]
 
 
@@ -71,6 +73,8 @@ def lib():
         L.gx_unitig_run.argtypes = [vp]
         L.gx_unitig_info.argtypes = [vp, u64p, u64p]
         L.gx_unitig_emit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        L.gx_build_add_fastq_pair.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
+        L.gx_build_readids.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -97,8 +101,24 @@ class GraphBuilder:
     def add_reads_device(self, ptr: int, nbytes: int):
         self._check(lib().gx_build_add_reads_device(self._h, ctypes.c_void_p(ptr), nbytes, GX_FMT_READIDS))
 
+    def add_fastq(self, mate1: bytes, mate2: bytes = None):
+        """FASTQ bytes of one library (single-end, or both files of a pair);
+        converted to readids lines on the device at finish."""
+        if mate2 is None:
+            self._check(lib().gx_build_add_reads(self._h, mate1, len(mate1), GX_FMT_FASTQ))
+        else:
+            self._check(lib().gx_build_add_fastq_pair(self._h, mate1, len(mate1), mate2, len(mate2)))
+
     def finish(self):
         self._check(lib().gx_build_finish(self._h))
+
+    def readids(self) -> bytes:
+        """The library-N.readids text made from FASTQ input (after finish)."""
+        n = ctypes.c_size_t()
+        self._check(lib().gx_build_readids(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(n.value, 1))
+        self._check(lib().gx_build_readids(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
 
     def part_info(self, part=0):
         n, b = ctypes.c_uint64(), ctypes.c_uint64()

@@ -114,6 +114,14 @@ def readids_text(m1, m2=None, first_read=0) -> bytes:
     return b"".join(ids[i] + rows[i * w:(i + 1) * w] for i in range(n))
 
 
+def fastq_text(m, name="r") -> bytes:
+    """Code matrix as FASTQ records (quality 'I')."""
+    n, L = m.shape
+    seqs = ASCII[m].tobytes()
+    qual = b"I" * L
+    return b"".join(b"@%s%d\n%s\n+\n%s\n" % (name.encode(), i, seqs[i * L:(i + 1) * L], qual) for i in range(n))
+
+
 def simulate(spec: ReadSetSpec) -> bytes:
     m1, m2 = reads(spec)
     return readids_text(m1, m2)

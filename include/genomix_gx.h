@@ -38,6 +38,8 @@ extern "C" {
 
 /* input formats */
 #define GX_FMT_READIDS 0 /* "readId\tseq[\tmateSeq]" lines (library-N.readids) */
+#define GX_FMT_FASTQ 1   /* single-end FASTQ of one library; converted on the device as
+                            GenomixDriver.convertAndUploadFastqToHDFS does (GenomixDriver.java:665-714) */
 
 /* edge-list order inside each FF/FR/RF/RR list */
 #define GX_EDGE_ORDER_CANONICAL 0 /* ascending neighbour k-mer (the reference's tests compare lists as sets) */
@@ -47,7 +49,9 @@ typedef struct gx_build gx_build;
 
 typedef struct {
   int k;                    /* GenomixJobConf KMER_LENGTH (GenomixJobConf.java:303); 3 <= k <= 63 */
-  int nparts;               /* output partitions P (Hyracks partitions); 1 = CLI default */
+  int nparts;               /* output partitions P, 1..65536 (Hyracks partitions; 1 = CLI default).  A record
+                               goes to part nonneg(hashBytes(key)) % P (KmerPartitionComputerFactory.java:28-53),
+                               each part in KmerPointable order: the reference's part-<p> for one input split */
   int max_readids_per_edge; /* GenomixJobConf MAX_READIDS_PER_EDGE (:333); validated >= 0, no effect (as in the reference) */
   int edge_order;           /* GX_EDGE_ORDER_* */
   int device;               /* HIP device ordinal */
@@ -69,6 +73,21 @@ int gx_build_add_reads(gx_build* b, const char* buf, size_t len, int fmt);
 /* Same, for text already resident in device memory (no host copy; the
  * buffer is copied device -> device into the handle before return). */
 int gx_build_add_reads_device(gx_build* b, const void* dev_buf, size_t len, int fmt);
+
+/* Paired-end FASTQ of one library: mate-1 and mate-2 file bytes, each
+ * appended to its own stream (chunks concatenate per file).  At finish the
+ * two files are converted on the device into "readId\tmate1\tmate2" lines,
+ * readId = 1-based line number of each sequence line (4i+2), both sequences
+ * String.trim()ed, as GenomixDriver.convertAndUploadFastqToHDFS
+ * (GenomixDriver.java:665-714); files with different line counts fail with
+ * GX_EINPUT ("didn't have the same number of lines").  A handle takes one
+ * input format: mixing formats across add calls is GX_EINVAL. */
+int gx_build_add_fastq_pair(gx_build* b, const char* mate1, size_t n1, const char* mate2, size_t n2);
+
+/* After finish on FASTQ input: the library-N.readids text the device made
+ * (the file convertAndUploadFastqToHDFS writes, GenomixDriver.java:669-712).
+ * dst == NULL stores the size in *written. */
+int gx_build_readids(gx_build* b, char* dst, size_t cap, size_t* written);
 
 /* Runs the whole build on the device (split reads, canonical k-mers, table
  * insert, aggregation, read heads, record serialisation) and waits for it.
