@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -27,6 +28,7 @@
 #include "../../include/genomix_gx.h"
 #include "gx_kmer.h"
 #include "gx_prims.h"
+#include "gx_sizer.h"
 
 namespace gx {
 
@@ -359,9 +361,14 @@ __device__ inline uint32_t ld_relaxed32(const uint32_t* p) {
 
 // Inserts key; returns the slot index (or ~0 when the probe limit was hit,
 // i.e. the table is too full) and whether this call claimed the slot.
+// Home slot by multiply-shift range reduction (any capacity, not only powers
+// of two), linear probing with wrap-around.
+__device__ inline uint64_t slot_home(uint64_t h, uint64_t cap) { return __umul64hi(h, cap); }
+__device__ inline uint64_t slot_next(uint64_t s, uint64_t cap) { return s + 1 == cap ? 0 : s + 1; }
+
 template <int W>
-__device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, uint64_t limit, bool* claimed) {
-  uint64_t s = khash<W>(key) & capmask;
+__device__ inline uint64_t table_insert(void* table, uint64_t cap, KV key, uint64_t limit, bool* claimed) {
+  uint64_t s = slot_home(khash<W>(key), cap);
   *claimed = false;
   if (W == 1) {
     Slot1* T = (Slot1*)table;
@@ -374,7 +381,7 @@ __device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, u
         if (prev == kEmpty) { *claimed = true; return s; }
         if (prev == key.lo) return s;
       }
-      s = (s + 1) & capmask;
+      s = slot_next(s, cap);
     }
     return ~0ull;
   } else {
@@ -403,7 +410,7 @@ __device__ inline uint64_t table_insert(void* table, uint64_t capmask, KV key, u
         continue;
       }
       if (cur == key.hi && ld_relaxed(&T[s].lo) == key.lo) return s;
-      s = (s + 1) & capmask;
+      s = slot_next(s, cap);
       probe++;
     }
     return ~0ull;
@@ -460,7 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
                                                    const uint64_t* __restrict__ words,
                                                    const uint64_t* __restrict__ occbase,
                                                    const uint64_t* __restrict__ nocc, int k, void* table,
-                                                   uint64_t capmask, uint64_t limit,
+                                                   uint64_t cap, uint64_t limit,
                                                    uint32_t* __restrict__ occ_slot, uint8_t* __restrict__ occ_code,
                                                    unsigned long long* err) {
   const int lane = threadIdx.x & 63;
@@ -499,7 +506,7 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
         bits |= 1u << (4 * t + x);
       }
       bool claimed;
-      uint64_t s = table_insert<W>(table, capmask, key, limit, &claimed);
+      uint64_t s = table_insert<W>(table, cap, key, limit, &claimed);
       (void)claimed;
       if (s == ~0ull) {
         report(err, ~0ull >> 8, ERR_TABLE);
@@ -514,6 +521,43 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
       occ_code[ob + p] = (uint8_t)code;
     }
   }
+}
+
+// HyperLogLog sketch (Flajolet et al. 2007; 2^12 registers) of the
+// canonical k-mers of the first mates, for the learned table sizing.  The
+// register of a k-mer is the top 12 bits of its 64-bit hash, its value the
+// position of the first set bit in the remaining 52 (1..53).
+constexpr int kHllBits = 12, kHllRegs = 1 << kHllBits;
+
+template <int W>
+__device__ inline uint64_t hll_hash(KV key) {
+  return mix64(khash<W>(key) + 0x632be59bd9b4e019ull);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_hll(uint64_t nmates, const uint64_t* __restrict__ wbase,
+                                                const uint64_t* __restrict__ words,
+                                                const uint64_t* __restrict__ nocc, int k,
+                                                uint32_t* __restrict__ regs) {
+  __shared__ uint32_t r[kHllRegs];
+  for (int i = threadIdx.x; i < kHllRegs; i += kBlock) r[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t m = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); m < nmates; m += nw) {
+    uint64_t nk = nocc[m];
+    const uint64_t* w = words + wbase[m];
+    for (uint32_t p = lane; p < nk; p += 64) {
+      KV key = canon<W>(load_kmer<W>(w, p, k), k, nullptr);
+      uint64_t h = hll_hash<W>(key);
+      uint32_t idx = (uint32_t)(h >> (64 - kHllBits));
+      uint64_t rest = (h << kHllBits) | (1ull << (kHllBits - 1));
+      atomicMax(&r[idx], (uint32_t)__builtin_clzll(rest) + 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHllRegs; i += kBlock)
+    if (r[i]) atomicMax(&regs[i], r[i]);
 }
 
 template <int W>
@@ -1191,9 +1235,9 @@ constexpr int kUEdges = 16;
 
 // Read-only lookup of a key that is known to be in the table.
 template <int W>
-__device__ inline uint64_t table_find(const void* table, uint64_t capmask, KV key) {
-  uint64_t s = khash<W>(key) & capmask;
-  for (uint64_t probe = 0; probe <= capmask; probe++) {
+__device__ inline uint64_t table_find(const void* table, uint64_t cap, KV key) {
+  uint64_t s = slot_home(khash<W>(key), cap);
+  for (uint64_t probe = 0; probe < cap; probe++) {
     if (W == 1) {
       uint64_t cur = ((const Slot1*)table)[s].key;
       if (cur == key.lo) return s;
@@ -1203,14 +1247,14 @@ __device__ inline uint64_t table_find(const void* table, uint64_t capmask, KV ke
       if (cur == key.hi && ((const Slot2*)table)[s].lo == key.lo) return s;
       if (cur == kEmpty) return ~0ull;
     }
-    s = (s + 1) & capmask;
+    s = slot_next(s, cap);
   }
   return ~0ull;
 }
 
 template <int W>
-__device__ inline uint32_t node_of_key(const void* table, uint64_t capmask, KV key) {
-  uint64_t s = table_find<W>(table, capmask, key);
+__device__ inline uint32_t node_of_key(const void* table, uint64_t cap, KV key) {
+  uint64_t s = table_find<W>(table, cap, key);
   if (s == ~0ull) return kEnd;
   return W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
 }
@@ -1226,7 +1270,7 @@ __device__ inline int back_side(uint32_t t) { return (t == ET_FF || t == ET_RF) 
 // (node << 1 | partner side) or kEnd.
 template <int W>
 __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
-                        const uint32_t* __restrict__ msk, const void* table, uint64_t capmask, int k,
+                        const uint32_t* __restrict__ msk, const void* table, uint64_t cap, int k,
                         uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     KV v = node_key<W>(klo, khi, i);
@@ -1240,7 +1284,7 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
         if (keq<W>(z, v)) tnd = true;
         int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
         if (side_deg(mask, d) == 1) {
-          uint32_t j = node_of_key<W>(table, capmask, z);
+          uint32_t j = node_of_key<W>(table, cap, z);
           if (j == kEnd) atomicMin(err, (unsigned long long)i);
           else l[d] = (j << 1) | (uint32_t)back_side(t);
         }
@@ -1377,7 +1421,7 @@ template <int W>
 __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, const uint64_t* __restrict__ ulen,
                                const uint32_t* __restrict__ order, const uint64_t* __restrict__ klo,
                                const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk,
-                               const void* table, uint64_t capmask, const uint32_t* __restrict__ urep, int k,
+                               const void* table, uint64_t cap, const uint32_t* __restrict__ urep, int k,
                                uint32_t* __restrict__ uedge, uint8_t* __restrict__ uecnt) {
   for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t ent[kUEdges];
@@ -1398,7 +1442,7 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
         for (uint32_t x = 0; x < 4; x++) {
           if (!((mask >> (4 * t + x)) & 1)) continue;
           KV z = neighbour<W>(v, k, t, x);
-          uint32_t zi = node_of_key<W>(table, capmask, z);
+          uint32_t zi = node_of_key<W>(table, cap, z);
           if (zi == kEnd) continue;  // cannot happen: the graph is symmetric
           uint32_t zr = urep[zi];
           bool z5 = (t == ET_FF || t == ET_RF);           // touches z's 5' end
@@ -1640,7 +1684,7 @@ struct gx_build {
   DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
   DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t stats[10] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t ev[2 * 16] = {};
@@ -1654,14 +1698,20 @@ struct gx_build {
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
+  // learned sizing
+  gx_mlp* sizer = nullptr;
+  DevBuf hregs, sfeat, spred;
+  float feat[GX_SIZER_FEATURES] = {};
+  uint8_t regs_h[GX_SIZER_HLL_REGISTERS] = {};
+  double pred[2] = {0, 0};
   std::vector<uint64_t> part_rec, part_off;  // per partition: first record, first byte (P + 1 entries)
 };
 
 static std::string g_create_err;
 
-enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, NPH };
+enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, NPH };
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay", "unitig"};
+                                       "k_replay", "unitig", "sizing"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -1721,6 +1771,51 @@ static void check_err(gx_build* b) {
     if (code == ERR_TABLE) throw GxError(GX_ENOMEM, "hash table overflow");
     throw GxError(GX_EINPUT, std::string(errname(code)) + " (line " + std::to_string(line + 1) + ")");
   }
+}
+
+// Sizing features of a read set from the HLL registers of its first mates
+// (the network's input layout):
+//   [0, 64)  histogram of register values / 4096, shifted so that the
+//            rounded mean register value lands in bin 32 (scale-free);
+//   64       HLL estimate of the sample's distinct k-mers / its occurrences;
+//   65       log1p(coverage estimate) / 5, coverage estimate =
+//            2 * max(0, 1 - estimate / occurrences) / sample fraction;
+//   66       k / 64;  67  mean valid mate length / 256;
+//   68       sample fraction * 10;  69  log2(partitions) / 16;
+//   70       0 (reserved);  71  1.
+// HLL estimate: alpha_m m^2 / sum 2^-M[j], linear counting m ln(m / zeros)
+// below 2.5 m (Flajolet et al. 2007).
+static void sizing_features(const uint8_t* regs, double occ_s, double valid_s, double frac, int k, int P,
+                            float* f) {
+  const int m = GX_SIZER_HLL_REGISTERS;
+  double hist[64] = {0}, inv = 0, sum = 0;
+  int zeros = 0;
+  for (int j = 0; j < m; j++) {
+    int r = regs[j] < 63 ? regs[j] : 63;
+    hist[r] += 1;
+    inv += std::ldexp(1.0, -(int)regs[j]);
+    sum += regs[j];
+    zeros += regs[j] == 0;
+  }
+  double E = 0.7213 / (1 + 1.079 / m) * m * (double)m / inv;
+  if (E <= 2.5 * m && zeros > 0) E = m * std::log((double)m / zeros);
+  int center = (int)std::lround(sum / m);
+  for (int i = 0; i < GX_SIZER_FEATURES; i++) f[i] = 0.f;
+  for (int v = 0; v < 64; v++) {
+    int bin = v - center + 32;
+    bin = bin < 0 ? 0 : bin > 63 ? 63 : bin;
+    f[bin] += (float)(hist[v] / m);
+  }
+  double ratio = occ_s > 0 ? std::min(2.0, E / occ_s) : 1.0;
+  double cov = frac > 0 ? 2.0 * std::max(0.0, 1.0 - ratio) / frac : 0.0;
+  f[64] = (float)ratio;
+  f[65] = (float)(std::log1p(cov) / 5.0);
+  f[66] = (float)(k / 64.0);
+  f[67] = (float)((valid_s > 0 ? occ_s / valid_s + k - 1 : 0.0) / 256.0);
+  f[68] = (float)(frac * 10.0);
+  f[69] = (float)(std::log2((double)P) / 16.0);
+  f[70] = 0.f;
+  f[71] = 1.f;
 }
 
 template <int W>
@@ -1791,13 +1886,55 @@ static void run_build(gx_build* b) {
   exclusive_scan<uint64_t, uint64_t>(hm, hidx, nmates, scratch_for(b, scan_scratch_bytes<uint64_t>(nmates)), st);
   uint64_t nheads = read1(st, hidx + nmates);
 
+  // 3b. sizing features: HLL sketch of the first 1% of the mates (at least
+  //     512), then, with a network attached, the predicted distinct count
+  ph_begin(b, PH_SIZING);
+  const uint64_t ns_m = std::min<uint64_t>(nmates, std::max<uint64_t>((nmates + 99) / 100, 512));
+  uint32_t* hr = buf<uint32_t>(b->hregs, kHllRegs);
+  GX_HIP(hipMemsetAsync(hr, 0, kHllRegs * 4, st));
+  if (ns_m) {
+    hipLaunchKernelGGL(k_hll<W>, dim3(grid_for(ns_m, kBlock / 64, 1024)), dim3(kBlock), 0, st, ns_m, wbase, words,
+                       nocc, k, hr);
+    GX_HIP(hipGetLastError());
+  }
+  const uint64_t occ_s = read1(st, occbase + ns_m), valid_s = read1(st, hidx + ns_m);
+  {
+    std::vector<uint32_t> rh(kHllRegs);
+    GX_HIP(hipMemcpyAsync(rh.data(), hr, kHllRegs * 4, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < kHllRegs; i++) b->regs_h[i] = (uint8_t)rh[i];
+  }
+  sizing_features(b->regs_h, (double)occ_s, (double)valid_s, nmates ? (double)ns_m / (double)nmates : 0.0, k,
+                  b->cfg.nparts, b->feat);
+  b->pred[0] = b->pred[1] = 0;
+  uint64_t learned_cap = 0;
+  if (b->sizer && !b->cfg.table_hint && N) {
+    float* df = buf<float>(b->sfeat, GX_SIZER_FEATURES);
+    float* dy = buf<float>(b->spred, 8);
+    GX_HIP(hipMemcpyAsync(df, b->feat, sizeof(b->feat), hipMemcpyHostToDevice, st));
+    int rc = gx_mlp_forward_device(b->sizer, df, 1, dy, st);
+    if (rc) throw GxError(rc, std::string("sizing network: ") + gx_mlp_last_error(b->sizer));
+    float y[2];
+    GX_HIP(hipMemcpyAsync(y, dy, sizeof(y), hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    double ly = std::isfinite(y[0]) ? std::min(0.0, std::max(-30.0, (double)y[0])) : 0.0;
+    b->pred[0] = std::max(1.0, std::floor((double)N * std::exp(ly)));
+    b->pred[1] = std::isfinite(y[1]) ? std::exp(std::min(10.0, std::max(-10.0, (double)y[1]))) : 1.0;
+    learned_cap = std::max<uint64_t>(1024, (uint64_t)std::ceil(1.5 * b->pred[0]));
+  }
+  ph_end(b, PH_SIZING);
+
   // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
   //    of two >= the expected distinct count (table_hint) or the occurrence
   //    count, so the load stays below 1 and is typically well below 1/2.
   ph_begin(b, PH_INSERT);
-  uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint * 2 : N;
-  uint64_t cap = 1024;
-  while (cap < expect) cap <<= 1;
+  uint64_t cap = learned_cap;
+  if (!cap) {
+    uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint * 2 : N;
+    cap = 1024;
+    while (cap < expect) cap <<= 1;
+  }
+  uint64_t rebuilds = 0;
   const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
   uint32_t* occ_slot = buf<uint32_t>(b->occ_slot, N + 1);
   uint8_t* occ_code = buf<uint8_t>(b->occ_code, N + 1);
@@ -1813,14 +1950,19 @@ static void run_build(gx_build* b) {
       unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
       ph_begin(b, PH_KINSERT);
       hipLaunchKernelGGL(k_insert<W>, dim3(g), dim3(kBlock), 0, st, nmates, mlen, wbase, words, occbase, nocc, k,
-                         table, cap - 1, limit, occ_slot, occ_code, errw);
+                         table, cap, limit, occ_slot, occ_code, errw);
       GX_HIP(hipGetLastError());
       ph_end(b, PH_KINSERT);
     }
     unsigned long long e = read1(st, errw);
     if (e == ~0ull) break;
     if ((e & 0xff) != ERR_TABLE || attempt > 8) check_err(b);
-    cap <<= 1;  // fallback: rebuild in a table twice as large
+    // fallback: rebuild in a table twice as large, and at least as large as
+    // the sizing from the occurrence count (the method without a prediction)
+    uint64_t plain = 1024;
+    while (plain < N) plain <<= 1;
+    cap = std::max(cap << 1, rebuilds == 0 ? plain : 0);
+    rebuilds++;
   }
   ph_end(b, PH_INSERT);
 
@@ -2033,6 +2175,8 @@ static void run_build(gx_build* b) {
   b->stats[4] = nrep;
   b->stats[5] = nsel;
   b->stats[6] = read1(st, errw + 3);
+  b->stats[7] = (uint64_t)b->pred[0];
+  b->stats[8] = rebuilds;
 }
 
 template <int W>
@@ -2057,7 +2201,7 @@ static void run_unitigs(gx_build* b) {
   GX_HIP(hipMemsetAsync(cut, 0, ne + 1, st));
   if (nn) {
     hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, table,
-                       cap - 1, k, lnk, tand, errw);
+                       cap, k, lnk, tand, errw);
     GX_HIP(hipGetLastError());
   }
   uint4* R[2] = {buf<uint4>(b->u_ptr[0], ne + 1), buf<uint4>(b->u_ptr[1], ne + 1)};
@@ -2126,7 +2270,7 @@ static void run_unitigs(gx_build* b) {
   uint8_t* uecnt = buf<uint8_t>(b->u_ecnt, nu + 1);
   if (nu) {
     hipLaunchKernelGGL(k_unitig_edges<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, order,
-                       klo, khi, msk, table, cap - 1, urep, k, uedge, uecnt);
+                       klo, khi, msk, table, cap, urep, k, uedge, uecnt);
     GX_HIP(hipGetLastError());
   }
   // heads
@@ -2445,7 +2589,7 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
 
 int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
-  for (int i = 0; i < n && i < 7; i++) out[i] = b->stats[i];
+  for (int i = 0; i < n && i < 9; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 
@@ -2478,6 +2622,36 @@ int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written) {
     if (b->u_bytes) GX_HIP(hipMemcpyAsync(dst, b->u_out.p, b->u_bytes, hipMemcpyDeviceToHost, b->st));
     GX_HIP(hipStreamSynchronize(b->st));
     if (written) *written = b->u_bytes;
+  });
+}
+
+int gx_build_set_sizer(gx_build* b, gx_mlp* m) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (m) {
+      int din = 0, dout = 0;
+      gx_mlp_dims(m, &din, &dout);
+      if (din != GX_SIZER_FEATURES || dout != GX_SIZER_OUTPUTS)
+        throw GxError(GX_EINVAL, "sizing network must map 72 features to 2 outputs");
+      if (gx_mlp_device_of(m) != b->cfg.device) throw GxError(GX_EINVAL, "sizing network lives on another device");
+    }
+    b->sizer = m;
+  });
+}
+
+int gx_build_features(gx_build* b, float* feat, int n, uint8_t* regs, double* prediction) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->finished) throw GxError(GX_ESTATE, "features before finish");
+    if (feat) {
+      if (n < GX_SIZER_FEATURES) throw GxError(GX_EINVAL, "feature buffer too small");
+      memcpy(feat, b->feat, sizeof(b->feat));
+    }
+    if (regs) memcpy(regs, b->regs_h, sizeof(b->regs_h));
+    if (prediction) {
+      prediction[0] = b->pred[0];
+      prediction[1] = b->pred[1];
+    }
   });
 }
 

@@ -29,6 +29,9 @@ EXPORTED = [
     "gx_build_reset", "gx_last_error", "gx_build_destroy", "gx_build_set_profiling",
     "gx_build_phase_times", "gx_build_phase_name", "gx_unitig_run", "gx_unitig_info", "gx_unitig_emit",
     "gx_build_add_fastq_pair", "gx_build_readids",
+    "gx_mlp_create", "gx_mlp_num_params", "gx_mlp_set_params", "gx_mlp_get_params", "gx_mlp_forward",
+    "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
+    "gx_build_features",
 ]
 
 
@@ -75,6 +78,20 @@ def lib():
         L.gx_unitig_emit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.gx_build_add_fastq_pair.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
         L.gx_build_readids.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.gx_mlp_create.argtypes = [i32, i32, i32, i32, ctypes.POINTER(vp)]
+        L.gx_mlp_num_params.argtypes = [vp]
+        L.gx_mlp_num_params.restype = sz
+        L.gx_mlp_set_params.argtypes = [vp, fp, sz]
+        L.gx_mlp_get_params.argtypes = [vp, fp, sz]
+        L.gx_mlp_forward.argtypes = [vp, fp, i32, fp]
+        L.gx_mlp_gradient.argtypes = [vp, fp, fp, i32, fp, fp]
+        L.gx_mlp_train_step.argtypes = [vp, fp, fp, i32, ctypes.c_float, fp]
+        L.gx_mlp_last_error.argtypes = [vp]
+        L.gx_mlp_last_error.restype = ctypes.c_char_p
+        L.gx_mlp_destroy.argtypes = [vp]
+        L.gx_build_set_sizer.argtypes = [vp, vp]
+        L.gx_build_features.argtypes = [vp, fp, i32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -82,14 +99,33 @@ def lib():
 class GraphBuilder:
     """One graph-build task on one GPU (a gx_build handle)."""
 
-    def __init__(self, k, nparts=1, edge_order="jdk7", device=0, table_hint=0, max_readids_per_edge=250):
+    def __init__(self, k, nparts=1, edge_order="jdk7", device=0, table_hint=0, max_readids_per_edge=250,
+                 sizer=None):
         self._h = ctypes.c_void_p()
+        self._sizer = None
         cfg = _Cfg(k, nparts, max_readids_per_edge, EDGE_ORDER[edge_order], device, table_hint)
         rc = lib().gx_build_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if rc != 0:
             raise GxError(rc, lib().gx_last_error(None).decode())
         self.k = k
         self.nparts = nparts
+        if sizer is not None:
+            self.set_sizer(sizer)
+
+    def set_sizer(self, sizer):
+        """Attach a trained sizing network (genomix_amd.sizer.Mlp) or None."""
+        self._check(lib().gx_build_set_sizer(self._h, sizer.handle if sizer is not None else None))
+        self._sizer = sizer  # keep it alive while attached
+
+    def features(self):
+        """(features, HLL registers, [predicted distinct, predicted skew]) of the last finish."""
+        import numpy as np
+        f = np.zeros(72, np.float32)
+        r = np.zeros(4096, np.uint8)
+        p = (ctypes.c_double * 2)()
+        self._check(lib().gx_build_features(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 72,
+                                            r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), p))
+        return f, r, [p[0], p[1]]
 
     def _check(self, rc):
         if rc != 0:
@@ -152,10 +188,10 @@ class GraphBuilder:
         return buf.raw[:w.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 7)()
-        self._check(lib().gx_build_stats(self._h, a, 7))
+        a = (ctypes.c_uint64 * 9)()
+        self._check(lib().gx_build_stats(self._h, a, 9))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
-                "max_coverage"]
+                "max_coverage", "predicted_distinct", "table_rebuilds"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

@@ -2,7 +2,9 @@
 
 Genome: iid uniform ACGT; optionally repeat families: copies of random
 templates, each copy with iid substitutions at the stated divergence, placed
-until the stated fraction of the genome is covered.  Reads: start uniform,
+until the stated fraction of the genome is covered (one template of a fixed
+length, or for a length range, families of 2-10 copies with lengths drawn
+from the range).  Reads: start uniform,
 strand uniform, iid substitution errors (one of the other three bases).
 Paired-end: fragment length ~ round(N(mean, sd)) clipped to [L, G]; mate 1 is
 the fragment's first L bases, mate 2 the reverse complement of its last L;
@@ -40,10 +42,14 @@ def genome(spec: ReadSetSpec, rng) -> np.ndarray:
         covered = 0
         target = int(spec.repeat_frac * spec.genome_len)
         fam = None
+        left = 0  # copies left of the current family (length ranges: 2..10 copies per family)
         while covered < target:
             if isinstance(spec.repeat_len, tuple):
-                ln = int(rng.integers(spec.repeat_len[0], spec.repeat_len[1] + 1))
-                fam = rng.integers(0, 4, ln, dtype=np.uint8)
+                if left == 0:
+                    ln = int(rng.integers(spec.repeat_len[0], spec.repeat_len[1] + 1))
+                    fam = rng.integers(0, 4, ln, dtype=np.uint8)
+                    left = int(rng.integers(2, 11))
+                left -= 1
             elif fam is None:
                 fam = rng.integers(0, 4, spec.repeat_len, dtype=np.uint8)
             ln = min(len(fam), spec.genome_len)

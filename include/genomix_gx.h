@@ -127,7 +127,9 @@ int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
 
 /* Statistics of the last finish: [0] occurrences, [1] distinct k-mers,
  * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed
- * occurrences, [6] largest coverage above 255 (0 if none). */
+ * occurrences, [6] largest coverage above 255 (0 if none), [7] distinct
+ * count predicted by an attached sizing network (0 if none, genomix_sizer.h),
+ * [8] table rebuilds after overflow. */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on

@@ -11,13 +11,17 @@ from .conftest import ROOT
 
 
 def _declared():
-    h = open(os.path.join(ROOT, "include", "genomix_gx.h")).read()
-    return sorted(set(re.findall(r"\b(gx_[a-z_]+)\s*\(", h)))
+    import glob
+    names = set()
+    for path in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        names |= set(re.findall(r"\b(gx_[a-z_0-9]+)\s*\(", open(path).read()))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
     names = _declared()
     assert "gx_build_create" in names and "gx_build_finish" in names and "gx_build_emit_part" in names
+    assert "gx_mlp_train_step" in names and "gx_build_set_sizer" in names
 
 
 def test_library_exports_all_declared_symbols():
