@@ -19,6 +19,14 @@ per-occurrence algorithmic bytes B = L/(L-k+1) + 2 * (8*ceil(k/32) + 8),
 measured with hipEvents on the library's stream) and a CPU baseline: the
 CPU oracle (a single-threaded C++ restatement of the reference's Hyracks
 dataflow) timed on a bounded sample of the same reads.
+
+"batched" (unless --no-batched): the batched workload -- 64 independent
+read sets, genomes log-uniform 100 kbp..10 Mbp, 100/150 bp SE/PE reads at
+20-80x, 0.5-2% error, k mixed 21/31/55 -- built (graph + unitigs) by the
+native batch scheduler with the tuned parameters and the learned table
+sizing, inputs resident in HBM; its k-mers/s beside the plain figure.
+--workload learned runs the learned-sizing workload (100 Mbp, 20% repeat
+families, 150 bp PE, 40x, k = 31) with and without the sizing network.
 """
 
 import argparse
@@ -69,12 +77,94 @@ def load_traffic(name):
     return None
 
 
+def _timed(fn, steps, warmup, label):
+    from genomix_amd import devmem
+    for i in range(warmup):
+        fn()
+        log(f"{label}: warmup step {i + 1}/{warmup} done")
+    devmem.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn()
+    devmem.synchronize()
+    el = time.perf_counter() - t0
+    log(f"{label}: timed {steps} steps in {el:.3f} s")
+    return el
+
+
+def batched_leg(steps, warmup, scale=1.0):
+    """The batched workload through the native batch scheduler."""
+    from genomix_amd import devmem, gx, sim, sizer
+    specs = sim.batched_sets(scale=scale)
+    t0 = time.perf_counter()
+    texts = sim.simulate_many([sp for sp, _ in specs])
+    log(f"batched: {len(texts)} read sets generated, {sum(map(len, texts)) / 1e9:.2f} GB in "
+        f"{time.perf_counter() - t0:.1f} s")
+    dev = [devmem.DeviceBytes(t) for t in texts]
+    sets = [{"k": k, "input": (d.address, d.n)} for (sp, k), d in zip(specs, dev)]
+    del texts
+    net = sizer.load() if os.path.exists(sizer.WEIGHTS) else None
+    tuning = load_tuning()
+    with gx.GraphBatch(sets, sizer=net, tuning=tuning) as b:
+        el = _timed(lambda: b.run(unitigs=True), steps, warmup, "batched")
+        st = [b.set_stats(i) for i in range(len(sets))]
+    for d in dev:
+        d.free()
+    occ = sum(x["occurrences"] for x in st)
+    return {"value": round(occ * steps / el, 1), "unit": "k-mers/s", "ms_per_step": round(1e3 * el / steps, 3),
+            "steps": steps, "warmup": warmup, "sets": len(sets), "kmer_occurrences": occ,
+            "distinct_kmers": sum(x["distinct"] for x in st),
+            "table_rebuilds": sum(x["table_rebuilds"] for x in st),
+            "k_mix": {str(k): sum(1 for _, kk in specs if kk == k) for k in (21, 31, 55)},
+            "workers": tuning["workers"] if tuning else 4, "learned_sizing": net is not None,
+            "workload": "64 read sets, genomes log-uniform 100 kbp-10 Mbp" + (f" x{scale}" if scale != 1 else "") +
+                        ", 100/150 bp SE/PE, 20-80x, 0.5-2% error, k 21/31/55; graph + unitigs per set"}
+
+
+def load_tuning():
+    from genomix_amd import gx
+    return gx.load_tuning()
+
+
+def learned_leg(steps, warmup):
+    """Learned-sizing workload: the same build with and without the network."""
+    from genomix_amd import devmem, gx, sim, sizer
+    spec = sim.LEARNED_100MBP
+    t0 = time.perf_counter()
+    text = sim.simulate(spec)
+    log(f"learned: {len(text) / 1e9:.2f} GB generated in {time.perf_counter() - t0:.1f} s")
+    d = devmem.DeviceBytes(text)
+    n = len(text)
+    del text
+    res = {}
+    for name, net in (("plain_sizing", None), ("learned_sizing", sizer.load())):
+        with gx.GraphBuilder(31, sizer=net) as b:
+            b.set_profiling(True)
+
+            def step():
+                b.reset()
+                b.add_reads_device(d.address, n)
+                b.finish()
+            el = _timed(step, steps, warmup, name)
+            st = b.stats()
+            res[name] = {"ms_per_step": round(1e3 * el / steps, 3), "value": round(st["occurrences"] * steps / el, 1),
+                         "table_capacity": st["table_capacity"], "table_bytes": st["table_capacity"] * 16,
+                         "predicted_distinct": st["predicted_distinct"], "distinct": st["distinct"],
+                         "table_rebuilds": st["table_rebuilds"], "phases_ms": b.phase_times()}
+    d.free()
+    return {"workload": "100 Mbp genome, 20% repeat families 1-10 kb at 2% divergence, 150 bp PE, 40x, "
+                        "1% error, k=31; graph build", "kmer_occurrences": st["occurrences"], **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-scale", type=float, default=1.0)
+    ap.add_argument("--workload", choices=["plain", "learned", "batched"], default="plain")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -83,25 +173,33 @@ def main():
                                    "DESIGN.md and no multi-GPU figure is produced", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import torch
-    from genomix_amd import gx, sim
+    from genomix_amd import devmem, gx, sim
+
+    if args.workload == "learned":
+        print(json.dumps({"metric": METRIC, "learned": learned_leg(max(1, args.steps // 2), 1)}), flush=True)
+        return
+    if args.workload == "batched":
+        print(json.dumps({"metric": METRIC, "batched": batched_leg(args.steps, args.warmup, args.batched_scale)}),
+              flush=True)
+        return
 
     spec = sim.PLAIN_GPU_5MBP
     k, L = 31, spec.read_len
     t0 = time.perf_counter()
     text = sim.simulate(spec)
     log(f"workload generated: {len(text)} bytes in {time.perf_counter() - t0:.1f} s")
-
-    dev = torch.device("cuda:0")
-    dtext = torch.frombuffer(bytearray(text), dtype=torch.uint8).to(dev)
-    torch.cuda.synchronize()
+    dtext = devmem.DeviceBytes(text)
 
     if args.copy_calibration:
-        a = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        import ctypes
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        h = devmem.hip()
+        assert h.hipMalloc(ctypes.byref(src), ctypes.c_size_t(1 << 30)) == 0
+        assert h.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(1 << 30)) == 0
+        assert h.hipMemcpy(dst, src, ctypes.c_size_t(1 << 30), 3) == 0  # device -> device
+        devmem.synchronize()
+        h.hipFree(src)
+        h.hipFree(dst)
         log("copy calibration: 1 GiB device copy done")
 
     builder = gx.GraphBuilder(k, edge_order="jdk7", device=0)
@@ -109,7 +207,7 @@ def main():
 
     def step():
         builder.reset()
-        builder.add_reads_device(dtext.data_ptr(), len(text))
+        builder.add_reads_device(dtext.address, len(text))
         builder.finish()
         builder.run_unitigs()
 
@@ -117,14 +215,14 @@ def main():
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
     st = builder.stats()
-    torch.cuda.synchronize()
+    devmem.synchronize()
     phases_sum = {}
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
         for kname, v in builder.phase_times().items():
             phases_sum[kname] = phases_sum.get(kname, 0.0) + v
-    torch.cuda.synchronize()
+    devmem.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
     ms_step = 1e3 * elapsed / max(args.steps, 1)
@@ -140,6 +238,16 @@ def main():
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "kernel": "k_insert", "algorithmic_bytes_per_kmer": round(B, 4),
                 "kernel_ms": round(phases.get("k_insert", 0.0), 4)}
+    unitigs = builder.unitig_count()
+    builder.close()
+    dtext.free()
+    batched = None
+    if not args.no_batched:
+        try:
+            batched = batched_leg(3, 1, args.batched_scale)
+            batched["vs_plain"] = round(batched["value"] / value, 3)
+        except Exception as e:  # the headline line is printed regardless
+            batched = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(text, k)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps,
@@ -150,10 +258,12 @@ def main():
                                "graph build to Node records + unitig path merge",
                    "k": k, "read_len": L, "read_pairs": text.count(b"\n"), "kmer_occurrences": N,
                    "distinct_kmers": st["distinct"], "replayed_nodes": st["replayed_nodes"],
-                   "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"], "max_coverage": st["max_coverage"],
-                   "unitigs": builder.unitig_count(), "edge_order": "jdk7", "parallelism": "single GPU"},
+                   "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"],
+                   "max_coverage": st["max_coverage"], "unitigs": unitigs, "edge_order": "jdk7",
+                   "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -6,9 +6,9 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgenomix_gx.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("gx_build.hip", "gx_sizer.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("gx_build.hip", "gx_sizer.hip", "gx_batch.hip")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("gx_prims.h", "gx_kmer.h", "gx_sizer.h")] + [
-    os.path.join(ROOT, "include", f) for f in ("genomix_gx.h", "genomix_sizer.h")]
+    os.path.join(ROOT, "include", f) for f in ("genomix_gx.h", "genomix_sizer.h", "genomix_batch.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

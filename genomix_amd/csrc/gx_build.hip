@@ -29,6 +29,7 @@
 #include "gx_kmer.h"
 #include "gx_prims.h"
 #include "gx_sizer.h"
+#include "../../include/genomix_batch.h"
 
 namespace gx {
 
@@ -352,9 +353,6 @@ constexpr uint64_t kPend = 1ull << 63;
 __device__ inline uint64_t ld_relaxed(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline uint32_t ld_relaxed32(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -386,20 +384,30 @@ __device__ inline uint64_t table_insert(void* table, uint64_t cap, KV key, uint6
     return ~0ull;
   } else {
     // Two-word keys without a 128-bit CAS: claim the slot by CAS-ing the
-    // high word from EMPTY to (hi | PENDING), store the low word, then
-    // publish hi with a release store.  Readers that see PENDING retry the
-    // same slot; the claiming lane publishes within the same loop
-    // iteration, so lanes of one wave never wait on each other.
+    // high word from EMPTY to (hi | PENDING), store the low word, wait for
+    // that store to complete, then publish hi.  Readers compare lo only
+    // after reading a published hi equal to the key's.  Readers that see
+    // PENDING retry the same slot; the claiming lane publishes within the
+    // same loop iteration, so lanes of one wave never wait on each other.
+    // A violated ordering could only show as one key in two slots, which
+    // the node pass checks for (k_dup_check) and reports as an error.
     Slot2* T = (Slot2*)table;
     uint64_t probe = 0;
     while (probe < limit) {
-      uint64_t cur = ld_acquire(&T[s].hi);
+      uint64_t cur = ld_relaxed(&T[s].hi);
       if (cur == kEmpty) {
         uint64_t prev = atomicCAS((unsigned long long*)&T[s].hi, (unsigned long long)kEmpty,
                                   (unsigned long long)(key.hi | kPend));
         if (prev == kEmpty) {
           __hip_atomic_store(&T[s].lo, key.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(&T[s].hi, key.hi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          // lo must be visible device-wide before hi loses PENDING.  Both
+          // are device-coherent (agent-scope atomic) stores, so waiting for
+          // the first one's completion orders them; an agent-scope release
+          // store would instead write back the whole L2 on every claim.
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __builtin_amdgcn_s_waitcnt(0);
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __hip_atomic_store(&T[s].hi, key.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           *claimed = true;
           return s;
         }
@@ -409,7 +417,12 @@ __device__ inline uint64_t table_insert(void* table, uint64_t cap, KV key, uint6
         probe++;  // bounded wait on a slot being published
         continue;
       }
-      if (cur == key.hi && ld_relaxed(&T[s].lo) == key.lo) return s;
+      if (cur == key.hi) {
+        // issued only after the published hi has been read (the compare
+        // needs it), so it sees the claimer's completed lo store
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (ld_relaxed(&T[s].lo) == key.lo) return s;
+      }
       s = slot_next(s, cap);
       probe++;
     }
@@ -671,6 +684,15 @@ __global__ void k_compact_list(const void* table, const uint32_t* __restrict__ c
 
 // Node arrays in output order.  The slot's count field is replaced by the
 // node's output rank once its count has been copied out.
+// Sorted node keys must be strictly increasing: one key in two table
+// slots would break the claim protocol's guarantee; report the first.
+template <int W>
+__global__ void k_dup_check(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
+                            unsigned long long* err) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < nn; i += (uint64_t)gridDim.x * blockDim.x)
+    if (klo[i] == klo[i - 1] && (W == 1 || khi[i] == khi[i - 1])) atomicMin(err, (unsigned long long)i);
+}
+
 template <int W>
 __global__ void k_gather_nodes(uint64_t nn, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ claimed,
                                void* table, uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
@@ -1700,6 +1722,8 @@ struct gx_build {
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
   // learned sizing
   gx_mlp* sizer = nullptr;
+  unsigned grid = 2048;  // blocks of grid-stride kernels (8 per CU)
+  float slack = 1.5f;    // learned table capacity / predicted distinct
   DevBuf hregs, sfeat, spred;
   float feat[GX_SIZER_FEATURES] = {};
   uint8_t regs_h[GX_SIZER_HLL_REGISTERS] = {};
@@ -1824,7 +1848,7 @@ static void run_build(gx_build* b) {
   const int k = b->cfg.k, nb = b->nb;
   const uint64_t n = b->src_len;
   const uint8_t* t = b->src;
-  const unsigned G = 2048;  // grid for grid-stride kernels (8 blocks per CU)
+  const unsigned G = b->grid;  // grid for grid-stride kernels
   unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
   GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
   GX_HIP(hipMemsetAsync(errw + 2, 0, 16, st));
@@ -1920,7 +1944,7 @@ static void run_build(gx_build* b) {
     double ly = std::isfinite(y[0]) ? std::min(0.0, std::max(-30.0, (double)y[0])) : 0.0;
     b->pred[0] = std::max(1.0, std::floor((double)N * std::exp(ly)));
     b->pred[1] = std::isfinite(y[1]) ? std::exp(std::min(10.0, std::max(-10.0, (double)y[1]))) : 1.0;
-    learned_cap = std::max<uint64_t>(1024, (uint64_t)std::ceil(1.5 * b->pred[0]));
+    learned_cap = std::max<uint64_t>(1024, (uint64_t)std::ceil((double)b->slack * b->pred[0]));
   }
   ph_end(b, PH_SIZING);
 
@@ -1945,7 +1969,9 @@ static void run_build(gx_build* b) {
     hipLaunchKernelGGL(k_table_clear<W>, dim3(grid_for(cap, kBlock, 65536)), dim3(kBlock), 0, st, table, cap);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
-    uint64_t limit = std::min<uint64_t>(cap, 1u << 16);
+    // probe bound: far above the longest probe of a table at load <= 0.7;
+    // a nearly full (under-sized) table fails fast and is rebuilt larger
+    uint64_t limit = std::min<uint64_t>(cap, 4096);
     if (nmates) {
       unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
       ph_begin(b, PH_KINSERT);
@@ -2000,6 +2026,12 @@ static void run_build(gx_build* b) {
     hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, claimed, table,
                        klo, khi, cnt, msk, nslot);
     GX_HIP(hipGetLastError());
+    GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
+    hipLaunchKernelGGL(k_dup_check<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, errw + 2);
+    GX_HIP(hipGetLastError());
+    unsigned long long dup = read1(st, errw + 2);
+    if (dup != ~0ull) throw GxError(GX_EDEVICE, "internal error: k-mer stored in two table slots (node " +
+                                                    std::to_string(dup) + ")");
   }
   ph_end(b, PH_NODES);
 
@@ -2184,7 +2216,7 @@ static void run_unitigs(gx_build* b) {
   hipStream_t st = b->st;
   const int k = b->cfg.k, nb = b->nb;
   const uint64_t nn = b->nnodes, cap = b->stats[3];
-  const unsigned G = 2048;
+  const unsigned G = b->grid;
   void* table = b->table.p;
   const uint64_t* klo = b->klo.as<uint64_t>();
   const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
@@ -2446,18 +2478,25 @@ static void fastq_convert(gx_build* b) {
 
 extern "C" {
 
+// Checks of a configuration (GenomixJobConf.java:417-423 for k); returns a
+// GX_* code and sets msg.
+static int check_cfg(const gx_build_cfg* cfg, std::string& msg) {
+  if (cfg->k < 3) { msg = "kmerLength must be at least 3!"; return GX_EINVAL; }
+  if (cfg->k > 63) { msg = "k > 63 is not supported"; return GX_EINVAL; }
+  if (cfg->max_readids_per_edge < 0) { msg = "maxReadIDsPerEdge must be non-negative!"; return GX_EINVAL; }
+  if (cfg->nparts < 1) { msg = "nparts must be >= 1"; return GX_EINVAL; }
+  if (cfg->nparts > 65536) { msg = "nparts above 65536 is not supported"; return GX_ENOTSUP; }
+  if (cfg->edge_order != GX_EDGE_ORDER_CANONICAL && cfg->edge_order != GX_EDGE_ORDER_JDK7) {
+    msg = "unknown edge_order";
+    return GX_EINVAL;
+  }
+  return GX_OK;
+}
+
 int gx_build_create(const gx_build_cfg* cfg, gx_build** out) {
   g_create_err.clear();
   if (!cfg || !out) { g_create_err = "null argument"; return GX_EINVAL; }
-  if (cfg->k < 3) { g_create_err = "kmerLength must be at least 3!"; return GX_EINVAL; }
-  if (cfg->k > 63) { g_create_err = "k > 63 is not supported"; return GX_EINVAL; }
-  if (cfg->max_readids_per_edge < 0) { g_create_err = "maxReadIDsPerEdge must be non-negative!"; return GX_EINVAL; }
-  if (cfg->nparts < 1) { g_create_err = "nparts must be >= 1"; return GX_EINVAL; }
-  if (cfg->nparts > 65536) { g_create_err = "nparts above 65536 is not supported"; return GX_ENOTSUP; }
-  if (cfg->edge_order != GX_EDGE_ORDER_CANONICAL && cfg->edge_order != GX_EDGE_ORDER_JDK7) {
-    g_create_err = "unknown edge_order";
-    return GX_EINVAL;
-  }
+  if (int rc = check_cfg(cfg, g_create_err)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     g_create_err = "no HIP device available: the graph build runs on the GPU only";
@@ -2622,6 +2661,35 @@ int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written) {
     if (b->u_bytes) GX_HIP(hipMemcpyAsync(dst, b->u_out.p, b->u_bytes, hipMemcpyDeviceToHost, b->st));
     GX_HIP(hipStreamSynchronize(b->st));
     if (written) *written = b->u_bytes;
+  });
+}
+
+int gx_build_set_tuning(gx_build* b, int grid_blocks, float table_slack) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (grid_blocks < 64 || grid_blocks > 65536) throw GxError(GX_EINVAL, "grid_blocks must be in [64, 65536]");
+    if (!(table_slack >= 1.05f && table_slack <= 16.f)) throw GxError(GX_EINVAL, "table_slack must be in [1.05, 16]");
+    b->grid = (unsigned)grid_blocks;
+    b->slack = table_slack;
+  });
+}
+
+int gx_build_set_config(gx_build* b, const gx_build_cfg* cfg) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!cfg) throw GxError(GX_EINVAL, "null argument");
+    std::string msg;
+    if (int rc = check_cfg(cfg, msg)) throw GxError(rc, msg);
+    if (cfg->device != b->cfg.device) throw GxError(GX_EINVAL, "a handle stays on its device");
+    b->cfg = *cfg;
+    b->W = cfg->k <= 31 ? 1 : 2;
+    b->nb = (cfg->k + 3) / 4;
+    b->text_len = b->text2_len = b->rtext_len = 0;
+    b->fmt = -1;
+    b->finished = false;
+    b->unitigs_done = false;
+    b->out_bytes = 0;
+    b->nnodes = 0;
   });
 }
 

@@ -31,7 +31,9 @@ EXPORTED = [
     "gx_build_add_fastq_pair", "gx_build_readids",
     "gx_mlp_create", "gx_mlp_num_params", "gx_mlp_set_params", "gx_mlp_get_params", "gx_mlp_forward",
     "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
-    "gx_build_features",
+    "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
+    "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
+    "gx_batch_destroy",
 ]
 
 
@@ -47,6 +49,7 @@ class _Cfg(ctypes.Structure):
 
 
 _lib = None
+BATCH_CONSUMER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
 
 
 def lib():
@@ -92,6 +95,17 @@ def lib():
         L.gx_mlp_destroy.argtypes = [vp]
         L.gx_build_set_sizer.argtypes = [vp, vp]
         L.gx_build_features.argtypes = [vp, fp, i32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_double)]
+        L.gx_build_set_tuning.argtypes = [vp, i32, ctypes.c_float]
+        L.gx_build_set_config.argtypes = [vp, ctypes.POINTER(_Cfg)]
+        L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
+        L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
+        L.gx_batch_set_sizer.argtypes = [vp, vp]
+        L.gx_batch_set_tuning.argtypes = [vp, i32, i32, u64p, ctypes.POINTER(ctypes.c_int), fp]
+        L.gx_batch_run.argtypes = [vp, i32, BATCH_CONSUMER, vp]
+        L.gx_batch_set_stats.argtypes = [vp, i32, u64p, i32, ctypes.POINTER(ctypes.c_double)]
+        L.gx_batch_last_error.argtypes = [vp]
+        L.gx_batch_last_error.restype = ctypes.c_char_p
+        L.gx_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -240,3 +254,136 @@ def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", 
         b.add_reads(text)
         b.finish()
         return [b.part_bytes(p) for p in range(nparts)]
+
+
+STAT_KEYS = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
+             "max_coverage", "predicted_distinct", "table_rebuilds"]
+
+
+class _View:
+    """The build handle a batch consumer receives (valid during the callback)."""
+
+    def __init__(self, h, nparts):
+        self._h = ctypes.c_void_p(h)
+        self.nparts = nparts
+
+    _check = GraphBuilder._check
+    part_info = GraphBuilder.part_info
+    part_bytes = GraphBuilder.part_bytes
+    stats = GraphBuilder.stats
+
+    def unitig_bytes(self):
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        w = ctypes.c_size_t()
+        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
+
+def load_tuning(path=os.path.join(HERE, "batch_tuning.json")):
+    """The autotuner's committed choice (genomix_amd/autotune.py), or None."""
+    if not os.path.exists(path):
+        return None
+    import json
+    with open(path) as f:
+        t = json.load(f)
+    return {k: t[k] for k in ("workers", "max_occ", "grid", "slack")}
+
+
+class GraphBatch:
+    """Many read sets (mixed k) built in one call by the native scheduler
+    (include/genomix_batch.h).  Inputs: bytes (host) or (device pointer,
+    length) pairs; they must stay alive until run() returns."""
+
+    def __init__(self, sets, device=0, sizer=None, tuning=None):
+        self._b = ctypes.c_void_p()
+        rc = lib().gx_batch_create(device, len(sets), ctypes.byref(self._b))
+        if rc != 0:
+            raise GxError(rc, "gx_batch_create failed")
+        self._keep = []
+        self.sets = sets
+        for i, s in enumerate(sets):
+            cfg = _Cfg(s["k"], s.get("nparts", 1), 250, EDGE_ORDER[s.get("edge_order", "jdk7")], device, 0)
+            data = s["input"]
+            if isinstance(data, tuple):
+                ptr, n, on_dev = ctypes.c_void_p(data[0]), data[1], 1
+            else:
+                buf = ctypes.create_string_buffer(data, len(data))
+                self._keep.append(buf)
+                ptr, n, on_dev = ctypes.cast(buf, ctypes.c_void_p), len(data), 0
+            self._check(lib().gx_batch_set(self._b, i, ctypes.byref(cfg), ptr, n, s.get("fmt", GX_FMT_READIDS),
+                                           on_dev))
+        self._sizer = sizer
+        if sizer is not None:
+            self._check(lib().gx_batch_set_sizer(self._b, sizer.handle))
+        if tuning is None:
+            tuning = load_tuning()
+        if tuning is not None:
+            self.set_tuning(**tuning)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GxError(rc, lib().gx_batch_last_error(self._b).decode(errors="replace"))
+
+    def set_tuning(self, workers, max_occ, grid, slack):
+        n = len(grid)
+        mo = (ctypes.c_uint64 * max(1, n - 1))(*max_occ)
+        g = (ctypes.c_int * n)(*grid)
+        sl = (ctypes.c_float * n)(*slack)
+        self._check(lib().gx_batch_set_tuning(self._b, workers, n, mo, g, sl))
+
+    def run(self, unitigs=True, collect=False, consumer=None):
+        """Builds every set.  collect=True returns {set: (parts, unitig bytes)};
+        consumer(set, view) is called on worker threads otherwise."""
+        import threading
+        out = {}
+        lock = threading.Lock()
+        errors = []
+
+        def cb(user, si, h):
+            try:
+                v = _View(h, self.sets[si].get("nparts", 1))
+                if collect:
+                    parts = [v.part_bytes(p) for p in range(v.nparts)]
+                    u = v.unitig_bytes() if unitigs else None
+                    with lock:
+                        out[si] = (parts, u)
+                elif consumer is not None:
+                    consumer(si, v)
+                return 0
+            except Exception as e:  # reported by the batch as a consumer failure
+                errors.append(e)
+                return -1
+
+        fn = BATCH_CONSUMER(cb) if (collect or consumer is not None) else BATCH_CONSUMER()
+        rc = lib().gx_batch_run(self._b, int(unitigs), fn, None)
+        if errors:
+            raise errors[0]
+        self._check(rc)
+        return out
+
+    def set_stats(self, i):
+        a = (ctypes.c_uint64 * 9)()
+        ms = ctypes.c_double()
+        self._check(lib().gx_batch_set_stats(self._b, i, a, 9, ctypes.byref(ms)))
+        d = dict(zip(STAT_KEYS, list(a)))
+        d["ms"] = ms.value
+        return d
+
+    def close(self):
+        if self._b:
+            lib().gx_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -138,3 +138,45 @@ PLAIN_CPU_50KBP = ReadSetSpec(genome_len=50_000, read_len=100, coverage=30, erro
 PLAIN_GPU_5MBP = ReadSetSpec(genome_len=5_000_000, read_len=150, coverage=50, error_rate=0.01, paired=True,
                              insert_mean=500, insert_sd=50, repeat_frac=0.05, repeat_len=2000, repeat_div=0.01,
                              seed=20261204)
+
+
+# Learned sizing applied: 100 Mbp genome, 20% repeat families of 1-10 kb at
+# 2% divergence, 150 bp PE, 40x, 1% error (built with k = 31).
+LEARNED_100MBP = ReadSetSpec(genome_len=100_000_000, read_len=150, coverage=40, error_rate=0.01, paired=True,
+                             insert_mean=500, insert_sd=50, repeat_frac=0.2, repeat_len=(1000, 10000),
+                             repeat_div=0.02, seed=20261207)
+
+
+def batched_sets(n=64, seed=20261206, scale=1.0):
+    """The batched workload: n independent read sets, genome sizes
+    log-uniform in [100 kbp, 10 Mbp] (times `scale`), 100 or 150 bp reads,
+    single or paired, 20-80x, 0.5-2% error, k drawn from 21/31/55.
+    Returns [(spec, k)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        G = int(np.exp(rng.uniform(np.log(1e5), np.log(1e7))) * scale)
+        L = int(rng.choice([100, 150]))
+        paired = bool(rng.random() < 0.5)
+        spec = ReadSetSpec(genome_len=max(G, 4 * L), read_len=L, coverage=float(rng.uniform(20, 80)),
+                           error_rate=float(rng.uniform(0.005, 0.02)), paired=paired,
+                           insert_mean=int(3.3 * L), insert_sd=L // 3, seed=seed + 1000 + i)
+        out.append((spec, int(rng.choice([21, 31, 55]))))
+    return out
+
+
+def _simulate_indexed(args):
+    i, spec = args
+    return i, simulate(spec)
+
+
+def simulate_many(specs, workers=12):
+    """simulate() of each spec, in worker processes; returns texts in order."""
+    import multiprocessing as mp
+    out = [None] * len(specs)
+    if workers <= 1:
+        return [simulate(s) for s in specs]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        for i, t in pool.imap_unordered(_simulate_indexed, list(enumerate(specs))):
+            out[i] = t
+    return out

@@ -1,0 +1,67 @@
+/* genomix_batch.h — many read sets, mixed k, one call.
+ *
+ * No counterpart in the reference: Genomix runs one Hyracks job per read
+ * set.  These entry points are for callers that build many sets at once
+ * (one pipeline run over a sample collection): a native scheduler hands the
+ * sets, largest first, to worker threads, each driving one build handle on
+ * its own HIP stream of the same GPU, so small sets and host-side steps of
+ * one set overlap device work of the others.  Each set's output is exactly
+ * what a separate gx_build handle would produce for it.
+ *
+ * All functions return GX_OK (0) or a negative GX_* code (genomix_gx.h).
+ */
+#ifndef GENOMIX_BATCH_H
+#define GENOMIX_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "genomix_gx.h"
+#include "genomix_sizer.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-handle launch and sizing knobs (defaults: 2048 blocks for grid-stride
+ * kernels, learned table = 1.5 x predicted distinct). */
+int gx_build_set_tuning(gx_build* b, int grid_blocks, float table_slack);
+
+/* Re-targets an existing handle to another configuration (k, nparts, edge
+ * order; same device), keeping its device buffers: a handle reused across
+ * read sets of different k. Clears any input added so far. */
+int gx_build_set_config(gx_build* b, const gx_build_cfg* cfg);
+
+typedef struct gx_batch gx_batch;
+
+/* Called on a worker thread once set `set` is built (and merged, if asked):
+ * the handle holds its records until the callback returns.  Callbacks of
+ * different workers run concurrently.  A non-zero return aborts the batch. */
+typedef int (*gx_batch_consumer)(void* user, int set, gx_build* handle);
+
+int gx_batch_create(int device, int nsets, gx_batch** out);
+/* Configuration and input of one set.  The input stays owned by the caller
+ * and must stay valid until gx_batch_run returns (on_device = 1: a device
+ * pointer on the batch's device). */
+int gx_batch_set(gx_batch* b, int set, const gx_build_cfg* cfg, const void* input, size_t len, int fmt,
+                 int on_device);
+int gx_batch_set_sizer(gx_batch* b, gx_mlp* m);
+/* Size classes by k-mer occurrences: class c holds sets with at most
+ * max_occ[c] occurrences (the last class takes the rest); each class has
+ * its grid and table slack.  workers = worker threads / streams (1..8). */
+int gx_batch_set_tuning(gx_batch* b, int workers, int nclasses, const uint64_t* max_occ, const int* grid_blocks,
+                        const float* table_slack);
+/* Builds every set (and runs unitig compression when unitigs != 0),
+ * calling consumer (may be NULL) after each. */
+int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* user);
+/* Per-set statistics of the last run (gx_build_stats layout, n <= 9) and
+ * its device time in milliseconds (finish + unitigs, host clock). */
+int gx_batch_set_stats(gx_batch* b, int set, uint64_t* out, int n, double* ms);
+const char* gx_batch_last_error(const gx_batch* b);
+void gx_batch_destroy(gx_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENOMIX_BATCH_H */
