@@ -430,6 +430,42 @@ __device__ inline uint64_t table_insert(void* table, uint64_t cap, KV key, uint6
   }
 }
 
+constexpr uint32_t kEnd = 0xffffffffu;
+
+// Read-only lookup of a key that is known to be in the table.
+template <int W>
+__device__ inline uint64_t table_find(const void* table, uint64_t cap, KV key) {
+  uint64_t s = slot_home(khash<W>(key), cap);
+  for (uint64_t probe = 0; probe < cap; probe++) {
+    if (W == 1) {
+      uint64_t cur = ((const Slot1*)table)[s].key;
+      if (cur == key.lo) return s;
+      if (cur == kEmpty) return ~0ull;
+    } else {
+      uint64_t cur = ((const Slot2*)table)[s].hi;
+      if (cur == key.hi && ((const Slot2*)table)[s].lo == key.lo) return s;
+      if (cur == kEmpty) return ~0ull;
+    }
+    s = slot_next(s, cap);
+  }
+  return ~0ull;
+}
+
+template <int W>
+__device__ inline uint32_t node_of_key(const void* table, uint64_t cap, KV key) {
+  uint64_t s = table_find<W>(table, cap, key);
+  if (s == ~0ull) return kEnd;
+  return W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
+}
+
+template <int W>
+__device__ inline KV node_key(const uint64_t* klo, const uint64_t* khi, uint64_t i) {
+  KV v;
+  v.lo = klo[i];
+  v.hi = W == 2 ? khi[i] : 0;
+  return v;
+}
+
 template <int W>
 __device__ inline KV load_kmer(const uint64_t* __restrict__ w, uint32_t p, int k) {
   // bits [2p, 2p+2k) of the packed stream
@@ -474,6 +510,50 @@ __device__ inline uint32_t nibble(uint32_t t, uint32_t x) {
   return 8u | ((t == ET_FR || t == ET_RF) ? 4u : 0u) | x;
 }
 
+// Canonical key of the k-mer at position p of a mate (L bases, packed
+// words w), its occurrence code and its neighbour bits (bit 4t + x).
+template <int W>
+__device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32_t p, int k, uint32_t* code_out,
+                              uint32_t* bits_out) {
+  KV v = load_kmer<W>(w, p, k);
+  bool fwd;
+  KV key = canon<W>(v, k, &fwd);
+  uint32_t code = 0, bits = 0;
+  if (p + k < L) {  // the read's next k-mer w' = w[1:]+c
+    uint32_t c = base_at(w, p + k);
+    KV nx = succ_of<W>(v, k, c);
+    bool nfwd = kle<W>(nx, revcomp<W>(nx, k));
+    // (cur F, next F) -> FF w'; (F, R) -> FR rc(w'); (R, F) -> RF w';
+    // (R, R) -> RR rc(w')   (setEdgesForCurAndNext, cur side)
+    uint32_t t = fwd ? (nfwd ? ET_FF : ET_FR) : (nfwd ? ET_RF : ET_RR);
+    uint32_t x = fwd ? c : 3 - c;  // letter relative to the canonical node
+    code |= nibble(t, x) << (fwd ? 0 : 4);
+    bits |= 1u << (4 * t + x);
+  }
+  if (p > 0) {  // the read's previous k-mer w'' = c+w[:-1]
+    uint32_t c = base_at(w, p - 1);
+    KV pv = pred_of<W>(v, k, c);
+    bool pfwd = kle<W>(pv, revcomp<W>(pv, k));
+    // (prev F, cur F) -> RR w''; (F, R) -> FR w''; (R, F) -> RF rc(w'');
+    // (R, R) -> FF rc(w'')   (setEdgesForCurAndNext, next side)
+    uint32_t t = fwd ? (pfwd ? ET_RR : ET_RF) : (pfwd ? ET_FR : ET_FF);
+    uint32_t x = fwd ? c : 3 - c;
+    code |= nibble(t, x) << (fwd ? 4 : 0);
+    bits |= 1u << (4 * t + x);
+  }
+  *code_out = code;
+  *bits_out = bits;
+  return key;
+}
+
+// Neighbour bits of an occurrence code (the inverse of the nibble packing).
+__device__ inline uint32_t code_bits(uint32_t code) {
+  uint32_t bits = 0, sn = code & 0xF, pn = (code >> 4) & 0xF;
+  if (sn & 8) bits |= 1u << (4 * ((sn & 4) ? ET_FR : ET_FF) + (sn & 3));
+  if (pn & 8) bits |= 1u << (4 * ((pn & 4) ? ET_RF : ET_RR) + (pn & 3));
+  return bits;
+}
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32_t* __restrict__ mlen,
                                                    const uint64_t* __restrict__ wbase,
@@ -492,32 +572,8 @@ __global__ void __launch_bounds__(kBlock) k_insert(uint64_t nmates, const uint32
     const uint64_t* w = words + wbase[m];
     uint64_t ob = occbase[m];
     for (uint32_t p = lane; p < nk; p += 64) {
-      KV v = load_kmer<W>(w, p, k);
-      bool fwd;
-      KV key = canon<W>(v, k, &fwd);
-      uint32_t code = 0, bits = 0;
-      if (p + k < L) {  // the read's next k-mer w' = w[1:]+c
-        uint32_t c = base_at(w, p + k);
-        KV nx = succ_of<W>(v, k, c);
-        bool nfwd = kle<W>(nx, revcomp<W>(nx, k));
-        // (cur F, next F) -> FF w'; (F, R) -> FR rc(w'); (R, F) -> RF w';
-        // (R, R) -> RR rc(w')   (setEdgesForCurAndNext, cur side)
-        uint32_t t = fwd ? (nfwd ? ET_FF : ET_FR) : (nfwd ? ET_RF : ET_RR);
-        uint32_t x = fwd ? c : 3 - c;  // letter relative to the canonical node
-        code |= nibble(t, x) << (fwd ? 0 : 4);
-        bits |= 1u << (4 * t + x);
-      }
-      if (p > 0) {  // the read's previous k-mer w'' = c+w[:-1]
-        uint32_t c = base_at(w, p - 1);
-        KV pv = pred_of<W>(v, k, c);
-        bool pfwd = kle<W>(pv, revcomp<W>(pv, k));
-        // (prev F, cur F) -> RR w''; (F, R) -> FR w''; (R, F) -> RF rc(w'');
-        // (R, R) -> FF rc(w'')   (setEdgesForCurAndNext, next side)
-        uint32_t t = fwd ? (pfwd ? ET_RR : ET_RF) : (pfwd ? ET_FR : ET_FF);
-        uint32_t x = fwd ? c : 3 - c;
-        code |= nibble(t, x) << (fwd ? 4 : 0);
-        bits |= 1u << (4 * t + x);
-      }
+      uint32_t code, bits;
+      KV key = occ_info<W>(w, L, p, k, &code, &bits);
       bool claimed;
       uint64_t s = table_insert<W>(table, cap, key, limit, &claimed);
       (void)claimed;
@@ -684,6 +740,497 @@ __global__ void k_compact_list(const void* table, const uint32_t* __restrict__ c
 
 // Node arrays in output order.  The slot's count field is replaced by the
 // node's output rank once its count has been copied out.
+// ---------------------------------------------------------------------------
+// 4'-5'. Sort-partitioned aggregation (the default method).  Every k-mer
+//    occurrence becomes a record (KmerPointable sort key, occurrence index
+//    << 8 | code); the records are sorted by (key, occurrence): two stable
+//    8-bit radix passes on the top 16 key bits -- the first fused with the
+//    k-mer extraction -- then a bitonic sort of each 16-bit bucket in LDS
+//    (buckets above kBucketCap records go through a global radix sort).
+//    Nodes are the runs of equal keys, already in output order; a run's
+//    codes are its occurrences' records in emission order, as the JDK-7
+//    replay needs them.  All HBM traffic is streaming; there is no
+//    random-access table in the aggregation.
+// ---------------------------------------------------------------------------
+
+template <int W> struct SortKeyOf { using T = uint64_t; };
+template <> struct SortKeyOf<2> { using T = Key128; };
+
+__device__ inline void make_sort_key(KV v, int nb, uint64_t& out) { out = sortkey64(v.lo, nb); }
+__device__ inline void make_sort_key(KV v, int nb, Key128& out) { out = sortkey128(v, nb); }
+
+// inverse of make_sort_key
+__device__ inline KV key_of_sort(uint64_t sk, int nb) {
+  KV v;
+  v.lo = __builtin_bswap64(sk << (64 - 8 * nb));
+  v.hi = 0;
+  return v;
+}
+__device__ inline KV key_of_sort(const Key128& r, int nb) {
+  int sh = 128 - 8 * nb;  // 0..64 for two-word keys
+  uint64_t lo, hi;
+  if (sh == 0) { lo = r.lo; hi = r.hi; }
+  else if (sh >= 64) { hi = r.lo << (sh - 64); lo = 0; }
+  else { hi = (r.hi << sh) | (r.lo >> (64 - sh)); lo = r.lo << sh; }
+  KV v;
+  v.lo = __builtin_bswap64(hi);
+  v.hi = __builtin_bswap64(lo);
+  return v;
+}
+
+__device__ inline bool sk_eq(uint64_t a, uint64_t b) { return a == b; }
+__device__ inline bool sk_eq(const Key128& a, const Key128& b) { return a.lo == b.lo && a.hi == b.hi; }
+
+// 16-bit bucket of a sort key: its top two bytes (nb >= 2), or its only byte.
+template <class SK>
+__device__ inline uint32_t bucket_of_key(const SK& sk, int nb) {
+  return nb >= 2 ? (key_byte(sk, nb - 1) << 8) | key_byte(sk, nb - 2) : key_byte(sk, 0);
+}
+
+// Key -> node index, built from the sorted nodes without global atomics:
+// the nodes of each 16-bit key bucket are contiguous (dir[b] = first node),
+// and each bucket owns a private open-addressing region of two slots per
+// node, filled in LDS by one block.  A lookup is the bucket's two dir
+// entries plus one or two slot probes.
+struct NodeIndex {
+  const uint32_t* dir;  // nbk + 1 entries
+  const void* slots;    // Slot1 / Slot2, 2 per node; count = node id
+  int nb;
+  uint32_t nbk;
+};
+
+template <int W>
+__device__ inline uint32_t node_bucket(KV key, int nb) {
+  typename SortKeyOf<W>::T sk;
+  make_sort_key(key, nb, sk);
+  return bucket_of_key(sk, nb);
+}
+
+template <int W>
+__device__ inline uint32_t node_lookup(const NodeIndex& ix, KV key) {
+  uint32_t b = node_bucket<W>(key, ix.nb);
+  uint32_t lo = ix.dir[b], hi = ix.dir[b + 1];
+  if (lo >= hi) return kEnd;
+  const uint64_t R = 2ull * (hi - lo), base = 2ull * lo;
+  uint64_t s = slot_home(khash<W>(key), R);
+  for (uint64_t probe = 0; probe < R; probe++) {
+    if (W == 1) {
+      const Slot1& e = ((const Slot1*)ix.slots)[base + s];
+      if (e.key == key.lo) return e.count;
+      if (e.key == kEmpty) return kEnd;
+    } else {
+      const Slot2& e = ((const Slot2*)ix.slots)[base + s];
+      if (e.hi == key.hi && e.lo == key.lo) return e.count;
+      if (e.hi == kEmpty) return kEnd;
+    }
+    s = s + 1 == R ? 0 : s + 1;
+  }
+  return kEnd;
+}
+
+template <int W>
+__global__ void k_nix_dir(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi, int nb,
+                          uint32_t nbk, uint32_t* __restrict__ dir) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= nn; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = j == 0 ? 0 : node_bucket<W>(node_key<W>(klo, khi, j - 1), nb) + 1;
+    uint32_t hi = j == nn ? nbk : node_bucket<W>(node_key<W>(klo, khi, j), nb);
+    for (uint32_t b = lo; b <= hi && b <= nbk; b++) dir[b] = (uint32_t)j;
+  }
+}
+
+constexpr int kNixLds = 4096;  // slots of one bucket region built in LDS
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
+                                                      const uint32_t* __restrict__ dir, void* slots) {
+  using S = typename std::conditional<W == 1, Slot1, Slot2>::type;
+  __shared__ S lt[kNixLds];
+  const uint32_t lo = dir[blockIdx.x], hi = dir[blockIdx.x + 1];
+  if (lo >= hi) return;
+  const uint64_t R = 2ull * (hi - lo), base = 2ull * lo;
+  const bool local = R <= kNixLds;
+  S* T = local ? lt : (S*)slots + base;
+  for (uint64_t i = threadIdx.x; i < R; i += kBlock) {
+    S e{};
+    if (W == 1) ((Slot1*)&e)->key = kEmpty;
+    else ((Slot2*)&e)->hi = kEmpty;
+    T[i] = e;
+  }
+  __syncthreads();
+  for (uint64_t n = lo + threadIdx.x; n < hi; n += kBlock) {
+    KV key = node_key<W>(klo, khi, n);
+    uint64_t s = slot_home(khash<W>(key), R);
+    for (uint64_t probe = 0; probe < R; probe++) {
+      if (W == 1) {
+        Slot1* e = (Slot1*)&T[s];
+        if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmpty, (unsigned long long)key.lo) ==
+            kEmpty) {
+          e->count = (uint32_t)n;
+          break;
+        }
+      } else {  // distinct keys: a taken slot always holds another key
+        Slot2* e = (Slot2*)&T[s];
+        if (atomicCAS((unsigned long long*)&e->hi, (unsigned long long)kEmpty, (unsigned long long)key.hi) ==
+            kEmpty) {
+          e->lo = key.lo;
+          e->count = (uint32_t)n;
+          break;
+        }
+      }
+      s = s + 1 == R ? 0 : s + 1;
+    }
+  }
+  __syncthreads();
+  if (local)
+    for (uint64_t i = threadIdx.x; i < R; i += kBlock) ((S*)slots)[base + i] = lt[i];
+}
+
+constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
+constexpr int kIdxBits = 13;      // log2(kBucketCap)
+
+// The bucket bytes are sorted least significant first (LSD): the fused
+// extraction pass scatters by byte nb-2, the radix pass after it by byte
+// nb-1; a one-byte key has a single pass on byte 0.
+__device__ __host__ inline int first_digit_byte(int nb) { return nb >= 2 ? nb - 2 : 0; }
+constexpr int kTileMateCap = 1024;
+
+// Mate holding occurrence j: the largest m in [lo, hi] with occbase[m] <= j.
+__device__ inline uint64_t mate_search(const uint64_t* __restrict__ ob, uint64_t lo, uint64_t hi, uint64_t j) {
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo + 1) / 2;
+    if (ob[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Occurrences [j0, j1) of a tile: finds their mates (a table of occbase in
+// LDS when the mate range is short) and calls f(index in tile, j, key, code)
+// for each, in rounds of kBlock consecutive occurrences.
+template <int W, class F>
+__device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t nmates,
+                                            const uint64_t* __restrict__ occbase, const uint32_t* __restrict__ mlen,
+                                            const uint64_t* __restrict__ wbase, const uint64_t* __restrict__ words,
+                                            int k, uint64_t* lob, F&& f) {
+  __shared__ uint64_t mrange[2];
+  if (threadIdx.x == 0) {
+    mrange[0] = mate_search(occbase, 0, nmates - 1, j0);
+    mrange[1] = mate_search(occbase, mrange[0], nmates - 1, j1 - 1);
+  }
+  __syncthreads();
+  const uint64_t m0 = mrange[0], m1 = mrange[1];
+  const bool local = m1 - m0 + 2 <= kTileMateCap;
+  if (local)
+    for (uint64_t i = threadIdx.x; i <= m1 - m0 + 1; i += kBlock) lob[i] = occbase[m0 + i];
+  __syncthreads();
+  for (int r = 0; r < kSortItems; r++) {
+    uint64_t j = j0 + (uint64_t)r * kBlock + threadIdx.x;
+    bool valid = j < j1;
+    KV key{0, 0};
+    uint32_t code = 0;
+    if (valid) {
+      uint64_t m = local ? m0 + mate_search(lob, 0, m1 - m0, j) : mate_search(occbase, m0, m1, j);
+      uint64_t mb = local ? lob[m - m0] : occbase[m];
+      uint32_t bits;
+      key = occ_info<W>(words + wbase[m], mlen[m], (uint32_t)(j - mb), k, &code, &bits);
+    }
+    f(r, valid, j, key, code);
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates, const uint64_t* __restrict__ occbase,
+                                                     const uint32_t* __restrict__ mlen,
+                                                     const uint64_t* __restrict__ wbase,
+                                                     const uint64_t* __restrict__ words, int k, int nb,
+                                                     uint32_t* __restrict__ hist, unsigned ntiles) {
+  using SK = typename SortKeyOf<W>::T;
+  __shared__ uint32_t h[256];
+  __shared__ uint64_t lob[kTileMateCap];
+  h[threadIdx.x] = 0;
+  uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
+  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
+                          [&](int, bool valid, uint64_t, KV key, uint32_t) {
+                            if (valid) {
+                              SK sk;
+                              make_sort_key(key, nb, sk);
+                              atomicAdd(&h[key_byte(sk, first_digit_byte(nb))], 1u);
+                            }
+                          });
+  __syncthreads();
+  hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of the tile's records by the lower bucket byte (same
+// ranking as k_radix_scatter: wave match + lower waves' counts).
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nmates,
+                                                        const uint64_t* __restrict__ occbase,
+                                                        const uint32_t* __restrict__ mlen,
+                                                        const uint64_t* __restrict__ wbase,
+                                                        const uint64_t* __restrict__ words, int k, int nb,
+                                                        const uint64_t* __restrict__ offs, unsigned ntiles,
+                                                        typename SortKeyOf<W>::T* __restrict__ okeys,
+                                                        uint64_t* __restrict__ ovals) {
+  using SK = typename SortKeyOf<W>::T;
+  __shared__ uint64_t run[256];
+  __shared__ uint16_t wcnt[kBlock / 64][256];
+  __shared__ uint64_t lob[kTileMateCap];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
+  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
+                          [&](int, bool valid, uint64_t j, KV key, uint32_t code) {
+    for (int w = 0; w < kBlock / 64; w++) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    SK sk{};
+    uint32_t d = 0;
+    if (valid) {
+      make_sort_key(key, nb, sk);
+      d = key_byte(sk, first_digit_byte(nb));
+    }
+    uint64_t act = __ballot(valid);
+    uint64_t match = act;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot((d >> b) & 1);
+      match &= ((d >> b) & 1) ? m : ~m;
+    }
+    uint32_t rank = __popcll(match & lt);
+    if (valid && rank == 0) wcnt[wid][d] = (uint16_t)__popcll(match);
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = run[d] + rank;
+      for (int w = 0; w < wid; w++) pos += wcnt[w][d];
+      okeys[pos] = sk;
+      ovals[pos] = (j << 8) | code;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; w++) tot += wcnt[w][threadIdx.x];
+    run[threadIdx.x] += tot;
+    __syncthreads();
+  });
+}
+
+// First record of every bucket (start[NB] = N; empty buckets start where
+// the next one does).
+template <class SK>
+__global__ void k_bucket_starts(uint64_t N, const SK* __restrict__ keys, int nb, uint32_t NB,
+                                uint64_t* __restrict__ start) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= N; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = j == 0 ? 0 : bucket_of_key(keys[j - 1], nb) + 1;
+    uint32_t hi = j == N ? NB : bucket_of_key(keys[j], nb);
+    for (uint32_t b = lo; b <= hi && b <= NB; b++) start[b] = j;
+  }
+}
+
+// In-bucket sort element: the key bits below the bucket, then the record's
+// index in the bucket (so ties keep the occurrence order).
+__device__ inline uint64_t bucket_elem(uint64_t sk, int nb, uint32_t idx) {
+  int bits = 8 * nb - 16;  // <= 48
+  uint64_t low = bits >= 64 ? sk : (sk & ((1ull << bits) - 1));
+  return (low << kIdxBits) | idx;
+}
+__device__ inline Key128 bucket_elem(const Key128& sk, int nb, uint32_t idx) {
+  int bits = 8 * nb - 16;  // 48..112
+  Key128 low = sk;
+  if (bits < 64) { low.lo &= (1ull << bits) - 1; low.hi = 0; }
+  else if (bits < 128) low.hi &= (bits == 64) ? 0ull : ((1ull << (bits - 64)) - 1);
+  Key128 r;
+  r.hi = (low.hi << kIdxBits) | (low.lo >> (64 - kIdxBits));
+  r.lo = (low.lo << kIdxBits) | idx;
+  return r;
+}
+__device__ inline uint32_t elem_idx(uint64_t e) { return (uint32_t)(e & (kBucketCap - 1)); }
+__device__ inline uint32_t elem_idx(const Key128& e) { return (uint32_t)(e.lo & (kBucketCap - 1)); }
+__device__ inline bool elem_gt(uint64_t a, uint64_t b) { return a > b; }
+__device__ inline bool elem_gt(const Key128& a, const Key128& b) { return a.hi > b.hi || (a.hi == b.hi && a.lo > b.lo); }
+__device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
+__device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
+
+// One block per bucket: bitonic sort of (key below the bucket, index) in
+// LDS, then the records are written in that order.  Oversized buckets are
+// copied unsorted and flagged for the global pass.
+// Two tiers: CAP = kBucketCap / 2 over every bucket (more blocks per CU),
+// leaving buckets above it marked 2 for the CAP = kBucketCap launch; that
+// one marks buckets above kBucketCap 1 for the global pass.
+template <int W, int CAP>
+__global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
+                                                        const uint64_t* __restrict__ iv,
+                                                        typename SortKeyOf<W>::T* __restrict__ ok,
+                                                        uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart,
+                                                        uint64_t N, int nb, uint32_t* __restrict__ big,
+                                                        unsigned long long* err) {
+  using SK = typename SortKeyOf<W>::T;
+  using E = SK;  // uint64_t or Key128
+  __shared__ E s[CAP];
+  const bool second = CAP == kBucketCap;
+  if (second && big[blockIdx.x] != 2) return;
+  const uint64_t a = bstart[blockIdx.x], e = bstart[blockIdx.x + 1];
+  if (a > e || e > N) {  // bucket bounds must be ordered: report, touch nothing
+    if (threadIdx.x == 0) atomicMin(err, (unsigned long long)blockIdx.x);
+    return;
+  }
+  const uint32_t n = (uint32_t)(e - a);
+  if (n == 0) return;
+  if (!second && n > CAP) {
+    if (threadIdx.x == 0) big[blockIdx.x] = n > kBucketCap ? 1 : 2;
+    if (n <= kBucketCap) return;  // sorted by the second tier
+  }
+  if (n > kBucketCap || n == 1 || nb < 2) {
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+      ok[a + i] = ik[a + i];
+      ov[a + i] = iv[a + i];
+    }
+    return;
+  }
+  uint32_t P = 2;
+  while (P < n) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += kBlock) {
+    E x;
+    if (i < n) x = bucket_elem(ik[a + i], nb, i);
+    else elem_max(x);
+    s[i] = x;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= P; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < P / 2; t += kBlock) {
+        uint32_t i = (t / stride) * stride * 2 + (t % stride), j = i + stride;
+        bool up = (i & size) == 0;
+        E x = s[i], y = s[j];
+        if (elem_gt(x, y) == up) {
+          s[i] = y;
+          s[j] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    uint32_t src = elem_idx(s[i]);
+    ok[a + i] = ik[a + src];
+    ov[a + i] = iv[a + src];
+  }
+}
+
+// Oversized buckets to and from a contiguous array (dir 0: gather, 1: back).
+template <class SK>
+__global__ void k_big_move(const uint64_t* __restrict__ bstart, const uint32_t* __restrict__ big,
+                           const uint64_t* __restrict__ boff, SK* __restrict__ keys, uint64_t* __restrict__ vals,
+                           SK* __restrict__ tk, uint64_t* __restrict__ tv, int dir) {
+  if (big[blockIdx.x] != 1) return;
+  const uint64_t a = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - a, o = boff[blockIdx.x];
+  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    if (dir == 0) {
+      tk[o + i] = keys[a + i];
+      tv[o + i] = vals[a + i];
+    } else {
+      keys[a + i] = tk[o + i];
+      vals[a + i] = tv[o + i];
+    }
+  }
+}
+
+__global__ void k_big_sizes(uint32_t NB, const uint64_t* __restrict__ bstart, const uint32_t* __restrict__ big,
+                            uint64_t* __restrict__ sz) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < NB; b += gridDim.x * blockDim.x)
+    sz[b] = (big[b] == 1 && bstart[b + 1] >= bstart[b]) ? bstart[b + 1] - bstart[b] : 0;
+}
+
+template <class SK>
+__global__ void k_run_flags(uint64_t N, const SK* __restrict__ keys, uint32_t* __restrict__ flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x)
+    flag[i] = (i == 0 || !sk_eq(keys[i], keys[i - 1])) ? 1u : 0u;
+}
+
+// Run starts -> nodes (first record, key), and each node's neighbour mask:
+// the OR of its records' code bits, gathered per block in LDS (a block's
+// 256 records touch at most 256 consecutive nodes); the first and last node
+// of a block may continue in a neighbouring block and are merged with a
+// global atomicOr, the others stored directly.  msk must start at zero.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename SortKeyOf<W>::T* __restrict__ keys,
+                                                      const uint64_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ flag,
+                                                      const uint64_t* __restrict__ rpos, int nb,
+                                                      uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
+                                                      uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk) {
+  __shared__ uint32_t lm[kBlock];
+  __shared__ uint64_t nfirst, nlast;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < N; base += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t last = base + kBlock < N ? base + kBlock - 1 : N - 1;
+    lm[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      nfirst = rpos[base + 1] - 1;
+      nlast = rpos[last + 1] - 1;
+    }
+    __syncthreads();
+    if (i < N) {
+      uint64_t n = rpos[i + 1] - 1;
+      if (flag[i]) {
+        nstart[n] = i;
+        KV v = key_of_sort(keys[i], nb);
+        klo[n] = v.lo;
+        if (W == 2) khi[n] = v.hi;
+      }
+      atomicOr(&lm[n - nfirst], code_bits((uint32_t)(vals[i] & 0xff)));
+    }
+    __syncthreads();
+    uint64_t n = nfirst + threadIdx.x;
+    if (n <= nlast) {
+      uint32_t bits = lm[threadIdx.x];
+      if (n == nfirst || n == nlast) atomicOr(&msk[n], bits);
+      else msk[n] = bits;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+__global__ void k_node_cnt(uint64_t nn, const uint64_t* __restrict__ nstart, uint32_t* __restrict__ cnt) {
+  for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nn; n += (uint64_t)gridDim.x * blockDim.x)
+    cnt[n] = (uint32_t)(nstart[n + 1] - nstart[n]);
+}
+
+// Key -> node lookup table (the unitig pass and the read heads find nodes
+// by key): distinct keys, so a failed claim always means another key.
+template <int W>
+__global__ void k_node_table(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
+                             void* table, uint64_t cap, unsigned long long* err) {
+  for (uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < nn; n += (uint64_t)gridDim.x * blockDim.x) {
+    KV key = node_key<W>(klo, khi, n);
+    uint64_t s = slot_home(khash<W>(key), cap);
+    bool done = false;
+    for (uint64_t probe = 0; probe < cap && !done; probe++) {
+      if (W == 1) {
+        Slot1* T = (Slot1*)table;
+        if (atomicCAS((unsigned long long*)&T[s].key, (unsigned long long)kEmpty, (unsigned long long)key.lo) ==
+            kEmpty) {
+          T[s].count = (uint32_t)n;
+          done = true;
+        }
+      } else {
+        Slot2* T = (Slot2*)table;
+        if (atomicCAS((unsigned long long*)&T[s].hi, (unsigned long long)kEmpty, (unsigned long long)key.hi) ==
+            kEmpty) {
+          T[s].lo = key.lo;
+          T[s].count = (uint32_t)n;
+          done = true;
+        }
+      }
+      s = slot_next(s, cap);
+    }
+    if (!done) report(err, ~0ull >> 8, ERR_TABLE);
+  }
+}
+
 // Sorted node keys must be strictly increasing: one key in two table
 // slots would break the claim protocol's guarantee; report the first.
 template <int W>
@@ -712,13 +1259,6 @@ __global__ void k_gather_nodes(uint64_t nn, const uint32_t* __restrict__ perm, c
   }
 }
 
-template <int W>
-__device__ inline KV node_key(const uint64_t* klo, const uint64_t* khi, uint64_t i) {
-  KV v;
-  v.lo = klo[i];
-  v.hi = W == 2 ? khi[i] : 0;
-  return v;
-}
 
 __device__ inline void type_sizes(uint32_t mask, int n[4]) {
   for (int t = 0; t < 4; t++) n[t] = __popc((mask >> (4 * t)) & 0xF);
@@ -739,7 +1279,7 @@ __global__ void k_node_info(uint64_t nn, const uint32_t* __restrict__ msk, const
     bool need = jdk7 && (n[0] > 1 || n[1] > 1 || n[2] > 1 || n[3] > 1);
     needs[i] = need ? 1 : 0;
     selcnt[i] = need ? cnt[i] : 0;
-    if (need) {
+    if (need && table) {
       uint32_t s = nslot[i];
       uint32_t* m = W == 1 ? &((Slot1*)table)[s].mask : &((Slot2*)table)[s].mask;
       *m |= kNeedsBit;
@@ -872,9 +1412,9 @@ __device__ inline LList j7_union(LList L, int sym, uint32_t hb) {
 // symbol class) -- a new letter, a letter already present, or none -- so
 // the last two transitions of each list are cached.  Codes are loaded four
 // at a time.
-template <int W>
+template <int W, class CodeT>
 __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, const uint64_t* __restrict__ seg,
-                         const uint32_t* __restrict__ codes, const uint64_t* __restrict__ klo,
+                         const CodeT* __restrict__ codes, const uint64_t* __restrict__ klo,
                          const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk, int k, int nb,
                          uint64_t* __restrict__ lists) {
   for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrep; q += (uint64_t)gridDim.x * blockDim.x) {
@@ -902,7 +1442,7 @@ __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, cons
     for (uint64_t r0 = a; r0 < e; r0 += 4) {
       uint32_t c4[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) c4[j] = (r0 + j < e) ? codes[r0 + j] : 0;
+      for (int j = 0; j < 4; j++) c4[j] = (r0 + j < e) ? (uint32_t)(codes[r0 + j] & 0xff) : 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (r0 + j >= e) break;
@@ -956,17 +1496,15 @@ __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, cons
 
 template <int W>
 __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, const uint64_t* __restrict__ nocc,
-                        const uint64_t* __restrict__ hidx, const uint64_t* __restrict__ occbase,
-                        const uint32_t* __restrict__ occ_slot, const void* table,
+                        const uint64_t* __restrict__ hidx, NodeIndex ix,
                         const int64_t* __restrict__ rid, const uint64_t* __restrict__ wbase,
                         const uint64_t* __restrict__ words, int k, uint64_t* __restrict__ hkey,
                         uint32_t* __restrict__ hval) {
   for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x) {
     if (!valid[m] || nocc[m] == 0) continue;
-    uint32_t s = occ_slot[occbase[m]];
-    uint64_t node = W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
     bool fwd;
-    (void)canon<W>(load_kmer<W>(words + wbase[m], 0, k), k, &fwd);
+    KV kk = canon<W>(load_kmer<W>(words + wbase[m], 0, k), k, &fwd);
+    uint64_t node = node_lookup<W>(ix, kk);
     uint64_t flipped = fwd ? 0 : 1;
     uint64_t key = (node << 31) | (flipped << 30) | ((m & 1) << 29) | (uint64_t)rid[m >> 1];
     uint64_t j = hidx[m];
@@ -1252,34 +1790,7 @@ __global__ void k_valid_u64(uint64_t nmates, const uint8_t* __restrict__ valid, 
 // at the link entering that k-mer.
 // ===========================================================================
 
-constexpr uint32_t kEnd = 0xffffffffu;
 constexpr int kUEdges = 16;
-
-// Read-only lookup of a key that is known to be in the table.
-template <int W>
-__device__ inline uint64_t table_find(const void* table, uint64_t cap, KV key) {
-  uint64_t s = slot_home(khash<W>(key), cap);
-  for (uint64_t probe = 0; probe < cap; probe++) {
-    if (W == 1) {
-      uint64_t cur = ((const Slot1*)table)[s].key;
-      if (cur == key.lo) return s;
-      if (cur == kEmpty) return ~0ull;
-    } else {
-      uint64_t cur = ((const Slot2*)table)[s].hi;
-      if (cur == key.hi && ((const Slot2*)table)[s].lo == key.lo) return s;
-      if (cur == kEmpty) return ~0ull;
-    }
-    s = slot_next(s, cap);
-  }
-  return ~0ull;
-}
-
-template <int W>
-__device__ inline uint32_t node_of_key(const void* table, uint64_t cap, KV key) {
-  uint64_t s = table_find<W>(table, cap, key);
-  if (s == ~0ull) return kEnd;
-  return W == 1 ? ((const Slot1*)table)[s].count : ((const Slot2*)table)[s].count;
-}
 
 __device__ inline int side_deg(uint32_t mask, int d) {
   return d == 0 ? __popc(mask & 0xFFu) : __popc((mask >> 8) & 0xFFu);
@@ -1292,7 +1803,7 @@ __device__ inline int back_side(uint32_t t) { return (t == ET_FF || t == ET_RF) 
 // (node << 1 | partner side) or kEnd.
 template <int W>
 __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
-                        const uint32_t* __restrict__ msk, const void* table, uint64_t cap, int k,
+                        const uint32_t* __restrict__ msk, NodeIndex ix, int k,
                         uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     KV v = node_key<W>(klo, khi, i);
@@ -1306,7 +1817,7 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
         if (keq<W>(z, v)) tnd = true;
         int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
         if (side_deg(mask, d) == 1) {
-          uint32_t j = node_of_key<W>(table, cap, z);
+          uint32_t j = node_lookup<W>(ix, z);
           if (j == kEnd) atomicMin(err, (unsigned long long)i);
           else l[d] = (j << 1) | (uint32_t)back_side(t);
         }
@@ -1443,7 +1954,7 @@ template <int W>
 __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, const uint64_t* __restrict__ ulen,
                                const uint32_t* __restrict__ order, const uint64_t* __restrict__ klo,
                                const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk,
-                               const void* table, uint64_t cap, const uint32_t* __restrict__ urep, int k,
+                               NodeIndex ix, const uint32_t* __restrict__ urep, int k,
                                uint32_t* __restrict__ uedge, uint8_t* __restrict__ uecnt) {
   for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t ent[kUEdges];
@@ -1464,7 +1975,7 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
         for (uint32_t x = 0; x < 4; x++) {
           if (!((mask >> (4 * t + x)) & 1)) continue;
           KV z = neighbour<W>(v, k, t, x);
-          uint32_t zi = node_of_key<W>(table, cap, z);
+          uint32_t zi = node_lookup<W>(ix, z);
           if (zi == kEnd) continue;  // cannot happen: the graph is symmetric
           uint32_t zr = urep[zi];
           bool z5 = (t == ET_FF || t == ET_RF);           // touches z's 5' end
@@ -1706,7 +2217,7 @@ struct gx_build {
   DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
   DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[10] = {};
+  uint64_t stats[12] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t ev[2 * 16] = {};
@@ -1720,10 +2231,13 @@ struct gx_build {
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
+  DevBuf sva, svb;  // sorted aggregation: record values
+  DevBuf nix_dir, nix_slots;  // key -> node index
   // learned sizing
   gx_mlp* sizer = nullptr;
   unsigned grid = 2048;  // blocks of grid-stride kernels (8 per CU)
   float slack = 1.5f;    // learned table capacity / predicted distinct
+  int method = GX_AGG_SORT;
   DevBuf hregs, sfeat, spred;
   float feat[GX_SIZER_FEATURES] = {};
   uint8_t regs_h[GX_SIZER_HLL_REGISTERS] = {};
@@ -1733,9 +2247,11 @@ struct gx_build {
 
 static std::string g_create_err;
 
-enum Phase { PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, NPH };
+enum Phase {
+  PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX, NPH
+};
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay", "unitig", "sizing"};
+                                       "k_replay", "unitig", "sizing", "node_index"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -1948,92 +2464,219 @@ static void run_build(gx_build* b) {
   }
   ph_end(b, PH_SIZING);
 
-  // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
-  //    of two >= the expected distinct count (table_hint) or the occurrence
-  //    count, so the load stays below 1 and is typically well below 1/2.
-  ph_begin(b, PH_INSERT);
-  uint64_t cap = learned_cap;
-  if (!cap) {
-    uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint * 2 : N;
-    cap = 1024;
-    while (cap < expect) cap <<= 1;
-  }
-  uint64_t rebuilds = 0;
-  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
-  uint32_t* occ_slot = buf<uint32_t>(b->occ_slot, N + 1);
-  uint8_t* occ_code = buf<uint8_t>(b->occ_code, N + 1);
+  // 4-5. aggregation: KmerPointable-ordered nodes (key, count, neighbour
+  //      mask) and, for the JDK-7 replay, each node's occurrence codes in
+  //      emission order; plus a key -> node table for lookups by key.
+  using SK = typename SortKeyOf<W>::T;
+  const bool sorted = b->method == GX_AGG_SORT;
+  uint64_t nn = 0, cap = 0, rebuilds = 0;
   void* table = nullptr;
-  for (int attempt = 0;; attempt++) {
-    if (cap > (1ull << 32)) throw GxError(GX_ENOTSUP, "hash table beyond 2^32 slots");
-    table = b->table.ensure(cap * slotsz);
-    hipLaunchKernelGGL(k_table_clear<W>, dim3(grid_for(cap, kBlock, 65536)), dim3(kBlock), 0, st, table, cap);
-    GX_HIP(hipGetLastError());
-    GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
-    // probe bound: far above the longest probe of a table at load <= 0.7;
-    // a nearly full (under-sized) table fails fast and is rebuilt larger
-    uint64_t limit = std::min<uint64_t>(cap, 4096);
-    if (nmates) {
-      unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
+  uint64_t *klo = nullptr, *khi = nullptr;
+  uint32_t *cnt = nullptr, *msk = nullptr, *nslot = nullptr, *occ_slot = nullptr;
+  const uint64_t* rseg = nullptr;  // sorted path: first record of each node
+  const uint64_t* rvals = nullptr; // sorted path: records (occurrence << 8 | code) in node order
+  uint8_t* occ_code = nullptr;
+  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
+  if (sorted) {
+    ph_begin(b, PH_INSERT);
+    SK* ka = buf<SK>(b->ckey, N + 2);
+    SK* kb = buf<SK>(b->ckey2, N + 2);
+    uint64_t* va = buf<uint64_t>(b->sva, N + 2);
+    uint64_t* vb = buf<uint64_t>(b->svb, N + 2);
+    const unsigned ntiles = (unsigned)std::max<uint64_t>(1, (N + kSortTile - 1) / kSortTile);
+    const size_t hn = (size_t)256 * ntiles;
+    uint32_t* hist = buf<uint32_t>(b->selflag, hn + 1);
+    uint64_t* hoff = buf<uint64_t>(b->seg, hn + 1);
+    if (N) {
       ph_begin(b, PH_KINSERT);
-      hipLaunchKernelGGL(k_insert<W>, dim3(g), dim3(kBlock), 0, st, nmates, mlen, wbase, words, occbase, nocc, k,
-                         table, cap, limit, occ_slot, occ_code, errw);
+      hipLaunchKernelGGL(k_occ_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
+                         nb, hist, ntiles);
+      GX_HIP(hipGetLastError());
+      exclusive_scan<uint32_t, uint64_t>(hist, hoff, hn, scratch_for(b, scan_scratch_bytes<uint64_t>(hn)), st);
+      hipLaunchKernelGGL(k_occ_scatter<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
+                         k, nb, hoff, ntiles, ka, va);
       GX_HIP(hipGetLastError());
       ph_end(b, PH_KINSERT);
     }
-    unsigned long long e = read1(st, errw);
-    if (e == ~0ull) break;
-    if ((e & 0xff) != ERR_TABLE || attempt > 8) check_err(b);
-    // fallback: rebuild in a table twice as large, and at least as large as
-    // the sizing from the occurrence count (the method without a prediction)
-    uint64_t plain = 1024;
-    while (plain < N) plain <<= 1;
-    cap = std::max(cap << 1, rebuilds == 0 ? plain : 0);
-    rebuilds++;
-  }
-  ph_end(b, PH_INSERT);
-
-  // 5. KmerPointable order
-  ph_begin(b, PH_NODES);
-  unsigned cblocks = grid_for(cap, kCTile, 0xffffffffu);
-  uint64_t* cb = buf<uint64_t>(b->blk, (size_t)cblocks + 1);
-  uint64_t* cbb = buf<uint64_t>(b->blk2, (size_t)cblocks + 1);
-  hipLaunchKernelGGL(k_compact_count<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cb);
-  GX_HIP(hipGetLastError());
-  exclusive_scan<uint64_t, uint64_t>(cb, cbb, cblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(cblocks)), st);
-  uint64_t nn = read1(st, cbb + cblocks);
-  if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
-  uint32_t* claimed = buf<uint32_t>(b->claimed, nn + 1);
-  hipLaunchKernelGGL(k_compact_write<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cbb, claimed);
-  GX_HIP(hipGetLastError());
-  using SK = typename std::conditional<W == 1, uint64_t, Key128>::type;
-  SK* ck = buf<SK>(b->ckey, nn + 1);
-  SK* ck2 = buf<SK>(b->ckey2, nn + 1);
-  uint32_t* cv = buf<uint32_t>(b->cval, nn + 1);
-  uint32_t* cv2 = buf<uint32_t>(b->cval2, nn + 1);
-  if (nn) {
-    hipLaunchKernelGGL((k_compact_list<W, SK>), dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, table, claimed,
-                       nn, nb, ck, cv);
-    GX_HIP(hipGetLastError());
-  }
-  bool fl = radix_sort<SK>(ck, cv, ck2, cv2, nn, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nn)), st);
-  uint32_t* perm = fl ? cv2 : cv;
-  uint64_t* klo = buf<uint64_t>(b->klo, nn + 1);
-  uint64_t* khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
-  uint32_t* cnt = buf<uint32_t>(b->cnt, nn + 1);
-  uint32_t* msk = buf<uint32_t>(b->msk, nn + 1);
-  uint32_t* nslot = buf<uint32_t>(b->nslot, nn + 1);
-  if (nn) {
-    hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, claimed, table,
-                       klo, khi, cnt, msk, nslot);
+    // the more significant bucket byte
+    SK *kx = ka, *ky = kb;
+    uint64_t *vx = va, *vy = vb;
+    if (nb >= 2 && N > 1) {
+      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, N, nb - 1, nb, scratch_for(b, radix_scratch_bytes<SK>(N)),
+                                        st);
+      if (f) { std::swap(kx, ky); std::swap(vx, vy); }
+    }
+    // buckets: LDS sort, oversized ones by a global radix sort
+    const uint32_t NBK = nb >= 2 ? 65536u : 256u;
+    uint64_t* bstart = buf<uint64_t>(b->hstart, NBK + 2);
+    uint32_t* big = buf<uint32_t>(b->hcu, NBK + 1);
+    uint64_t* bsz = buf<uint64_t>(b->htot, NBK + 1);
+    uint64_t* boff = buf<uint64_t>(b->upos, NBK + 2);
+    GX_HIP(hipMemsetAsync(big, 0, (NBK + 1) * 4, st));
+    hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(N + 1, kBlock, G)), dim3(kBlock), 0, st, N, kx, nb, NBK,
+                       bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
-    hipLaunchKernelGGL(k_dup_check<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, errw + 2);
+    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2>), dim3(NBK), dim3(kBlock), 0, st, kx, vx, ky, vy, bstart, N,
+                       nb, big, errw + 2);
     GX_HIP(hipGetLastError());
-    unsigned long long dup = read1(st, errw + 2);
-    if (dup != ~0ull) throw GxError(GX_EDEVICE, "internal error: k-mer stored in two table slots (node " +
-                                                    std::to_string(dup) + ")");
+    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap>), dim3(NBK), dim3(kBlock), 0, st, kx, vx, ky, vy, bstart, N, nb,
+                       big, errw + 2);
+    GX_HIP(hipGetLastError());
+    if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
+    hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(NBK, kBlock, G)), dim3(kBlock), 0, st, NBK, bstart, big, bsz);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint64_t, uint64_t>(bsz, boff, NBK, scratch_for(b, scan_scratch_bytes<uint64_t>(NBK)), st);
+    const uint64_t nbig = read1(st, boff + NBK);
+    if (nbig) {
+      SK* tk = buf<SK>(b->hkey, nbig + 1);
+      SK* tk2 = buf<SK>(b->hkey2, nbig + 1);
+      uint64_t* tv = buf<uint64_t>(b->hval, nbig + 1);
+      uint64_t* tv2 = buf<uint64_t>(b->hval2, nbig + 1);
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(NBK), dim3(kBlock), 0, st, bstart, big, boff, ky, vy, tk, tv, 0);
+      GX_HIP(hipGetLastError());
+      // all key bytes: several oversized buckets share the array, and the
+      // stable sort keeps each bucket's records in occurrence order
+      bool f = radix_sort<SK, uint64_t>(tk, tv, tk2, tv2, nbig, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nbig)),
+                                        st);
+      if (f) { std::swap(tk, tk2); std::swap(tv, tv2); }
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(NBK), dim3(kBlock), 0, st, bstart, big, boff, ky, vy, tk, tv, 1);
+      GX_HIP(hipGetLastError());
+    }
+    b->stats[9] = nbig;
+    ph_end(b, PH_INSERT);
+
+    // 5'. nodes = runs of equal keys
+    ph_begin(b, PH_NODES);
+    uint32_t* rflag = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
+    uint64_t* rpos = vx;
+    if (N) {
+      hipLaunchKernelGGL(k_run_flags<SK>, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, ky, rflag);
+      GX_HIP(hipGetLastError());
+    }
+    exclusive_scan<uint32_t, uint64_t>(rflag, rpos, N, scratch_for(b, scan_scratch_bytes<uint64_t>(N)), st);
+    nn = read1(st, rpos + N);
+    if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
+    klo = buf<uint64_t>(b->klo, nn + 1);
+    khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
+    cnt = buf<uint32_t>(b->cnt, nn + 1);
+    msk = buf<uint32_t>(b->msk, nn + 1);
+    uint64_t* nst = buf<uint64_t>(b->nslot, nn + 2);
+    GX_HIP(hipMemsetAsync(msk, 0, (nn + 1) * 4, st));
+    if (N) {
+      // rpos[i + 1] - 1 is the node of record i (rpos = exclusive scan of
+      // the run-start flags, N + 1 entries)
+      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(N, kBlock, 1u << 16)), dim3(kBlock), 0, st, N, ky, vy, rflag,
+                         rpos, nb, klo, khi, nst, msk);
+      GX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, N);
+    GX_HIP(hipGetLastError());
+    if (nn) {
+      hipLaunchKernelGGL(k_node_cnt, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, nst, cnt);
+      GX_HIP(hipGetLastError());
+    }
+    rseg = nst;
+    rvals = vy;
+    ph_end(b, PH_NODES);
+  } else {
+    // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
+    //    of two >= the expected distinct count (table_hint) or the occurrence
+    //    count, so the load stays below 1 and is typically well below 1/2.
+    ph_begin(b, PH_INSERT);
+    cap = learned_cap;
+    if (!cap) {
+      uint64_t expect = b->cfg.table_hint ? b->cfg.table_hint * 2 : N;
+      cap = 1024;
+      while (cap < expect) cap <<= 1;
+    }
+    occ_slot = buf<uint32_t>(b->occ_slot, N + 1);
+    occ_code = buf<uint8_t>(b->occ_code, N + 1);
+    for (int attempt = 0;; attempt++) {
+      if (cap > (1ull << 32)) throw GxError(GX_ENOTSUP, "hash table beyond 2^32 slots");
+      table = b->table.ensure(cap * slotsz);
+      hipLaunchKernelGGL(k_table_clear<W>, dim3(grid_for(cap, kBlock, 65536)), dim3(kBlock), 0, st, table, cap);
+      GX_HIP(hipGetLastError());
+      GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
+      // probe bound: far above the longest probe of a table at load <= 0.7;
+      // a nearly full (under-sized) table fails fast and is rebuilt larger
+      uint64_t limit = std::min<uint64_t>(cap, 4096);
+      if (nmates) {
+        unsigned g = grid_for(nmates, kBlock / 64, 1u << 16);
+        ph_begin(b, PH_KINSERT);
+        hipLaunchKernelGGL(k_insert<W>, dim3(g), dim3(kBlock), 0, st, nmates, mlen, wbase, words, occbase, nocc, k,
+                           table, cap, limit, occ_slot, occ_code, errw);
+        GX_HIP(hipGetLastError());
+        ph_end(b, PH_KINSERT);
+      }
+      unsigned long long e = read1(st, errw);
+      if (e == ~0ull) break;
+      if ((e & 0xff) != ERR_TABLE || attempt > 8) check_err(b);
+      // fallback: rebuild in a table twice as large, and at least as large as
+      // the sizing from the occurrence count (the method without a prediction)
+      uint64_t plain = 1024;
+      while (plain < N) plain <<= 1;
+      cap = std::max(cap << 1, rebuilds == 0 ? plain : 0);
+      rebuilds++;
+    }
+    ph_end(b, PH_INSERT);
+
+    // 5. KmerPointable order
+    ph_begin(b, PH_NODES);
+    unsigned cblocks = grid_for(cap, kCTile, 0xffffffffu);
+    uint64_t* cb = buf<uint64_t>(b->blk, (size_t)cblocks + 1);
+    uint64_t* cbb = buf<uint64_t>(b->blk2, (size_t)cblocks + 1);
+    hipLaunchKernelGGL(k_compact_count<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cb);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint64_t, uint64_t>(cb, cbb, cblocks, scratch_for(b, scan_scratch_bytes<uint64_t>(cblocks)), st);
+    nn = read1(st, cbb + cblocks);
+    if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
+    uint32_t* claimed = buf<uint32_t>(b->claimed, nn + 1);
+    hipLaunchKernelGGL(k_compact_write<W>, dim3(cblocks), dim3(kBlock), 0, st, table, cap, cbb, claimed);
+    GX_HIP(hipGetLastError());
+    SK* ck = buf<SK>(b->ckey, nn + 1);
+    SK* ck2 = buf<SK>(b->ckey2, nn + 1);
+    uint32_t* cv = buf<uint32_t>(b->cval, nn + 1);
+    uint32_t* cv2 = buf<uint32_t>(b->cval2, nn + 1);
+    if (nn) {
+      hipLaunchKernelGGL((k_compact_list<W, SK>), dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, table, claimed,
+                         nn, nb, ck, cv);
+      GX_HIP(hipGetLastError());
+    }
+    bool fl = radix_sort<SK>(ck, cv, ck2, cv2, nn, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nn)), st);
+    uint32_t* perm = fl ? cv2 : cv;
+    klo = buf<uint64_t>(b->klo, nn + 1);
+    khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
+    cnt = buf<uint32_t>(b->cnt, nn + 1);
+    msk = buf<uint32_t>(b->msk, nn + 1);
+    nslot = buf<uint32_t>(b->nslot, nn + 1);
+    if (nn) {
+      hipLaunchKernelGGL(k_gather_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, perm, claimed, table,
+                         klo, khi, cnt, msk, nslot);
+      GX_HIP(hipGetLastError());
+      GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
+      hipLaunchKernelGGL(k_dup_check<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, errw + 2);
+      GX_HIP(hipGetLastError());
+      unsigned long long dup = read1(st, errw + 2);
+      if (dup != ~0ull) throw GxError(GX_EDEVICE, "internal error: k-mer stored in two table slots (node " +
+                                                      std::to_string(dup) + ")");
+    }
+    ph_end(b, PH_NODES);
+
   }
-  ph_end(b, PH_NODES);
+
+  // 5b. key -> node index (heads, unitig links)
+  ph_begin(b, PH_NIX);
+  const uint32_t nbk = nb >= 2 ? 65536u : 256u;
+  uint32_t* ndir = buf<uint32_t>(b->nix_dir, nbk + 1);
+  void* nslots = b->nix_slots.ensure(std::max<uint64_t>(1, 2 * nn) * slotsz);
+  hipLaunchKernelGGL(k_nix_dir<W>, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, nbk, ndir);
+  GX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_nix_build<W>, dim3(nbk), dim3(kBlock), 0, st, klo, khi, ndir, nslots);
+  GX_HIP(hipGetLastError());
+  const NodeIndex nix{ndir, nslots, nb, nbk};
+  if (sorted) cap = 2 * nn;
+  ph_end(b, PH_NIX);
 
   // 6. replay
   ph_begin(b, PH_REPLAY);
@@ -2042,8 +2685,8 @@ static void run_build(gx_build* b) {
   uint64_t* selcnt = buf<uint64_t>(b->selcnt, nn + 1);
   uint64_t* seg = buf<uint64_t>(b->seg, nn + 1);
   if (nn) {
-    hipLaunchKernelGGL(k_node_info<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, msk, cnt, nslot, table,
-                       jdk7, needs, selcnt, errw + 3);
+    hipLaunchKernelGGL(k_node_info<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, msk, cnt, nslot,
+                       sorted ? nullptr : table, jdk7, needs, selcnt, errw + 3);
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint64_t, uint64_t>(selcnt, seg, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
@@ -2051,6 +2694,8 @@ static void run_build(gx_build* b) {
   uint64_t* lists = buf<uint64_t>(b->lists, nn + 1);
   uint64_t nrep = 0;
   if (nsel) {
+    const uint32_t* codes = nullptr;
+    if (!sorted) {
     uint32_t* flag = buf<uint32_t>(b->selflag, N + 1);
     uint64_t* pos = buf<uint64_t>(b->selpos, N + 1);
     hipLaunchKernelGGL(k_sel_flags<W>, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, occ_slot, table, flag);
@@ -2067,7 +2712,8 @@ static void run_build(gx_build* b) {
     while (nbytes_id < 4 && (nn >> (8 * nbytes_id)) > 0) nbytes_id++;
     bool f2 = radix_sort<uint32_t>(sk, sv, sk2, sv2, nsel, 0, nbytes_id,
                                    scratch_for(b, radix_scratch_bytes<uint32_t>(nsel)), st);
-    uint32_t* codes = f2 ? sv2 : sv;
+    codes = f2 ? sv2 : sv;
+    }
     // replay list ordered by descending record count
     uint64_t* rpos = buf<uint64_t>(b->rpos, nn + 1);
     exclusive_scan<uint32_t, uint64_t>(needs, rpos, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
@@ -2082,8 +2728,12 @@ static void run_build(gx_build* b) {
                                    st);
     uint32_t* rlist = f4 ? rv2 : rv;
     ph_begin(b, PH_KREPLAY);
-    hipLaunchKernelGGL(k_replay<W>, dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep, rlist, seg,
-                       codes, klo, khi, msk, k, nb, lists);
+    if (sorted)
+      hipLaunchKernelGGL((k_replay<W, uint64_t>), dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep,
+                         rlist, rseg, rvals, klo, khi, msk, k, nb, lists);
+    else
+      hipLaunchKernelGGL((k_replay<W, uint32_t>), dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep,
+                         rlist, seg, codes, klo, khi, msk, k, nb, lists);
     GX_HIP(hipGetLastError());
     ph_end(b, PH_KREPLAY);
   }
@@ -2097,7 +2747,7 @@ static void run_build(gx_build* b) {
   uint32_t* hv2 = buf<uint32_t>(b->hval2, nheads + 1);
   if (nmates) {
     hipLaunchKernelGGL(k_heads<W>, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, valid, nocc, hidx,
-                       occbase, occ_slot, table, rid, wbase, words, k, hk, hv);
+                       nix, rid, wbase, words, k, hk, hv);
     GX_HIP(hipGetLastError());
   }
   int hbytes = 1;
@@ -2215,9 +2865,9 @@ template <int W>
 static void run_unitigs(gx_build* b) {
   hipStream_t st = b->st;
   const int k = b->cfg.k, nb = b->nb;
-  const uint64_t nn = b->nnodes, cap = b->stats[3];
+  const uint64_t nn = b->nnodes;
   const unsigned G = b->grid;
-  void* table = b->table.p;
+  const NodeIndex nix{b->nix_dir.as<uint32_t>(), b->nix_slots.p, nb, nb >= 2 ? 65536u : 256u};
   const uint64_t* klo = b->klo.as<uint64_t>();
   const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
   const uint32_t* cnt = b->cnt.as<uint32_t>();
@@ -2232,8 +2882,8 @@ static void run_unitigs(gx_build* b) {
   uint8_t* cut = buf<uint8_t>(b->u_cut, ne + 1);
   GX_HIP(hipMemsetAsync(cut, 0, ne + 1, st));
   if (nn) {
-    hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, table,
-                       cap, k, lnk, tand, errw);
+    hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, nix, k, lnk,
+                       tand, errw);
     GX_HIP(hipGetLastError());
   }
   uint4* R[2] = {buf<uint4>(b->u_ptr[0], ne + 1), buf<uint4>(b->u_ptr[1], ne + 1)};
@@ -2302,7 +2952,7 @@ static void run_unitigs(gx_build* b) {
   uint8_t* uecnt = buf<uint8_t>(b->u_ecnt, nu + 1);
   if (nu) {
     hipLaunchKernelGGL(k_unitig_edges<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, order,
-                       klo, khi, msk, table, cap, urep, k, uedge, uecnt);
+                       klo, khi, msk, nix, urep, k, uedge, uecnt);
     GX_HIP(hipGetLastError());
   }
   // heads
@@ -2671,6 +3321,14 @@ int gx_build_set_tuning(gx_build* b, int grid_blocks, float table_slack) {
     if (!(table_slack >= 1.05f && table_slack <= 16.f)) throw GxError(GX_EINVAL, "table_slack must be in [1.05, 16]");
     b->grid = (unsigned)grid_blocks;
     b->slack = table_slack;
+  });
+}
+
+int gx_build_set_method(gx_build* b, int method) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (method != GX_AGG_SORT && method != GX_AGG_HASH) throw GxError(GX_EINVAL, "unknown aggregation method");
+    b->method = method;
   });
 }
 

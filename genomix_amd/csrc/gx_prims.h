@@ -160,7 +160,8 @@ static void exclusive_scan(const Tin* in, T* out, size_t n, char* scratch, hipSt
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort, 8-bit digits.  Keys are up to 128 bits (KeyT =
 // uint32_t, uint64_t or Key128); the digits sorted are bytes
-// [byte_lo, byte_hi) of the key's integer value.  Values are uint32_t.
+// [byte_lo, byte_hi) of the key's integer value.  Values are uint32_t or
+// uint64_t.
 // ---------------------------------------------------------------------------
 
 struct Key128 {
@@ -196,11 +197,11 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const KeyT* __restrict__ 
 // 256 consecutive elements; inside a round the rank of an element among equal
 // digits comes from a wave-level match (8 ballots) plus the counts of the
 // lower waves, so ties keep their input order.
-template <class KeyT, bool kVals>
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict__ keys, const uint32_t* __restrict__ vals,
+template <class KeyT, bool kVals, class ValT = uint32_t>
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict__ keys, const ValT* __restrict__ vals,
                                                           size_t n, int byte, const uint64_t* __restrict__ offs,
                                                           unsigned ntiles, KeyT* __restrict__ okeys,
-                                                          uint32_t* __restrict__ ovals) {
+                                                          ValT* __restrict__ ovals) {
   __shared__ uint64_t run[256];
   __shared__ uint16_t wcnt[kBlock / 64][256];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -213,7 +214,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict
     size_t j = base + (size_t)r * kBlock + threadIdx.x;
     bool valid = j < n;
     KeyT key{};
-    uint32_t val = 0, d = 0;
+    ValT val = 0;
+    uint32_t d = 0;
     if (valid) {
       key = keys[j];
       if (kVals) val = vals[j];
@@ -253,8 +255,8 @@ static size_t radix_scratch_bytes(size_t n) {
 
 // Sorts (keys, vals) by bytes [byte_lo, byte_hi).  Uses (k2, v2) as the
 // ping-pong buffers; returns true when the result ended in (k2, v2).
-template <class KeyT>
-static bool radix_sort(KeyT* k1, uint32_t* v1, KeyT* k2, uint32_t* v2, size_t n, int byte_lo, int byte_hi,
+template <class KeyT, class ValT = uint32_t>
+static bool radix_sort(KeyT* k1, ValT* v1, KeyT* k2, ValT* v2, size_t n, int byte_lo, int byte_hi,
                        char* scratch, hipStream_t st) {
   if (n <= 1) return false;
   unsigned ntiles = (unsigned)((n + kSortTile - 1) / kSortTile);
@@ -266,17 +268,17 @@ static bool radix_sort(KeyT* k1, uint32_t* v1, KeyT* k2, uint32_t* v2, size_t n,
   for (int b = byte_lo; b < byte_hi; b++) {
     KeyT* ik = flip ? k2 : k1;
     KeyT* ok = flip ? k1 : k2;
-    uint32_t* iv = flip ? v2 : v1;
-    uint32_t* ov = flip ? v1 : v2;
+    ValT* iv = flip ? v2 : v1;
+    ValT* ov = flip ? v1 : v2;
     hipLaunchKernelGGL((k_radix_hist<KeyT>), dim3(ntiles), dim3(kBlock), 0, st, ik, n, b, hist, ntiles);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(hist, offs, h, sc, st);
     if (v1)
-      hipLaunchKernelGGL((k_radix_scatter<KeyT, true>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b, offs,
+      hipLaunchKernelGGL((k_radix_scatter<KeyT, true, ValT>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b, offs,
                          ntiles, ok, ov);
     else
-      hipLaunchKernelGGL((k_radix_scatter<KeyT, false>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b, offs,
-                         ntiles, ok, ov);
+      hipLaunchKernelGGL((k_radix_scatter<KeyT, false, ValT>), dim3(ntiles), dim3(kBlock), 0, st, ik, iv, n, b,
+                         offs, ntiles, ok, ov);
     GX_HIP(hipGetLastError());
     flip = !flip;
   }

@@ -33,7 +33,7 @@ EXPORTED = [
     "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
     "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
-    "gx_batch_destroy",
+    "gx_batch_destroy", "gx_build_set_method",
 ]
 
 
@@ -97,6 +97,7 @@ def lib():
         L.gx_build_features.argtypes = [vp, fp, i32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_double)]
         L.gx_build_set_tuning.argtypes = [vp, i32, ctypes.c_float]
         L.gx_build_set_config.argtypes = [vp, ctypes.POINTER(_Cfg)]
+        L.gx_build_set_method.argtypes = [vp, i32]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -114,7 +115,7 @@ class GraphBuilder:
     """One graph-build task on one GPU (a gx_build handle)."""
 
     def __init__(self, k, nparts=1, edge_order="jdk7", device=0, table_hint=0, max_readids_per_edge=250,
-                 sizer=None):
+                 sizer=None, method=None):
         self._h = ctypes.c_void_p()
         self._sizer = None
         cfg = _Cfg(k, nparts, max_readids_per_edge, EDGE_ORDER[edge_order], device, table_hint)
@@ -125,6 +126,13 @@ class GraphBuilder:
         self.nparts = nparts
         if sizer is not None:
             self.set_sizer(sizer)
+        if method is not None:
+            self.set_method(method)
+
+    def set_method(self, method):
+        """'sort' (default: sort-partitioned aggregation) or 'hash' (atomic
+        hash table, sized by the learned predictor when one is attached)."""
+        self._check(lib().gx_build_set_method(self._h, {"sort": 0, "hash": 1}[method]))
 
     def set_sizer(self, sizer):
         """Attach a trained sizing network (genomix_amd.sizer.Mlp) or None."""
@@ -248,9 +256,9 @@ def build_and_merge(text: bytes, k: int, edge_order: str = "jdk7", device: int =
         return g, b.unitigs()
 
 
-def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", device: int = 0):
+def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", device: int = 0, method=None):
     """Graph build of one readids buffer; returns the per-part record streams."""
-    with GraphBuilder(k, nparts=nparts, edge_order=edge_order, device=device) as b:
+    with GraphBuilder(k, nparts=nparts, edge_order=edge_order, device=device, method=method) as b:
         b.add_reads(text)
         b.finish()
         return [b.part_bytes(p) for p in range(nparts)]

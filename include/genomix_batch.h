@@ -32,6 +32,16 @@ int gx_build_set_tuning(gx_build* b, int grid_blocks, float table_slack);
  * read sets of different k. Clears any input added so far. */
 int gx_build_set_config(gx_build* b, const gx_build_cfg* cfg);
 
+/* Aggregation method of a handle.  GX_AGG_SORT (default): occurrences
+ * sorted by (key, occurrence) -- radix passes on the top key bits, then
+ * per-bucket sorts in LDS -- and nodes taken as runs; streaming HBM traffic
+ * only.  GX_AGG_HASH: an open-addressing table with atomic count/mask
+ * updates, sized up front (learned sizing applies here) and rebuilt on
+ * overflow.  Output bytes are identical. */
+#define GX_AGG_SORT 0
+#define GX_AGG_HASH 1
+int gx_build_set_method(gx_build* b, int method);
+
 typedef struct gx_batch gx_batch;
 
 /* Called on a worker thread once set `set` is built (and merged, if asked):

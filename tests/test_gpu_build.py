@@ -29,8 +29,8 @@ def gx():
     return g
 
 
-def gpu_records(gx, text, k, edge_order="jdk7"):
-    return gx.build_graph(text, k, edge_order=edge_order)[0]
+def gpu_records(gx, text, k, edge_order="jdk7", method=None):
+    return gx.build_graph(text, k, edge_order=edge_order, method=method)[0]
 
 
 def ora_records(text, k):
@@ -67,9 +67,27 @@ def _sim(seed, G, L, cov, err, paired, n_mut=0):
 
 @pytest.mark.parametrize("k", [3, 4, 5, 8, 11, 21, 31, 32, 33, 47, 55, 63])
 @pytest.mark.parametrize("paired", [False, True], ids=["se", "pe"])
-def test_random_reads_byte_exact(gx, k, paired):
+@pytest.mark.parametrize("method", ["sort", "hash"])
+def test_random_reads_byte_exact(gx, k, paired, method):
     text = _sim(1000 + k, G=3000, L=max(k + 20, 70), cov=12, err=0.01, paired=paired)
-    assert_same(gpu_records(gx, text, k), ora_records(text, k))
+    assert_same(gpu_records(gx, text, k, method=method), ora_records(text, k))
+
+
+@pytest.mark.parametrize("k", [4, 9, 31, 55])
+def test_oversized_buckets_byte_exact(gx, k):
+    # thousands of homopolymer and dinucleotide reads put far more than one
+    # LDS bucket's worth of records on a few keys (global-sort fallback)
+    rng = np.random.default_rng(k)
+    lines = []
+    for i in range(2500):
+        unit = ["A", "C", "AC", "GT"][i % 4]
+        s = (unit * 60)[:rng.integers(k + 5, 110)]
+        s = "".join(c if rng.random() > 0.01 else rng.choice(list("ACGT")) for c in s)
+        lines.append(f"{4 * i + 2}\t{s}\n")
+    text = "".join(lines).encode()
+    want = ora_records(text, k)
+    for method in ("sort", "hash"):
+        assert_same(gpu_records(gx, text, k, method=method), want)
 
 
 def test_low_complexity_byte_exact(gx):

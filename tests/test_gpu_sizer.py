@@ -6,7 +6,8 @@
   features equal to float32 rounding.
 * Sizing never changes the graph: identical bytes with and without the
   network, and with a network that predicts far too few k-mers (the
-  grow-and-rebuild fallback runs and reports itself).
+  grow-and-rebuild fallback runs and reports itself).  Sizing applies to
+  the hash-table aggregation method.
 * What the trained network gains: on read sets outside its training seeds,
   a table several times smaller than the occurrence-count sizing, without
   rebuilds.
@@ -117,14 +118,14 @@ def test_sizing_never_changes_the_graph(gx, sizer):
     p = np.zeros(sizer.Mlp().n, np.float32)
     p[-2] = -20.0
     tiny = sizer.Mlp(params=p)
-    with gx.GraphBuilder(31, sizer=tiny) as b:
+    with gx.GraphBuilder(31, sizer=tiny, method="hash") as b:
         b.add_reads(text)
         b.finish()
         assert b.part_bytes(0) == base
         st = b.stats()
         assert st["table_rebuilds"] >= 1 and st["predicted_distinct"] >= 1
         u1 = b.unitigs()
-    with gx.GraphBuilder(31) as b:
+    with gx.GraphBuilder(31, method="hash") as b:
         b.add_reads(text)
         b.finish()
         assert b.stats()["table_rebuilds"] == 0
@@ -143,12 +144,12 @@ def test_trained_network_sizes_the_table(gx, sizer):
              sim.ReadSetSpec(genome_len=1_000_000, read_len=100, coverage=30, error_rate=0.015, seed=990002)]
     for spec, k in zip(cases, (31, 21)):
         text = sim.simulate(spec)
-        with gx.GraphBuilder(k, sizer=net) as b:
+        with gx.GraphBuilder(k, sizer=net, method="hash") as b:
             b.add_reads(text)
             b.finish()
             got = b.part_bytes(0)
             st = b.stats()
-        with gx.GraphBuilder(k) as b:
+        with gx.GraphBuilder(k, method="hash") as b:
             b.add_reads(text)
             b.finish()
             assert b.part_bytes(0) == got
