@@ -1056,8 +1056,8 @@ __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 // Two tiers: CAP = kBucketCap / 2 over every bucket (more blocks per CU),
 // leaving buckets above it marked 2 for the CAP = kBucketCap launch; that
 // one marks buckets above kBucketCap 1 for the global pass.
-template <int W, int CAP>
-__global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
+template <int W, int CAP, int NT>
+__global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
                                                         const uint64_t* __restrict__ iv,
                                                         typename SortKeyOf<W>::T* __restrict__ ok,
                                                         uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart,
@@ -1080,7 +1080,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf
     if (n <= kBucketCap) return;  // sorted by the second tier
   }
   if (n > kBucketCap || n == 1 || nb < 2) {
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       ok[a + i] = ik[a + i];
       ov[a + i] = iv[a + i];
     }
@@ -1088,7 +1088,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf
   }
   uint32_t P = 2;
   while (P < n) P <<= 1;
-  for (uint32_t i = threadIdx.x; i < P; i += kBlock) {
+  for (uint32_t i = threadIdx.x; i < P; i += NT) {
     E x;
     if (i < n) x = bucket_elem(ik[a + i], nb, i);
     else elem_max(x);
@@ -1097,7 +1097,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf
   __syncthreads();
   for (uint32_t size = 2; size <= P; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t t = threadIdx.x; t < P / 2; t += kBlock) {
+      for (uint32_t t = threadIdx.x; t < P / 2; t += NT) {
         uint32_t i = (t / stride) * stride * 2 + (t % stride), j = i + stride;
         bool up = (i & size) == 0;
         E x = s[i], y = s[j];
@@ -1109,7 +1109,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_sort(const typename SortKeyOf
       __syncthreads();
     }
   }
-  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+  for (uint32_t i = threadIdx.x; i < n; i += NT) {
     uint32_t src = elem_idx(s[i]);
     ok[a + i] = ik[a + src];
     ov[a + i] = iv[a + src];
@@ -2517,11 +2517,11 @@ static void run_build(gx_build* b) {
                        bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
-    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2>), dim3(NBK), dim3(kBlock), 0, st, kx, vx, ky, vy, bstart, N,
-                       nb, big, errw + 2);
+    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(NBK), dim3(512), 0, st, kx, vx, ky, vy, bstart,
+                       N, nb, big, errw + 2);
     GX_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap>), dim3(NBK), dim3(kBlock), 0, st, kx, vx, ky, vy, bstart, N, nb,
-                       big, errw + 2);
+    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(NBK), dim3(1024), 0, st, kx, vx, ky, vy, bstart, N,
+                       nb, big, errw + 2);
     GX_HIP(hipGetLastError());
     if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(NBK, kBlock, G)), dim3(kBlock), 0, st, NBK, bstart, big, bsz);
