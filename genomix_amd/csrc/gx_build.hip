@@ -1157,7 +1157,8 @@ __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename
                                                       const uint32_t* __restrict__ flag,
                                                       const uint64_t* __restrict__ rpos, int nb,
                                                       uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
-                                                      uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk) {
+                                                      uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk,
+                                                      uint8_t* __restrict__ rcode) {
   __shared__ uint32_t lm[kBlock];
   __shared__ uint64_t nfirst, nlast;
   for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < N; base += (uint64_t)gridDim.x * kBlock) {
@@ -1177,7 +1178,9 @@ __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename
         klo[n] = v.lo;
         if (W == 2) khi[n] = v.hi;
       }
-      atomicOr(&lm[n - nfirst], code_bits((uint32_t)(vals[i] & 0xff)));
+      uint32_t code = (uint32_t)(vals[i] & 0xff);
+      rcode[i] = (uint8_t)code;
+      atomicOr(&lm[n - nfirst], code_bits(code));
     }
     __syncthreads();
     uint64_t n = nfirst + threadIdx.x;
@@ -1188,6 +1191,11 @@ __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename
     }
     __syncthreads();
   }
+}
+
+__global__ void k_narrow_codes(uint64_t n, const uint32_t* __restrict__ in, uint8_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint8_t)in[i];
 }
 
 __global__ void k_set_u64(uint64_t* p, uint64_t v) {
@@ -1405,16 +1413,96 @@ __device__ inline LList j7_union(LList L, int sym, uint32_t hb) {
   return o;
 }
 
+// Long runs (a repeat k-mer carries thousands of records) are replayed by
+// four lanes per node, one per edge type: the four lists are independent
+// state machines over the same records, so the serial chain per lane is a
+// quarter of the work.  Lanes 4q..4q+3 hold node q; the four 16-bit results
+// are merged with lane shuffles.
+constexpr uint32_t kLongReplay = 64;
+
+__device__ inline int code_sym(uint32_t code, int t) {
+  uint32_t sn = code & 0xF, pn = (code >> 4) & 0xF;
+  if (t == ET_FF || t == ET_FR) return ((sn & 8) && (((sn & 4) != 0) == (t == ET_FR))) ? (int)(sn & 3) : -1;
+  return ((pn & 8) && (((pn & 4) != 0) == (t == ET_RF))) ? (int)(pn & 3) : -1;
+}
+
+template <int W>
+__global__ void k_replay_long(uint64_t nlong, const uint32_t* __restrict__ rlist, const uint64_t* __restrict__ seg,
+                              const uint8_t* __restrict__ codes, const uint64_t* __restrict__ klo,
+                              const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk, int k, int nb,
+                              uint64_t* __restrict__ lists) {
+  for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < 4 * nlong;
+       gid += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = gid >> 2;
+    const int t = (int)(gid & 3);
+    const uint32_t i = rlist[q];
+    const uint32_t mask = msk[i];
+    const bool active = __popc((mask >> (4 * t)) & 0xF) > 1;
+    uint64_t part = 0;
+    if (active) {
+      KV v = node_key<W>(klo, khi, i);
+      uint32_t hb = 0;
+      for (uint32_t x = 0; x < 4; x++)
+        hb |= ((uint32_t)jdk7_spread(java_hash_vkmer<W>(neighbour<W>(v, k, (uint32_t)t, x), k, nb)) & 7u) << (3 * x);
+      LList l{0, 0};
+      uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu;
+      const uint64_t a = seg[i], e = seg[i + 1];
+      uint64_t g = a & ~15ull;
+      uint4 cur = *reinterpret_cast<const uint4*>(codes + g);
+      while (g < e) {
+        const uint64_t gn = g + 16;
+        const uint4 nxt = gn < e ? *reinterpret_cast<const uint4*>(codes + gn) : make_uint4(0, 0, 0, 0);
+        const uint64_t lo64 = (uint64_t)cur.x | ((uint64_t)cur.y << 32), hi64 = (uint64_t)cur.z | ((uint64_t)cur.w << 32);
+        const int j0 = g < a ? (int)(a - g) : 0, j1 = e - g < 16 ? (int)(e - g) : 16;
+        for (int j = j0; j < j1; j++) {
+          uint32_t code = (uint32_t)((j < 8 ? lo64 >> (8 * j) : hi64 >> (8 * (j - 8))) & 0xff);
+          int sy = code_sym(code, t);
+          if (l.n <= 1 && (sy < 0 || (l.n == 1 && (int)ll_get(l, 0) == sy))) continue;
+          if (l.n == 0) { l.bits = (uint32_t)sy; l.n = 1; continue; }
+          uint32_t cls = 5;
+          if (sy >= 0) {
+            cls = (uint32_t)sy;
+            for (int jj = 0; jj < l.n; jj++)
+              if ((int)ll_get(l, jj) == sy) cls = 4;
+          }
+          uint32_t key = ((uint32_t)l.n << 11) | (l.bits << 3) | cls;
+          if ((c0 >> 11) == key) { l.n = (c0 >> 8) & 7; l.bits = c0 & 0xff; continue; }
+          if ((c1 >> 11) == key) { l.n = (c1 >> 8) & 7; l.bits = c1 & 0xff; continue; }
+          LList o = j7_union(l, sy, hb);
+          c1 = c0;
+          c0 = (key << 11) | ((uint32_t)o.n << 8) | o.bits;
+          l = o;
+        }
+        cur = nxt;
+        g = gn;
+      }
+      LList f = j7_union(l, -1, hb);  // cross aggregator init
+      part = ((uint64_t)f.n | ((uint64_t)f.bits << 3)) << (16 * t);
+    }
+    uint32_t plo = (uint32_t)part, phi = (uint32_t)(part >> 32);
+    plo |= __shfl_xor(plo, 1);
+    phi |= __shfl_xor(phi, 1);
+    plo |= __shfl_xor(plo, 2);
+    phi |= __shfl_xor(phi, 2);
+    if (t == 0) lists[i] = ((uint64_t)phi << 32) | plo;
+  }
+}
+
+__global__ void k_count_long(uint64_t n, const uint32_t* __restrict__ rkey, uint32_t lim,
+                             unsigned long long* __restrict__ cnt) {
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (uint64_t)gridDim.x * blockDim.x)
+    if (rkey[q] <= lim) atomicAdd(cnt, 1ull);
+}
+
 // One thread per node of the replay list.  Only lists that end with >= 2
 // k-mers are simulated; a list with <= 1 k-mer has one order.  Fast paths:
 // unions that cannot reorder (empty or one-element list, no new element)
 // are skipped, and a union's result depends only on (list size, order,
 // symbol class) -- a new letter, a letter already present, or none -- so
-// the last two transitions of each list are cached.  Codes are loaded four
-// at a time.
-template <int W, class CodeT>
+// the last two transitions of each list are cached.
+template <int W>
 __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, const uint64_t* __restrict__ seg,
-                         const CodeT* __restrict__ codes, const uint64_t* __restrict__ klo,
+                         const uint8_t* __restrict__ codes, const uint64_t* __restrict__ klo,
                          const uint64_t* __restrict__ khi, const uint32_t* __restrict__ msk, int k, int nb,
                          uint64_t* __restrict__ lists) {
   for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrep; q += (uint64_t)gridDim.x * blockDim.x) {
@@ -1439,14 +1527,19 @@ __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, cons
     uint32_t cache[4][2];
     for (int t = 0; t < 4; t++) cache[t][0] = cache[t][1] = 0xffffffffu;
     uint64_t a = seg[i], e = seg[i + 1];
-    for (uint64_t r0 = a; r0 < e; r0 += 4) {
-      uint32_t c4[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) c4[j] = (r0 + j < e) ? (uint32_t)(codes[r0 + j] & 0xff) : 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (r0 + j >= e) break;
-        uint32_t code = c4[j];
+    // Codes are bytes, read 16 at a time with aligned 16-byte loads; the
+    // next group is loaded before the current one is replayed, so a long
+    // list (a repeat k-mer has thousands of records) waits on memory only
+    // once per group.  The code array is padded to a multiple of 16 bytes.
+    uint64_t g = a & ~15ull;
+    uint4 cur = a < e ? *reinterpret_cast<const uint4*>(codes + g) : make_uint4(0, 0, 0, 0);
+    while (g < e) {
+      const uint64_t gn = g + 16;
+      const uint4 nxt = gn < e ? *reinterpret_cast<const uint4*>(codes + gn) : make_uint4(0, 0, 0, 0);
+      const uint64_t lo64 = (uint64_t)cur.x | ((uint64_t)cur.y << 32), hi64 = (uint64_t)cur.z | ((uint64_t)cur.w << 32);
+      const int j0 = g < a ? (int)(a - g) : 0, j1 = e - g < 16 ? (int)(e - g) : 16;
+      for (int j = j0; j < j1; j++) {
+        uint32_t code = (uint32_t)((j < 8 ? lo64 >> (8 * j) : hi64 >> (8 * (j - 8))) & 0xff);
         int sym[4] = {-1, -1, -1, -1};
         uint32_t sn = code & 0xF, pn = (code >> 4) & 0xF;
         if (sn & 8) sym[(sn & 4) ? ET_FR : ET_FF] = (int)(sn & 3);
@@ -1474,6 +1567,8 @@ __global__ void k_replay(uint64_t nrep, const uint32_t* __restrict__ rlist, cons
           l = o;
         }
       }
+      cur = nxt;
+      g = gn;
     }
     uint64_t out = 0;
     for (int t = 0; t < 4; t++) {
@@ -2233,6 +2328,7 @@ struct gx_build {
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
   DevBuf sva, svb;  // sorted aggregation: record values
   DevBuf nix_dir, nix_slots;  // key -> node index
+  DevBuf rcode;               // replay codes, one byte per record in node order
   // learned sizing
   gx_mlp* sizer = nullptr;
   unsigned grid = 2048;  // blocks of grid-stride kernels (8 per CU)
@@ -2474,7 +2570,7 @@ static void run_build(gx_build* b) {
   uint64_t *klo = nullptr, *khi = nullptr;
   uint32_t *cnt = nullptr, *msk = nullptr, *nslot = nullptr, *occ_slot = nullptr;
   const uint64_t* rseg = nullptr;  // sorted path: first record of each node
-  const uint64_t* rvals = nullptr; // sorted path: records (occurrence << 8 | code) in node order
+  const uint8_t* rbytes = nullptr;  // sorted path: record codes in node order
   uint8_t* occ_code = nullptr;
   const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
   if (sorted) {
@@ -2562,12 +2658,13 @@ static void run_build(gx_build* b) {
     cnt = buf<uint32_t>(b->cnt, nn + 1);
     msk = buf<uint32_t>(b->msk, nn + 1);
     uint64_t* nst = buf<uint64_t>(b->nslot, nn + 2);
+    uint8_t* rcodes = buf<uint8_t>(b->rcode, N + 32);
     GX_HIP(hipMemsetAsync(msk, 0, (nn + 1) * 4, st));
     if (N) {
       // rpos[i + 1] - 1 is the node of record i (rpos = exclusive scan of
       // the run-start flags, N + 1 entries)
       hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(N, kBlock, 1u << 16)), dim3(kBlock), 0, st, N, ky, vy, rflag,
-                         rpos, nb, klo, khi, nst, msk);
+                         rpos, nb, klo, khi, nst, msk, rcodes);
       GX_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, N);
@@ -2577,7 +2674,7 @@ static void run_build(gx_build* b) {
       GX_HIP(hipGetLastError());
     }
     rseg = nst;
-    rvals = vy;
+    rbytes = rcodes;
     ph_end(b, PH_NODES);
   } else {
     // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
@@ -2713,6 +2810,11 @@ static void run_build(gx_build* b) {
     bool f2 = radix_sort<uint32_t>(sk, sv, sk2, sv2, nsel, 0, nbytes_id,
                                    scratch_for(b, radix_scratch_bytes<uint32_t>(nsel)), st);
     codes = f2 ? sv2 : sv;
+    uint8_t* cb = buf<uint8_t>(b->rcode, nsel + 32);
+    hipLaunchKernelGGL(k_narrow_codes, dim3(grid_for(nsel, kBlock, G)), dim3(kBlock), 0, st, nsel, codes, cb);
+    GX_HIP(hipGetLastError());
+    rbytes = cb;
+    rseg = seg;
     }
     // replay list ordered by descending record count
     uint64_t* rpos = buf<uint64_t>(b->rpos, nn + 1);
@@ -2727,14 +2829,24 @@ static void run_build(gx_build* b) {
     bool f4 = radix_sort<uint32_t>(rk, rv, rk2, rv2, nrep, 0, 2, scratch_for(b, radix_scratch_bytes<uint32_t>(nrep)),
                                    st);
     uint32_t* rlist = f4 ? rv2 : rv;
-    ph_begin(b, PH_KREPLAY);
-    if (sorted)
-      hipLaunchKernelGGL((k_replay<W, uint64_t>), dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep,
-                         rlist, rseg, rvals, klo, khi, msk, k, nb, lists);
-    else
-      hipLaunchKernelGGL((k_replay<W, uint32_t>), dim3(grid_for(nrep, kBlock, 1u << 16)), dim3(kBlock), 0, st, nrep,
-                         rlist, seg, codes, klo, khi, msk, k, nb, lists);
+    const uint32_t* rkeys = f4 ? rk2 : rk;
+    // the longest runs (a prefix of the list) go to the four-lane kernel
+    GX_HIP(hipMemsetAsync(errw + 2, 0, 8, st));
+    hipLaunchKernelGGL(k_count_long, dim3(grid_for(nrep, kBlock, G)), dim3(kBlock), 0, st, nrep, rkeys,
+                       0xffffu - kLongReplay, errw + 2);
     GX_HIP(hipGetLastError());
+    const uint64_t nlong = read1(st, errw + 2);
+    ph_begin(b, PH_KREPLAY);
+    if (nlong) {
+      hipLaunchKernelGGL(k_replay_long<W>, dim3(grid_for(4 * nlong, kBlock, 1u << 16)), dim3(kBlock), 0, st, nlong,
+                         rlist, rseg, rbytes, klo, khi, msk, k, nb, lists);
+      GX_HIP(hipGetLastError());
+    }
+    if (nrep > nlong) {
+      hipLaunchKernelGGL(k_replay<W>, dim3(grid_for(nrep - nlong, kBlock, 1u << 16)), dim3(kBlock), 0, st,
+                         nrep - nlong, rlist + nlong, rseg, rbytes, klo, khi, msk, k, nb, lists);
+      GX_HIP(hipGetLastError());
+    }
     ph_end(b, PH_KREPLAY);
   }
   ph_end(b, PH_REPLAY);

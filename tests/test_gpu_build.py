@@ -73,6 +73,20 @@ def test_random_reads_byte_exact(gx, k, paired, method):
     assert_same(gpu_records(gx, text, k, method=method), ora_records(text, k))
 
 
+@pytest.mark.parametrize("k,G,cov", [(21, 20000, 100), (33, 8000, 150)])
+def test_repeat_rich_long_replays_byte_exact(gx, k, G, cov):
+    # repeat families at low divergence: k-mers with thousands of records and
+    # branching neighbours (the long-run replay path)
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=G, read_len=100, coverage=cov, error_rate=0.01, paired=True,
+                           insert_mean=300, insert_sd=30, repeat_frac=0.3, repeat_len=(200, 400),
+                           repeat_div=0.01, seed=31337 + k)
+    text = sim.simulate(spec)
+    want = ora_records(text, k)
+    for method in ("sort", "hash"):
+        assert_same(gpu_records(gx, text, k, method=method), want)
+
+
 @pytest.mark.parametrize("k", [4, 9, 31, 55])
 def test_oversized_buckets_byte_exact(gx, k):
     # thousands of homopolymer and dinucleotide reads put far more than one
