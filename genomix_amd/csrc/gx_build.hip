@@ -1678,22 +1678,22 @@ __device__ inline LList edge_list(const SerCtx& c, uint64_t i, int t, KV v) {
     l.bits = (uint32_t)(w >> 3) & 0xFF;
     return l;
   }
-  // ascending neighbour k-mer (numeric V = VKmer byte order)
+  // ascending neighbour k-mer (numeric V = VKmer byte order): the rank of
+  // each letter is the number of smaller neighbours (the neighbours of one
+  // type are distinct); fully unrolled so everything stays in registers
   KV nk[4];
-  int idx[4], n = 0;
-  for (uint32_t x = 0; x < 4; x++) {
+#pragma unroll
+  for (int x = 0; x < 4; x++) nk[x] = ((sel >> x) & 1) ? neighbour<W>(v, c.k, (uint32_t)t, (uint32_t)x) : KV{0, 0};
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
     if (!((sel >> x) & 1)) continue;
-    nk[n] = neighbour<W>(v, c.k, (uint32_t)t, x);
-    idx[n] = (int)x;
-    n++;
+    int r = 0;
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+      if (y != x && ((sel >> y) & 1) && kle<W>(nk[y], nk[x])) r++;
+    l.bits |= (uint32_t)x << (2 * r);
   }
-  for (int a = 1; a < n; a++)
-    for (int b = a; b > 0 && !kle<W>(nk[b - 1], nk[b]); b--) {
-      KV tk = nk[b]; nk[b] = nk[b - 1]; nk[b - 1] = tk;
-      int ti = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = ti;
-    }
-  for (int a = 0; a < n; a++) l.bits |= (uint32_t)idx[a] << (2 * a);
-  l.n = n;
+  l.n = __popc(sel);
   return l;
 }
 
@@ -1797,7 +1797,7 @@ __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uin
 // thread writes its record into LDS, then the block stores the whole span
 // with coalesced 4-byte writes.  Spans larger than the staging buffer are
 // written directly.
-constexpr int kSerStage = 40960;
+constexpr int kSerStage = 30720;
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* __restrict__ roff,
