@@ -2202,11 +2202,39 @@ __global__ void k_useq_size(USerCtx c, uint64_t* __restrict__ rsize) {
     rsize[u] = unitig_record_size(c, u);
 }
 
+// Sequences longer than kUseqLong bases are written by k_useq_long (a wave
+// per unitig, lanes over bytes); k_useq_write leaves their bytes to it and
+// records where they start.
+constexpr uint64_t kUseqLong = 256;
+
 template <int W>
-__global__ void k_useq_write(USerCtx c, const uint64_t* __restrict__ roff, uint8_t* __restrict__ out) {
-  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < c.nu; u += (uint64_t)gridDim.x * blockDim.x) {
-    uint8_t* p = out + roff[u];
-    uint64_t rec = roff[u + 1] - roff[u];
+__device__ inline uint8_t useq_byte(const USerCtx& c, const uint32_t* ord, uint64_t len, uint64_t q) {
+  // byte q from the least significant end holds bases 4q..4q+3 (letter 0 low)
+  uint32_t byte = 0;
+  uint32_t o0 = ord[0];
+  for (int s2 = 0; s2 < 4; s2++) {
+    uint64_t j = 4 * q + s2;
+    if (j >= len) break;
+    uint32_t x;
+    if (j < (uint64_t)c.k) {
+      x = oriented_letter<W>(c.klo, c.khi, o0 & 0x7fffffffu, !(o0 & 0x80000000u), c.k, (int)j);
+    } else {
+      uint32_t o = ord[j - c.k + 1];
+      x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
+    }
+    byte |= x << (2 * s2);
+  }
+  return (uint8_t)byte;
+}
+
+// Records of kBlock consecutive unitigs are contiguous: each thread writes
+// its record into an LDS stage and the block stores the span with
+// coalesced 4-byte writes (spans larger than the stage go direct).
+template <int W>
+__device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t* rs, uint64_t rec,
+                                           uint64_t rec_at, uint64_t* __restrict__ seq_at) {
+  {
+    uint8_t* p = rs;
     uint32_t rep = c.urep_node[u];
     KV v = node_key<W>(c.klo, c.khi, rep);
     put32(p, (uint32_t)(rec - 8));
@@ -2258,23 +2286,11 @@ __global__ void k_useq_write(USerCtx c, const uint64_t* __restrict__ roff, uint8
       put32(p, (uint32_t)len);
       uint64_t nbl = (len + 3) / 4;
       const uint32_t* ord = c.order + c.ubase[u];
-      uint32_t o0 = ord[0];
-      for (uint64_t bq = 0; bq < nbl; bq++) {
-        uint64_t q = nbl - 1 - bq;  // byte index from the least significant end
-        uint32_t byte = 0;
-        for (int s2 = 0; s2 < 4; s2++) {
-          uint64_t j = 4 * q + s2;
-          if (j >= len) break;
-          uint32_t x;
-          if (j < (uint64_t)c.k) {
-            x = oriented_letter<W>(c.klo, c.khi, o0 & 0x7fffffffu, !(o0 & 0x80000000u), c.k, (int)j);
-          } else {
-            uint32_t o = ord[j - c.k + 1];
-            x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
-          }
-          byte |= x << (2 * s2);
-        }
-        *p++ = (uint8_t)byte;
+      if (len > kUseqLong) {
+        seq_at[u] = rec_at + (uint64_t)(p - rs);
+        p += nbl;
+      } else {
+        for (uint64_t bq = 0; bq < nbl; bq++) *p++ = useq_byte<W>(c, ord, len, nbl - 1 - bq);
       }
     }
     double cov = (double)c.ucov[u] / (double)m;
@@ -2282,6 +2298,47 @@ __global__ void k_useq_write(USerCtx c, const uint64_t* __restrict__ roff, uint8
     uint32_t bits;
     memcpy(&bits, &fc, 4);
     put32(p, bits);
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t* __restrict__ roff,
+                                                       uint8_t* __restrict__ out, uint64_t* __restrict__ seq_at) {
+  __shared__ __attribute__((aligned(16))) uint8_t stage[kSerStage + 8];
+  const uint64_t first = (uint64_t)blockIdx.x * kBlock;
+  if (first >= c.nu) return;
+  const uint64_t last = first + kBlock < c.nu ? first + kBlock : c.nu;
+  const uint64_t s0 = roff[first], s1 = roff[last];
+  const uint64_t base = s0 & ~3ull;
+  const bool staged = (s1 - base) <= kSerStage;
+  const uint64_t u = first + threadIdx.x;
+  if (u < last)
+    write_unitig_record<W>(c, u, staged ? stage + (roff[u] - base) : out + roff[u], roff[u + 1] - roff[u], roff[u],
+                           seq_at);
+  if (!staged) return;
+  __syncthreads();
+  uint64_t a4 = (s0 + 3) & ~3ull, e4 = s1 & ~3ull;
+  if (a4 > e4) { a4 = s1; e4 = s1; }
+  for (uint64_t x = s0 + threadIdx.x; x < a4; x += kBlock) out[x] = stage[x - base];
+  for (uint64_t x = e4 + threadIdx.x; x < s1; x += kBlock) out[x] = stage[x - base];
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
+  uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
+  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_useq_long(USerCtx c, const uint64_t* __restrict__ seq_at,
+                                                     uint8_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t u = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); u < c.nu; u += nw) {
+    if (c.ulen[u] <= 1) continue;
+    const uint64_t len = useq_len(c, u);
+    if (len <= kUseqLong) continue;
+    const uint64_t nbl = (len + 3) / 4;
+    const uint32_t* ord = c.order + c.ubase[u];
+    uint8_t* p = out + seq_at[u];
+    for (uint64_t bq = lane; bq < nbl; bq += 64) p[bq] = useq_byte<W>(c, ord, len, nbl - 1 - bq);
   }
 }
 
@@ -2329,6 +2386,7 @@ struct gx_build {
   DevBuf sva, svb;  // sorted aggregation: record values
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
+  DevBuf u_seqat;             // unitig pass: byte position of long sequences
   // learned sizing
   gx_mlp* sizer = nullptr;
   unsigned grid = 2048;  // blocks of grid-stride kernels (8 per CU)
@@ -3122,7 +3180,12 @@ static void run_unitigs(gx_build* b) {
   uint64_t total = read1(st, roff + nu);
   uint8_t* out = buf<uint8_t>(b->u_out, total + 1);
   if (nu) {
-    hipLaunchKernelGGL(k_useq_write<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, c, roff, out);
+    uint64_t* seq_at = buf<uint64_t>(b->u_seqat, nu + 1);
+    // one block per kBlock unitigs (no grid-stride loop: every block stages its own span)
+    hipLaunchKernelGGL(k_useq_write<W>, dim3(grid_for(nu, kBlock, 0xffffffffu)), dim3(kBlock), 0, st, c, roff, out,
+                       seq_at);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_useq_long<W>, dim3(grid_for(nu, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, c, seq_at, out);
     GX_HIP(hipGetLastError());
   }
   ph_end(b, PH_UNITIG);
