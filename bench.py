@@ -14,9 +14,10 @@ Workload (default): seeded 5 Mbp genome with 5% in 2 kb repeat copies at
 1% divergence; 150 bp paired-end reads, insert 500/50, 50x, 1% substitution
 errors; k = 31.  Synthetic data, generated at start-up.
 
-The JSON line also carries the roofline of the insert kernel (k_insert:
-per-occurrence algorithmic bytes B = L/(L-k+1) + 2 * (8*ceil(k/32) + 8),
-measured with hipEvents on the library's stream) and a CPU baseline: the
+The JSON line also carries the roofline of the dominant kernel (the
+pointer-jumping round of the unitig pass: 48 algorithmic bytes per node end
+per launch, every launch timed with hipEvents on the library's stream), the
+same for the k-mer pass of the aggregation, and a CPU baseline: the
 CPU oracle (a single-threaded C++ restatement of the reference's Hyracks
 dataflow) timed on a bounded sample of the same reads.
 
@@ -44,11 +45,6 @@ METRIC = "canonical k-mers/s into graph (bit-exact) + % HBM roofline; batched vs
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def occurrences_per_byte_model(L, k):
-    S = 8 * ((k + 31) // 32) + 8
-    return L / (L - k + 1) + 2 * S
 
 
 def cpu_baseline(text, k, target_pairs=16000):
@@ -229,15 +225,28 @@ def main():
     N = st["occurrences"]
     value = N / (elapsed / args.steps)
     phases = {kname: v / args.steps for kname, v in phases_sum.items()}
-    B = occurrences_per_byte_model(L, k)
-    t_ins = phases.get("k_insert", 0.0) * 1e-3
-    achieved = B * N / t_ins / 1e9 if t_ins > 0 else 0.0
-    traffic = load_traffic("k_insert_traffic_plain_gpu_5mbp.json")
+    # dominant kernel: the pointer-jumping round of the unitig pass, 2 elements
+    # (node ends) per distinct k-mer, each round reads its own 16-byte record
+    # and its successor's and writes its new one: 48 algorithmic bytes per
+    # element per launch; hipEvents around every launch on the library stream
+    rounds = max(1, st["pj_rounds"])
+    t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
+    elems = 2 * st["distinct"]
+    achieved = 48.0 * elems / t_pj / 1e9 if t_pj > 0 else 0.0
+    traffic = load_traffic("k_pj_round_traffic_plain_gpu_5mbp.json")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "kernel": "k_insert", "algorithmic_bytes_per_kmer": round(B, 4),
-                "kernel_ms": round(phases.get("k_insert", 0.0), 4)}
+                "kernel": "k_pj_round", "algorithmic_bytes_per_launch": 48 * elems,
+                "kernel_ms": round(t_pj * 1e3, 4), "launches_per_step": rounds}
+    # the k-mer pass of the aggregation (k-mer extraction fused with the first
+    # radix scatter): two reads of the 2-bit packed mates, 16-byte record out
+    Bk = 2 * (L / (L - k + 1)) / 4 + 16
+    t_k = phases.get("k_insert", 0.0) * 1e-3
+    ach_k = Bk * N / t_k / 1e9 if t_k > 0 else 0.0
+    roofline_kmer = {"bound": "hbm", "achieved": round(ach_k, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
+                     "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
     unitigs = builder.unitig_count()
     builder.close()
     dtext.free()
@@ -263,6 +272,7 @@ def main():
                    "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
+        "roofline_kmer_pass": roofline_kmer,
         "batched": batched,
         "cpu_baseline": cpu,
     }

@@ -2350,6 +2350,8 @@ __global__ void __launch_bounds__(kBlock) k_useq_long(USerCtx c, const uint64_t*
 
 using namespace gx;
 
+constexpr int kMaxTimedRounds = 64;  // pointer-jumping rounds timed per unitig run
+
 struct gx_build {
   gx_build_cfg cfg{};
   int W = 1, nb = 0;
@@ -2372,6 +2374,8 @@ struct gx_build {
   uint64_t stats[12] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
+  hipEvent_t evpj[2 * kMaxTimedRounds] = {};  // per pointer-jumping round
+  int npj = 0;
   hipEvent_t ev[2 * 16] = {};
   bool ev_used[16] = {};
   double phase_ms[16] = {};
@@ -2402,10 +2406,11 @@ struct gx_build {
 static std::string g_create_err;
 
 enum Phase {
-  PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX, NPH
+  PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX,
+  PH_KPJ, NPH
 };
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay", "unitig", "sizing", "node_index"};
+                                       "k_replay", "unitig", "sizing", "node_index", "k_pj_round"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -2428,6 +2433,32 @@ static void ph_collect(gx_build* b) {
     }
     b->ev_used[i] = false;
   }
+  if (b->prof && b->npj) {
+    double tot = 0;
+    for (int r = 0; r < b->npj; r++) {
+      float ms = 0;
+      GX_HIP(hipEventElapsedTime(&ms, b->evpj[2 * r], b->evpj[2 * r + 1]));
+      tot += ms;
+    }
+    b->phase_ms[PH_KPJ] = tot;
+  }
+  b->npj = 0;
+}
+
+// events around each launch of a repeated kernel (summed at collect)
+static void pj_begin(gx_build* b) {
+  if (!b->prof || b->npj >= kMaxTimedRounds) return;
+  int r = b->npj;
+  if (!b->evpj[2 * r]) {
+    GX_HIP(hipEventCreate(&b->evpj[2 * r]));
+    GX_HIP(hipEventCreate(&b->evpj[2 * r + 1]));
+  }
+  GX_HIP(hipEventRecord(b->evpj[2 * r], b->st));
+}
+static void pj_end(gx_build* b) {
+  if (!b->prof || b->npj >= kMaxTimedRounds) return;
+  GX_HIP(hipEventRecord(b->evpj[2 * b->npj + 1], b->st));
+  b->npj++;
 }
 
 static char* scratch_for(gx_build* b, size_t bytes) { return b->scratch.ensure(bytes) ? b->scratch.as<char>() : nullptr; }
@@ -3073,8 +3104,10 @@ static void run_unitigs(gx_build* b) {
     uint32_t act = nn ? 1 : 0;
     for (int r = 0; r <= rmax && act; r++) {
       unsigned g = grid_for(ne, kBlock, jgrid);
+      pj_begin(b);
       hipLaunchKernelGGL(k_pj_round, dim3(g), dim3(kBlock), 0, st, ne, R[cur], R[cur ^ 1], bflag);
       GX_HIP(hipGetLastError());
+      pj_end(b);
       hipLaunchKernelGGL(k_flag_or, dim3(1), dim3(kBlock), 0, st, bflag, g, anyflag);
       GX_HIP(hipGetLastError());
       cur ^= 1;
@@ -3453,7 +3486,8 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
 
 int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
-  for (int i = 0; i < n && i < 9; i++) out[i] = b->stats[i];
+  b->stats[10] = b->u_rounds;
+  for (int i = 0; i < n && i < 11; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 
@@ -3591,6 +3625,8 @@ void gx_build_destroy(gx_build* b) {
   if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->st) (void)hipStreamDestroy(b->st);
   for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : b->evpj)
     if (e) (void)hipEventDestroy(e);
   delete b;
 }

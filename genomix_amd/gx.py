@@ -210,10 +210,10 @@ class GraphBuilder:
         return buf.raw[:w.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 9)()
-        self._check(lib().gx_build_stats(self._h, a, 9))
+        a = (ctypes.c_uint64 * 11)()
+        self._check(lib().gx_build_stats(self._h, a, 11))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
-                "max_coverage", "predicted_distinct", "table_rebuilds"]
+                "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

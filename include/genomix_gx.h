@@ -129,7 +129,9 @@ int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
  * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed
  * occurrences, [6] largest coverage above 255 (0 if none), [7] distinct
  * count predicted by an attached sizing network (0 if none, genomix_sizer.h),
- * [8] table rebuilds after overflow. */
+ * [8] table rebuilds after overflow, [9] records sorted by the global
+ * fallback for oversized key buckets (sorted aggregation), [10] pointer-
+ * jumping rounds of the last unitig run. */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on
