@@ -2391,6 +2391,7 @@ struct gx_build {
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
+  ReclaimList reclaim;        // buffers the last build reads no more
   // learned sizing
   gx_mlp* sizer = nullptr;
   unsigned grid = 2048;  // blocks of grid-stride kernels (8 per CU)
@@ -2764,6 +2765,7 @@ static void run_build(gx_build* b) {
     }
     rseg = nst;
     rbytes = rcodes;
+    b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->ckey, &b->ckey2, &b->sva, &b->svb});  // records are dead
     ph_end(b, PH_NODES);
   } else {
     // 4. table insert (grow-and-rebuild on overflow).  Capacity: the next power
@@ -2847,6 +2849,7 @@ static void run_build(gx_build* b) {
       if (dup != ~0ull) throw GxError(GX_EDEVICE, "internal error: k-mer stored in two table slots (node " +
                                                       std::to_string(dup) + ")");
     }
+    b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->ckey, &b->ckey2, &b->cval, &b->cval2, &b->claimed});
     ph_end(b, PH_NODES);
 
   }
@@ -2904,6 +2907,10 @@ static void run_build(gx_build* b) {
     GX_HIP(hipGetLastError());
     rbytes = cb;
     rseg = seg;
+    // occurrence records, selection and table are dead now
+    b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->occ_slot, &b->occ_code, &b->selflag, &b->selpos, &b->selk,
+                                                   &b->selv, &b->selk2, &b->selv2, &b->table});
+    table = nullptr;
     }
     // replay list ordered by descending record count
     uint64_t* rpos = buf<uint64_t>(b->rpos, nn + 1);
@@ -3442,6 +3449,8 @@ int gx_build_finish(gx_build* b) {
       b->src_len = b->text_len;
     }
     b->unitigs_done = false;
+    b->reclaim.dead.clear();
+    ReclaimScope rs(&b->reclaim, b->st);
     if (b->W == 1) run_build<1>(b);
     else run_build<2>(b);
     b->finished = true;
@@ -3496,6 +3505,7 @@ int gx_unitig_run(gx_build* b) {
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "unitig_run before finish");
     GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
     if (b->W == 1) run_unitigs<1>(b);
     else run_unitigs<2>(b);
     b->unitigs_done = true;

@@ -24,6 +24,17 @@ struct HipError : std::runtime_error {
   HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
 };
 
+struct DevBuf;
+
+// Buffers that the running build reads no more.  They stay allocated for the
+// next build on the handle and are freed only when an allocation would
+// otherwise fail; ensure() of a listed buffer takes it off the list.
+struct ReclaimList {
+  hipStream_t st = nullptr;
+  std::vector<DevBuf*> dead;
+};
+inline thread_local ReclaimList* tl_reclaim = nullptr;
+
 // Growable device allocation, reused across builds on one handle.
 struct DevBuf {
   void* p = nullptr;
@@ -37,17 +48,66 @@ struct DevBuf {
     p = nullptr;
     cap = 0;
   }
-  void* ensure(size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    if (bytes > cap) {
-      release();
-      GX_HIP(hipMalloc(&p, bytes));
-      cap = bytes;
-    }
-    return p;
-  }
+  inline void* ensure(size_t bytes);
   template <class T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// Frees the listed buffers (after the stream has drained: kernels in flight
+// may still read them); false if there was nothing to free.
+inline bool reclaim_dead(const DevBuf* self) {
+  ReclaimList* r = tl_reclaim;
+  if (!r || r->dead.empty()) return false;
+  (void)hipStreamSynchronize(r->st);
+  bool any = false;
+  for (DevBuf* d : r->dead)
+    if (d != self && d->p) {
+      d->release();
+      any = true;
+    }
+  r->dead.clear();
+  return any;
+}
+
+inline void* DevBuf::ensure(size_t bytes) {
+  if (tl_reclaim) {
+    auto& v = tl_reclaim->dead;
+    for (size_t i = 0; i < v.size(); i++)
+      if (v[i] == this) {
+        v[i] = v.back();
+        v.pop_back();
+        break;
+      }
+  }
+  if (bytes == 0) bytes = 256;
+  if (bytes > cap) {
+    release();
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory && reclaim_dead(this)) {
+      (void)hipGetLastError();
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw HipError("hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e), (int)e);
+    }
+    cap = bytes;
+  }
+  return p;
+}
+
+// Installs a handle's reclaim list for the calling thread while a build
+// phase runs.
+struct ReclaimScope {
+  ReclaimList* prev;
+  ReclaimScope(ReclaimList* r, hipStream_t st) : prev(tl_reclaim) {
+    r->st = st;
+    tl_reclaim = r;
+  }
+  ~ReclaimScope() { tl_reclaim = prev; }
+  ReclaimScope(const ReclaimScope&) = delete;
+  ReclaimScope& operator=(const ReclaimScope&) = delete;
 };
 
 constexpr int kBlock = 256;
