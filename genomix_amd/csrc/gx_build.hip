@@ -30,6 +30,7 @@
 #include "gx_prims.h"
 #include "gx_sizer.h"
 #include "../../include/genomix_batch.h"
+#include "../../include/genomix_stream.h"
 
 namespace gx {
 
@@ -886,6 +887,9 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
 }
 
 constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
+// HBM per node of a sorted build (key, count, mask, run start, index slots,
+// replay and head arrays, serialised record), by key width: sizes key ranges
+constexpr double kNodeBytes[2] = {224.0, 268.0};
 constexpr int kIdxBits = 13;      // log2(kBucketCap)
 
 // The bucket bytes are sorted least significant first (LSD): the fused
@@ -943,7 +947,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
                                                      const uint32_t* __restrict__ mlen,
                                                      const uint64_t* __restrict__ wbase,
                                                      const uint64_t* __restrict__ words, int k, int nb,
-                                                     uint32_t* __restrict__ hist, unsigned ntiles) {
+                                                     uint32_t* __restrict__ hist, unsigned ntiles, uint32_t tlo,
+                                                     uint32_t thi) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
   __shared__ uint64_t lob[kTileMateCap];
@@ -954,7 +959,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
                             if (valid) {
                               SK sk;
                               make_sort_key(key, nb, sk);
-                              atomicAdd(&h[key_byte(sk, first_digit_byte(nb))], 1u);
+                              const uint32_t t = key_byte(sk, nb - 1);
+                              if (t >= tlo && t < thi) atomicAdd(&h[key_byte(sk, first_digit_byte(nb))], 1u);
                             }
                           });
   __syncthreads();
@@ -971,7 +977,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         const uint64_t* __restrict__ words, int k, int nb,
                                                         const uint64_t* __restrict__ offs, unsigned ntiles,
                                                         typename SortKeyOf<W>::T* __restrict__ okeys,
-                                                        uint64_t* __restrict__ ovals) {
+                                                        uint64_t* __restrict__ ovals, uint32_t tlo, uint32_t thi) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint64_t run[256];
   __shared__ uint16_t wcnt[kBlock / 64][256];
@@ -989,6 +995,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     if (valid) {
       make_sort_key(key, nb, sk);
       d = key_byte(sk, first_digit_byte(nb));
+      const uint32_t t = key_byte(sk, nb - 1);
+      valid = t >= tlo && t < thi;  // key-range pass: other records wait for their range
     }
     uint64_t act = __ballot(valid);
     uint64_t match = act;
@@ -1012,6 +1020,31 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     run[threadIdx.x] += tot;
     __syncthreads();
   });
+}
+
+// Occurrences per value of the sort key's top byte (the key ranges of a
+// multi-pass build are cut on it).
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_top_hist(uint64_t N, uint64_t nmates, const uint64_t* __restrict__ occbase,
+                                                     const uint32_t* __restrict__ mlen,
+                                                     const uint64_t* __restrict__ wbase,
+                                                     const uint64_t* __restrict__ words, int k, int nb,
+                                                     unsigned long long* __restrict__ out) {
+  using SK = typename SortKeyOf<W>::T;
+  __shared__ uint32_t h[256];
+  __shared__ uint64_t lob[kTileMateCap];
+  h[threadIdx.x] = 0;
+  uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
+  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
+                          [&](int, bool valid, uint64_t, KV key, uint32_t) {
+                            if (valid) {
+                              SK sk;
+                              make_sort_key(key, nb, sk);
+                              atomicAdd(&h[key_byte(sk, nb - 1)], 1u);
+                            }
+                          });
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
 // First record of every bucket (start[NB] = N; empty buckets start where
@@ -1594,11 +1627,22 @@ __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, cons
                         const uint64_t* __restrict__ hidx, NodeIndex ix,
                         const int64_t* __restrict__ rid, const uint64_t* __restrict__ wbase,
                         const uint64_t* __restrict__ words, int k, uint64_t* __restrict__ hkey,
-                        uint32_t* __restrict__ hval) {
+                        uint32_t* __restrict__ hval, uint32_t tlo, uint32_t thi) {
+  using SK = typename SortKeyOf<W>::T;
   for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nmates; m += (uint64_t)gridDim.x * blockDim.x) {
     if (!valid[m] || nocc[m] == 0) continue;
     bool fwd;
     KV kk = canon<W>(load_kmer<W>(words + wbase[m], 0, k), k, &fwd);
+    if (tlo != 0 || thi != 256) {
+      SK sk;
+      make_sort_key(kk, ix.nb, sk);
+      const uint32_t t = key_byte(sk, ix.nb - 1);
+      if (t < tlo || t >= thi) {  // another key range holds this head's node
+        hkey[hidx[m]] = ~0ull;
+        hval[hidx[m]] = 0;
+        continue;
+      }
+    }
     uint64_t node = node_lookup<W>(ix, kk);
     uint64_t flipped = fwd ? 0 : 1;
     uint64_t key = (node << 31) | (flipped << 30) | ((m & 1) << 29) | (uint64_t)rid[m >> 1];
@@ -1612,7 +1656,7 @@ __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, cons
 // of unflipped / flipped heads.
 __global__ void k_head_unique(uint64_t nh, const uint64_t* __restrict__ hkey, uint32_t* __restrict__ uflag) {
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x)
-    uflag[j] = (j == 0 || hkey[j] != hkey[j - 1]) ? 1 : 0;
+    uflag[j] = hkey[j] != ~0ull && (j == 0 || hkey[j] != hkey[j - 1]) ? 1 : 0;  // ~0: head of another range
 }
 
 __global__ void k_head_compact(uint64_t nh, const uint64_t* __restrict__ hkey, const uint32_t* __restrict__ hval,
@@ -2391,6 +2435,12 @@ struct gx_build {
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
+  DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
+  uint64_t range_occ = 0;     // occurrences per key-range pass (0: from the HBM size)
+  gx_part_sink sink = nullptr;
+  void* sink_user = nullptr;
+  bool streamed = false;      // the last build's records went to the sink
+  uint32_t nranges = 1;
   ReclaimList reclaim;        // buffers the last build reads no more
   // learned sizing
   gx_mlp* sizer = nullptr;
@@ -2425,12 +2475,13 @@ static void ph_begin(gx_build* b, int ph) {
 static void ph_end(gx_build* b, int ph) {
   if (b->prof) GX_HIP(hipEventRecord(b->ev[2 * ph + 1], b->st));
 }
-static void ph_collect(gx_build* b) {
+// add: sum into the phase totals (a build of several key ranges)
+static void ph_collect(gx_build* b, bool add = false) {
   for (int i = 0; i < NPH; i++) {
     if (b->prof && b->ev_used[i]) {
       float ms = 0;
       GX_HIP(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
-      b->phase_ms[i] = ms;
+      b->phase_ms[i] = add ? b->phase_ms[i] + ms : ms;
     }
     b->ev_used[i] = false;
   }
@@ -2544,9 +2595,13 @@ static void sizing_features(const uint8_t* regs, double occ_s, double valid_s, d
   f[71] = 1.f;
 }
 
+static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind);
+
 template <int W>
 static void run_build(gx_build* b) {
   hipStream_t st = b->st;
+  for (int i = 0; i < NPH; i++)
+    if (i != PH_UNITIG && i != PH_KPJ) b->phase_ms[i] = 0;
   const int k = b->cfg.k, nb = b->nb;
   const uint64_t n = b->src_len;
   const uint8_t* t = b->src;
@@ -2655,40 +2710,96 @@ static void run_build(gx_build* b) {
   //      emission order; plus a key -> node table for lookups by key.
   using SK = typename SortKeyOf<W>::T;
   const bool sorted = b->method == GX_AGG_SORT;
-  uint64_t nn = 0, cap = 0, rebuilds = 0;
+  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
+  const uint32_t P = (uint32_t)b->cfg.nparts;
+
+  // 4'. key ranges.  The sorted method takes the occurrences in passes over
+  //     ranges of the sort key's top byte when one pass would not fit in
+  //     HBM (or when the caller bounds the occurrences per pass): every node
+  //     lies in one range, the ranges follow KmerPointable order, and each
+  //     range's records are serialised and handed on before the next.
+  std::vector<uint32_t> tb{0u, 256u};
+  std::vector<uint64_t> rocc{N};
+  if (sorted && N) {
+    uint64_t target = b->range_occ;
+    if (!target) {
+      size_t fr = 0, tot = 0;
+      GX_HIP(hipMemGetInfo(&fr, &tot));
+      size_t held = 0;
+      for (DevBuf* d : {&b->text, &b->text2, &b->rtext, &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase,
+                        &b->words, &b->valid, &b->nocc, &b->occbase, &b->hmask, &b->hidx})
+        held += d->cap;
+      const double dfrac = b->pred[0] > 0 ? std::min(1.0, b->pred[0] / (double)N) : 0.6;
+      const double per_occ = std::max<double>(2.0 * sizeof(SK) + 17.0, dfrac * (double)kNodeBytes[W - 1] + 1.0);
+      const double budget = 0.7 * (double)tot - (double)held - 48.0 * (double)nheads;
+      target = budget > per_occ ? (uint64_t)(budget / per_occ) : 1;
+    }
+    if (target < N) {
+      unsigned long long* th = buf<unsigned long long>(b->tophist, 256);
+      GX_HIP(hipMemsetAsync(th, 0, 256 * 8, st));
+      const unsigned ntiles = (unsigned)((N + kSortTile - 1) / kSortTile);
+      hipLaunchKernelGGL(k_top_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
+                         nb, th);
+      GX_HIP(hipGetLastError());
+      uint64_t h[256];
+      GX_HIP(hipMemcpyAsync(h, th, sizeof(h), hipMemcpyDeviceToHost, st));
+      GX_HIP(hipStreamSynchronize(st));
+      tb.assign(1, 0u);
+      rocc.clear();
+      uint64_t cur = 0;
+      for (uint32_t t = 0; t < 256; t++) {
+        if (cur && cur + h[t] > target) {  // close the range before this byte value
+          tb.push_back(t);
+          rocc.push_back(cur);
+          cur = 0;
+        }
+        cur += h[t];
+      }
+      tb.push_back(256u);
+      rocc.push_back(cur);
+    }
+  }
+  const uint32_t R = (uint32_t)rocc.size();
+  uint64_t cap = 0, rebuilds = 0, tot_nn = 0, tot_nrep = 0, tot_nsel = 0, tot_bytes = 0, acc_len = 0;
+  std::vector<uint64_t> seg_src, seg_len;  // several ranges, no sink: range r's part p bytes in outacc
+  std::vector<uint64_t> prec(P + 1, 0), pbyt(P + 1, 0);  // per partition: records and bytes so far
+  b->nuheads = 0;
+  for (uint32_t ri = 0; ri < R; ri++) {
+  const uint32_t tlo = tb[ri], thi = tb[ri + 1];
+  const uint64_t Nr = rocc[ri];
+  uint64_t nn = 0;
   void* table = nullptr;
   uint64_t *klo = nullptr, *khi = nullptr;
   uint32_t *cnt = nullptr, *msk = nullptr, *nslot = nullptr, *occ_slot = nullptr;
   const uint64_t* rseg = nullptr;  // sorted path: first record of each node
   const uint8_t* rbytes = nullptr;  // sorted path: record codes in node order
   uint8_t* occ_code = nullptr;
-  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
   if (sorted) {
     ph_begin(b, PH_INSERT);
-    SK* ka = buf<SK>(b->ckey, N + 2);
-    SK* kb = buf<SK>(b->ckey2, N + 2);
-    uint64_t* va = buf<uint64_t>(b->sva, N + 2);
-    uint64_t* vb = buf<uint64_t>(b->svb, N + 2);
+    SK* ka = buf<SK>(b->ckey, Nr + 2);
+    SK* kb = buf<SK>(b->ckey2, Nr + 2);
+    uint64_t* va = buf<uint64_t>(b->sva, Nr + 2);
+    uint64_t* vb = buf<uint64_t>(b->svb, Nr + 2);
     const unsigned ntiles = (unsigned)std::max<uint64_t>(1, (N + kSortTile - 1) / kSortTile);
     const size_t hn = (size_t)256 * ntiles;
     uint32_t* hist = buf<uint32_t>(b->selflag, hn + 1);
     uint64_t* hoff = buf<uint64_t>(b->seg, hn + 1);
-    if (N) {
+    if (Nr) {
       ph_begin(b, PH_KINSERT);
       hipLaunchKernelGGL(k_occ_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
-                         nb, hist, ntiles);
+                         nb, hist, ntiles, tlo, thi);
       GX_HIP(hipGetLastError());
       exclusive_scan<uint32_t, uint64_t>(hist, hoff, hn, scratch_for(b, scan_scratch_bytes<uint64_t>(hn)), st);
       hipLaunchKernelGGL(k_occ_scatter<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
-                         k, nb, hoff, ntiles, ka, va);
+                         k, nb, hoff, ntiles, ka, va, tlo, thi);
       GX_HIP(hipGetLastError());
       ph_end(b, PH_KINSERT);
     }
     // the more significant bucket byte
     SK *kx = ka, *ky = kb;
     uint64_t *vx = va, *vy = vb;
-    if (nb >= 2 && N > 1) {
-      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, N, nb - 1, nb, scratch_for(b, radix_scratch_bytes<SK>(N)),
+    if (nb >= 2 && Nr > 1) {
+      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, Nr, nb - 1, nb, scratch_for(b, radix_scratch_bytes<SK>(Nr)),
                                         st);
       if (f) { std::swap(kx, ky); std::swap(vx, vy); }
     }
@@ -2699,14 +2810,14 @@ static void run_build(gx_build* b) {
     uint64_t* bsz = buf<uint64_t>(b->htot, NBK + 1);
     uint64_t* boff = buf<uint64_t>(b->upos, NBK + 2);
     GX_HIP(hipMemsetAsync(big, 0, (NBK + 1) * 4, st));
-    hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(N + 1, kBlock, G)), dim3(kBlock), 0, st, N, kx, nb, NBK,
+    hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(Nr + 1, kBlock, G)), dim3(kBlock), 0, st, Nr, kx, nb, NBK,
                        bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
     hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(NBK), dim3(512), 0, st, kx, vx, ky, vy, bstart,
-                       N, nb, big, errw + 2);
+                       Nr, nb, big, errw + 2);
     GX_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(NBK), dim3(1024), 0, st, kx, vx, ky, vy, bstart, N,
+    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(NBK), dim3(1024), 0, st, kx, vx, ky, vy, bstart, Nr,
                        nb, big, errw + 2);
     GX_HIP(hipGetLastError());
     if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
@@ -2736,28 +2847,28 @@ static void run_build(gx_build* b) {
     ph_begin(b, PH_NODES);
     uint32_t* rflag = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
     uint64_t* rpos = vx;
-    if (N) {
-      hipLaunchKernelGGL(k_run_flags<SK>, dim3(grid_for(N, kBlock, G)), dim3(kBlock), 0, st, N, ky, rflag);
+    if (Nr) {
+      hipLaunchKernelGGL(k_run_flags<SK>, dim3(grid_for(Nr, kBlock, G)), dim3(kBlock), 0, st, Nr, ky, rflag);
       GX_HIP(hipGetLastError());
     }
-    exclusive_scan<uint32_t, uint64_t>(rflag, rpos, N, scratch_for(b, scan_scratch_bytes<uint64_t>(N)), st);
-    nn = read1(st, rpos + N);
+    exclusive_scan<uint32_t, uint64_t>(rflag, rpos, Nr, scratch_for(b, scan_scratch_bytes<uint64_t>(Nr)), st);
+    nn = read1(st, rpos + Nr);
     if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
     klo = buf<uint64_t>(b->klo, nn + 1);
     khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
     cnt = buf<uint32_t>(b->cnt, nn + 1);
     msk = buf<uint32_t>(b->msk, nn + 1);
     uint64_t* nst = buf<uint64_t>(b->nslot, nn + 2);
-    uint8_t* rcodes = buf<uint8_t>(b->rcode, N + 32);
+    uint8_t* rcodes = buf<uint8_t>(b->rcode, Nr + 32);
     GX_HIP(hipMemsetAsync(msk, 0, (nn + 1) * 4, st));
-    if (N) {
+    if (Nr) {
       // rpos[i + 1] - 1 is the node of record i (rpos = exclusive scan of
-      // the run-start flags, N + 1 entries)
-      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(N, kBlock, 1u << 16)), dim3(kBlock), 0, st, N, ky, vy, rflag,
+      // the run-start flags, Nr + 1 entries)
+      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(Nr, kBlock, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, vy, rflag,
                          rpos, nb, klo, khi, nst, msk, rcodes);
       GX_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, N);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, Nr);
     GX_HIP(hipGetLastError());
     if (nn) {
       hipLaunchKernelGGL(k_node_cnt, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, nst, cnt);
@@ -2955,7 +3066,7 @@ static void run_build(gx_build* b) {
   uint32_t* hv2 = buf<uint32_t>(b->hval2, nheads + 1);
   if (nmates) {
     hipLaunchKernelGGL(k_heads<W>, dim3(grid_for(nmates, kBlock, G)), dim3(kBlock), 0, st, nmates, valid, nocc, hidx,
-                       nix, rid, wbase, words, k, hk, hv);
+                       nix, rid, wbase, words, k, hk, hv, tlo, thi);
     GX_HIP(hipGetLastError());
   }
   int hbytes = 1;
@@ -2977,7 +3088,7 @@ static void run_build(gx_build* b) {
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint32_t, uint64_t>(uflag, upos, nheads, scratch_for(b, scan_scratch_bytes<uint64_t>(nheads)), st);
-  b->nuheads = read1(st, upos + nheads);
+  b->nuheads += read1(st, upos + nheads);
   if (nheads) {
     hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, hv, uflag,
                        upos, uval, hcu, hcf);
@@ -3001,7 +3112,6 @@ static void run_build(gx_build* b) {
   c.perm = nullptr;
   GX_HIP(hipMemsetAsync(errw + 1, 0xff, 8, st));
   // P > 1: records are grouped by output partition, each group in key order
-  const uint32_t P = (uint32_t)b->cfg.nparts;
   uint64_t* pstart = nullptr;
   if (P > 1) {
     uint32_t* pid = buf<uint32_t>(b->ppid, nn + 1);
@@ -3052,21 +3162,81 @@ static void run_build(gx_build* b) {
   }
   ph_end(b, PH_SER);
   GX_HIP(hipStreamSynchronize(st));
-  ph_collect(b);
+  ph_collect(b, true);
   unsigned long long badnode = read1(st, errw + 1);
   if (badnode != ~0ull)
     throw GxError(GX_EDEVICE, "internal error: replayed edge list inconsistent at node " + std::to_string(badnode));
-  b->out_bytes = total;
+  tot_nn += nn;
+  tot_nrep += nrep;
+  tot_nsel += nsel;
+  tot_bytes += total;
   b->nnodes = nn;
+  if (b->sink) {  // the range's records, partition by partition, to the caller
+    for (uint32_t p = 0; p < P; p++) {
+      const uint64_t o = b->part_off[p], nbyt = b->part_off[p + 1] - o, nrec = b->part_rec[p + 1] - b->part_rec[p];
+      if (!nbyt) continue;
+      int rc = b->sink(b->sink_user, (int)p, out + o, nbyt, nrec);
+      if (rc) throw GxError(rc < 0 ? rc : GX_EINVAL, "sink returned " + std::to_string(rc));
+    }
+  } else if (R > 1) {  // kept in HBM; partitions are assembled after the last range
+    append_to(b, b->outacc, acc_len, out, total, hipMemcpyDeviceToDevice);
+    for (uint32_t p = 0; p < P; p++) {
+      seg_src.push_back(acc_len - total + b->part_off[p]);
+      seg_len.push_back(b->part_off[p + 1] - b->part_off[p]);
+    }
+  }
+  for (uint32_t p = 0; p < P; p++) {
+    prec[p + 1] += b->part_rec[p + 1] - b->part_rec[p];
+    pbyt[p + 1] += b->part_off[p + 1] - b->part_off[p];
+  }
+  if (R > 1)  // the next range takes these buffers back as it needs them
+    b->reclaim.dead.insert(b->reclaim.dead.end(),
+                           {&b->klo, &b->khi, &b->cnt, &b->msk, &b->nslot, &b->nix_slots, &b->needs, &b->selcnt,
+                            &b->seg, &b->lists, &b->rpos, &b->rkey, &b->rval, &b->rkey2, &b->rval2, &b->rcode,
+                            &b->hcu, &b->hcf, &b->htot, &b->hstart, &b->uval, &b->rsize, &b->roff, &b->out, &b->ppid,
+                            &b->pidx, &b->ppid2, &b->pidx2});
+  }  // key ranges
+
+  for (uint32_t p = 0; p < P; p++) {
+    prec[p + 1] += prec[p];
+    pbyt[p + 1] += pbyt[p];
+  }
+  if (!b->sink && R > 1) {
+    // part p = its bytes of range 0, then of range 1, ...
+    if (P == 1) {
+      std::swap(b->out.p, b->outacc.p);
+      std::swap(b->out.cap, b->outacc.cap);
+      b->out.ensure(tot_bytes + 1);  // live again: off the reclaim list (no allocation)
+    } else {
+      uint8_t* out = buf<uint8_t>(b->out, tot_bytes + 1);
+      uint64_t dst = 0;
+      for (uint32_t p = 0; p < P; p++)
+        for (uint32_t r = 0; r < R; r++) {
+          const size_t i = (size_t)r * P + p;
+          if (seg_len[i])
+            GX_HIP(hipMemcpyAsync(out + dst, b->outacc.as<uint8_t>() + seg_src[i], seg_len[i],
+                                  hipMemcpyDeviceToDevice, st));
+          dst += seg_len[i];
+        }
+      GX_HIP(hipStreamSynchronize(st));
+    }
+    b->reclaim.dead.push_back(&b->outacc);
+  }
+  b->part_rec = prec;
+  b->part_off = pbyt;
+  b->streamed = b->sink != nullptr;
+  b->nranges = R;
+  b->out_bytes = tot_bytes;
   b->stats[0] = N;
-  b->stats[1] = nn;
+  b->stats[1] = tot_nn;
   b->stats[2] = nheads;
   b->stats[3] = cap;
-  b->stats[4] = nrep;
-  b->stats[5] = nsel;
+  b->stats[4] = tot_nrep;
+  b->stats[5] = tot_nsel;
   b->stats[6] = read1(st, errw + 3);
   b->stats[7] = (uint64_t)b->pred[0];
   b->stats[8] = rebuilds;
+  b->stats[11] = R;
 }
 
 template <int W>
@@ -3471,6 +3641,7 @@ int gx_build_emit_part(gx_build* b, int part, uint8_t* dst, size_t cap, size_t* 
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "emit before finish");
+    if (b->streamed) throw GxError(GX_ESTATE, "the records went to the sink");
     if (part < 0 || part >= b->cfg.nparts) throw GxError(GX_EINVAL, "no such partition");
     uint64_t off = b->part_off[part], n = b->part_off[part + 1] - off;
     if (cap < n) throw GxError(GX_EINVAL, "destination too small");
@@ -3486,6 +3657,7 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "part_device before finish");
+    if (b->streamed) throw GxError(GX_ESTATE, "the records went to the sink");
     if (part < 0 || part >= b->cfg.nparts) throw GxError(GX_EINVAL, "no such partition");
     if (!dev_ptr || !nbytes) throw GxError(GX_EINVAL, "null argument");
     *dev_ptr = b->out.as<uint8_t>() + b->part_off[part];
@@ -3496,7 +3668,7 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
 int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
   b->stats[10] = b->u_rounds;
-  for (int i = 0; i < n && i < 11; i++) out[i] = b->stats[i];
+  for (int i = 0; i < n && i < 12; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 
@@ -3504,6 +3676,9 @@ int gx_unitig_run(gx_build* b) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "unitig_run before finish");
+    if (b->nranges > 1)
+      throw GxError(GX_ENOTSUP, "the unitig pass needs the whole graph in HBM; this build ran in " +
+                                    std::to_string(b->nranges) + " key ranges");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     if (b->W == 1) run_unitigs<1>(b);
@@ -3548,6 +3723,15 @@ int gx_build_set_method(gx_build* b, int method) {
   return guarded(b, [&] {
     if (method != GX_AGG_SORT && method != GX_AGG_HASH) throw GxError(GX_EINVAL, "unknown aggregation method");
     b->method = method;
+  });
+}
+
+int gx_build_set_stream(gx_build* b, uint64_t range_occurrences, gx_part_sink sink, void* user) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    b->range_occ = range_occurrences;
+    b->sink = sink;
+    b->sink_user = user;
   });
 }
 

@@ -43,6 +43,14 @@ class DeviceBytes:
             pass
 
 
+def to_host(address, n):
+    """`n` bytes at device address `address`, copied to host memory."""
+    buf = ctypes.create_string_buffer(max(1, n))
+    if n and hip().hipMemcpy(buf, ctypes.c_void_p(address), ctypes.c_size_t(n), 2) != 0:  # device -> host
+        raise RuntimeError("hipMemcpy failed")
+    return buf.raw[:n]
+
+
 def synchronize():
     if hip().hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")

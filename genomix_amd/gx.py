@@ -33,7 +33,7 @@ EXPORTED = [
     "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
     "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
-    "gx_batch_destroy", "gx_build_set_method",
+    "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream",
 ]
 
 
@@ -50,6 +50,8 @@ class _Cfg(ctypes.Structure):
 
 _lib = None
 BATCH_CONSUMER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
+PART_SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                             ctypes.c_uint64)
 
 
 def lib():
@@ -98,6 +100,7 @@ def lib():
         L.gx_build_set_tuning.argtypes = [vp, i32, ctypes.c_float]
         L.gx_build_set_config.argtypes = [vp, ctypes.POINTER(_Cfg)]
         L.gx_build_set_method.argtypes = [vp, i32]
+        L.gx_build_set_stream.argtypes = [vp, ctypes.c_uint64, PART_SINK, vp]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -134,6 +137,25 @@ class GraphBuilder:
         hash table, sized by the learned predictor when one is attached)."""
         self._check(lib().gx_build_set_method(self._h, {"sort": 0, "hash": 1}[method]))
 
+    def set_stream(self, range_occurrences=0, sink=None):
+        """Key-range passes (include/genomix_stream.h): at most about
+        `range_occurrences` k-mer occurrences per pass (0: as HBM needs) and,
+        with `sink`, each range's records handed to
+        ``sink(part, device_address, nbytes, nrec)`` instead of kept."""
+        self._sink_exc = None
+        if sink is None:
+            cb = ctypes.cast(None, PART_SINK)
+        else:
+            def call(_user, part, addr, nbytes, nrec):
+                try:
+                    return int(sink(part, addr, nbytes, nrec) or 0)
+                except BaseException as e:  # reported by finish()
+                    self._sink_exc = e
+                    return GX_EINVAL
+            cb = PART_SINK(call)
+        self._check(lib().gx_build_set_stream(self._h, range_occurrences, cb, None))
+        self._sink_cb = cb  # alive while the library holds it
+
     def set_sizer(self, sizer):
         """Attach a trained sizing network (genomix_amd.sizer.Mlp) or None."""
         self._check(lib().gx_build_set_sizer(self._h, sizer.handle if sizer is not None else None))
@@ -168,7 +190,12 @@ class GraphBuilder:
             self._check(lib().gx_build_add_fastq_pair(self._h, mate1, len(mate1), mate2, len(mate2)))
 
     def finish(self):
-        self._check(lib().gx_build_finish(self._h))
+        rc = lib().gx_build_finish(self._h)
+        exc = getattr(self, "_sink_exc", None)
+        if exc is not None:
+            self._sink_exc = None
+            raise exc
+        self._check(rc)
 
     def readids(self) -> bytes:
         """The library-N.readids text made from FASTQ input (after finish)."""
@@ -210,10 +237,11 @@ class GraphBuilder:
         return buf.raw[:w.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 11)()
-        self._check(lib().gx_build_stats(self._h, a, 11))
+        a = (ctypes.c_uint64 * 12)()
+        self._check(lib().gx_build_stats(self._h, a, 12))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
-                "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds"]
+                "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds",
+                "key_ranges"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

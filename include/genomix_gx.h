@@ -131,7 +131,8 @@ int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
  * count predicted by an attached sizing network (0 if none, genomix_sizer.h),
  * [8] table rebuilds after overflow, [9] records sorted by the global
  * fallback for oversized key buckets (sorted aggregation), [10] pointer-
- * jumping rounds of the last unitig run. */
+ * jumping rounds of the last unitig run, [11] key-range passes of the build
+ * (genomix_stream.h). */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on

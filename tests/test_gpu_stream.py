@@ -1,0 +1,123 @@
+"""Key-range passes and the record sink (include/genomix_stream.h): a build
+taken in several ranges of the sort key's top byte gives, partition by
+partition, the bytes of a single-pass build and of the oracle; a sink
+receives the same bytes range by range."""
+
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gx():
+    import genomix_amd.gx as g
+    return g
+
+
+def _text(seed, G=40000, L=100, cov=20, paired=True):
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=G, read_len=L, coverage=cov, error_rate=0.01, paired=paired,
+                           insert_mean=3 * L, insert_sd=L // 4, repeat_frac=0.05, repeat_len=(300, 900),
+                           repeat_div=0.01, seed=seed)
+    return sim.simulate(spec)
+
+
+def _parts(b, P):
+    return [b.part_bytes(p) for p in range(P)]
+
+
+@pytest.mark.parametrize("k,P", [(21, 1), (31, 3), (55, 2), (33, 1)])
+def test_ranges_match_single_pass_and_oracle(gx, k, P):
+    text = _text(900 + k, paired=k != 33)
+    with gx.GraphBuilder(k, nparts=P) as b:
+        b.add_reads(text)
+        b.finish()
+        want = _parts(b, P)
+        info1 = [b.part_info(p) for p in range(P)]
+        st1 = b.stats()
+        assert st1["key_ranges"] == 1
+        u1 = b.unitigs() if P == 1 else None
+    assert want == oracle.build_graph(text, k, nparts=P)
+    N = st1["occurrences"]
+    for per in (N // 5, N // 40, 1):
+        with gx.GraphBuilder(k, nparts=P) as b:
+            b.set_stream(range_occurrences=per)
+            b.add_reads(text)
+            b.finish()
+            st = b.stats()
+            assert st["key_ranges"] > 1
+            if per == 1:
+                assert st["key_ranges"] >= 200  # one range per top-byte value that occurs
+            assert _parts(b, P) == want, f"range_occurrences={per}"
+            for key in ("occurrences", "distinct", "heads", "replayed_nodes", "replayed_occurrences"):
+                assert st[key] == st1[key], key
+            assert [b.part_info(p) for p in range(P)] == info1
+            if P == 1:
+                with pytest.raises(gx.GxError) as e:
+                    b.run_unitigs()
+                assert e.value.code == gx.GX_ENOTSUP
+    if u1 is not None:  # a one-range build keeps the unitig pass
+        with gx.GraphBuilder(k) as b:
+            b.set_stream(range_occurrences=10 * N)
+            b.add_reads(text)
+            b.finish()
+            assert b.stats()["key_ranges"] == 1
+            assert b.unitigs() == u1
+
+
+@pytest.mark.parametrize("method", ["sort", "hash"])
+def test_sink_receives_each_partition_in_order(gx, method):
+    from genomix_amd import devmem
+    k, P = 31, 4
+    text = _text(77)
+    want = oracle.build_graph(text, k, nparts=P)
+    got = [[] for _ in range(P)]
+    nrec = [0] * P
+
+    def sink(part, addr, nbytes, n):
+        got[part].append(devmem.to_host(addr, nbytes))
+        nrec[part] += n
+        return 0
+
+    with gx.GraphBuilder(k, nparts=P, method=method) as b:
+        b.set_stream(range_occurrences=50_000 if method == "sort" else 0, sink=sink)
+        b.add_reads(text)
+        b.finish()
+        st = b.stats()
+        assert st["key_ranges"] > 1 if method == "sort" else st["key_ranges"] == 1
+        assert [b"".join(c) for c in got] == want
+        for p in range(P):
+            assert b.part_info(p) == (nrec[p], len(want[p]))
+        with pytest.raises(gx.GxError) as e:
+            b.part_bytes(0)
+        assert e.value.code == gx.GX_ESTATE
+        # back to records kept in HBM on the same handle
+        b.set_stream()
+        b.reset()
+        b.add_reads(text)
+        b.finish()
+        assert _parts(b, P) == want
+
+
+def test_sink_error_aborts_the_build(gx):
+    text = _text(5, G=10000)
+
+    def bad(part, addr, nbytes, n):
+        raise ValueError("consumer full")
+
+    def code(part, addr, nbytes, n):
+        return -12
+
+    with gx.GraphBuilder(21) as b:
+        b.set_stream(range_occurrences=1000, sink=bad)
+        b.add_reads(text)
+        with pytest.raises(ValueError, match="consumer full"):
+            b.finish()
+    with gx.GraphBuilder(21) as b:
+        b.set_stream(sink=code)
+        b.add_reads(text)
+        with pytest.raises(gx.GxError) as e:
+            b.finish()
+        assert e.value.code == -12
