@@ -28,6 +28,9 @@ native batch scheduler with the tuned parameters and the learned table
 sizing, inputs resident in HBM; its k-mers/s beside the plain figure.
 --workload learned runs the learned-sizing workload (100 Mbp, 20% repeat
 families, 150 bp PE, 40x, k = 31) with and without the sizing network.
+--workload streamed runs the large streamed workload (500 Mbp, 30% repeat
+families, 150 bp PE, 30x, k = 55): readids text in ~1 GB chunks, key-range
+passes, each range's records handed to a sink.
 """
 
 import argparse
@@ -162,6 +165,76 @@ def learned_leg(steps, warmup):
                         "1% error, k=31; graph build, hash-table aggregation", "kmer_occurrences": occ, **res}
 
 
+def streamed_leg(steps, warmup, workers=12):
+    """Large streamed workload: the read set goes to the builder in ~1 GB
+    readids chunks; the build runs in key-range passes sized from HBM and
+    hands each range's records to a sink (which counts them).  Timed twice:
+    chunks already in HBM (the value), and once from host memory (PCIe
+    included)."""
+    from genomix_amd import devmem, gx, sim
+    spec, k = sim.STREAMED_500MBP, 55
+    t0 = time.perf_counter()
+    pieces, cur, size = [], [], 0
+    for c in sim.simulate_chunks(spec, chunk_reads=1_000_000, workers=workers):
+        cur.append(c)
+        size += len(c)
+        if size >= 900_000_000:
+            pieces.append(b"".join(cur))
+            cur, size = [], 0
+    if cur:
+        pieces.append(b"".join(cur))
+    nbytes = sum(len(p) for p in pieces)
+    log(f"streamed: {nbytes / 1e9:.2f} GB in {len(pieces)} chunks generated in {time.perf_counter() - t0:.1f} s")
+    sunk = {"bytes": 0, "records": 0, "calls": 0}
+
+    def sink(part, addr, n, nrec):
+        sunk["bytes"] += n
+        sunk["records"] += nrec
+        sunk["calls"] += 1
+        return 0
+
+    res = {"workload": "500 Mbp genome, 30% repeat families 1-10 kb at 2% divergence, 150 bp PE insert 400/40, "
+                       "30x, 1% error, k=55; graph build in key-range passes, records to a sink",
+           "input_bytes": nbytes, "chunks": len(pieces)}
+    with gx.GraphBuilder(k) as b:
+        b.set_profiling(True)
+        b.set_stream(0, sink)
+        # from host memory, once: H2D copies of the chunks + build
+        t1 = time.perf_counter()
+        for p in pieces:
+            b.add_reads(p)
+        t2 = time.perf_counter()
+        b.finish()
+        devmem.synchronize()
+        t3 = time.perf_counter()
+        st = b.stats()
+        res["from_host"] = {"seconds": round(t3 - t1, 3), "h2d_seconds": round(t2 - t1, 3),
+                            "h2d_GBps": round(nbytes / (t2 - t1) / 1e9, 2),
+                            "value": round(st["occurrences"] / (t3 - t1), 1)}
+        log(f"streamed: from host {t3 - t1:.2f} s ({st['key_ranges']} key ranges)")
+        first = dict(sunk)
+        dev = [devmem.DeviceBytes(p) for p in pieces]
+        del pieces
+
+        def step():
+            sunk.update(bytes=0, records=0, calls=0)
+            b.reset()
+            for d in dev:
+                b.add_reads_device(d.address, d.n)
+            b.finish()
+        el = _timed(step, steps, warmup, "streamed")
+        st = b.stats()
+        assert sunk["records"] == st["distinct"] == first["records"] and sunk["bytes"] == first["bytes"]
+        res.update({"ms_per_step": round(1e3 * el / steps, 3), "value": round(st["occurrences"] * steps / el, 1),
+                    "unit": "k-mer occurrences/s", "kmer_occurrences": st["occurrences"], "distinct": st["distinct"],
+                    "key_ranges": st["key_ranges"], "graph_bytes": sunk["bytes"], "sink_calls": sunk["calls"],
+                    "heads": st["heads"],
+                    "phases_ms": {kk: round(v, 3) for kk, v in b.phase_times().items()}})
+        for d in dev:
+            d.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +243,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-scale", type=float, default=1.0)
-    ap.add_argument("--workload", choices=["plain", "learned", "batched"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "learned", "batched", "streamed"], default="plain")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -183,6 +256,9 @@ def main():
 
     if args.workload == "learned":
         print(json.dumps({"metric": METRIC, "learned": learned_leg(max(1, args.steps // 2), 1)}), flush=True)
+        return
+    if args.workload == "streamed":
+        print(json.dumps({"metric": METRIC, "streamed": streamed_leg(max(1, args.steps // 2), 1)}), flush=True)
         return
     if args.workload == "batched":
         print(json.dumps({"metric": METRIC, "batched": batched_leg(args.steps, args.warmup, args.batched_scale)}),

@@ -788,6 +788,16 @@ __device__ inline uint32_t bucket_of_key(const SK& sk, int nb) {
   return nb >= 2 ? (key_byte(sk, nb - 1) << 8) | key_byte(sk, nb - 2) : key_byte(sk, 0);
 }
 
+// Record-sort bucket: the top `bkb` bytes of the sort key (2, or 3 for
+// passes of more than 2^28 records, so buckets stay LDS-sized; 1 when the
+// key has one byte).
+template <class SK>
+__device__ inline uint32_t sort_bucket(const SK& sk, int nb, int bkb) {
+  uint32_t v = 0;
+  for (int i = 1; i <= bkb; i++) v = (v << 8) | key_byte(sk, nb - i);
+  return v;
+}
+
 // Key -> node index, built from the sorted nodes without global atomics:
 // the nodes of each 16-bit key bucket are contiguous (dir[b] = first node),
 // and each bucket owns a private open-addressing region of two slots per
@@ -893,9 +903,8 @@ constexpr double kNodeBytes[2] = {224.0, 268.0};
 constexpr int kIdxBits = 13;      // log2(kBucketCap)
 
 // The bucket bytes are sorted least significant first (LSD): the fused
-// extraction pass scatters by byte nb-2, the radix pass after it by byte
-// nb-1; a one-byte key has a single pass on byte 0.
-__device__ __host__ inline int first_digit_byte(int nb) { return nb >= 2 ? nb - 2 : 0; }
+// extraction pass scatters by byte nb-bkb, radix passes after it by bytes
+// nb-bkb+1 .. nb-1; a one-byte key has a single pass on byte 0.
 constexpr int kTileMateCap = 1024;
 
 // Mate holding occurrence j: the largest m in [lo, hi] with occbase[m] <= j.
@@ -948,7 +957,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
                                                      const uint64_t* __restrict__ wbase,
                                                      const uint64_t* __restrict__ words, int k, int nb,
                                                      uint32_t* __restrict__ hist, unsigned ntiles, uint32_t tlo,
-                                                     uint32_t thi) {
+                                                     uint32_t thi, int dbyte) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
   __shared__ uint64_t lob[kTileMateCap];
@@ -960,7 +969,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
                               SK sk;
                               make_sort_key(key, nb, sk);
                               const uint32_t t = key_byte(sk, nb - 1);
-                              if (t >= tlo && t < thi) atomicAdd(&h[key_byte(sk, first_digit_byte(nb))], 1u);
+                              if (t >= tlo && t < thi) atomicAdd(&h[key_byte(sk, dbyte)], 1u);
                             }
                           });
   __syncthreads();
@@ -977,7 +986,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         const uint64_t* __restrict__ words, int k, int nb,
                                                         const uint64_t* __restrict__ offs, unsigned ntiles,
                                                         typename SortKeyOf<W>::T* __restrict__ okeys,
-                                                        uint64_t* __restrict__ ovals, uint32_t tlo, uint32_t thi) {
+                                                        uint64_t* __restrict__ ovals, uint32_t tlo, uint32_t thi,
+                                                        int dbyte) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint64_t run[256];
   __shared__ uint16_t wcnt[kBlock / 64][256];
@@ -994,7 +1004,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     uint32_t d = 0;
     if (valid) {
       make_sort_key(key, nb, sk);
-      d = key_byte(sk, first_digit_byte(nb));
+      d = key_byte(sk, dbyte);
       const uint32_t t = key_byte(sk, nb - 1);
       valid = t >= tlo && t < thi;  // key-range pass: other records wait for their range
     }
@@ -1050,24 +1060,24 @@ __global__ void __launch_bounds__(kBlock) k_top_hist(uint64_t N, uint64_t nmates
 // First record of every bucket (start[NB] = N; empty buckets start where
 // the next one does).
 template <class SK>
-__global__ void k_bucket_starts(uint64_t N, const SK* __restrict__ keys, int nb, uint32_t NB,
+__global__ void k_bucket_starts(uint64_t N, const SK* __restrict__ keys, int nb, int bkb, uint32_t NB,
                                 uint64_t* __restrict__ start) {
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= N; j += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t lo = j == 0 ? 0 : bucket_of_key(keys[j - 1], nb) + 1;
-    uint32_t hi = j == N ? NB : bucket_of_key(keys[j], nb);
+    uint32_t lo = j == 0 ? 0 : sort_bucket(keys[j - 1], nb, bkb) + 1;
+    uint32_t hi = j == N ? NB : sort_bucket(keys[j], nb, bkb);
     for (uint32_t b = lo; b <= hi && b <= NB; b++) start[b] = j;
   }
 }
 
 // In-bucket sort element: the key bits below the bucket, then the record's
 // index in the bucket (so ties keep the occurrence order).
-__device__ inline uint64_t bucket_elem(uint64_t sk, int nb, uint32_t idx) {
-  int bits = 8 * nb - 16;  // <= 48
+__device__ inline uint64_t bucket_elem(uint64_t sk, int nb, int bkb, uint32_t idx) {
+  int bits = 8 * (nb - bkb);  // <= 48
   uint64_t low = bits >= 64 ? sk : (sk & ((1ull << bits) - 1));
   return (low << kIdxBits) | idx;
 }
-__device__ inline Key128 bucket_elem(const Key128& sk, int nb, uint32_t idx) {
-  int bits = 8 * nb - 16;  // 48..112
+__device__ inline Key128 bucket_elem(const Key128& sk, int nb, int bkb, uint32_t idx) {
+  int bits = 8 * (nb - bkb);  // 40..112
   Key128 low = sk;
   if (bits < 64) { low.lo &= (1ull << bits) - 1; low.hi = 0; }
   else if (bits < 128) low.hi &= (bits == 64) ? 0ull : ((1ull << (bits - 64)) - 1);
@@ -1084,33 +1094,40 @@ __device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
 __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 
 // One block per bucket: bitonic sort of (key below the bucket, index) in
-// LDS, then the records are written in that order.  Oversized buckets are
-// copied unsorted and flagged for the global pass.
-// Two tiers: CAP = kBucketCap / 2 over every bucket (more blocks per CU),
-// leaving buckets above it marked 2 for the CAP = kBucketCap launch; that
-// one marks buckets above kBucketCap 1 for the global pass.
+// LDS, then the records are written in that order.  Three tiers: one wave
+// per bucket (CAP = kSmallBucket) over every bucket of the key range; it
+// lists the buckets above that for the CAP = kBucketCap / 2 blocks
+// (marked 3) and the CAP = kBucketCap blocks (marked 2), which run over
+// their lists only.  Oversized buckets (above kBucketCap) are copied
+// unsorted and marked 1 for the global pass.
+constexpr int kSmallBucket = 256;
 template <int W, int CAP, int NT>
-__global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
-                                                        const uint64_t* __restrict__ iv,
-                                                        typename SortKeyOf<W>::T* __restrict__ ok,
-                                                        uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart,
-                                                        uint64_t N, int nb, uint32_t* __restrict__ big,
-                                                        unsigned long long* err) {
+__device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
+                                       const typename SortKeyOf<W>::T* __restrict__ ik,
+                                       const uint64_t* __restrict__ iv, typename SortKeyOf<W>::T* __restrict__ ok,
+                                       uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart, uint64_t N,
+                                       int nb, int bkb, uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
+                                       uint32_t* __restrict__ nlist, uint32_t NBK, unsigned long long* err) {
   using SK = typename SortKeyOf<W>::T;
   using E = SK;  // uint64_t or Key128
-  __shared__ E s[CAP];
-  const bool second = CAP == kBucketCap;
-  if (second && big[blockIdx.x] != 2) return;
-  const uint64_t a = bstart[blockIdx.x], e = bstart[blockIdx.x + 1];
+  const bool first = CAP == kSmallBucket;
+  const uint64_t a = bstart[bk], e = bstart[bk + 1];
   if (a > e || e > N) {  // bucket bounds must be ordered: report, touch nothing
-    if (threadIdx.x == 0) atomicMin(err, (unsigned long long)blockIdx.x);
+    if (threadIdx.x == 0) atomicMin(err, (unsigned long long)bk);
     return;
   }
   const uint32_t n = (uint32_t)(e - a);
   if (n == 0) return;
-  if (!second && n > CAP) {
-    if (threadIdx.x == 0) big[blockIdx.x] = n > kBucketCap ? 1 : 2;
-    if (n <= kBucketCap) return;  // sorted by the second tier
+  if (first && n > CAP) {
+    if (n <= kBucketCap) {  // to the tier that fits it
+      if (threadIdx.x == 0) {
+        const int t = n <= kBucketCap / 2 ? 0 : 1;
+        big[bk] = t == 0 ? 3 : 2;
+        lists[(size_t)t * NBK + atomicAdd(&nlist[t], 1u)] = bk;
+      }
+      return;
+    }
+    if (threadIdx.x == 0) big[bk] = 1;
   }
   if (n > kBucketCap || n == 1 || nb < 2) {
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
@@ -1123,7 +1140,7 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
   while (P < n) P <<= 1;
   for (uint32_t i = threadIdx.x; i < P; i += NT) {
     E x;
-    if (i < n) x = bucket_elem(ik[a + i], nb, i);
+    if (i < n) x = bucket_elem(ik[a + i], nb, bkb, i);
     else elem_max(x);
     s[i] = x;
   }
@@ -1149,20 +1166,43 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
   }
 }
 
+// Blocks loop over the buckets (grid x block stays below 2^32 work-items):
+// the first tier over buckets [bucket0, bucket1), the others over a list.
+template <int W, int CAP, int NT>
+__global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
+                                                        const uint64_t* __restrict__ iv,
+                                                        typename SortKeyOf<W>::T* __restrict__ ok,
+                                                        uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart,
+                                                        uint64_t N, int nb, int bkb, uint32_t bucket0,
+                                                        uint32_t count, const uint32_t* __restrict__ list,
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
+                                                        uint32_t* __restrict__ nlist, uint32_t NBK,
+                                                        unsigned long long* err) {
+  __shared__ typename SortKeyOf<W>::T s[CAP];
+  for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const uint32_t bk = list ? list[q] : bucket0 + q;
+    sort_one_bucket<W, CAP, NT>(bk, s, ik, iv, ok, ov, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
+    __syncthreads();  // s is reused by the next bucket
+  }
+}
+
 // Oversized buckets to and from a contiguous array (dir 0: gather, 1: back).
 template <class SK>
 __global__ void k_big_move(const uint64_t* __restrict__ bstart, const uint32_t* __restrict__ big,
                            const uint64_t* __restrict__ boff, SK* __restrict__ keys, uint64_t* __restrict__ vals,
-                           SK* __restrict__ tk, uint64_t* __restrict__ tv, int dir) {
-  if (big[blockIdx.x] != 1) return;
-  const uint64_t a = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - a, o = boff[blockIdx.x];
-  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    if (dir == 0) {
-      tk[o + i] = keys[a + i];
-      tv[o + i] = vals[a + i];
-    } else {
-      keys[a + i] = tk[o + i];
-      vals[a + i] = tv[o + i];
+                           SK* __restrict__ tk, uint64_t* __restrict__ tv, int dir, uint32_t bucket0,
+                           uint32_t bucket1) {
+  for (uint32_t bk = bucket0 + blockIdx.x; bk < bucket1; bk += gridDim.x) {
+    if (big[bk] != 1) continue;
+    const uint64_t a = bstart[bk], n = bstart[bk + 1] - a, o = boff[bk];
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      if (dir == 0) {
+        tk[o + i] = keys[a + i];
+        tv[o + i] = vals[a + i];
+      } else {
+        keys[a + i] = tk[o + i];
+        vals[a + i] = tv[o + i];
+      }
     }
   }
 }
@@ -2436,6 +2476,7 @@ struct gx_build {
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
+  DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   uint64_t range_occ = 0;     // occurrences per key-range pass (0: from the HBM size)
   gx_part_sink sink = nullptr;
   void* sink_user = nullptr;
@@ -2764,6 +2805,7 @@ static void run_build(gx_build* b) {
   std::vector<uint64_t> seg_src, seg_len;  // several ranges, no sink: range r's part p bytes in outacc
   std::vector<uint64_t> prec(P + 1, 0), pbyt(P + 1, 0);  // per partition: records and bytes so far
   b->nuheads = 0;
+  b->stats[9] = 0;
   for (uint32_t ri = 0; ri < R; ri++) {
   const uint32_t tlo = tb[ri], thi = tb[ri + 1];
   const uint64_t Nr = rocc[ri];
@@ -2781,45 +2823,67 @@ static void run_build(gx_build* b) {
     uint64_t* va = buf<uint64_t>(b->sva, Nr + 2);
     uint64_t* vb = buf<uint64_t>(b->svb, Nr + 2);
     const unsigned ntiles = (unsigned)std::max<uint64_t>(1, (N + kSortTile - 1) / kSortTile);
+    // record-sort buckets: the top bkb key bytes (LDS-sized buckets)
+    const int bkb = nb < 2 ? 1 : (nb >= 4 && Nr > (1ull << 28)) ? 3 : 2;
+    const int dbyte = nb - bkb;
     const size_t hn = (size_t)256 * ntiles;
     uint32_t* hist = buf<uint32_t>(b->selflag, hn + 1);
     uint64_t* hoff = buf<uint64_t>(b->seg, hn + 1);
     if (Nr) {
       ph_begin(b, PH_KINSERT);
       hipLaunchKernelGGL(k_occ_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
-                         nb, hist, ntiles, tlo, thi);
+                         nb, hist, ntiles, tlo, thi, dbyte);
       GX_HIP(hipGetLastError());
       exclusive_scan<uint32_t, uint64_t>(hist, hoff, hn, scratch_for(b, scan_scratch_bytes<uint64_t>(hn)), st);
       hipLaunchKernelGGL(k_occ_scatter<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
-                         k, nb, hoff, ntiles, ka, va, tlo, thi);
+                         k, nb, hoff, ntiles, ka, va, tlo, thi, dbyte);
       GX_HIP(hipGetLastError());
       ph_end(b, PH_KINSERT);
     }
-    // the more significant bucket byte
+    // the more significant bucket bytes
     SK *kx = ka, *ky = kb;
     uint64_t *vx = va, *vy = vb;
-    if (nb >= 2 && Nr > 1) {
-      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, Nr, nb - 1, nb, scratch_for(b, radix_scratch_bytes<SK>(Nr)),
-                                        st);
+    if (bkb >= 2 && Nr > 1) {
+      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, Nr, nb - bkb + 1, nb,
+                                        scratch_for(b, radix_scratch_bytes<SK>(Nr)), st);
       if (f) { std::swap(kx, ky); std::swap(vx, vy); }
     }
-    // buckets: LDS sort, oversized ones by a global radix sort
-    const uint32_t NBK = nb >= 2 ? 65536u : 256u;
+    // buckets: LDS sort, oversized ones by a global radix sort; the blocks
+    // cover the buckets of this key range only
+    const uint32_t NBK = 1u << (8 * bkb);
+    const uint32_t bk0 = tlo << (8 * (bkb - 1)), bk1 = thi << (8 * (bkb - 1));
     uint64_t* bstart = buf<uint64_t>(b->hstart, NBK + 2);
     uint32_t* big = buf<uint32_t>(b->hcu, NBK + 1);
     uint64_t* bsz = buf<uint64_t>(b->htot, NBK + 1);
     uint64_t* boff = buf<uint64_t>(b->upos, NBK + 2);
     GX_HIP(hipMemsetAsync(big, 0, (NBK + 1) * 4, st));
-    hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(Nr + 1, kBlock, G)), dim3(kBlock), 0, st, Nr, kx, nb, NBK,
-                       bstart);
+    hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(Nr + 1, kBlock, G)), dim3(kBlock), 0, st, Nr, kx, nb, bkb,
+                       NBK, bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
-    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(NBK), dim3(512), 0, st, kx, vx, ky, vy, bstart,
-                       Nr, nb, big, errw + 2);
+    uint32_t* blists = buf<uint32_t>(b->blist, 2 * (size_t)NBK + 2);
+    uint32_t* nlist = blists + 2 * (size_t)NBK;
+    GX_HIP(hipMemsetAsync(nlist, 0, 8, st));
+    const unsigned kMaxSortBlocks = 1u << 20;
+    hipLaunchKernelGGL((k_bucket_sort<W, kSmallBucket, 64>), dim3(std::min(bk1 - bk0, kMaxSortBlocks)), dim3(64), 0,
+                       st, kx, vx, ky, vy, bstart, Nr, nb, bkb, bk0, bk1 - bk0, nullptr, big, blists, nlist, NBK,
+                       errw + 2);
     GX_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(NBK), dim3(1024), 0, st, kx, vx, ky, vy, bstart, Nr,
-                       nb, big, errw + 2);
-    GX_HIP(hipGetLastError());
+    uint32_t nl[2];
+    GX_HIP(hipMemcpyAsync(nl, nlist, 8, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    if (nl[0] > bk1 - bk0 || nl[1] > bk1 - bk0) throw GxError(GX_EDEVICE, "internal error: bucket list overflow");
+    if (nl[0]) {
+      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(std::min(nl[0], 65536u)), dim3(512), 0, st, kx,
+                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[0], blists, big, nullptr, nullptr, NBK, errw + 2);
+      GX_HIP(hipGetLastError());
+    }
+    if (nl[1]) {
+      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(std::min(nl[1], 65536u)), dim3(1024), 0, st, kx,
+                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[1], blists + NBK, big, nullptr, nullptr, NBK,
+                         errw + 2);
+      GX_HIP(hipGetLastError());
+    }
     if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(NBK, kBlock, G)), dim3(kBlock), 0, st, NBK, bstart, big, bsz);
     GX_HIP(hipGetLastError());
@@ -2830,17 +2894,19 @@ static void run_build(gx_build* b) {
       SK* tk2 = buf<SK>(b->hkey2, nbig + 1);
       uint64_t* tv = buf<uint64_t>(b->hval, nbig + 1);
       uint64_t* tv2 = buf<uint64_t>(b->hval2, nbig + 1);
-      hipLaunchKernelGGL(k_big_move<SK>, dim3(NBK), dim3(kBlock), 0, st, bstart, big, boff, ky, vy, tk, tv, 0);
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, ky,
+                         vy, tk, tv, 0, bk0, bk1);
       GX_HIP(hipGetLastError());
       // all key bytes: several oversized buckets share the array, and the
       // stable sort keeps each bucket's records in occurrence order
       bool f = radix_sort<SK, uint64_t>(tk, tv, tk2, tv2, nbig, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nbig)),
                                         st);
       if (f) { std::swap(tk, tk2); std::swap(tv, tv2); }
-      hipLaunchKernelGGL(k_big_move<SK>, dim3(NBK), dim3(kBlock), 0, st, bstart, big, boff, ky, vy, tk, tv, 1);
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, ky,
+                         vy, tk, tv, 1, bk0, bk1);
       GX_HIP(hipGetLastError());
     }
-    b->stats[9] = nbig;
+    b->stats[9] += nbig;
     ph_end(b, PH_INSERT);
 
     // 5'. nodes = runs of equal keys

@@ -74,32 +74,38 @@ def _rc(codes):
     return (3 - codes[..., ::-1]).astype(np.uint8)
 
 
-def reads(spec: ReadSetSpec, g=None, rng=None):
-    """Returns (mate1, mate2 or None) code matrices of shape (n, L)."""
+def n_reads(spec: ReadSetSpec, G=None):
+    """Reads (single-end) or pairs (paired-end) of a read set."""
+    G = spec.genome_len if G is None else G
+    return int(round(spec.coverage * G / ((2 if spec.paired else 1) * spec.read_len)))
+
+
+def reads(spec: ReadSetSpec, g=None, rng=None, n=None):
+    """Returns (mate1, mate2 or None) code matrices of shape (n, L); n
+    defaults to the spec's coverage."""
     rng = rng if rng is not None else np.random.default_rng(spec.seed)
     if g is None:
         g = genome(spec, rng)
     G, L = len(g), spec.read_len
+    if n is None:
+        n = n_reads(spec, G)
     if spec.paired:
-        n = int(round(spec.coverage * G / (2 * L)))
         frag = np.rint(rng.normal(spec.insert_mean, spec.insert_sd, n)).astype(np.int64)
         frag = np.clip(frag, L, G)
         start = (rng.random(n) * (G - frag + 1)).astype(np.int64)
         rev = rng.random(n) < 0.5
-        idx = np.arange(L, dtype=np.int64)
-        a = g[start[:, None] + idx[None, :]]
-        b = g[(start + frag - L)[:, None] + idx[None, :]]
+        win = np.lib.stride_tricks.sliding_window_view(g, L)  # row s = g[s:s + L]
+        a = win[start]
+        b = win[start + frag - L]
         # forward: (start, rc(end)); reverse strand = the reverse-complemented
         # fragment: (rc(end), rc(rc(start)) = start)
         m1 = np.where(rev[:, None], _rc(b), a)
         m2 = np.where(rev[:, None], a, _rc(b))
         return _add_errors(m1.astype(np.uint8), spec.error_rate, rng), _add_errors(
             m2.astype(np.uint8), spec.error_rate, rng)
-    n = int(round(spec.coverage * G / L))
     start = (rng.random(n) * (G - L + 1)).astype(np.int64)
     rev = rng.random(n) < 0.5
-    idx = np.arange(L, dtype=np.int64)
-    r = g[start[:, None] + idx[None, :]]
+    r = np.lib.stride_tricks.sliding_window_view(g, L)[start]
     r = np.where(rev[:, None], _rc(r), r)
     return _add_errors(r.astype(np.uint8), spec.error_rate, rng), None
 
@@ -118,6 +124,57 @@ def readids_text(m1, m2=None, first_read=0) -> bytes:
     rows = body.tobytes()
     w = body.shape[1]
     return b"".join(ids[i] + rows[i * w:(i + 1) * w] for i in range(n))
+
+
+def readids_text_np(m1, m2=None, first_read=0) -> bytes:
+    """readids_text() with no per-read Python work (the same bytes): the
+    lines are laid out in a byte matrix whose id columns are left-padded
+    with zero bytes, which are then dropped."""
+    n, L = m1.shape
+    x = 4 * (first_read + np.arange(n, dtype=np.int64)) + 2
+    D = len(str(int(x[-1]))) if n else 1
+    dig = np.zeros((n, D), np.uint8)
+    for c in range(D - 1, -1, -1):
+        dig[:, c] = np.where((x > 0) | (c == D - 1), 48 + x % 10, 0)
+        x //= 10
+    cols = [dig, np.full((n, 1), ord("\t"), np.uint8), ASCII[m1]]
+    if m2 is not None:
+        cols += [np.full((n, 1), ord("\t"), np.uint8), ASCII[m2]]
+    cols.append(np.full((n, 1), ord("\n"), np.uint8))
+    flat = np.concatenate(cols, axis=1).ravel()
+    return flat[flat != 0].tobytes()
+
+
+_chunk_genome = {}
+
+
+def _chunk_text(args):
+    spec, c, first, n = args
+    g = _chunk_genome.get(spec.seed)
+    if g is None:
+        _chunk_genome.clear()
+        g = _chunk_genome[spec.seed] = genome(spec, np.random.default_rng(spec.seed))
+    m1, m2 = reads(spec, g, np.random.default_rng([spec.seed, c + 1]), n)
+    return readids_text_np(m1, m2, first_read=first)
+
+
+def simulate_chunks(spec: ReadSetSpec, chunk_reads=1 << 20, workers=1):
+    """A read set too large for one array, as readids text chunks of
+    `chunk_reads` reads (pairs) each, in order, numbered on across chunks.
+    The genome is the one simulate() draws for the spec; chunk c's reads
+    come from a generator seeded with (seed, c + 1), so the chunks are fixed
+    by the seed and the chunk size and can be drawn by `workers` processes
+    (each draws the genome once)."""
+    total = n_reads(spec)
+    jobs = [(spec, c, s, min(chunk_reads, total - s)) for c, s in enumerate(range(0, total, chunk_reads))]
+    if workers <= 1:
+        for j in jobs:
+            yield _chunk_text(j)
+        return
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(workers) as pool:
+        for t in pool.imap(_chunk_text, jobs):
+            yield t
 
 
 def fastq_text(m, name="r") -> bytes:
@@ -145,6 +202,13 @@ PLAIN_GPU_5MBP = ReadSetSpec(genome_len=5_000_000, read_len=150, coverage=50, er
 LEARNED_100MBP = ReadSetSpec(genome_len=100_000_000, read_len=150, coverage=40, error_rate=0.01, paired=True,
                              insert_mean=500, insert_sd=50, repeat_frac=0.2, repeat_len=(1000, 10000),
                              repeat_div=0.02, seed=20261207)
+
+
+# Large streamed: 500 Mbp genome, 30% repeat families of 1-10 kb at 2%
+# divergence, 150 bp PE (insert 400/40), 30x, 1% error (built with k = 55).
+STREAMED_500MBP = ReadSetSpec(genome_len=500_000_000, read_len=150, coverage=30, error_rate=0.01, paired=True,
+                              insert_mean=400, insert_sd=40, repeat_frac=0.3, repeat_len=(1000, 10000),
+                              repeat_div=0.02, seed=20261208)
 
 
 def batched_sets(n=64, seed=20261206, scale=1.0):

@@ -121,3 +121,31 @@ def test_sink_error_aborts_the_build(gx):
         with pytest.raises(gx.GxError) as e:
             b.finish()
         assert e.value.code == -12
+
+
+@pytest.mark.parametrize("k", [31, 55])
+def test_large_pass_three_byte_buckets(gx, k):
+    """A pass of more than 2^28 records sorts in buckets of the top three key
+    bytes; the same read set in smaller passes (two-byte buckets) and through
+    the hash-table aggregation gives the same bytes."""
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=6_000_000, read_len=150, coverage=70, error_rate=0.01, paired=True,
+                           insert_mean=500, insert_sd=50, repeat_frac=0.05, repeat_len=2000, repeat_div=0.01,
+                           seed=20261299)
+    text = sim.simulate(spec)
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        st = b.stats()
+        assert st["occurrences"] > 1 << 28 and st["key_ranges"] == 1
+        one = b.part_bytes(0)
+        b.set_stream(range_occurrences=st["occurrences"] // 3)
+        b.reset()
+        b.add_reads(text)
+        b.finish()
+        assert b.stats()["key_ranges"] >= 3
+        assert b.part_bytes(0) == one
+    with gx.GraphBuilder(k, method="hash") as b:
+        b.add_reads(text)
+        b.finish()
+        assert b.part_bytes(0) == one
