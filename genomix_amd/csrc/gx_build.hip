@@ -1057,15 +1057,16 @@ __global__ void __launch_bounds__(kBlock) k_top_hist(uint64_t N, uint64_t nmates
   if (h[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
-// First record of every bucket (start[NB] = N; empty buckets start where
-// the next one does).
+// First record of every bucket of [b0, b1) (start[b1] = N; empty buckets
+// start where the next one does).  Every record's bucket lies in [b0, b1);
+// entries outside [b0, b1] are not written.
 template <class SK>
-__global__ void k_bucket_starts(uint64_t N, const SK* __restrict__ keys, int nb, int bkb, uint32_t NB,
+__global__ void k_bucket_starts(uint64_t N, const SK* __restrict__ keys, int nb, int bkb, uint32_t b0, uint32_t b1,
                                 uint64_t* __restrict__ start) {
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= N; j += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t lo = j == 0 ? 0 : sort_bucket(keys[j - 1], nb, bkb) + 1;
-    uint32_t hi = j == N ? NB : sort_bucket(keys[j], nb, bkb);
-    for (uint32_t b = lo; b <= hi && b <= NB; b++) start[b] = j;
+    uint32_t lo = j == 0 ? b0 : sort_bucket(keys[j - 1], nb, bkb) + 1;
+    uint32_t hi = j == N ? b1 : sort_bucket(keys[j], nb, bkb);
+    for (uint32_t b = lo; b <= hi && b <= b1; b++) start[b] = j;
   }
 }
 
@@ -2858,7 +2859,7 @@ static void run_build(gx_build* b) {
     uint64_t* boff = buf<uint64_t>(b->upos, NBK + 2);
     GX_HIP(hipMemsetAsync(big, 0, (NBK + 1) * 4, st));
     hipLaunchKernelGGL(k_bucket_starts<SK>, dim3(grid_for(Nr + 1, kBlock, G)), dim3(kBlock), 0, st, Nr, kx, nb, bkb,
-                       NBK, bstart);
+                       bk0, bk1, bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
     uint32_t* blists = buf<uint32_t>(b->blist, 2 * (size_t)NBK + 2);
