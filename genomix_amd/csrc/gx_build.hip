@@ -782,12 +782,6 @@ __device__ inline KV key_of_sort(const Key128& r, int nb) {
 __device__ inline bool sk_eq(uint64_t a, uint64_t b) { return a == b; }
 __device__ inline bool sk_eq(const Key128& a, const Key128& b) { return a.lo == b.lo && a.hi == b.hi; }
 
-// 16-bit bucket of a sort key: its top two bytes (nb >= 2), or its only byte.
-template <class SK>
-__device__ inline uint32_t bucket_of_key(const SK& sk, int nb) {
-  return nb >= 2 ? (key_byte(sk, nb - 1) << 8) | key_byte(sk, nb - 2) : key_byte(sk, 0);
-}
-
 // Record-sort bucket: the top `bkb` bytes of the sort key (2, or 3 for
 // passes of more than 2^28 records, so buckets stay LDS-sized; 1 when the
 // key has one byte).
@@ -799,27 +793,29 @@ __device__ inline uint32_t sort_bucket(const SK& sk, int nb, int bkb) {
 }
 
 // Key -> node index, built from the sorted nodes without global atomics:
-// the nodes of each 16-bit key bucket are contiguous (dir[b] = first node),
-// and each bucket owns a private open-addressing region of two slots per
-// node, filled in LDS by one block.  A lookup is the bucket's two dir
-// entries plus one or two slot probes.
+// the nodes of each key bucket (the top bkb bytes of the sort key: 2, or 3
+// above 2^26 nodes) are contiguous (dir[b] = first node), and each bucket
+// owns a private open-addressing region of two slots per node, filled in
+// LDS by one block.  A lookup is the bucket's two dir entries plus one or
+// two slot probes.
 struct NodeIndex {
   const uint32_t* dir;  // nbk + 1 entries
   const void* slots;    // Slot1 / Slot2, 2 per node; count = node id
   int nb;
   uint32_t nbk;
+  int bkb;
 };
 
 template <int W>
-__device__ inline uint32_t node_bucket(KV key, int nb) {
+__device__ inline uint32_t node_bucket(KV key, int nb, int bkb) {
   typename SortKeyOf<W>::T sk;
   make_sort_key(key, nb, sk);
-  return bucket_of_key(sk, nb);
+  return sort_bucket(sk, nb, bkb);
 }
 
 template <int W>
 __device__ inline uint32_t node_lookup(const NodeIndex& ix, KV key) {
-  uint32_t b = node_bucket<W>(key, ix.nb);
+  uint32_t b = node_bucket<W>(key, ix.nb, ix.bkb);
   uint32_t lo = ix.dir[b], hi = ix.dir[b + 1];
   if (lo >= hi) return kEnd;
   const uint64_t R = 2ull * (hi - lo), base = 2ull * lo;
@@ -839,61 +835,72 @@ __device__ inline uint32_t node_lookup(const NodeIndex& ix, KV key) {
   return kEnd;
 }
 
+// dir over buckets [b0, b1] (every node's bucket lies in [b0, b1)); the
+// entries outside are filled by k_fill_u32.
 template <int W>
 __global__ void k_nix_dir(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi, int nb,
-                          uint32_t nbk, uint32_t* __restrict__ dir) {
+                          int bkb, uint32_t b0, uint32_t b1, uint32_t* __restrict__ dir) {
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= nn; j += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t lo = j == 0 ? 0 : node_bucket<W>(node_key<W>(klo, khi, j - 1), nb) + 1;
-    uint32_t hi = j == nn ? nbk : node_bucket<W>(node_key<W>(klo, khi, j), nb);
-    for (uint32_t b = lo; b <= hi && b <= nbk; b++) dir[b] = (uint32_t)j;
+    uint32_t lo = j == 0 ? b0 : node_bucket<W>(node_key<W>(klo, khi, j - 1), nb, bkb) + 1;
+    uint32_t hi = j == nn ? b1 : node_bucket<W>(node_key<W>(klo, khi, j), nb, bkb);
+    for (uint32_t b = lo; b <= hi && b <= b1; b++) dir[b] = (uint32_t)j;
   }
 }
 
-constexpr int kNixLds = 4096;  // slots of one bucket region built in LDS
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t from, uint64_t to, uint32_t v) {
+  for (uint64_t i = from + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < to; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
 
-template <int W>
+constexpr int kNixLds = 4096;  // slots of one bucket region built in LDS (two-byte buckets)
+constexpr int kNixLdsSmall = 1024;  // three-byte buckets: smaller regions, more blocks per CU
+template <int W, int LCAP>
 __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
-                                                      const uint32_t* __restrict__ dir, void* slots) {
+                                                      const uint32_t* __restrict__ dir, void* slots, uint32_t b0,
+                                                      uint32_t b1) {
   using S = typename std::conditional<W == 1, Slot1, Slot2>::type;
-  __shared__ S lt[kNixLds];
-  const uint32_t lo = dir[blockIdx.x], hi = dir[blockIdx.x + 1];
-  if (lo >= hi) return;
-  const uint64_t R = 2ull * (hi - lo), base = 2ull * lo;
-  const bool local = R <= kNixLds;
-  S* T = local ? lt : (S*)slots + base;
-  for (uint64_t i = threadIdx.x; i < R; i += kBlock) {
-    S e{};
-    if (W == 1) ((Slot1*)&e)->key = kEmpty;
-    else ((Slot2*)&e)->hi = kEmpty;
-    T[i] = e;
-  }
-  __syncthreads();
-  for (uint64_t n = lo + threadIdx.x; n < hi; n += kBlock) {
-    KV key = node_key<W>(klo, khi, n);
-    uint64_t s = slot_home(khash<W>(key), R);
-    for (uint64_t probe = 0; probe < R; probe++) {
-      if (W == 1) {
-        Slot1* e = (Slot1*)&T[s];
-        if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmpty, (unsigned long long)key.lo) ==
-            kEmpty) {
-          e->count = (uint32_t)n;
-          break;
-        }
-      } else {  // distinct keys: a taken slot always holds another key
-        Slot2* e = (Slot2*)&T[s];
-        if (atomicCAS((unsigned long long*)&e->hi, (unsigned long long)kEmpty, (unsigned long long)key.hi) ==
-            kEmpty) {
-          e->lo = key.lo;
-          e->count = (uint32_t)n;
-          break;
-        }
-      }
-      s = s + 1 == R ? 0 : s + 1;
+  __shared__ S lt[LCAP];
+  for (uint32_t bk = b0 + blockIdx.x; bk < b1; bk += gridDim.x) {
+    const uint32_t lo = dir[bk], hi = dir[bk + 1];
+    if (lo >= hi) continue;
+    const uint64_t R = 2ull * (hi - lo), base = 2ull * lo;
+    const bool local = R <= LCAP;
+    S* T = local ? lt : (S*)slots + base;
+    for (uint64_t i = threadIdx.x; i < R; i += kBlock) {
+      S e{};
+      if (W == 1) ((Slot1*)&e)->key = kEmpty;
+      else ((Slot2*)&e)->hi = kEmpty;
+      T[i] = e;
     }
+    __syncthreads();
+    for (uint64_t n = lo + threadIdx.x; n < hi; n += kBlock) {
+      KV key = node_key<W>(klo, khi, n);
+      uint64_t s = slot_home(khash<W>(key), R);
+      for (uint64_t probe = 0; probe < R; probe++) {
+        if (W == 1) {
+          Slot1* e = (Slot1*)&T[s];
+          if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmpty, (unsigned long long)key.lo) ==
+              kEmpty) {
+            e->count = (uint32_t)n;
+            break;
+          }
+        } else {  // distinct keys: a taken slot always holds another key
+          Slot2* e = (Slot2*)&T[s];
+          if (atomicCAS((unsigned long long*)&e->hi, (unsigned long long)kEmpty, (unsigned long long)key.hi) ==
+              kEmpty) {
+            e->lo = key.lo;
+            e->count = (uint32_t)n;
+            break;
+          }
+        }
+        s = s + 1 == R ? 0 : s + 1;
+      }
+    }
+    __syncthreads();
+    if (local)
+      for (uint64_t i = threadIdx.x; i < R; i += kBlock) ((S*)slots)[base + i] = lt[i];
+    __syncthreads();  // lt is reused by the next bucket
   }
-  __syncthreads();
-  if (local)
-    for (uint64_t i = threadIdx.x; i < R; i += kBlock) ((S*)slots)[base + i] = lt[i];
 }
 
 constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
@@ -2483,6 +2490,7 @@ struct gx_build {
   void* sink_user = nullptr;
   bool streamed = false;      // the last build's records went to the sink
   uint32_t nranges = 1;
+  int nix_bkb = 2;            // key bytes of the node index directory
   ReclaimList reclaim;        // buffers the last build reads no more
   // learned sizing
   gx_mlp* sizer = nullptr;
@@ -3034,14 +3042,32 @@ static void run_build(gx_build* b) {
 
   // 5b. key -> node index (heads, unitig links)
   ph_begin(b, PH_NIX);
-  const uint32_t nbk = nb >= 2 ? 65536u : 256u;
+  const int xbkb = nb < 2 ? 1 : (nb >= 4 && nn > (1ull << 26)) ? 3 : 2;
+  const uint32_t nbk = 1u << (8 * xbkb);
+  const uint32_t xb0 = tlo << (8 * (xbkb - 1)), xb1 = thi << (8 * (xbkb - 1));
+  b->nix_bkb = xbkb;
   uint32_t* ndir = buf<uint32_t>(b->nix_dir, nbk + 1);
   void* nslots = b->nix_slots.ensure(std::max<uint64_t>(1, 2 * nn) * slotsz);
-  hipLaunchKernelGGL(k_nix_dir<W>, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, nbk, ndir);
+  if (xb0 > 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(xb0, kBlock, G)), dim3(kBlock), 0, st, ndir, 0, xb0, 0u);
+    GX_HIP(hipGetLastError());
+  }
+  if (xb1 < nbk) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nbk - xb1, kBlock, G)), dim3(kBlock), 0, st, ndir, (uint64_t)xb1 + 1,
+                       (uint64_t)nbk + 1, (uint32_t)nn);
+    GX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_nix_dir<W>, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, xbkb, xb0,
+                     xb1, ndir);
   GX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_nix_build<W>, dim3(nbk), dim3(kBlock), 0, st, klo, khi, ndir, nslots);
+  if (xbkb == 3)
+    hipLaunchKernelGGL((k_nix_build<W, kNixLdsSmall>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo,
+                       khi, ndir, nslots, xb0, xb1);
+  else
+    hipLaunchKernelGGL((k_nix_build<W, kNixLds>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo, khi,
+                       ndir, nslots, xb0, xb1);
   GX_HIP(hipGetLastError());
-  const NodeIndex nix{ndir, nslots, nb, nbk};
+  const NodeIndex nix{ndir, nslots, nb, nbk, xbkb};
   if (sorted) cap = 2 * nn;
   ph_end(b, PH_NIX);
 
@@ -3312,7 +3338,7 @@ static void run_unitigs(gx_build* b) {
   const int k = b->cfg.k, nb = b->nb;
   const uint64_t nn = b->nnodes;
   const unsigned G = b->grid;
-  const NodeIndex nix{b->nix_dir.as<uint32_t>(), b->nix_slots.p, nb, nb >= 2 ? 65536u : 256u};
+  const NodeIndex nix{b->nix_dir.as<uint32_t>(), b->nix_slots.p, nb, 1u << (8 * b->nix_bkb), b->nix_bkb};
   const uint64_t* klo = b->klo.as<uint64_t>();
   const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
   const uint32_t* cnt = b->cnt.as<uint32_t>();

@@ -137,8 +137,13 @@ def test_large_pass_three_byte_buckets(gx, k):
         b.add_reads(text)
         b.finish()
         st = b.stats()
-        assert st["occurrences"] > 1 << 28 and st["key_ranges"] == 1
+        # more than 2^28 records in one pass, more than 2^26 nodes (three-byte
+        # node index directory)
+        assert st["occurrences"] > 1 << 28 and st["key_ranges"] == 1 and st["distinct"] > 1 << 26
         one = b.part_bytes(0)
+        b.run_unitigs()  # every neighbour lookup goes through the node index
+        nu = b.unitig_count()
+        assert 0 < nu < st["distinct"]
         b.set_stream(range_occurrences=st["occurrences"] // 3)
         b.reset()
         b.add_reads(text)
@@ -149,3 +154,5 @@ def test_large_pass_three_byte_buckets(gx, k):
         b.add_reads(text)
         b.finish()
         assert b.part_bytes(0) == one
+        b.run_unitigs()
+        assert b.unitig_count() == nu
