@@ -994,13 +994,16 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         const uint64_t* __restrict__ offs, unsigned ntiles,
                                                         typename SortKeyOf<W>::T* __restrict__ okeys,
                                                         uint64_t* __restrict__ ovals, uint32_t tlo, uint32_t thi,
-                                                        int dbyte) {
+                                                        int dbyte, uint32_t nlo, uint32_t nhi, int ndbyte,
+                                                        uint32_t* __restrict__ nhist) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint64_t run[256];
   __shared__ uint16_t wcnt[kBlock / 64][256];
   __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint32_t h2[256];  // the next key range's digit counts (as k_occ_hist would give)
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+  h2[threadIdx.x] = 0;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
   for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
@@ -1013,6 +1016,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
       make_sort_key(key, nb, sk);
       d = key_byte(sk, dbyte);
       const uint32_t t = key_byte(sk, nb - 1);
+      if (nhist && t >= nlo && t < nhi) atomicAdd(&h2[key_byte(sk, ndbyte)], 1u);
       valid = t >= tlo && t < thi;  // key-range pass: other records wait for their range
     }
     uint64_t act = __ballot(valid);
@@ -1037,6 +1041,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     run[threadIdx.x] += tot;
     __syncthreads();
   });
+  if (nhist) nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h2[threadIdx.x];
 }
 
 // Occurrences per value of the sort key's top byte (the key ranges of a
@@ -1102,12 +1107,12 @@ __device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
 __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 
 // One block per bucket: bitonic sort of (key below the bucket, index) in
-// LDS, then the records are written in that order.  Three tiers: one wave
+// LDS, then the records are written in that order.  Four tiers: one wave
 // per bucket (CAP = kSmallBucket) over every bucket of the key range; it
-// lists the buckets above that for the CAP = kBucketCap / 2 blocks
-// (marked 3) and the CAP = kBucketCap blocks (marked 2), which run over
-// their lists only.  Oversized buckets (above kBucketCap) are copied
-// unsorted and marked 1 for the global pass.
+// lists the buckets above that for the CAP = kBucketCap / 8 (256 threads),
+// kBucketCap / 2 (512) and kBucketCap (1024) blocks, which run over their
+// lists only (buckets marked 4, 3, 2).  Oversized buckets (above
+// kBucketCap) are copied unsorted and marked 1 for the global pass.
 constexpr int kSmallBucket = 256;
 template <int W, int CAP, int NT>
 __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
@@ -1129,8 +1134,8 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
   if (first && n > CAP) {
     if (n <= kBucketCap) {  // to the tier that fits it
       if (threadIdx.x == 0) {
-        const int t = n <= kBucketCap / 2 ? 0 : 1;
-        big[bk] = t == 0 ? 3 : 2;
+        const int t = n <= kBucketCap / 8 ? 0 : n <= kBucketCap / 2 ? 1 : 2;
+        big[bk] = 4 - t;
         lists[(size_t)t * NBK + atomicAdd(&nlist[t], 1u)] = bk;
       }
       return;
@@ -1698,6 +1703,23 @@ __global__ void k_heads(uint64_t nmates, const uint8_t* __restrict__ valid, cons
     hkey[j] = key;
     hval[j] = (uint32_t)m;
   }
+}
+
+// Heads of the current key range (the others carry key ~0), kept in mate
+// order by a scan of these flags.
+__global__ void k_head_in_range(uint64_t nh, const uint64_t* __restrict__ hkey, uint32_t* __restrict__ flag) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x)
+    flag[j] = hkey[j] != ~0ull ? 1u : 0u;
+}
+
+__global__ void k_head_keep(uint64_t nh, const uint64_t* __restrict__ hkey, const uint32_t* __restrict__ hval,
+                            const uint32_t* __restrict__ flag, const uint64_t* __restrict__ pos,
+                            uint64_t* __restrict__ okey, uint32_t* __restrict__ oval) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += (uint64_t)gridDim.x * blockDim.x)
+    if (flag[j]) {
+      okey[pos[j]] = hkey[j];
+      oval[pos[j]] = hval[j];
+    }
 }
 
 // Unique heads (TreeSet keeps the first of equal keys) and per-node counts
@@ -2485,6 +2507,7 @@ struct gx_build {
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
+  DevBuf hist2;               // key ranges: the next range's per-tile digit counts
   uint64_t range_occ = 0;     // occurrences per key-range pass (0: from the HBM size)
   gx_part_sink sink = nullptr;
   void* sink_user = nullptr;
@@ -2815,6 +2838,10 @@ static void run_build(gx_build* b) {
   std::vector<uint64_t> prec(P + 1, 0), pbyt(P + 1, 0);  // per partition: records and bytes so far
   b->nuheads = 0;
   b->stats[9] = 0;
+  // record-sort bucket bytes of a pass of n records (LDS-sized buckets)
+  auto bkb_of = [&](uint64_t n) { return nb < 2 ? 1 : (nb >= 4 && n > (1ull << 28)) ? 3 : 2; };
+  DevBuf *hcur = &b->selflag, *hnext = &b->hist2;  // per-tile digit counts: this range's, the next one's
+  bool hist_ready = false;
   for (uint32_t ri = 0; ri < R; ri++) {
   const uint32_t tlo = tb[ri], thi = tb[ri + 1];
   const uint64_t Nr = rocc[ri];
@@ -2833,22 +2860,32 @@ static void run_build(gx_build* b) {
     uint64_t* vb = buf<uint64_t>(b->svb, Nr + 2);
     const unsigned ntiles = (unsigned)std::max<uint64_t>(1, (N + kSortTile - 1) / kSortTile);
     // record-sort buckets: the top bkb key bytes (LDS-sized buckets)
-    const int bkb = nb < 2 ? 1 : (nb >= 4 && Nr > (1ull << 28)) ? 3 : 2;
+    const int bkb = bkb_of(Nr);
     const int dbyte = nb - bkb;
     const size_t hn = (size_t)256 * ntiles;
-    uint32_t* hist = buf<uint32_t>(b->selflag, hn + 1);
+    uint32_t* hist = buf<uint32_t>(*hcur, hn + 1);
     uint64_t* hoff = buf<uint64_t>(b->seg, hn + 1);
+    bool next_hist = false;
     if (Nr) {
       ph_begin(b, PH_KINSERT);
-      hipLaunchKernelGGL(k_occ_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
-                         nb, hist, ntiles, tlo, thi, dbyte);
-      GX_HIP(hipGetLastError());
+      if (!hist_ready) {
+        hipLaunchKernelGGL(k_occ_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
+                           k, nb, hist, ntiles, tlo, thi, dbyte);
+        GX_HIP(hipGetLastError());
+      }
       exclusive_scan<uint32_t, uint64_t>(hist, hoff, hn, scratch_for(b, scan_scratch_bytes<uint64_t>(hn)), st);
+      // the same extraction pass counts the next range's digits
+      next_hist = ri + 1 < R && rocc[ri + 1] > 0;
+      uint32_t* nh = next_hist ? buf<uint32_t>(*hnext, hn + 1) : nullptr;
+      const uint32_t nlo = next_hist ? tb[ri + 1] : 0, nhi = next_hist ? tb[ri + 2] : 0;
+      const int ndbyte = next_hist ? nb - bkb_of(rocc[ri + 1]) : 0;
       hipLaunchKernelGGL(k_occ_scatter<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
-                         k, nb, hoff, ntiles, ka, va, tlo, thi, dbyte);
+                         k, nb, hoff, ntiles, ka, va, tlo, thi, dbyte, nlo, nhi, ndbyte, nh);
       GX_HIP(hipGetLastError());
       ph_end(b, PH_KINSERT);
     }
+    hist_ready = next_hist;
+    std::swap(hcur, hnext);
     // the more significant bucket bytes
     SK *kx = ka, *ky = kb;
     uint64_t *vx = va, *vy = vb;
@@ -2870,27 +2907,34 @@ static void run_build(gx_build* b) {
                        bk0, bk1, bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
-    uint32_t* blists = buf<uint32_t>(b->blist, 2 * (size_t)NBK + 2);
-    uint32_t* nlist = blists + 2 * (size_t)NBK;
-    GX_HIP(hipMemsetAsync(nlist, 0, 8, st));
+    uint32_t* blists = buf<uint32_t>(b->blist, 3 * (size_t)NBK + 3);
+    uint32_t* nlist = blists + 3 * (size_t)NBK;
+    GX_HIP(hipMemsetAsync(nlist, 0, 12, st));
     const unsigned kMaxSortBlocks = 1u << 20;
     hipLaunchKernelGGL((k_bucket_sort<W, kSmallBucket, 64>), dim3(std::min(bk1 - bk0, kMaxSortBlocks)), dim3(64), 0,
                        st, kx, vx, ky, vy, bstart, Nr, nb, bkb, bk0, bk1 - bk0, nullptr, big, blists, nlist, NBK,
                        errw + 2);
     GX_HIP(hipGetLastError());
-    uint32_t nl[2];
-    GX_HIP(hipMemcpyAsync(nl, nlist, 8, hipMemcpyDeviceToHost, st));
+    uint32_t nl[3];
+    GX_HIP(hipMemcpyAsync(nl, nlist, 12, hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
-    if (nl[0] > bk1 - bk0 || nl[1] > bk1 - bk0) throw GxError(GX_EDEVICE, "internal error: bucket list overflow");
+    if (nl[0] > bk1 - bk0 || nl[1] > bk1 - bk0 || nl[2] > bk1 - bk0)
+      throw GxError(GX_EDEVICE, "internal error: bucket list overflow");
     if (nl[0]) {
-      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(std::min(nl[0], 65536u)), dim3(512), 0, st, kx,
+      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 8, 256>), dim3(std::min(nl[0], 65536u)), dim3(256), 0, st, kx,
                          vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[0], blists, big, nullptr, nullptr, NBK, errw + 2);
       GX_HIP(hipGetLastError());
     }
     if (nl[1]) {
-      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(std::min(nl[1], 65536u)), dim3(1024), 0, st, kx,
+      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(std::min(nl[1], 65536u)), dim3(512), 0, st, kx,
                          vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[1], blists + NBK, big, nullptr, nullptr, NBK,
                          errw + 2);
+      GX_HIP(hipGetLastError());
+    }
+    if (nl[2]) {
+      hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(std::min(nl[2], 65536u)), dim3(1024), 0, st, kx,
+                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[2], blists + 2 * (size_t)NBK, big, nullptr, nullptr,
+                         NBK, errw + 2);
       GX_HIP(hipGetLastError());
     }
     if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
@@ -3162,13 +3206,25 @@ static void run_build(gx_build* b) {
                        nix, rid, wbase, words, k, hk, hv, tlo, thi);
     GX_HIP(hipGetLastError());
   }
-  int hbytes = 1;
-  while (hbytes < 8 && (((nn << 31) | ((1ull << 31) - 1)) >> (8 * hbytes)) > 0) hbytes++;
-  bool f3 = radix_sort<uint64_t>(hk, hv, hk2, hv2, nheads, 0, hbytes,
-                                 scratch_for(b, radix_scratch_bytes<uint64_t>(nheads)), st);
-  if (f3) { std::swap(hk, hk2); std::swap(hv, hv2); }
   uint32_t* uflag = buf<uint32_t>(b->uflag, nheads + 1);
   uint64_t* upos = buf<uint64_t>(b->upos, nheads + 1);
+  uint64_t nh = nheads;  // heads of this key range
+  if (R > 1 && nheads) {  // keep this range's heads (in mate order) before sorting
+    hipLaunchKernelGGL(k_head_in_range, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, uflag);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(uflag, upos, nheads, scratch_for(b, scan_scratch_bytes<uint64_t>(nheads)), st);
+    nh = read1(st, upos + nheads);
+    hipLaunchKernelGGL(k_head_keep, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, hv, uflag,
+                       upos, hk2, hv2);
+    GX_HIP(hipGetLastError());
+    std::swap(hk, hk2);
+    std::swap(hv, hv2);
+  }
+  int hbytes = 1;
+  while (hbytes < 8 && (((nn << 31) | ((1ull << 31) - 1)) >> (8 * hbytes)) > 0) hbytes++;
+  bool f3 = radix_sort<uint64_t>(hk, hv, hk2, hv2, nh, 0, hbytes, scratch_for(b, radix_scratch_bytes<uint64_t>(nh)),
+                                 st);
+  if (f3) { std::swap(hk, hk2); std::swap(hv, hv2); }
   uint32_t* uval = buf<uint32_t>(b->uval, nheads + 1);
   uint32_t* hcu = buf<uint32_t>(b->hcu, nn + 1);
   uint32_t* hcf = buf<uint32_t>(b->hcf, nn + 1);
@@ -3176,15 +3232,15 @@ static void run_build(gx_build* b) {
   uint64_t* hstart = buf<uint64_t>(b->hstart, nn + 1);
   GX_HIP(hipMemsetAsync(hcu, 0, (nn + 1) * 4, st));
   GX_HIP(hipMemsetAsync(hcf, 0, (nn + 1) * 4, st));
-  if (nheads) {
-    hipLaunchKernelGGL(k_head_unique, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, uflag);
+  if (nh) {
+    hipLaunchKernelGGL(k_head_unique, dim3(grid_for(nh, kBlock, G)), dim3(kBlock), 0, st, nh, hk, uflag);
     GX_HIP(hipGetLastError());
   }
-  exclusive_scan<uint32_t, uint64_t>(uflag, upos, nheads, scratch_for(b, scan_scratch_bytes<uint64_t>(nheads)), st);
-  b->nuheads += read1(st, upos + nheads);
-  if (nheads) {
-    hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nheads, kBlock, G)), dim3(kBlock), 0, st, nheads, hk, hv, uflag,
-                       upos, uval, hcu, hcf);
+  exclusive_scan<uint32_t, uint64_t>(uflag, upos, nh, scratch_for(b, scan_scratch_bytes<uint64_t>(nh)), st);
+  b->nuheads += read1(st, upos + nh);
+  if (nh) {
+    hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nh, kBlock, G)), dim3(kBlock), 0, st, nh, hk, hv, uflag, upos,
+                       uval, hcu, hcf);
     GX_HIP(hipGetLastError());
   }
   if (nn) {
