@@ -1835,25 +1835,70 @@ __global__ void k_ser_size(SerCtx c, uint64_t* __restrict__ rsize) {
     rsize[i] = record_size(c, ser_node(c, i));
 }
 
-__device__ inline void put32(uint8_t*& p, uint32_t v) {
-  p[0] = v >> 24; p[1] = v >> 16; p[2] = v >> 8; p[3] = v;
+// Record bytes are written with 2-, 4- and 8-byte stores at any alignment
+// (gfx950 runs in unaligned access mode; the compiler merges these copies
+// into single stores), never past the bytes asked for.
+
+// The n (<= 8) low bytes of x, least significant first.
+__device__ inline void put_le(uint8_t*& p, uint64_t x, int n) {
+  if (n == 8) {
+    memcpy(p, &x, 8);
+    p += 8;
+    return;
+  }
+  if (n & 4) {
+    uint32_t y = (uint32_t)x;
+    memcpy(p, &y, 4);
+    p += 4;
+    x >>= 32;
+  }
+  if (n & 2) {
+    uint16_t y = (uint16_t)x;
+    memcpy(p, &y, 2);
+    p += 2;
+    x >>= 16;
+  }
+  if (n & 1) *p++ = (uint8_t)x;
+}
+
+__device__ inline void put32(uint8_t*& p, uint32_t v) {  // big endian
+  uint32_t b = __builtin_bswap32(v);
+  memcpy(p, &b, 4);
   p += 4;
 }
 
+__device__ inline void put64(uint8_t*& p, uint64_t v) {  // big endian
+  uint64_t b = __builtin_bswap64(v);
+  memcpy(p, &b, 8);
+  p += 8;
+}
+
+// VKmer bytes: BE32(k), then the nb-byte big-endian form of V (whose
+// little-endian image is the sort key).
 template <int W>
 __device__ inline void put_key(uint8_t*& p, KV v, int k, int nb) {
   put32(p, (uint32_t)k);
-  for (int j = 0; j < nb; j++) *p++ = be_byte<W>(v, nb, j);
+  if constexpr (W == 1) {
+    put_le(p, sortkey64(v.lo, nb), nb);
+  } else {
+    Key128 s = sortkey128(v, nb);
+    put_le(p, s.lo, nb < 8 ? nb : 8);
+    if (nb > 8) put_le(p, s.hi, nb - 8);
+  }
 }
 
-// A read as VKmer bytes: BE32(L) then the packed stream's bytes reversed.
+// A read as VKmer bytes: BE32(L) then the packed stream's bytes reversed,
+// eight at a time as byte-swapped 64-bit windows of the stream.
 __device__ inline void put_read(uint8_t*& p, const uint64_t* w, uint32_t L) {
   put32(p, L);
-  uint32_t nbl = nbytes_of(L);
-  for (uint32_t j = 0; j < nbl; j++) {
-    uint32_t q = nbl - 1 - j;
-    *p++ = (uint8_t)(w[q >> 3] >> (8 * (q & 7)));
+  const int64_t nbl = nbytes_of(L);
+  int64_t q = nbl - 8;  // lowest stream byte of the next window
+  for (; q >= 0; q -= 8) {
+    const uint64_t a = (uint64_t)q >> 3, sh = ((uint64_t)q & 7) * 8;
+    uint64_t x = sh ? (w[a] >> sh) | (w[a + 1] << (64 - sh)) : w[a];
+    put64(p, x);
   }
+  for (int64_t r = q + 7; r >= 0; r--) *p++ = (uint8_t)(w[r >> 3] >> (8 * (r & 7)));
 }
 
 template <int W>
@@ -1894,7 +1939,7 @@ __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uin
       *p++ = Lm > 0 ? 1 : 0;
       uint64_t off = f ? (uint64_t)(c.k - 1) : 0;
       uint64_t uuid = (off << 40) | ((uint64_t)(m & 1) << 35) | (uint64_t)c.rid[m >> 1];
-      for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
+      put64(p, uuid);
       put_read(p, c.words + c.wbase[m], L);
       if (Lm > 0) put_read(p, c.words + c.wbase[m ^ 1], Lm);
     }
@@ -2388,7 +2433,7 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
         *p++ = Lm > 0 ? 1 : 0;
         uint64_t off = (c.hkey[j] >> 30) & ((1ull << 23) - 1);
         uint64_t uuid = (off << 40) | ((uint64_t)(mi & 1) << 35) | (uint64_t)c.rid[mi >> 1];
-        for (int s = 56; s >= 0; s -= 8) *p++ = (uint8_t)(uuid >> s);
+        put64(p, uuid);
         put_read(p, c.words + c.wbase[mi], L);
         if (Lm > 0) put_read(p, c.words + c.wbase[mi ^ 1], Lm);
       }
