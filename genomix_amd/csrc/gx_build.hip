@@ -904,9 +904,20 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
 }
 
 constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
-// HBM per node of a sorted build (key, count, mask, run start, index slots,
-// replay and head arrays, serialised record), by key width: sizes key ranges
-constexpr double kNodeBytes[2] = {224.0, 268.0};
+// HBM of a key-range pass, per occurrence (sizes the ranges): the records
+// (two key and two value arrays, a replay code byte) and, per node (dfrac
+// nodes per occurrence), every node array of the pass -- key words, count,
+// mask, first record, two index slots, replay flags, counts, offsets,
+// lists and order, head counts and starts, record sizes and offsets -- and
+// the record itself, estimated as three VKmers and headers.  Counted as
+// live together: freeing dead buffers on demand works, but re-allocating
+// 100 GB-scale buffers every pass costs more than an extra pass.
+__host__ inline double pass_bytes_per_occurrence(int W, int nb, double dfrac) {
+  const double rec = 2.0 * (W == 1 ? 8 : 16) + 16 + 1;
+  const double node = 8.0 * W + 4 + 4 + 8 + 2.0 * (W == 1 ? 16 : 24) + 4 + 8 + 8 + 8 + 8 + 16 + 8 + 16 + 16;
+  const double out = 3.0 * (4 + nb) + 16;
+  return rec + dfrac * (node + out);
+}
 constexpr int kIdxBits = 13;      // log2(kBucketCap)
 
 // The bucket bytes are sorted least significant first (LSD): the fused
@@ -2715,6 +2726,27 @@ static void sizing_features(const uint8_t* regs, double occ_s, double valid_s, d
 
 static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind);
 
+// Device bytes the handle holds (all of its buffers).
+static size_t handle_bytes(const gx_build* b) {
+  size_t s = b->u_ptr[0].cap + b->u_ptr[1].cap;
+  for (const DevBuf* d :
+       {&b->text2, &b->rtext, &b->fterm1, &b->fterm2, &b->fsz, &b->foff, &b->text, &b->scratch, &b->blk, &b->blk2,
+        &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase, &b->words, &b->valid, &b->nocc, &b->occbase,
+        &b->hmask, &b->hidx, &b->table, &b->occ_slot, &b->occ_code, &b->claimed, &b->ckey, &b->cval, &b->ckey2,
+        &b->cval2, &b->nslot, &b->klo, &b->khi, &b->cnt, &b->msk, &b->needs, &b->selcnt, &b->seg, &b->selflag,
+        &b->selpos, &b->selk, &b->selv, &b->selk2, &b->selv2, &b->lists, &b->rpos, &b->rkey, &b->rval, &b->rkey2,
+        &b->rval2, &b->hkey, &b->hval, &b->hkey2, &b->hval2, &b->uflag, &b->upos, &b->uval, &b->hcu, &b->hcf,
+        &b->htot, &b->hstart, &b->rsize, &b->roff, &b->out, &b->errw, &b->u_lnk, &b->u_tand, &b->u_cut, &b->u_flag,
+        &b->u_any, &b->u_rep, &b->u_pos, &b->u_isrep, &b->u_lenrep, &b->u_uid, &b->u_repnode, &b->u_len,
+        &b->u_base, &b->u_order, &b->u_cov, &b->u_edge, &b->u_ecnt, &b->u_hk, &b->u_hk2, &b->u_hv, &b->u_hv2,
+        &b->u_uflag, &b->u_uposh, &b->u_hu, &b->u_hf, &b->u_htot, &b->u_hstart, &b->u_hklo, &b->u_hvc, &b->u_rsize,
+        &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
+        &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
+        &b->hregs, &b->sfeat, &b->spred})
+    s += d->cap;
+  return s;
+}
+
 template <int W>
 static void run_build(gx_build* b) {
   hipStream_t st = b->st;
@@ -2841,15 +2873,19 @@ static void run_build(gx_build* b) {
   if (sorted && N) {
     uint64_t target = b->range_occ;
     if (!target) {
+      // HBM free now plus what this handle holds beyond the parsed reads
+      // (its buffers from an earlier build are reused), less a margin
       size_t fr = 0, tot = 0;
       GX_HIP(hipMemGetInfo(&fr, &tot));
-      size_t held = 0;
+      size_t live = 0;
       for (DevBuf* d : {&b->text, &b->text2, &b->rtext, &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase,
                         &b->words, &b->valid, &b->nocc, &b->occbase, &b->hmask, &b->hidx})
-        held += d->cap;
+        live += d->cap;
+      const double avail = (double)fr + (double)handle_bytes(b) - (double)live;
+      const double tiles = (double)((N + kSortTile - 1) / kSortTile);
       const double dfrac = b->pred[0] > 0 ? std::min(1.0, b->pred[0] / (double)N) : 0.6;
-      const double per_occ = std::max<double>(2.0 * sizeof(SK) + 17.0, dfrac * (double)kNodeBytes[W - 1] + 1.0);
-      const double budget = 0.7 * (double)tot - (double)held - 48.0 * (double)nheads;
+      const double per_occ = pass_bytes_per_occurrence(W, nb, dfrac);
+      const double budget = avail - 0.1 * (double)tot - 48.0 * (double)nheads - 16.0 * 256.0 * tiles;
       target = budget > per_occ ? (uint64_t)(budget / per_occ) : 1;
     }
     if (target < N) {
@@ -3239,6 +3275,9 @@ static void run_build(gx_build* b) {
     ph_end(b, PH_KREPLAY);
   }
   ph_end(b, PH_REPLAY);
+  // replay inputs and order are dead (lists, needs and the node arrays stay)
+  b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->nslot, &b->rcode, &b->selcnt, &b->seg, &b->rpos, &b->rkey,
+                                                 &b->rval, &b->rkey2, &b->rval2});
 
   // 7. heads
   ph_begin(b, PH_HEADS);
@@ -3294,6 +3333,10 @@ static void run_build(gx_build* b) {
   }
   exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   ph_end(b, PH_HEADS);
+  // head sort arrays are dead; with several key ranges so is the node index
+  // (the unitig pass, which looks nodes up, needs a single-range build)
+  b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->hkey, &b->hval, &b->hkey2, &b->hval2, &b->uflag, &b->upos});
+  if (R > 1) b->reclaim.dead.push_back(&b->nix_slots);
 
   // 8. serialisation
   ph_begin(b, PH_SER);
@@ -3335,6 +3378,7 @@ static void run_build(gx_build* b) {
   }
   exclusive_scan<uint64_t, uint64_t>(rsize, roff, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   uint64_t total = read1(st, roff + nn);
+  b->reclaim.dead.push_back(&b->rsize);
   b->part_rec.assign(P + 1, 0);
   b->part_off.assign(P + 1, 0);
   if (P > 1) {
