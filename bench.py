@@ -356,7 +356,8 @@ def main():
                    "k": k, "read_len": L, "read_pairs": text.count(b"\n"), "kmer_occurrences": N,
                    "distinct_kmers": st["distinct"], "replayed_nodes": st["replayed_nodes"],
                    "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"],
-                   "max_coverage": st["max_coverage"], "unitigs": unitigs, "edge_order": "jdk7",
+                   "max_coverage": st["max_coverage"], "oversized_bucket_records": st["oversized_bucket_records"],
+                   "unitigs": unitigs, "edge_order": "jdk7",
                    "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,

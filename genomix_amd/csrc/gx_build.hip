@@ -2460,7 +2460,39 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
         seq_at[u] = rec_at + (uint64_t)(p - rs);
         p += nbl;
       } else {
-        for (uint64_t bq = 0; bq < nbl; bq++) *p++ = useq_byte<W>(c, ord, len, nbl - 1 - bq);
+        // bytes from the most significant end, eight per store; the first
+        // node's k letters come from its key in registers, each later
+        // letter is the last letter of the next node
+        const uint32_t o0 = ord[0];
+        const KV v0 = node_key<W>(c.klo, c.khi, o0 & 0x7fffffffu);
+        const bool f0 = !(o0 & 0x80000000u);
+        uint64_t acc = 0;
+        int nacc = 0;
+        for (int64_t q = (int64_t)nbl - 1; q >= 0; q--) {
+          uint32_t byte = 0;
+          for (int s2 = 0; s2 < 4; s2++) {
+            const uint64_t j = 4 * (uint64_t)q + s2;
+            if (j >= len) break;
+            uint32_t x;
+            if (j < (uint64_t)c.k) {
+              const int qq = f0 ? (int)j : c.k - 1 - (int)j;
+              x = (uint32_t)((qq < 32 ? v0.lo >> (2 * qq) : v0.hi >> (2 * (qq - 32))) & 3);
+              if (!f0) x = 3 - x;
+            } else {
+              const uint32_t o = ord[j - c.k + 1];
+              x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
+            }
+            byte |= x << (2 * s2);
+          }
+          acc |= (uint64_t)byte << (8 * nacc);
+          if (++nacc == 8) {
+            memcpy(p, &acc, 8);
+            p += 8;
+            acc = 0;
+            nacc = 0;
+          }
+        }
+        put_le(p, acc, nacc);
       }
     }
     double cov = (double)c.ucov[u] / (double)m;
