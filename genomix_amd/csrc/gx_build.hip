@@ -31,6 +31,7 @@
 #include "gx_sizer.h"
 #include "../../include/genomix_batch.h"
 #include "../../include/genomix_stream.h"
+#include "../../include/genomix_clean.h"
 
 namespace gx {
 
@@ -2596,6 +2597,7 @@ struct gx_build {
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   DevBuf hist2;               // key ranges: the next range's per-tile digit counts
+  DevBuf c_keep, c_pos, c_klo, c_khi, c_cnt, c_msk, c_hcu, c_hcf, c_htot, c_hstart, c_uval;  // graph cleaning
   uint64_t range_occ = 0;     // occurrences per key-range pass (0: from the HBM size)
   gx_part_sink sink = nullptr;
   void* sink_user = nullptr;
@@ -2758,6 +2760,43 @@ static void sizing_features(const uint8_t* regs, double occ_s, double valid_s, d
 
 static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind);
 
+// Key -> node index over nodes [0, nn) in key order whose keys' top byte
+// lies in [tlo, thi).
+template <int W>
+static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo, const uint64_t* khi, uint32_t tlo,
+                                  uint32_t thi) {
+  hipStream_t st = b->st;
+  const int nb = b->nb;
+  const unsigned G = b->grid;
+  const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
+  const int xbkb = nb < 2 ? 1 : (nb >= 4 && nn > (1ull << 26)) ? 3 : 2;
+  const uint32_t nbk = 1u << (8 * xbkb);
+  const uint32_t xb0 = tlo << (8 * (xbkb - 1)), xb1 = thi << (8 * (xbkb - 1));
+  b->nix_bkb = xbkb;
+  uint32_t* ndir = buf<uint32_t>(b->nix_dir, nbk + 1);
+  void* nslots = b->nix_slots.ensure(std::max<uint64_t>(1, 2 * nn) * slotsz);
+  if (xb0 > 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(xb0, kBlock, G)), dim3(kBlock), 0, st, ndir, 0, xb0, 0u);
+    GX_HIP(hipGetLastError());
+  }
+  if (xb1 < nbk) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nbk - xb1, kBlock, G)), dim3(kBlock), 0, st, ndir, (uint64_t)xb1 + 1,
+                       (uint64_t)nbk + 1, (uint32_t)nn);
+    GX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_nix_dir<W>, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, xbkb, xb0,
+                     xb1, ndir);
+  GX_HIP(hipGetLastError());
+  if (xbkb == 3)
+    hipLaunchKernelGGL((k_nix_build<W, kNixLdsSmall>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo,
+                       khi, ndir, nslots, xb0, xb1);
+  else
+    hipLaunchKernelGGL((k_nix_build<W, kNixLds>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo, khi,
+                       ndir, nslots, xb0, xb1);
+  GX_HIP(hipGetLastError());
+  return NodeIndex{ndir, nslots, nb, nbk, xbkb};
+}
+
 // Device bytes the handle holds (all of its buffers).
 static size_t handle_bytes(const gx_build* b) {
   size_t s = b->u_ptr[0].cap + b->u_ptr[1].cap;
@@ -2774,7 +2813,8 @@ static size_t handle_bytes(const gx_build* b) {
         &b->u_uflag, &b->u_uposh, &b->u_hu, &b->u_hf, &b->u_htot, &b->u_hstart, &b->u_hklo, &b->u_hvc, &b->u_rsize,
         &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
         &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
-        &b->hregs, &b->sfeat, &b->spred})
+        &b->hregs, &b->sfeat, &b->spred, &b->c_keep, &b->c_pos, &b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk,
+        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval})
     s += d->cap;
   return s;
 }
@@ -3199,32 +3239,7 @@ static void run_build(gx_build* b) {
 
   // 5b. key -> node index (heads, unitig links)
   ph_begin(b, PH_NIX);
-  const int xbkb = nb < 2 ? 1 : (nb >= 4 && nn > (1ull << 26)) ? 3 : 2;
-  const uint32_t nbk = 1u << (8 * xbkb);
-  const uint32_t xb0 = tlo << (8 * (xbkb - 1)), xb1 = thi << (8 * (xbkb - 1));
-  b->nix_bkb = xbkb;
-  uint32_t* ndir = buf<uint32_t>(b->nix_dir, nbk + 1);
-  void* nslots = b->nix_slots.ensure(std::max<uint64_t>(1, 2 * nn) * slotsz);
-  if (xb0 > 0) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(xb0, kBlock, G)), dim3(kBlock), 0, st, ndir, 0, xb0, 0u);
-    GX_HIP(hipGetLastError());
-  }
-  if (xb1 < nbk) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nbk - xb1, kBlock, G)), dim3(kBlock), 0, st, ndir, (uint64_t)xb1 + 1,
-                       (uint64_t)nbk + 1, (uint32_t)nn);
-    GX_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_nix_dir<W>, dim3(grid_for(nn + 1, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, nb, xbkb, xb0,
-                     xb1, ndir);
-  GX_HIP(hipGetLastError());
-  if (xbkb == 3)
-    hipLaunchKernelGGL((k_nix_build<W, kNixLdsSmall>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo,
-                       khi, ndir, nslots, xb0, xb1);
-  else
-    hipLaunchKernelGGL((k_nix_build<W, kNixLds>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo, khi,
-                       ndir, nslots, xb0, xb1);
-  GX_HIP(hipGetLastError());
-  const NodeIndex nix{ndir, nslots, nb, nbk, xbkb};
+  const NodeIndex nix = build_node_index<W>(b, nn, klo, khi, tlo, thi);
   if (sorted) cap = 2 * nn;
   ph_end(b, PH_NIX);
 
@@ -3781,6 +3796,136 @@ static void fastq_convert(gx_build* b) {
   b->rtext_len = total;
 }
 
+// ===========================================================================
+// Graph cleaning: REMOVE_BAD_COVERAGE, then MERGE (the pipeline step after
+// the first merge, GenomixJobConf.java:511-517).  The reference removes each
+// merged vertex whose average coverage lies below minCoverage or above
+// maxCoverage, and its neighbours drop their edges to it
+// (RemoveBadCoverageVertex.java:38-70); the next MERGE joins the rest.  On
+// the device the node graph is cut instead: every node of a removed merge
+// path goes, surviving nodes lose their edge bits towards removed nodes,
+// and the unitig pass runs again over the surviving nodes (still in key
+// order) -- the same merged records as merging the surviving vertices.
+// ===========================================================================
+
+// keep[i]: the coverage node i's merged record carries lies in [minc, maxc]
+__global__ void k_cov_keep(uint64_t nn, const uint32_t* __restrict__ urep, const uint64_t* __restrict__ uid_of_rep,
+                           const unsigned long long* __restrict__ ucov, const uint64_t* __restrict__ ulen, float minc,
+                           float maxc, uint32_t* __restrict__ keep) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t u = uid_of_rep[urep[i] & 0x7fffffffu];
+    const float c = (float)((double)ucov[u] / (double)ulen[u]);
+    keep[i] = (c < minc || c > maxc) ? 0u : 1u;
+  }
+}
+
+// Surviving nodes, compacted in key order, with the edge bits towards
+// removed nodes cleared.
+template <int W>
+__global__ void k_keep_nodes(uint64_t nn, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos,
+                             const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
+                             const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ msk, NodeIndex ix, int k,
+                             const uint32_t* __restrict__ hcu, const uint32_t* __restrict__ hcf,
+                             uint64_t* __restrict__ klo2, uint64_t* __restrict__ khi2, uint32_t* __restrict__ cnt2,
+                             uint32_t* __restrict__ msk2, uint32_t* __restrict__ hcu2, uint32_t* __restrict__ hcf2,
+                             uint64_t* __restrict__ htot2, unsigned long long* err) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!keep[i]) continue;
+    const uint64_t j = pos[i];
+    const KV v = node_key<W>(klo, khi, i);
+    uint32_t m = msk[i];
+    for (uint32_t t = 0; t < 4; t++)
+      for (uint32_t x = 0; x < 4; x++) {
+        if (!((m >> (4 * t + x)) & 1)) continue;
+        const uint32_t q = node_lookup<W>(ix, neighbour<W>(v, k, t, x));
+        if (q == kEnd) atomicMin(err, (unsigned long long)i);
+        else if (!keep[q]) m &= ~(1u << (4 * t + x));
+      }
+    klo2[j] = klo[i];
+    if (W == 2) khi2[j] = khi[i];
+    cnt2[j] = cnt[i];
+    msk2[j] = m;
+    hcu2[j] = hcu[i];
+    hcf2[j] = hcf[i];
+    htot2[j] = (uint64_t)hcu[i] + hcf[i];
+  }
+}
+
+__global__ void k_keep_heads(uint64_t nn, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos,
+                             const uint64_t* __restrict__ hstart, const uint32_t* __restrict__ hcu,
+                             const uint32_t* __restrict__ hcf, const uint32_t* __restrict__ uval,
+                             const uint64_t* __restrict__ hstart2, uint32_t* __restrict__ uval2) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (!keep[i]) continue;
+    const uint64_t a = hstart[i], o = hstart2[pos[i]], n = (uint64_t)hcu[i] + hcf[i];
+    for (uint64_t q = 0; q < n; q++) uval2[o + q] = uval[a + q];
+  }
+}
+
+static void swap_buf(DevBuf& x, DevBuf& y) {
+  std::swap(x.p, y.p);
+  std::swap(x.cap, y.cap);
+}
+
+template <int W>
+static void clean_bad_coverage(gx_build* b, float minc, float maxc) {
+  hipStream_t st = b->st;
+  const uint64_t nn = b->nnodes;
+  const unsigned G = b->grid;
+  const NodeIndex nix{b->nix_dir.as<uint32_t>(), b->nix_slots.p, b->nb, 1u << (8 * b->nix_bkb), b->nix_bkb};
+  unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
+  GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
+  uint32_t* keep = buf<uint32_t>(b->c_keep, nn + 1);
+  uint64_t* pos = buf<uint64_t>(b->c_pos, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_cov_keep, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, b->u_rep.as<uint32_t>(),
+                       b->u_uid.as<uint64_t>(), b->u_cov.as<unsigned long long>(), b->u_len.as<uint64_t>(), minc,
+                       maxc, keep);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(keep, pos, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+  const uint64_t nn2 = read1(st, pos + nn);
+  uint64_t* klo2 = buf<uint64_t>(b->c_klo, nn2 + 1);
+  uint64_t* khi2 = W == 2 ? buf<uint64_t>(b->c_khi, nn2 + 1) : nullptr;
+  uint32_t* cnt2 = buf<uint32_t>(b->c_cnt, nn2 + 1);
+  uint32_t* msk2 = buf<uint32_t>(b->c_msk, nn2 + 1);
+  uint32_t* hcu2 = buf<uint32_t>(b->c_hcu, nn2 + 1);
+  uint32_t* hcf2 = buf<uint32_t>(b->c_hcf, nn2 + 1);
+  uint64_t* htot2 = buf<uint64_t>(b->c_htot, nn2 + 1);
+  uint64_t* hstart2 = buf<uint64_t>(b->c_hstart, nn2 + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_keep_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, keep, pos,
+                       b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, b->cnt.as<uint32_t>(),
+                       b->msk.as<uint32_t>(), nix, b->cfg.k, b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), klo2, khi2,
+                       cnt2, msk2, hcu2, hcf2, htot2, errw);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(htot2, hstart2, nn2, scratch_for(b, scan_scratch_bytes<uint64_t>(nn2)), st);
+  const uint64_t nh2 = read1(st, hstart2 + nn2);
+  uint32_t* uval2 = buf<uint32_t>(b->c_uval, nh2 + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_keep_heads, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, keep, pos,
+                       b->hstart.as<uint64_t>(), b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), b->uval.as<uint32_t>(),
+                       hstart2, uval2);
+    GX_HIP(hipGetLastError());
+  }
+  if (read1(st, errw) != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the graph");
+  // the surviving graph becomes the handle's node graph (the build's own
+  // records are unchanged)
+  swap_buf(b->klo, b->c_klo);
+  if (W == 2) swap_buf(b->khi, b->c_khi);
+  swap_buf(b->cnt, b->c_cnt);
+  swap_buf(b->msk, b->c_msk);
+  swap_buf(b->hcu, b->c_hcu);
+  swap_buf(b->hcf, b->c_hcf);
+  swap_buf(b->hstart, b->c_hstart);
+  swap_buf(b->uval, b->c_uval);
+  b->nnodes = nn2;
+  b->nuheads = nh2;
+  build_node_index<W>(b, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, 0, 256);
+  run_unitigs<W>(b);
+}
+
 extern "C" {
 
 // Checks of a configuration (GenomixJobConf.java:417-423 for k); returns a
@@ -3940,6 +4085,21 @@ int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   b->stats[10] = b->u_rounds;
   for (int i = 0; i < n && i < 12; i++) out[i] = b->stats[i];
   return GX_OK;
+}
+
+int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_bad_coverage before unitig_run");
+    if (!(min_cov >= 0.f) || !(max_cov >= 0.f))  // GenomixJobConf.java:442-447
+      throw GxError(GX_EINVAL, "coverage bounds must not be negative");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
+    b->unitigs_done = false;
+    if (b->W == 1) clean_bad_coverage<1>(b, min_cov, max_cov);
+    else clean_bad_coverage<2>(b, min_cov, max_cov);
+    b->unitigs_done = true;
+  });
 }
 
 int gx_unitig_run(gx_build* b) {

@@ -33,7 +33,7 @@ EXPORTED = [
     "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
     "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
-    "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream",
+    "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage",
 ]
 
 
@@ -101,6 +101,7 @@ def lib():
         L.gx_build_set_config.argtypes = [vp, ctypes.POINTER(_Cfg)]
         L.gx_build_set_method.argtypes = [vp, i32]
         L.gx_build_set_stream.argtypes = [vp, ctypes.c_uint64, PART_SINK, vp]
+        L.gx_clean_bad_coverage.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -235,6 +236,21 @@ class GraphBuilder:
         w = ctypes.c_size_t()
         self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
         return buf.raw[:w.value]
+
+    def unitig_records(self) -> bytes:
+        """The merged records of the last unitig pass or cleaning step."""
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        w = ctypes.c_size_t()
+        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
+    def clean_bad_coverage(self, min_cov=3.0, max_cov=float("inf")) -> bytes:
+        """REMOVE_BAD_COVERAGE then MERGE (include/genomix_clean.h) on the
+        graph after a unitig pass; returns the merged records."""
+        self._check(lib().gx_clean_bad_coverage(self._h, ctypes.c_float(min_cov), ctypes.c_float(max_cov)))
+        return self.unitig_records()
 
     def stats(self):
         a = (ctypes.c_uint64 * 12)()

@@ -1,0 +1,35 @@
+/* genomix_clean.h — graph cleaning on the built graph.
+ *
+ * Genomix's default pipeline after BUILD and MERGE runs REMOVE_BAD_COVERAGE
+ * then MERGE again (GenomixJobConf.java:511-517).  The entry point here
+ * performs both on the graph a handle holds.
+ *
+ * All functions return GX_OK (0) or a negative GX_* code (genomix_gx.h).
+ */
+#ifndef GENOMIX_CLEAN_H
+#define GENOMIX_CLEAN_H
+
+#include "genomix_gx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* REMOVE_BAD_COVERAGE, then MERGE.  Replaces the Pregelix jobs
+ * RemoveBadCoverageVertex (RemoveBadCoverageVertex.java:38-70: a merged
+ * vertex whose average coverage is below min_cov or above max_cov is
+ * removed and its neighbours drop their edges to it; conf keys
+ * genomix.removeBadCoverage.minCoverage / maxCoverage, defaults 3.0 and
+ * +inf, GenomixJobConf.java:330-331,498-502) and the P4 merge that follows.
+ * Call after gx_unitig_run.  Afterwards gx_unitig_info / gx_unitig_emit
+ * give the merged records of the cleaned graph, and the handle's graph is
+ * the cleaned node graph (a further gx_unitig_run merges it again); the
+ * build's own records (gx_build_emit_part) are unchanged.  Negative bounds
+ * fail with GX_EINVAL, as the reference's configuration check does. */
+int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENOMIX_CLEAN_H */

@@ -1009,7 +1009,7 @@ struct GNode {
 static inline int et_side(int t) { return (t == FF || t == FR) ? 0 : 1; }
 static inline int et_mirror(int t) { return t == FF ? RR : t == RR ? FF : t; }
 
-static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
+static std::vector<GNode> parse_graph(const uint8_t* recs, size_t n, int k) {
   std::vector<GNode> g;
   size_t off = 0;
   while (off < n) {
@@ -1023,8 +1023,35 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     g.push_back(std::move(x));
     off += 8 + recLen;
   }
-  const size_t N = g.size();
+  return g;
+}
+
+static std::vector<uint8_t> graph_records(const std::vector<GNode>& g) {
+  std::vector<uint8_t> out;
+  for (const auto& x : g) {
+    std::vector<uint8_t> kb, vb;
+    x.key.write(kb);
+    x.node.write(vb);
+    put_be32(out, (uint32_t)(kb.size() + vb.size()));
+    put_be32(out, (uint32_t)kb.size());
+    out.insert(out.end(), kb.begin(), kb.end());
+    out.insert(out.end(), vb.begin(), vb.end());
+  }
+  return out;
+}
+
+// Maximal merge paths of a node graph (the P4 fixed point), see unitigs().
+struct Elem { size_t node; bool fwd; };
+struct Paths {
   std::map<std::vector<uint8_t>, size_t> idx;
+  std::vector<long> comp_of;
+  std::vector<std::vector<Elem>> comps;
+};
+
+static Paths merge_paths(const std::vector<GNode>& g) {
+  Paths P;
+  const size_t N = g.size();
+  auto& idx = P.idx;
   for (size_t i = 0; i < N; i++) idx[g[i].key.b] = i;
   std::vector<int> deg(2 * N, 0);
   std::vector<char> tandem(N, 0);
@@ -1051,9 +1078,9 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     if (deg[2 * i + d] != 1 || tandem[i] || j < 0 || (size_t)j == i) return false;
     return deg[2 * j + nbr_side[2 * i + d]] == 1 && !tandem[j];
   };
-  struct Elem { size_t node; bool fwd; };
-  std::vector<long> comp_of(N, -1);
-  std::vector<std::vector<Elem>> comps;
+  auto& comp_of = P.comp_of;
+  auto& comps = P.comps;
+  comp_of.assign(N, -1);
   for (size_t i = 0; i < N; i++) {
     if (comp_of[i] >= 0) continue;
     // walk exiting F (i forward) and exiting R (i reversed)
@@ -1103,6 +1130,25 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     for (auto& e : path) comp_of[e.node] = (long)comps.size();
     comps.push_back(path);
   }
+  return P;
+}
+
+// Coverage of a merged path: sum of node coverages / number of k-mers
+// (Node.mergeCoverage is this weighted average, computed pairwise in float
+// by the reference).
+static float path_coverage(const std::vector<GNode>& g, const std::vector<Elem>& path) {
+  double csum = 0;
+  for (auto& e : path) csum += g[e.node].node.cov;
+  return (float)(csum / (double)path.size());
+}
+
+static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
+  const std::vector<GNode> g = parse_graph(recs, n, k);
+  const size_t N = g.size();
+  const Paths P = merge_paths(g);
+  const auto& idx = P.idx;
+  const auto& comp_of = P.comp_of;
+  const auto& comps = P.comps;
   // position / orientation of every node in its unitig
   std::vector<size_t> pos(N);
   std::vector<char> ori(N);
@@ -1139,10 +1185,8 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     }
     // coverage: sum of node coverages / number of k-mers (Node.mergeCoverage
     // is this weighted average, computed pairwise in float by the reference)
-    double csum = 0;
-    for (auto& e : path) csum += g[e.node].node.cov;
     u.has_cov = true;
-    u.cov = (float)(csum / (double)m);
+    u.cov = path_coverage(g, path);
     // read heads: offset = k-mer position in the unitig; flipped heads sit
     // at position + k - 1; a reversed node swaps the two sets
     // (Node.mergeUnflippedAndFlippedReadIDs, Node.java:740-806)
@@ -1194,6 +1238,41 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     out.insert(out.end(), vb.begin(), vb.end());
   }
   return out;
+}
+
+// REMOVE_BAD_COVERAGE then MERGE, restated on the node graph.  The
+// reference removes every merged vertex whose average coverage is below
+// minCoverage or above maxCoverage and its neighbours drop the edges to it
+// (RemoveBadCoverageVertex.java:38-60, edge removal :62-70;
+// GenomixJobConf.java:498-502 defaults 3.0 / +inf); the next MERGE joins
+// what is left.  Here: the build's nodes are merged (merge_paths), every
+// node of a removed path is dropped, surviving nodes lose their edges to
+// dropped ones (list order kept, as VKmerList.remove), and the surviving
+// graph is merged again -- the same paths, sequences, coverages, heads and
+// outer edges as merging the surviving merged vertices.
+static std::vector<uint8_t> clean_bad_coverage(const uint8_t* recs, size_t n, int k, float minc, float maxc) {
+  std::vector<GNode> g = parse_graph(recs, n, k);
+  const Paths P = merge_paths(g);
+  std::vector<char> dead(g.size(), 0);
+  for (const auto& path : P.comps) {
+    const float c = path_coverage(g, path);
+    if (c < minc || c > maxc)
+      for (auto& e : path) dead[e.node] = 1;
+  }
+  std::vector<GNode> keep;
+  for (size_t i = 0; i < g.size(); i++) {
+    if (dead[i]) continue;
+    GNode x = g[i];
+    for (int t = 0; t < 4; t++) {
+      std::vector<VKmer> l;
+      for (auto& z : x.node.edges[t])
+        if (!dead[P.idx.at(z.b)]) l.push_back(z);
+      x.node.edges[t] = l;
+    }
+    keep.push_back(std::move(x));
+  }
+  const std::vector<uint8_t> recs2 = graph_records(keep);
+  return unitigs(recs2.data(), recs2.size(), k);
 }
 
 }  // namespace ora
@@ -1373,6 +1452,15 @@ int ora_unitig_summary(const uint8_t* recs, size_t n, int k, double* out4) {
       off += 8 + recLen;
     }
     out4[0] = nrec; out4[1] = nk; out4[2] = cv; out4[3] = asc ? 1 : 0;
+  });
+}
+
+int ora_clean_bad_coverage(const uint8_t* recs, size_t n, int k, float minc, float maxc, uint8_t** out,
+                           size_t* outlen) {
+  return guard([&] {
+    auto r = clean_bad_coverage(recs, n, k, minc, maxc);
+    *out = dup_bytes(r.data(), r.size());
+    *outlen = r.size();
   });
 }
 

@@ -88,6 +88,9 @@ def lib():
         L.ora_unitig_summary.argtypes = [vp, sz, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_unitigs"):
             L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        if hasattr(L, "ora_clean_bad_coverage"):
+            L.ora_clean_bad_coverage.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp),
+                                                 ctypes.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -217,6 +220,18 @@ def unitigs(records: bytes, k: int) -> bytes:
     sequence oriented so that k-mer reads forward), sorted by key."""
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_unitigs(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def clean_bad_coverage(records: bytes, k: int, min_cov=3.0, max_cov=float("inf")) -> bytes:
+    """REMOVE_BAD_COVERAGE then MERGE (RemoveBadCoverageVertex.java:38-60;
+    defaults GenomixJobConf.java:498-502) on a graph record stream: merged
+    paths whose average coverage lies outside [min_cov, max_cov] are
+    removed with their nodes, survivors lose the edges to them, and the
+    surviving graph is merged again.  Returns the merged records."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_clean_bad_coverage(records, len(records), k, ctypes.c_float(min_cov), ctypes.c_float(max_cov),
+                                        ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
 
 

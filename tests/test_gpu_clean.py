@@ -1,0 +1,83 @@
+"""REMOVE_BAD_COVERAGE then MERGE on the device (include/genomix_clean.h)
+against the oracle's restatement, byte for byte."""
+
+import math
+
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gx():
+    import genomix_amd.gx as g
+    return g
+
+
+def _text(seed, G=40000, L=100, cov=12, paired=True):
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=G, read_len=L, coverage=cov, error_rate=0.01, paired=paired,
+                           insert_mean=3 * L, insert_sd=L // 4, repeat_frac=0.05, repeat_len=(300, 900),
+                           repeat_div=0.01, seed=seed)
+    return sim.simulate(spec)
+
+
+@pytest.mark.parametrize("k,paired", [(21, False), (31, True), (55, True), (25, True)])
+def test_clean_matches_oracle(gx, k, paired):
+    text = _text(3100 + k, paired=paired)
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        assert b.part_bytes(0) == recs
+        assert b.unitigs() == oracle.unitigs(recs, k)
+        got = b.clean_bad_coverage(3.0)
+        assert got == oracle.clean_bad_coverage(recs, k, 3.0)
+        assert b.part_bytes(0) == recs  # the build's records are unchanged
+
+
+def test_bounds(gx):
+    k = 31
+    text = _text(77, cov=20)
+    recs = oracle.build_graph(text, k)[0]
+    for lo, hi in ((0.0, math.inf), (5.0, 40.0), (2.5, 12.0), (10.0, 5.0)):
+        with gx.GraphBuilder(k) as b:
+            b.add_reads(text)
+            b.finish()
+            b.run_unitigs()
+            got = b.clean_bad_coverage(lo, hi)
+        assert got == oracle.clean_bad_coverage(recs, k, lo, hi), (lo, hi)
+        if lo == 0.0:
+            assert got == oracle.unitigs(recs, k)
+        if lo > hi:
+            assert got == b""
+
+
+def test_clean_after_hash_method_and_twice(gx):
+    k = 21
+    text = _text(91, G=20000, cov=15, paired=False)
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k, method="hash") as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        first = b.clean_bad_coverage(3.0)
+        assert first == oracle.clean_bad_coverage(recs, k, 3.0)
+        # the handle now holds the cleaned graph: merging it again changes nothing
+        b.run_unitigs()
+        assert b.unitig_records() == first
+
+
+def test_errors(gx):
+    with gx.GraphBuilder(21) as b:
+        b.add_reads(_text(5, G=5000, paired=False))
+        b.finish()
+        with pytest.raises(gx.GxError) as e:
+            b.clean_bad_coverage(3.0)
+        assert e.value.code == gx.GX_ESTATE
+        b.run_unitigs()
+        with pytest.raises(gx.GxError) as e:
+            b.clean_bad_coverage(-1.0)
+        assert e.value.code == gx.GX_EINVAL
