@@ -336,6 +336,19 @@ def main():
                      "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
                      "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
     unitigs = builder.unitig_count()
+    # graph cleaning after the step, as the reference pipeline runs it:
+    # REMOVE_BAD_COVERAGE (min 3.0) then MERGE, on the graph in HBM (timed
+    # on its second call, its buffers allocated)
+    for rep in range(2):
+        step()
+        devmem.synchronize()
+        t1 = time.perf_counter()
+        builder.clean_bad_coverage(3.0)
+        devmem.synchronize()
+        t_clean = time.perf_counter() - t1
+    clean = {"ms": round(1e3 * t_clean, 3), "min_coverage": 3.0,
+             "unitigs_before": unitigs, "unitigs_after": builder.unitig_count(),
+             "unitig_pass_ms": round(builder.phase_times().get("unitig", 0.0), 3)}
     builder.close()
     dtext.free()
     batched = None
@@ -362,6 +375,7 @@ def main():
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
         "roofline_kmer_pass": roofline_kmer,
+        "clean_bad_coverage": clean,
         "batched": batched,
         "cpu_baseline": cpu,
     }

@@ -3808,14 +3808,38 @@ static void fastq_convert(gx_build* b) {
 // order) -- the same merged records as merging the surviving vertices.
 // ===========================================================================
 
-// keep[i]: the coverage node i's merged record carries lies in [minc, maxc]
-__global__ void k_cov_keep(uint64_t nn, const uint32_t* __restrict__ urep, const uint64_t* __restrict__ uid_of_rep,
-                           const unsigned long long* __restrict__ ucov, const uint64_t* __restrict__ ulen, float minc,
-                           float maxc, uint32_t* __restrict__ keep) {
+// The removal rules, applied to the merged vertex (unitig) of each node.
+enum CleanRule { CLEAN_COVERAGE = 0, CLEAN_TIP = 1 };
+struct CleanArgs {
+  int rule;
+  float minc, maxc;  // CLEAN_COVERAGE: keep average coverage in [minc, maxc]
+  uint64_t maxlen;   // CLEAN_TIP: remove tips of at most maxlen letters
+  int k;
+};
+
+// keep[i]: node i's merged vertex survives the rule.  Coverage: the value
+// its record carries.  Tip: one outer edge on one side, none on the other
+// (Node.degree counts the FF+FR and RF+RR lists), and k + m - 1 letters.
+__global__ void k_unitig_keep(uint64_t nn, const uint32_t* __restrict__ urep, const uint64_t* __restrict__ uid_of_rep,
+                              const unsigned long long* __restrict__ ucov, const uint64_t* __restrict__ ulen,
+                              const uint32_t* __restrict__ uedge, const uint8_t* __restrict__ uecnt, CleanArgs a,
+                              uint32_t* __restrict__ keep) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t u = uid_of_rep[urep[i] & 0x7fffffffu];
-    const float c = (float)((double)ucov[u] / (double)ulen[u]);
-    keep[i] = (c < minc || c > maxc) ? 0u : 1u;
+    bool gone;
+    if (a.rule == CLEAN_COVERAGE) {
+      const float c = (float)((double)ucov[u] / (double)ulen[u]);
+      gone = c < a.minc || c > a.maxc;
+    } else {
+      int deg[2] = {0, 0};
+      for (int e = 0; e < uecnt[u]; e++) {
+        const uint32_t t = uedge[kUEdges * u + e] >> 30;
+        deg[(t == ET_FF || t == ET_FR) ? 0 : 1]++;
+      }
+      const bool tip = (deg[0] == 1 && deg[1] == 0) || (deg[1] == 1 && deg[0] == 0);
+      gone = tip && ulen[u] + (uint64_t)a.k - 1 <= a.maxlen;
+    }
+    keep[i] = gone ? 0u : 1u;
   }
 }
 
@@ -3868,7 +3892,7 @@ static void swap_buf(DevBuf& x, DevBuf& y) {
 }
 
 template <int W>
-static void clean_bad_coverage(gx_build* b, float minc, float maxc) {
+static void clean_graph(gx_build* b, const CleanArgs& ca) {
   hipStream_t st = b->st;
   const uint64_t nn = b->nnodes;
   const unsigned G = b->grid;
@@ -3878,9 +3902,9 @@ static void clean_bad_coverage(gx_build* b, float minc, float maxc) {
   uint32_t* keep = buf<uint32_t>(b->c_keep, nn + 1);
   uint64_t* pos = buf<uint64_t>(b->c_pos, nn + 1);
   if (nn) {
-    hipLaunchKernelGGL(k_cov_keep, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, b->u_rep.as<uint32_t>(),
-                       b->u_uid.as<uint64_t>(), b->u_cov.as<unsigned long long>(), b->u_len.as<uint64_t>(), minc,
-                       maxc, keep);
+    hipLaunchKernelGGL(k_unitig_keep, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
+                       b->u_rep.as<uint32_t>(), b->u_uid.as<uint64_t>(), b->u_cov.as<unsigned long long>(),
+                       b->u_len.as<uint64_t>(), b->u_edge.as<uint32_t>(), b->u_ecnt.as<uint8_t>(), ca, keep);
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint32_t, uint64_t>(keep, pos, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
@@ -4096,8 +4120,23 @@ int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov) {
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->unitigs_done = false;
-    if (b->W == 1) clean_bad_coverage<1>(b, min_cov, max_cov);
-    else clean_bad_coverage<2>(b, min_cov, max_cov);
+    const CleanArgs ca{CLEAN_COVERAGE, min_cov, max_cov, 0, b->cfg.k};
+    if (b->W == 1) clean_graph<1>(b, ca);
+    else clean_graph<2>(b, ca);
+    b->unitigs_done = true;
+  });
+}
+
+int gx_clean_tips(gx_build* b, int max_len) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_tips before unitig_run");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
+    b->unitigs_done = false;
+    const CleanArgs ca{CLEAN_TIP, 0.f, 0.f, (uint64_t)(max_len < 0 ? b->cfg.k : max_len), b->cfg.k};
+    if (b->W == 1) clean_graph<1>(b, ca);
+    else clean_graph<2>(b, ca);
     b->unitigs_done = true;
   });
 }

@@ -33,7 +33,7 @@ EXPORTED = [
     "gx_mlp_gradient", "gx_mlp_train_step", "gx_mlp_last_error", "gx_mlp_destroy", "gx_build_set_sizer",
     "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
-    "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage",
+    "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
         L.gx_build_set_method.argtypes = [vp, i32]
         L.gx_build_set_stream.argtypes = [vp, ctypes.c_uint64, PART_SINK, vp]
         L.gx_clean_bad_coverage.argtypes = [vp, ctypes.c_float, ctypes.c_float]
+        L.gx_clean_tips.argtypes = [vp, i32]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -250,6 +251,12 @@ class GraphBuilder:
         """REMOVE_BAD_COVERAGE then MERGE (include/genomix_clean.h) on the
         graph after a unitig pass; returns the merged records."""
         self._check(lib().gx_clean_bad_coverage(self._h, ctypes.c_float(min_cov), ctypes.c_float(max_cov)))
+        return self.unitig_records()
+
+    def clean_tips(self, max_len=-1) -> bytes:
+        """TIP_SINGLE_NODE then MERGE (include/genomix_clean.h); max_len < 0:
+        k.  Returns the merged records."""
+        self._check(lib().gx_clean_tips(self._h, max_len))
         return self.unitig_records()
 
     def stats(self):

@@ -28,6 +28,16 @@ extern "C" {
  * fail with GX_EINVAL, as the reference's configuration check does. */
 int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov);
 
+/* TIP_SINGLE_NODE, then MERGE.  Replaces SingleNodeTipRemoveVertex
+ * (SingleNodeTipRemoveVertex.java:47-85): a merged vertex with exactly one
+ * edge on one side and none on the other (Node.degree, Node.java:820-828)
+ * whose sequence has at most max_len letters is removed and its neighbour
+ * drops the edge to it; then the P4 merge.  max_len < 0 means the
+ * reference's default, k (genomix.tipRemove.maxLength,
+ * GenomixJobConf.java:332,504-505).  Same calling rules as
+ * gx_clean_bad_coverage. */
+int gx_clean_tips(gx_build* b, int max_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1240,23 +1240,25 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
   return out;
 }
 
-// REMOVE_BAD_COVERAGE then MERGE, restated on the node graph.  The
-// reference removes every merged vertex whose average coverage is below
-// minCoverage or above maxCoverage and its neighbours drop the edges to it
-// (RemoveBadCoverageVertex.java:38-60, edge removal :62-70;
-// GenomixJobConf.java:498-502 defaults 3.0 / +inf); the next MERGE joins
-// what is left.  Here: the build's nodes are merged (merge_paths), every
-// node of a removed path is dropped, surviving nodes lose their edges to
+// Graph cleaning, restated on the node graph: the build's nodes are merged
+// (merge_paths, serialised as unitigs() does), every node of a merged
+// vertex the rule removes is dropped, surviving nodes lose their edges to
 // dropped ones (list order kept, as VKmerList.remove), and the surviving
 // graph is merged again -- the same paths, sequences, coverages, heads and
 // outer edges as merging the surviving merged vertices.
-static std::vector<uint8_t> clean_bad_coverage(const uint8_t* recs, size_t n, int k, float minc, float maxc) {
+static std::vector<uint8_t> clean_nodes(const uint8_t* recs, size_t n, int k,
+                                        const std::function<bool(const GNode&)>& removed) {
   std::vector<GNode> g = parse_graph(recs, n, k);
   const Paths P = merge_paths(g);
+  const std::vector<uint8_t> merged = unitigs(recs, n, k);
+  std::set<std::vector<uint8_t>> gone;  // keys of removed merged vertices
+  for (const GNode& v : parse_graph(merged.data(), merged.size(), k))
+    if (removed(v)) gone.insert(v.key.b);
   std::vector<char> dead(g.size(), 0);
   for (const auto& path : P.comps) {
-    const float c = path_coverage(g, path);
-    if (c < minc || c > maxc)
+    size_t mn = path[0].node;
+    for (auto& e : path) mn = std::min(mn, e.node);
+    if (gone.count(g[mn].key.b))  // a merged vertex is keyed by its smallest k-mer
       for (auto& e : path) dead[e.node] = 1;
   }
   std::vector<GNode> keep;
@@ -1271,8 +1273,48 @@ static std::vector<uint8_t> clean_bad_coverage(const uint8_t* recs, size_t n, in
     }
     keep.push_back(std::move(x));
   }
-  const std::vector<uint8_t> recs2 = graph_records(keep);
+  return graph_records(keep);  // the surviving node graph
+}
+
+static std::vector<uint8_t> clean_graph(const uint8_t* recs, size_t n, int k,
+                                        const std::function<bool(const GNode&)>& removed) {
+  const std::vector<uint8_t> recs2 = clean_nodes(recs, n, k, removed);
   return unitigs(recs2.data(), recs2.size(), k);
+}
+
+// REMOVE_BAD_COVERAGE then MERGE: a merged vertex whose average coverage
+// is below minCoverage or above maxCoverage is removed and its neighbours
+// drop the edges to it (RemoveBadCoverageVertex.java:38-70; defaults
+// GenomixJobConf.java:498-502: 3.0 / +inf).
+static bool bad_coverage(const GNode& v, float minc, float maxc) { return v.node.cov < minc || v.node.cov > maxc; }
+
+static std::vector<uint8_t> clean_bad_coverage(const uint8_t* recs, size_t n, int k, float minc, float maxc) {
+  return clean_graph(recs, n, k, [&](const GNode& v) { return bad_coverage(v, minc, maxc); });
+}
+
+// TIP_SINGLE_NODE then MERGE: a merged vertex with degree 1 on one side and
+// 0 on the other whose sequence is at most maxLength letters long is
+// removed and its neighbour drops the edge to it
+// (SingleNodeTipRemoveVertex.java:47-85; Node.degree :820-828,
+// getKmerLength :191-193; default maxLength = k, GenomixJobConf.java:504-505).
+static bool short_tip(const GNode& v, int k, int maxlen) {
+  int deg[2] = {0, 0};
+  for (int t = 0; t < 4; t++) deg[et_side(t)] += (int)v.node.edges[t].size();
+  const int len = v.node.has_internal ? v.node.internal.k : k;
+  const bool tip = (deg[0] == 1 && deg[1] == 0) || (deg[1] == 1 && deg[0] == 0);
+  return tip && len <= maxlen;
+}
+
+static std::vector<uint8_t> clean_tips(const uint8_t* recs, size_t n, int k, int maxlen) {
+  return clean_graph(recs, n, k, [&](const GNode& v) { return short_tip(v, k, maxlen); });
+}
+
+// The reference pipeline's cleaning prefix (GenomixJobConf.java:511-517):
+// REMOVE_BAD_COVERAGE, MERGE, TIP_SINGLE_NODE, MERGE.
+static std::vector<uint8_t> clean_coverage_then_tips(const uint8_t* recs, size_t n, int k, float minc, float maxc,
+                                                     int maxlen) {
+  const std::vector<uint8_t> g1 = clean_nodes(recs, n, k, [&](const GNode& v) { return bad_coverage(v, minc, maxc); });
+  return clean_tips(g1.data(), g1.size(), k, maxlen);
 }
 
 }  // namespace ora
@@ -1459,6 +1501,23 @@ int ora_clean_bad_coverage(const uint8_t* recs, size_t n, int k, float minc, flo
                            size_t* outlen) {
   return guard([&] {
     auto r = clean_bad_coverage(recs, n, k, minc, maxc);
+    *out = dup_bytes(r.data(), r.size());
+    *outlen = r.size();
+  });
+}
+
+int ora_clean_tips(const uint8_t* recs, size_t n, int k, int maxlen, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto r = clean_tips(recs, n, k, maxlen);
+    *out = dup_bytes(r.data(), r.size());
+    *outlen = r.size();
+  });
+}
+
+int ora_clean_coverage_then_tips(const uint8_t* recs, size_t n, int k, float minc, float maxc, int maxlen,
+                                 uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto r = clean_coverage_then_tips(recs, n, k, minc, maxc, maxlen);
     *out = dup_bytes(r.data(), r.size());
     *outlen = r.size();
   });

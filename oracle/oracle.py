@@ -88,6 +88,11 @@ def lib():
         L.ora_unitig_summary.argtypes = [vp, sz, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_unitigs"):
             L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        if hasattr(L, "ora_clean_coverage_then_tips"):
+            L.ora_clean_coverage_then_tips.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, i32,
+                                                       ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        if hasattr(L, "ora_clean_tips"):
+            L.ora_clean_tips.argtypes = [vp, sz, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
         if hasattr(L, "ora_clean_bad_coverage"):
             L.ora_clean_bad_coverage.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp),
                                                  ctypes.POINTER(sz)]
@@ -232,6 +237,28 @@ def clean_bad_coverage(records: bytes, k: int, min_cov=3.0, max_cov=float("inf")
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_clean_bad_coverage(records, len(records), k, ctypes.c_float(min_cov), ctypes.c_float(max_cov),
                                         ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def clean_tips(records: bytes, k: int, max_len=None) -> bytes:
+    """TIP_SINGLE_NODE then MERGE (SingleNodeTipRemoveVertex.java:47-85;
+    default max_len = k, GenomixJobConf.java:504-505) on a graph record
+    stream: merged vertices with one edge on one side and none on the other
+    and at most max_len letters are removed, their neighbours drop the edge,
+    and the surviving graph is merged again.  Returns the merged records."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_clean_tips(records, len(records), k, k if max_len is None else max_len, ctypes.byref(p),
+                                ctypes.byref(n)))
+    return _take(p, n)
+
+
+def clean_coverage_then_tips(records: bytes, k: int, min_cov=3.0, max_cov=float("inf"), max_len=None) -> bytes:
+    """The reference pipeline's cleaning prefix (GenomixJobConf.java:511-517):
+    REMOVE_BAD_COVERAGE, MERGE, TIP_SINGLE_NODE, MERGE."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_clean_coverage_then_tips(records, len(records), k, ctypes.c_float(min_cov),
+                                              ctypes.c_float(max_cov), k if max_len is None else max_len,
+                                              ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
 
 

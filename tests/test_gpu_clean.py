@@ -81,3 +81,33 @@ def test_errors(gx):
         with pytest.raises(gx.GxError) as e:
             b.clean_bad_coverage(-1.0)
         assert e.value.code == gx.GX_EINVAL
+
+
+@pytest.mark.parametrize("k,paired", [(21, False), (31, True), (55, True)])
+def test_tips_match_oracle(gx, k, paired):
+    text = _text(4100 + k, paired=paired)
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        assert b.clean_tips() == oracle.clean_tips(recs, k)
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        assert b.clean_tips(3 * k) == oracle.clean_tips(recs, k, 3 * k)
+
+
+@pytest.mark.parametrize("k", [21, 31])
+def test_pipeline_cleaning_prefix(gx, k):
+    """REMOVE_BAD_COVERAGE, MERGE, TIP_SINGLE_NODE, MERGE (the reference's
+    default pipeline after BUILD, MERGE)."""
+    text = _text(5200 + k, cov=20)
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        b.clean_bad_coverage(3.0)
+        assert b.clean_tips() == oracle.clean_coverage_then_tips(recs, k, 3.0)

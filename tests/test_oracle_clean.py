@@ -59,3 +59,27 @@ def test_one_error_cleans_back_to_the_genome():
     assert len(formats.decode_records(oracle.unitigs(recs, k))) > 1  # the error splits the genome path
     cleaned = formats.decode_records(oracle.clean_bad_coverage(recs, k, 3.0))
     assert [n.internal for _, n in cleaned] == seqs
+
+
+def test_end_error_tip_is_removed():
+    """An error on a read's last base makes one erroneous k-mer hanging off
+    the genome path (a single-node tip); TIP_SINGLE_NODE removes it and the
+    merge gives back the genome as one unitig.  An error mid-read makes a
+    longer branch, which the rule keeps."""
+    rng = np.random.default_rng(12)
+    G, L, k = 600, 60, 21
+    genome = "".join("ACGT"[x] for x in rng.integers(0, 4, G))
+    reads = [genome[s:s + L] for s in range(0, G - L + 1, 3)]
+    base = formats.decode_records(oracle.unitigs(oracle.build_graph(_readids(reads), k)[0], k))
+    assert len(base) == 1
+    sub = lambda s, j: s[:j] + "ACGT"[("ACGT".index(s[j]) + 1) % 4] + s[j + 1:]
+    tip = list(reads)
+    tip[40] = sub(tip[40], L - 1)
+    recs = oracle.build_graph(_readids(tip), k)[0]
+    assert len(formats.decode_records(oracle.unitigs(recs, k))) == 3  # the genome in two at the branch, and the tip
+    cleaned = formats.decode_records(oracle.clean_tips(recs, k))
+    assert [n.internal for _, n in cleaned] == [base[0][1].internal]
+    mid = list(reads)
+    mid[40] = sub(mid[40], 30)
+    recs = oracle.build_graph(_readids(mid), k)[0]
+    assert oracle.clean_tips(recs, k) == oracle.unitigs(recs, k)
