@@ -2344,6 +2344,30 @@ struct USerCtx {
 
 __device__ inline uint64_t useq_len(const USerCtx& c, uint64_t u) { return c.ulen[u] + c.k - 1; }
 
+// Unitigs with more than kHeadsLong read heads have their heads written by
+// k_uheads_long (a wave per unitig, lanes over heads); k_useq_write leaves
+// the bytes to it.
+constexpr uint32_t kHeadsLong = 32;
+
+__device__ inline uint64_t uhead_bytes(const USerCtx& c, uint64_t j) {
+  const uint32_t m = c.hval[j] & 0x7fffffffu;
+  const uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
+  return 1 + 8 + 4 + (L + 3) / 4 + (Lm > 0 ? 4 + (Lm + 3) / 4 : 0);
+}
+
+// One read head of unitig records: mate flag, uuid (offset in the unitig,
+// mate, readId), the read and its mate as VKmers.
+__device__ inline void write_uhead(const USerCtx& c, uint64_t j, uint8_t*& p) {
+  const uint32_t mi = c.hval[j] & 0x7fffffffu;
+  const uint32_t L = c.mlen[mi], Lm = c.mlen[mi ^ 1];
+  *p++ = Lm > 0 ? 1 : 0;
+  const uint64_t off = (c.hkey[j] >> 30) & ((1ull << 23) - 1);
+  const uint64_t uuid = (off << 40) | ((uint64_t)(mi & 1) << 35) | (uint64_t)c.rid[mi >> 1];
+  put64(p, uuid);
+  put_read(p, c.words + c.wbase[mi], L);
+  if (Lm > 0) put_read(p, c.words + c.wbase[mi ^ 1], Lm);
+}
+
 __device__ inline uint64_t unitig_record_size(const USerCtx& c, uint64_t u) {
   uint64_t val = 1 + 4;  // flags + coverage
   int ne = c.uecnt[u];
@@ -2358,11 +2382,7 @@ __device__ inline uint64_t unitig_record_size(const USerCtx& c, uint64_t u) {
     if (!n) continue;
     val += 1 + 4;
     uint64_t a = h0 + (f ? nu_ : 0);
-    for (uint64_t j = a; j < a + n; j++) {
-      uint32_t m = c.hval[j] & 0x7fffffffu;
-      uint32_t L = c.mlen[m], Lm = c.mlen[m ^ 1];
-      val += 1 + 8 + 4 + (L + 3) / 4 + (Lm > 0 ? 4 + (Lm + 3) / 4 : 0);
-    }
+    for (uint64_t j = a; j < a + n; j++) val += uhead_bytes(c, j);
   }
   if (c.ulen[u] > 1) val += 4 + (useq_len(c, u) + 3) / 4;
   return 8 + 4 + c.nb + val;
@@ -2403,7 +2423,8 @@ __device__ inline uint8_t useq_byte(const USerCtx& c, const uint32_t* ord, uint6
 // coalesced 4-byte writes (spans larger than the stage go direct).
 template <int W>
 __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t* rs, uint64_t rec,
-                                           uint64_t rec_at, uint64_t* __restrict__ seq_at) {
+                                           uint64_t rec_at, uint64_t* __restrict__ seq_at,
+                                           uint64_t* __restrict__ head_at) {
   {
     uint8_t* p = rs;
     uint32_t rep = c.urep_node[u];
@@ -2439,15 +2460,11 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
       *p++ = 1;
       put32(p, n);
       uint64_t a = h0 + (f ? nu_ : 0);
-      for (uint64_t j = a; j < a + n; j++) {
-        uint32_t mi = c.hval[j] & 0x7fffffffu;
-        uint32_t L = c.mlen[mi], Lm = c.mlen[mi ^ 1];
-        *p++ = Lm > 0 ? 1 : 0;
-        uint64_t off = (c.hkey[j] >> 30) & ((1ull << 23) - 1);
-        uint64_t uuid = (off << 40) | ((uint64_t)(mi & 1) << 35) | (uint64_t)c.rid[mi >> 1];
-        put64(p, uuid);
-        put_read(p, c.words + c.wbase[mi], L);
-        if (Lm > 0) put_read(p, c.words + c.wbase[mi ^ 1], Lm);
+      if (nu_ + nf > kHeadsLong) {  // k_uheads_long writes these heads
+        head_at[2 * u + f] = rec_at + (uint64_t)(p - rs);
+        for (uint64_t j = a; j < a + n; j++) p += uhead_bytes(c, j);
+      } else {
+        for (uint64_t j = a; j < a + n; j++) write_uhead(c, j, p);
       }
     }
     if (m > 1) {
@@ -2506,7 +2523,8 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t* __restrict__ roff,
-                                                       uint8_t* __restrict__ out, uint64_t* __restrict__ seq_at) {
+                                                       uint8_t* __restrict__ out, uint64_t* __restrict__ seq_at,
+                                                       uint64_t* __restrict__ head_at) {
   __shared__ __attribute__((aligned(16))) uint8_t stage[kSerStage + 8];
   const uint64_t first = (uint64_t)blockIdx.x * kBlock;
   if (first >= c.nu) return;
@@ -2517,7 +2535,7 @@ __global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t
   const uint64_t u = first + threadIdx.x;
   if (u < last)
     write_unitig_record<W>(c, u, staged ? stage + (roff[u] - base) : out + roff[u], roff[u + 1] - roff[u], roff[u],
-                           seq_at);
+                           seq_at, head_at);
   if (!staged) return;
   __syncthreads();
   uint64_t a4 = (s0 + 3) & ~3ull, e4 = s1 & ~3ull;
@@ -2527,6 +2545,39 @@ __global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t
   const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
   uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
   for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
+}
+
+// Heads of unitigs with many of them: a wave per unitig; each pass places
+// 64 heads by a wave prefix sum of their sizes.
+__global__ void __launch_bounds__(kBlock) k_uheads_long(USerCtx c, const uint64_t* __restrict__ head_at,
+                                                       uint8_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t u = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); u < c.nu; u += nw) {
+    const uint32_t nu_ = c.hu[u], nf = c.hf[u];
+    if (nu_ + nf <= kHeadsLong) continue;
+    for (int f = 0; f < 2; f++) {
+      const uint32_t n = f ? nf : nu_;
+      if (!n) continue;
+      const uint64_t a = c.hstart[u] + (f ? nu_ : 0);
+      uint64_t at = head_at[2 * u + f];
+      for (uint32_t q0 = 0; q0 < n; q0 += 64) {
+        const uint32_t q = q0 + lane;
+        const uint32_t sz = q < n ? (uint32_t)uhead_bytes(c, a + q) : 0u;  // < 2^16 per head
+        uint32_t inc = sz;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(inc, o, 64);
+          if (lane >= o) inc += y;
+        }
+        if (q < n) {
+          uint8_t* p = out + at + (inc - sz);
+          write_uhead(c, a + q, p);
+        }
+        at += (uint32_t)__shfl(inc, 63, 64);
+      }
+    }
+  }
 }
 
 template <int W>
@@ -2594,6 +2645,7 @@ struct gx_build {
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
+  DevBuf u_headat;            // unitig pass: byte position of the heads of head-rich unitigs
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   DevBuf hist2;               // key ranges: the next range's per-tile digit counts
@@ -2814,7 +2866,7 @@ static size_t handle_bytes(const gx_build* b) {
         &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
         &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
         &b->hregs, &b->sfeat, &b->spred, &b->c_keep, &b->c_pos, &b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk,
-        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval})
+        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->u_headat})
     s += d->cap;
   return s;
 }
@@ -3676,9 +3728,13 @@ static void run_unitigs(gx_build* b) {
   uint8_t* out = buf<uint8_t>(b->u_out, total + 1);
   if (nu) {
     uint64_t* seq_at = buf<uint64_t>(b->u_seqat, nu + 1);
+    uint64_t* head_at = buf<uint64_t>(b->u_headat, 2 * nu + 2);
     // one block per kBlock unitigs (no grid-stride loop: every block stages its own span)
     hipLaunchKernelGGL(k_useq_write<W>, dim3(grid_for(nu, kBlock, 0xffffffffu)), dim3(kBlock), 0, st, c, roff, out,
-                       seq_at);
+                       seq_at, head_at);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_uheads_long, dim3(grid_for(nu, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, c, head_at,
+                       out);
     GX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_useq_long<W>, dim3(grid_for(nu, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, c, seq_at, out);
     GX_HIP(hipGetLastError());
