@@ -32,6 +32,7 @@
 #include "../../include/genomix_batch.h"
 #include "../../include/genomix_stream.h"
 #include "../../include/genomix_clean.h"
+#include "../../include/genomix_stats.h"
 
 namespace gx {
 
@@ -2646,6 +2647,11 @@ struct gx_build {
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
   DevBuf u_headat;            // unitig pass: byte position of the heads of head-rich unitigs
+  DevBuf s_acc, s_k1, s_k2, s_v1, s_v2, s_flag, s_pos, s_bval, s_bst, s_out;  // statistics, FASTA
+  int st_graph = -1;                                                          // graph of the last statistics
+  std::vector<uint64_t> st_tot;
+  std::vector<int64_t> st_sum, st_max;
+  std::vector<std::vector<std::pair<int64_t, uint64_t>>> st_bins;
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   DevBuf hist2;               // key ranges: the next range's per-tile digit counts
@@ -2866,7 +2872,8 @@ static size_t handle_bytes(const gx_build* b) {
         &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
         &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
         &b->hregs, &b->sfeat, &b->spred, &b->c_keep, &b->c_pos, &b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk,
-        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->u_headat})
+        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
+        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out})
     s += d->cap;
   return s;
 }
@@ -4006,6 +4013,308 @@ static void clean_graph(gx_build* b, const CleanArgs& ca) {
   run_unitigs<W>(b);
 }
 
+// ===========================================================================
+// Graph statistics and FASTA dump (GraphStatistics.java:79-146,
+// DriverUtils.dumpGraph :85-128) over the node graph or the last merge.
+// A vertex's properties: out-degree (FF + FR entries), in-degree (RF + RR),
+// sequence length (the internal k-mer, or k), average coverage, unflipped
+// and flipped read heads, self edges per type.
+// ===========================================================================
+
+struct VertexCtx {
+  int merged;  // 0: nodes, 1: unitigs of the last merge
+  int k;
+  uint64_t n;
+  const uint64_t *klo, *khi;
+  // nodes
+  const uint32_t *cnt, *msk, *hcu, *hcf;
+  // unitigs
+  const uint64_t* ulen;
+  const unsigned long long* ucov;
+  const uint32_t* uedge;
+  const uint8_t* uecnt;
+  const uint32_t *urep_node, *hu, *hf;
+  const uint64_t* ubase;
+  const uint32_t* order;
+};
+
+struct VProps {
+  uint32_t deg[2];  // [0] forward (out), [1] reverse (in)
+  uint64_t len;
+  float cov;
+  uint32_t nu, nf;
+  uint32_t self[4];
+};
+
+template <int W>
+__device__ inline VProps vertex_props(const VertexCtx& c, uint64_t i) {
+  VProps p{};
+  if (!c.merged) {
+    const uint32_t m = c.msk[i];
+    p.deg[0] = __popc(m & 0xFFu);
+    p.deg[1] = __popc((m >> 8) & 0xFFu);
+    p.len = (uint64_t)c.k;
+    p.cov = (float)(c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i]);
+    p.nu = c.hcu[i];
+    p.nf = c.hcf[i];
+    const KV v = node_key<W>(c.klo, c.khi, i);
+    for (uint32_t t = 0; t < 4; t++)
+      for (uint32_t x = 0; x < 4; x++)
+        if (((m >> (4 * t + x)) & 1) && keq<W>(neighbour<W>(v, c.k, t, x), v)) p.self[t]++;
+  } else {
+    const uint32_t rep = c.urep_node[i];
+    for (int e = 0; e < c.uecnt[i]; e++) {
+      const uint32_t x = c.uedge[kUEdges * i + e], t = x >> 30;
+      p.deg[(t == ET_FF || t == ET_FR) ? 0 : 1]++;
+      if ((x & 0x3fffffffu) == rep) p.self[t]++;
+    }
+    p.len = c.ulen[i] + (uint64_t)c.k - 1;
+    p.cov = (float)((double)c.ucov[i] / (double)c.ulen[i]);
+    p.nu = c.hu[i];
+    p.nf = c.hf[i];
+  }
+  return p;
+}
+
+// Java Math.round(float) (floor of x + 0.5f, in float) and the seed score
+// (Node.calculateSeedScore, Node.java:859-862: length x read heads).
+__device__ inline int64_t java_round(float x) { return (int64_t)floorf(x + 0.5f); }
+
+enum { ST_DEGREE, ST_KLEN, ST_COV, ST_UNFLIPPED, ST_FLIPPED, ST_SEED, ST_KLEN_F, ST_COV_F, ST_SEED_F, ST_KLEN_R,
+       ST_COV_R, ST_SEED_R, NSTAT };
+enum { TOT_NODES, TOT_PATH, TOT_TIPS_F, TOT_TIPS_R, TOT_TIPS_BOTH, TOT_TIPS_ONE, TOT_SELF_FF, NTOT = TOT_SELF_FF + 4 };
+
+// The value of statistic s for vertex i, or nothing (GraphStatistics
+// counts scaffoldSeedScore only for coverage within mean 0 +- 1000, and the
+// -with-d statistics only where degree(d) != 0).
+template <int W>
+__device__ inline bool stat_value(const VertexCtx& c, const VProps& p, int s, int64_t* out) {
+  const bool win = p.cov >= -1000.f && p.cov <= 1000.f;
+  const int64_t seed = (int64_t)(int32_t)((uint32_t)p.len * (p.nu + p.nf));
+  switch (s) {
+    case ST_DEGREE: *out = p.deg[0] + p.deg[1]; return true;
+    case ST_KLEN: *out = (int64_t)p.len; return true;
+    case ST_COV: *out = java_round(p.cov); return true;
+    case ST_UNFLIPPED: *out = p.nu; return true;
+    case ST_FLIPPED: *out = p.nf; return true;
+    case ST_SEED: *out = seed; return win;
+    case ST_KLEN_F: *out = (int64_t)p.len; return p.deg[0] != 0;
+    case ST_COV_F: *out = java_round(p.cov); return p.deg[0] != 0;
+    case ST_SEED_F: *out = seed; return p.deg[0] != 0 && win;
+    case ST_KLEN_R: *out = (int64_t)p.len; return p.deg[1] != 0;
+    case ST_COV_R: *out = java_round(p.cov); return p.deg[1] != 0;
+    default: *out = seed; return p.deg[1] != 0 && win;
+  }
+}
+
+// Values of statistic s as sortable keys (value + 2^63; excluded vertices
+// ~0, sorted last), with their count, sum and largest value.
+template <int W>
+__global__ void k_stat_values(VertexCtx c, int s, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                              unsigned long long* __restrict__ acc) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const VProps p = vertex_props<W>(c, i);
+    int64_t v;
+    const bool in = stat_value<W>(c, p, s, &v);
+    keys[i] = in ? (uint64_t)v + 0x8000000000000000ull : ~0ull;
+    vals[i] = 0;
+    if (in) {
+      atomicAdd(&acc[0], 1ull);
+      atomicAdd(&acc[1], (unsigned long long)v);
+      atomicMax(&acc[2], (unsigned long long)((uint64_t)v + 0x8000000000000000ull));
+    }
+  }
+}
+
+template <int W>
+__global__ void k_stat_totals(VertexCtx c, unsigned long long* __restrict__ tot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const VProps p = vertex_props<W>(c, i);
+    atomicAdd(&tot[TOT_NODES], 1ull);
+    if (p.deg[0] == 1 && p.deg[1] == 1) atomicAdd(&tot[TOT_PATH], 1ull);
+    if (p.deg[0] == 0) atomicAdd(&tot[TOT_TIPS_F], 1ull);
+    if (p.deg[1] == 0) atomicAdd(&tot[TOT_TIPS_R], 1ull);
+    if (p.deg[0] == 0 && p.deg[1] == 0) atomicAdd(&tot[TOT_TIPS_BOTH], 1ull);
+    if ((p.deg[0] == 0) != (p.deg[1] == 0)) atomicAdd(&tot[TOT_TIPS_ONE], 1ull);
+    for (int t = 0; t < 4; t++)
+      if (p.self[t]) atomicAdd(&tot[TOT_SELF_FF + t], (unsigned long long)p.self[t]);
+  }
+}
+
+// Distinct sorted values and their counts (first `m` keys are the values).
+__global__ void k_bin_starts(uint64_t m, const uint64_t* __restrict__ keys, uint32_t* __restrict__ flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_bin_write(uint64_t m, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+                            const uint64_t* __restrict__ pos, int64_t* __restrict__ bval, uint64_t* __restrict__ bstart) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+    if (flag[i]) {
+      bval[pos[i]] = (int64_t)(keys[i] - 0x8000000000000000ull);
+      bstart[pos[i]] = i;
+    }
+}
+
+// FASTA: ">node_" + key letters + "\n" + sequence letters + "\n" per vertex
+// (the internal k-mer of a merged vertex, else its key).
+constexpr char kAcgt[4] = {'A', 'C', 'G', 'T'};
+
+template <int W>
+__device__ inline uint64_t fasta_seq_len(const VertexCtx& c, uint64_t i) {
+  return c.merged ? c.ulen[i] + (uint64_t)c.k - 1 : (uint64_t)c.k;
+}
+
+template <int W>
+__global__ void k_fasta_size(VertexCtx c, uint64_t* __restrict__ sz) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x)
+    sz[i] = 6 + (uint64_t)c.k + 1 + fasta_seq_len<W>(c, i) + 1;
+}
+
+// Letter j of vertex i's sequence.
+template <int W>
+__device__ inline uint32_t fasta_letter(const VertexCtx& c, uint64_t i, uint64_t j) {
+  if (!c.merged) return oriented_letter<W>(c.klo, c.khi, (uint32_t)i, true, c.k, (int)j);
+  const uint32_t* ord = c.order + c.ubase[i];
+  const uint32_t o = j < (uint64_t)c.k ? ord[0] : ord[j - c.k + 1];
+  const int q = j < (uint64_t)c.k ? (int)j : c.k - 1;
+  return oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, q);
+}
+
+// A wave per vertex; lanes over the letters.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_fasta_write(VertexCtx c, const uint64_t* __restrict__ off,
+                                                       uint8_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t i = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < c.n; i += nw) {
+    uint8_t* p = out + off[i];
+    const uint32_t key = c.merged ? c.urep_node[i] : (uint32_t)i;
+    const uint64_t len = fasta_seq_len<W>(c, i);
+    if (lane < 6) p[lane] = ">node_"[lane];
+    for (int j = lane; j < c.k; j += 64) p[6 + j] = kAcgt[oriented_letter<W>(c.klo, c.khi, key, true, c.k, j)];
+    if (lane == 0) p[6 + c.k] = '\n';
+    uint8_t* q = p + 6 + c.k + 1;
+    for (uint64_t j = lane; j < len; j += 64) q[j] = kAcgt[fasta_letter<W>(c, i, j)];
+    if (lane == 0) q[len] = '\n';
+  }
+}
+
+template <int W>
+static VertexCtx vertex_ctx(gx_build* b, int merged) {
+  VertexCtx c{};
+  c.merged = merged;
+  c.k = b->cfg.k;
+  c.klo = b->klo.as<uint64_t>();
+  c.khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
+  if (!merged) {
+    c.n = b->nnodes;
+    c.cnt = b->cnt.as<uint32_t>();
+    c.msk = b->msk.as<uint32_t>();
+    c.hcu = b->hcu.as<uint32_t>();
+    c.hcf = b->hcf.as<uint32_t>();
+  } else {
+    c.n = b->u_count;
+    c.ulen = b->u_len.as<uint64_t>();
+    c.ucov = b->u_cov.as<unsigned long long>();
+    c.uedge = b->u_edge.as<uint32_t>();
+    c.uecnt = b->u_ecnt.as<uint8_t>();
+    c.urep_node = b->u_repnode.as<uint32_t>();
+    c.hu = b->u_hu.as<uint32_t>();
+    c.hf = b->u_hf.as<uint32_t>();
+    c.ubase = b->u_base.as<uint64_t>();
+    c.order = b->u_order.as<uint32_t>();
+  }
+  return c;
+}
+
+template <int W>
+static void graph_stats(gx_build* b, int merged) {
+  hipStream_t st = b->st;
+  const unsigned G = b->grid;
+  const VertexCtx c = vertex_ctx<W>(b, merged);
+  const uint64_t n = c.n;
+  unsigned long long* acc = buf<unsigned long long>(b->s_acc, 4 + NTOT);
+  GX_HIP(hipMemsetAsync(acc, 0, (4 + NTOT) * 8, st));
+  if (n) {
+    hipLaunchKernelGGL(k_stat_totals<W>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, c, acc + 4);
+    GX_HIP(hipGetLastError());
+  }
+  b->st_tot.assign(NTOT, 0);
+  GX_HIP(hipMemcpyAsync(b->st_tot.data(), acc + 4, NTOT * 8, hipMemcpyDeviceToHost, st));
+  uint64_t* k1 = buf<uint64_t>(b->s_k1, n + 1);
+  uint64_t* k2 = buf<uint64_t>(b->s_k2, n + 1);
+  uint32_t* v1 = buf<uint32_t>(b->s_v1, n + 1);
+  uint32_t* v2 = buf<uint32_t>(b->s_v2, n + 1);
+  uint32_t* flag = buf<uint32_t>(b->s_flag, n + 1);
+  uint64_t* pos = buf<uint64_t>(b->s_pos, n + 1);
+  b->st_bins.assign(NSTAT, {});
+  b->st_sum.assign(NSTAT, 0);
+  b->st_max.assign(NSTAT, 0);
+  for (int s = 0; s < NSTAT; s++) {
+    GX_HIP(hipMemsetAsync(acc, 0, 24, st));
+    if (n) {
+      hipLaunchKernelGGL(k_stat_values<W>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, c, s, k1, v1, acc);
+      GX_HIP(hipGetLastError());
+    }
+    unsigned long long a[3];
+    GX_HIP(hipMemcpyAsync(a, acc, 24, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    const uint64_t m = a[0];
+    b->st_sum[s] = (int64_t)a[1];
+    const int64_t mx = m ? (int64_t)(a[2] - 0x8000000000000000ull) : 0;
+    b->st_max[s] = mx > 0 ? mx : 0;  // the reference's maximum counter starts at 0
+    if (!m) continue;
+    bool f = radix_sort<uint64_t>(k1, v1, k2, v2, n, 0, 8, scratch_for(b, radix_scratch_bytes<uint64_t>(n)), st);
+    uint64_t* ks = f ? k2 : k1;
+    hipLaunchKernelGGL(k_bin_starts, dim3(grid_for(m, kBlock, G)), dim3(kBlock), 0, st, m, ks, flag);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(flag, pos, m, scratch_for(b, scan_scratch_bytes<uint64_t>(m)), st);
+    const uint64_t nbin = read1(st, pos + m);
+    int64_t* bval = buf<int64_t>(b->s_bval, nbin + 1);
+    uint64_t* bst = buf<uint64_t>(b->s_bst, nbin + 1);
+    hipLaunchKernelGGL(k_bin_write, dim3(grid_for(m, kBlock, G)), dim3(kBlock), 0, st, m, ks, flag, pos, bval, bst);
+    GX_HIP(hipGetLastError());
+    std::vector<int64_t> hv(nbin);
+    std::vector<uint64_t> hs(nbin);
+    GX_HIP(hipMemcpyAsync(hv.data(), bval, nbin * 8, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipMemcpyAsync(hs.data(), bst, nbin * 8, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    auto& bins = b->st_bins[s];
+    bins.resize(nbin);
+    for (uint64_t q = 0; q < nbin; q++) bins[q] = {hv[q], (q + 1 < nbin ? hs[q + 1] : m) - hs[q]};
+  }
+  b->st_graph = merged;
+  b->reclaim.dead.insert(b->reclaim.dead.end(),
+                         {&b->s_k1, &b->s_k2, &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst});
+}
+
+template <int W>
+static uint64_t fasta_bytes(gx_build* b, int merged, uint8_t* dst, size_t cap) {
+  hipStream_t st = b->st;
+  const unsigned G = b->grid;
+  const VertexCtx c = vertex_ctx<W>(b, merged);
+  const uint64_t n = c.n;
+  uint64_t* sz = buf<uint64_t>(b->s_k1, n + 1);
+  uint64_t* off = buf<uint64_t>(b->s_k2, n + 1);
+  if (n) {
+    hipLaunchKernelGGL(k_fasta_size<W>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, c, sz);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint64_t, uint64_t>(sz, off, n, scratch_for(b, scan_scratch_bytes<uint64_t>(n)), st);
+  const uint64_t total = read1(st, off + n);
+  if (!dst) return total;
+  if (cap < total) throw GxError(GX_EINVAL, "destination too small");
+  uint8_t* out = buf<uint8_t>(b->s_out, total + 1);
+  if (n) {
+    hipLaunchKernelGGL(k_fasta_write<W>, dim3(grid_for(n, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, c, off, out);
+    GX_HIP(hipGetLastError());
+  }
+  if (total) GX_HIP(hipMemcpyAsync(dst, out, total, hipMemcpyDeviceToHost, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return total;
+}
+
 extern "C" {
 
 // Checks of a configuration (GenomixJobConf.java:417-423 for k); returns a
@@ -4194,6 +4503,68 @@ int gx_clean_tips(gx_build* b, int max_len) {
     if (b->W == 1) clean_graph<1>(b, ca);
     else clean_graph<2>(b, ca);
     b->unitigs_done = true;
+  });
+}
+
+static void check_graph(gx_build* b, int graph) {
+  if (graph != GX_GRAPH_NODES && graph != GX_GRAPH_UNITIGS) throw GxError(GX_EINVAL, "unknown graph");
+  if (!b->finished) throw GxError(GX_ESTATE, "statistics before finish");
+  if (b->nranges > 1) throw GxError(GX_ENOTSUP, "the node graph of a multi-range build is not held in HBM");
+  if (graph == GX_GRAPH_UNITIGS && !b->unitigs_done) throw GxError(GX_ESTATE, "no unitig pass to report on");
+}
+
+int gx_graph_stats(gx_build* b, int graph) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    check_graph(b, graph);
+    GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
+    if (b->W == 1) graph_stats<1>(b, graph);
+    else graph_stats<2>(b, graph);
+  });
+}
+
+int gx_graph_stat_totals(gx_build* b, uint64_t* out, int n) {
+  if (!b || !out) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (b->st_graph < 0) throw GxError(GX_ESTATE, "gx_graph_stats first");
+    for (int i = 0; i < n && i < (int)b->st_tot.size(); i++) out[i] = b->st_tot[i];
+  });
+}
+
+int gx_graph_stat(gx_build* b, int stat, int64_t* total, int64_t* maximum, uint64_t* nbins) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (b->st_graph < 0) throw GxError(GX_ESTATE, "gx_graph_stats first");
+    if (stat < 0 || stat >= NSTAT) throw GxError(GX_EINVAL, "unknown statistic");
+    if (total) *total = b->st_sum[stat];
+    if (maximum) *maximum = b->st_max[stat];
+    if (nbins) *nbins = b->st_bins[stat].size();
+  });
+}
+
+int gx_graph_stat_bins(gx_build* b, int stat, int64_t* values, uint64_t* counts, size_t cap) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (b->st_graph < 0) throw GxError(GX_ESTATE, "gx_graph_stats first");
+    if (stat < 0 || stat >= NSTAT) throw GxError(GX_EINVAL, "unknown statistic");
+    const auto& bins = b->st_bins[stat];
+    if (cap < bins.size()) throw GxError(GX_EINVAL, "destination too small");
+    for (size_t q = 0; q < bins.size(); q++) {
+      if (values) values[q] = bins[q].first;
+      if (counts) counts[q] = bins[q].second;
+    }
+  });
+}
+
+int gx_dump_fasta(gx_build* b, int graph, uint8_t* dst, size_t cap, size_t* written) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    check_graph(b, graph);
+    GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
+    const uint64_t n = b->W == 1 ? fasta_bytes<1>(b, graph, dst, cap) : fasta_bytes<2>(b, graph, dst, cap);
+    if (written) *written = n;
   });
 }
 

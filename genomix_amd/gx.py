@@ -34,7 +34,14 @@ EXPORTED = [
     "gx_build_features", "gx_build_set_tuning", "gx_build_set_config", "gx_batch_create", "gx_batch_set",
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
+    "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
 ]
+GRAPHS = {"nodes": 0, "unitigs": 1}
+STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
+              "kmerLength-with-FORWARD", "coverage-with-FORWARD", "scaffoldSeedScore-with-FORWARD",
+              "kmerLength-with-REVERSE", "coverage-with-REVERSE", "scaffoldSeedScore-with-REVERSE"]
+TOTAL_NAMES = ["nodes", "pathNode", "tips-FORWARD", "tips-REVERSE", "tips-BOTH", "tips-ONE", "selfEdge-FF",
+               "selfEdge-FR", "selfEdge-RF", "selfEdge-RR"]
 
 
 class GxError(RuntimeError):
@@ -103,6 +110,12 @@ def lib():
         L.gx_build_set_stream.argtypes = [vp, ctypes.c_uint64, PART_SINK, vp]
         L.gx_clean_bad_coverage.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.gx_clean_tips.argtypes = [vp, i32]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.gx_graph_stats.argtypes = [vp, i32]
+        L.gx_graph_stat_totals.argtypes = [vp, u64p, i32]
+        L.gx_graph_stat.argtypes = [vp, i32, i64p, i64p, u64p]
+        L.gx_graph_stat_bins.argtypes = [vp, i32, i64p, u64p, sz]
+        L.gx_dump_fasta.argtypes = [vp, i32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -258,6 +271,36 @@ class GraphBuilder:
         k.  Returns the merged records."""
         self._check(lib().gx_clean_tips(self._h, max_len))
         return self.unitig_records()
+
+    def graph_stats(self, graph="unitigs") -> dict:
+        """GraphStatistics counters (include/genomix_stats.h) of the node graph
+        or of the last merge, laid out as the reference's counter groups:
+        {"totals": {...}, "maximum": {...}, "<name>-bins": {value: count}}
+        (counters that stay 0 are left out, as Hadoop never creates them)."""
+        self._check(lib().gx_graph_stats(self._h, GRAPHS[graph]))
+        tot = (ctypes.c_uint64 * len(TOTAL_NAMES))()
+        self._check(lib().gx_graph_stat_totals(self._h, tot, len(TOTAL_NAMES)))
+        out = {"totals": {n: v for n, v in zip(TOTAL_NAMES, tot) if v}, "maximum": {}}
+        for sid, name in enumerate(STAT_NAMES):
+            total, mx, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint64()
+            self._check(lib().gx_graph_stat(self._h, sid, ctypes.byref(total), ctypes.byref(mx), ctypes.byref(nb)))
+            if nb.value == 0:
+                continue
+            vals = (ctypes.c_int64 * nb.value)()
+            cnts = (ctypes.c_uint64 * nb.value)()
+            self._check(lib().gx_graph_stat_bins(self._h, sid, vals, cnts, nb.value))
+            out[name + "-bins"] = {str(v): c for v, c in zip(vals, cnts)}
+            out["totals"][name] = total.value
+            out["maximum"][name] = mx.value
+        return out
+
+    def fasta(self, graph="unitigs") -> bytes:
+        """DriverUtils.dumpGraph of the node graph or of the last merge."""
+        n = ctypes.c_size_t()
+        self._check(lib().gx_dump_fasta(self._h, GRAPHS[graph], None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(n.value, 1))
+        self._check(lib().gx_dump_fasta(self._h, GRAPHS[graph], buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
 
     def stats(self):
         a = (ctypes.c_uint64 * 12)()

@@ -1317,6 +1317,81 @@ static std::vector<uint8_t> clean_coverage_then_tips(const uint8_t* recs, size_t
   return clean_tips(g1.data(), g1.size(), k, maxlen);
 }
 
+// GraphStatistics.map (GraphStatistics.java:79-146) over a record stream:
+// counters as JSON {"totals": {...}, "maximum": {...}, "<name>-bins":
+// {"value": count}}.  updateStats (:131-139): bin count, total, and a
+// maximum counter that starts at 0.
+static std::string graph_stats(const uint8_t* recs, size_t n, int k) {
+  const std::vector<GNode> g = parse_graph(recs, n, k);
+  std::map<std::string, std::map<std::string, long long>> groups;
+  auto update = [&](const std::string& name, long long v) {
+    groups[name + "-bins"][std::to_string(v)] += 1;
+    groups["totals"][name] += v;
+    long long& mx = groups["maximum"][name];
+    if (mx < v) mx = v;
+  };
+  static const char* kEt[4] = {"FF", "FR", "RF", "RR"};
+  for (const GNode& x : g) {
+    const Node& v = x.node;
+    groups["totals"]["nodes"] += 1;
+    const long long out = (long long)(v.edges[FF].size() + v.edges[FR].size());
+    const long long in = (long long)(v.edges[RF].size() + v.edges[RR].size());
+    update("degree", in + out);
+    const int len = (v.has_internal && v.internal.k > 0) ? v.internal.k : x.key.k;
+    update("kmerLength", len);
+    const long long covr = (long long)std::floor(v.cov + 0.5f);  // Math.round(float)
+    update("coverage", covr);
+    update("unflippedReadIds", (long long)v.unflipped.size());
+    update("flippedReadIds", (long long)v.flipped.size());
+    const float cov = v.cov;
+    const bool win = cov >= 0.f - 1000.f && cov <= 0.f + 1000.f;
+    const long long seed = (long long)(int32_t)((uint32_t)len * (uint32_t)(v.unflipped.size() + v.flipped.size()));
+    if (win) update("scaffoldSeedScore", seed);
+    for (int t = 0; t < 4; t++)
+      for (auto& z : v.edges[t])
+        if (z == x.key) groups["totals"][std::string("selfEdge-") + kEt[t]] += 1;
+    if (in == 1 && out == 1) groups["totals"]["pathNode"] += 1;
+    const long long deg[2] = {out, in};
+    static const char* kDir[2] = {"FORWARD", "REVERSE"};
+    for (int d = 0; d < 2; d++) {
+      if (deg[d] == 0) {
+        groups["totals"][std::string("tips-") + kDir[d]] += 1;
+      } else {
+        update(std::string("kmerLength-with-") + kDir[d], len);
+        update(std::string("coverage-with-") + kDir[d], covr);
+        if (win) update(std::string("scaffoldSeedScore-with-") + kDir[d], seed);
+      }
+    }
+    if (in == 0 && out == 0) groups["totals"]["tips-BOTH"] += 1;
+    if ((in == 0 && out != 0) || (in != 0 && out == 0)) groups["totals"]["tips-ONE"] += 1;
+  }
+  std::string js = "{";
+  bool fg = true;
+  for (auto& gp : groups) {
+    js += (fg ? "" : ",") + std::string("\"") + gp.first + "\":{";
+    fg = false;
+    bool fc = true;
+    for (auto& c : gp.second) {
+      js += (fc ? "" : ",") + std::string("\"") + c.first + "\":" + std::to_string(c.second);
+      fc = false;
+    }
+    js += "}";
+  }
+  return js + "}";
+}
+
+// DriverUtils.dumpGraph (DriverUtils.java:109-114): ">node_<key>\n" then the
+// internal k-mer (or the key) and "\n", record by record.
+static std::string dump_fasta(const uint8_t* recs, size_t n, int k) {
+  std::string o;
+  for (const GNode& x : parse_graph(recs, n, k)) {
+    o += ">node_" + x.key.str() + "\n";
+    o += (x.node.has_internal && x.node.internal.k > 0) ? x.node.internal.str() : x.key.str();
+    o += "\n";
+  }
+  return o;
+}
+
 }  // namespace ora
 
 // ---------------------------------------------------------------------------
@@ -1518,6 +1593,22 @@ int ora_clean_coverage_then_tips(const uint8_t* recs, size_t n, int k, float min
                                  uint8_t** out, size_t* outlen) {
   return guard([&] {
     auto r = clean_coverage_then_tips(recs, n, k, minc, maxc, maxlen);
+    *out = dup_bytes(r.data(), r.size());
+    *outlen = r.size();
+  });
+}
+
+int ora_graph_stats(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto r = graph_stats(recs, n, k);
+    *out = dup_bytes(r.data(), r.size());
+    *outlen = r.size();
+  });
+}
+
+int ora_dump_fasta(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto r = dump_fasta(recs, n, k);
     *out = dup_bytes(r.data(), r.size());
     *outlen = r.size();
   });

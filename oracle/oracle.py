@@ -88,6 +88,9 @@ def lib():
         L.ora_unitig_summary.argtypes = [vp, sz, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_unitigs"):
             L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        for name in ("ora_graph_stats", "ora_dump_fasta"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
         if hasattr(L, "ora_clean_coverage_then_tips"):
             L.ora_clean_coverage_then_tips.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, i32,
                                                        ctypes.POINTER(vp), ctypes.POINTER(sz)]
@@ -259,6 +262,22 @@ def clean_coverage_then_tips(records: bytes, k: int, min_cov=3.0, max_cov=float(
     _check(lib().ora_clean_coverage_then_tips(records, len(records), k, ctypes.c_float(min_cov),
                                               ctypes.c_float(max_cov), k if max_len is None else max_len,
                                               ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def graph_stats(records: bytes, k: int) -> dict:
+    """GraphStatistics counters (GraphStatistics.java:79-146) of a record
+    stream: {"totals": {...}, "maximum": {...}, "<name>-bins": {value: n}}."""
+    import json
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_graph_stats(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
+    return json.loads(_take(p, n).decode())
+
+
+def dump_fasta(records: bytes, k: int) -> bytes:
+    """DriverUtils.dumpGraph (DriverUtils.java:109-114) of a record stream."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_dump_fasta(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
 
 
