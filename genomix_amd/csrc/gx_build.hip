@@ -4046,7 +4046,7 @@ struct VProps {
   uint32_t self[4];
 };
 
-template <int W>
+template <int W, bool SELF>
 __device__ inline VProps vertex_props(const VertexCtx& c, uint64_t i) {
   VProps p{};
   if (!c.merged) {
@@ -4057,10 +4057,12 @@ __device__ inline VProps vertex_props(const VertexCtx& c, uint64_t i) {
     p.cov = (float)(c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i]);
     p.nu = c.hcu[i];
     p.nf = c.hcf[i];
-    const KV v = node_key<W>(c.klo, c.khi, i);
-    for (uint32_t t = 0; t < 4; t++)
-      for (uint32_t x = 0; x < 4; x++)
-        if (((m >> (4 * t + x)) & 1) && keq<W>(neighbour<W>(v, c.k, t, x), v)) p.self[t]++;
+    if (SELF) {
+      const KV v = node_key<W>(c.klo, c.khi, i);
+      for (uint32_t t = 0; t < 4; t++)
+        for (uint32_t x = 0; x < 4; x++)
+          if (((m >> (4 * t + x)) & 1) && keq<W>(neighbour<W>(v, c.k, t, x), v)) p.self[t]++;
+    }
   } else {
     const uint32_t rep = c.urep_node[i];
     for (int e = 0; e < c.uecnt[i]; e++) {
@@ -4107,37 +4109,83 @@ __device__ inline bool stat_value(const VertexCtx& c, const VProps& p, int s, in
   }
 }
 
-// Values of statistic s as sortable keys (value + 2^63; excluded vertices
-// ~0, sorted last), with their count, sum and largest value.
+// Values of statistic s (as value + 2^63, ordered like the signed values)
+// and whether each vertex has one; with their count, sum, least and
+// largest value.
+// Wave sums, minima and maxima of 64-bit values (lane 0 holds the result).
+__device__ inline uint64_t wave_sum(uint64_t x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ inline uint64_t wave_min(uint64_t x) {
+  for (int o = 32; o > 0; o >>= 1) x = min(x, (uint64_t)__shfl_xor(x, o, 64));
+  return x;
+}
+__device__ inline uint64_t wave_max(uint64_t x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, (uint64_t)__shfl_xor(x, o, 64));
+  return x;
+}
+
 template <int W>
-__global__ void k_stat_values(VertexCtx c, int s, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+__global__ void k_stat_values(VertexCtx c, int s, uint64_t* __restrict__ keys, uint32_t* __restrict__ has,
                               unsigned long long* __restrict__ acc) {
+  uint64_t cnt = 0, sum = 0, lo = ~0ull, hi = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const VProps p = vertex_props<W>(c, i);
+    const VProps p = vertex_props<W, false>(c, i);
     int64_t v;
     const bool in = stat_value<W>(c, p, s, &v);
-    keys[i] = in ? (uint64_t)v + 0x8000000000000000ull : ~0ull;
-    vals[i] = 0;
+    const uint64_t key = (uint64_t)v + 0x8000000000000000ull;
+    keys[i] = key;
+    has[i] = in ? 1u : 0u;
     if (in) {
-      atomicAdd(&acc[0], 1ull);
-      atomicAdd(&acc[1], (unsigned long long)v);
-      atomicMax(&acc[2], (unsigned long long)((uint64_t)v + 0x8000000000000000ull));
+      cnt++;
+      sum += (uint64_t)v;
+      lo = min(lo, key);
+      hi = max(hi, key);
     }
   }
+  // one set of atomics per wave
+  cnt = wave_sum(cnt);
+  sum = wave_sum(sum);
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0 && cnt) {
+    atomicAdd(&acc[0], (unsigned long long)cnt);
+    atomicAdd(&acc[1], (unsigned long long)sum);
+    atomicMin(&acc[3], (unsigned long long)lo);
+    atomicMax(&acc[2], (unsigned long long)hi);
+  }
+}
+
+// The values present, compacted and offset by the least: sortable on the
+// few low bytes their range needs.
+__global__ void k_stat_compact(uint64_t n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ has,
+                               const uint64_t* __restrict__ pos, uint64_t lo, uint64_t* __restrict__ out,
+                               uint32_t* __restrict__ vals) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (has[i]) {
+      out[pos[i]] = keys[i] - lo;
+      vals[pos[i]] = 0;
+    }
 }
 
 template <int W>
 __global__ void k_stat_totals(VertexCtx c, unsigned long long* __restrict__ tot) {
+  uint64_t t[NTOT] = {};
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const VProps p = vertex_props<W>(c, i);
-    atomicAdd(&tot[TOT_NODES], 1ull);
-    if (p.deg[0] == 1 && p.deg[1] == 1) atomicAdd(&tot[TOT_PATH], 1ull);
-    if (p.deg[0] == 0) atomicAdd(&tot[TOT_TIPS_F], 1ull);
-    if (p.deg[1] == 0) atomicAdd(&tot[TOT_TIPS_R], 1ull);
-    if (p.deg[0] == 0 && p.deg[1] == 0) atomicAdd(&tot[TOT_TIPS_BOTH], 1ull);
-    if ((p.deg[0] == 0) != (p.deg[1] == 0)) atomicAdd(&tot[TOT_TIPS_ONE], 1ull);
-    for (int t = 0; t < 4; t++)
-      if (p.self[t]) atomicAdd(&tot[TOT_SELF_FF + t], (unsigned long long)p.self[t]);
+    const VProps p = vertex_props<W, true>(c, i);
+    t[TOT_NODES]++;
+    t[TOT_PATH] += p.deg[0] == 1 && p.deg[1] == 1;
+    t[TOT_TIPS_F] += p.deg[0] == 0;
+    t[TOT_TIPS_R] += p.deg[1] == 0;
+    t[TOT_TIPS_BOTH] += p.deg[0] == 0 && p.deg[1] == 0;
+    t[TOT_TIPS_ONE] += (p.deg[0] == 0) != (p.deg[1] == 0);
+    for (int e = 0; e < 4; e++) t[TOT_SELF_FF + e] += p.self[e];
+  }
+#pragma unroll
+  for (int q = 0; q < NTOT; q++) {
+    const uint64_t x = wave_sum(t[q]);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(&tot[q], (unsigned long long)x);
   }
 }
 
@@ -4147,11 +4195,11 @@ __global__ void k_bin_starts(uint64_t m, const uint64_t* __restrict__ keys, uint
     flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-__global__ void k_bin_write(uint64_t m, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+__global__ void k_bin_write(uint64_t m, const uint64_t* __restrict__ keys, uint64_t lo, const uint32_t* __restrict__ flag,
                             const uint64_t* __restrict__ pos, int64_t* __restrict__ bval, uint64_t* __restrict__ bstart) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
     if (flag[i]) {
-      bval[pos[i]] = (int64_t)(keys[i] - 0x8000000000000000ull);
+      bval[pos[i]] = (int64_t)(keys[i] + lo - 0x8000000000000000ull);
       bstart[pos[i]] = i;
     }
 }
@@ -4252,28 +4300,39 @@ static void graph_stats(gx_build* b, int merged) {
   b->st_sum.assign(NSTAT, 0);
   b->st_max.assign(NSTAT, 0);
   for (int s = 0; s < NSTAT; s++) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(acc) + 3, ~0ull);
     GX_HIP(hipMemsetAsync(acc, 0, 24, st));
     if (n) {
-      hipLaunchKernelGGL(k_stat_values<W>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, c, s, k1, v1, acc);
+      hipLaunchKernelGGL(k_stat_values<W>, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, c, s, k1, flag, acc);
       GX_HIP(hipGetLastError());
     }
-    unsigned long long a[3];
-    GX_HIP(hipMemcpyAsync(a, acc, 24, hipMemcpyDeviceToHost, st));
+    unsigned long long a[4];
+    GX_HIP(hipMemcpyAsync(a, acc, 32, hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
     const uint64_t m = a[0];
     b->st_sum[s] = (int64_t)a[1];
     const int64_t mx = m ? (int64_t)(a[2] - 0x8000000000000000ull) : 0;
     b->st_max[s] = mx > 0 ? mx : 0;  // the reference's maximum counter starts at 0
     if (!m) continue;
-    bool f = radix_sort<uint64_t>(k1, v1, k2, v2, n, 0, 8, scratch_for(b, radix_scratch_bytes<uint64_t>(n)), st);
-    uint64_t* ks = f ? k2 : k1;
+    const uint64_t lo = a[3], range = a[2] - a[3];
+    int nbytes = 0;
+    while (nbytes < 8 && (range >> (8 * nbytes)) != 0) nbytes++;
+    exclusive_scan<uint32_t, uint64_t>(flag, pos, n, scratch_for(b, scan_scratch_bytes<uint64_t>(n)), st);
+    hipLaunchKernelGGL(k_stat_compact, dim3(grid_for(n, kBlock, G)), dim3(kBlock), 0, st, n, k1, flag, pos, lo, k2,
+                       v2);
+    GX_HIP(hipGetLastError());
+    bool f = nbytes ? radix_sort<uint64_t>(k2, v2, k1, v1, m, 0, nbytes,
+                                           scratch_for(b, radix_scratch_bytes<uint64_t>(m)), st)
+                    : false;
+    uint64_t* ks = f ? k1 : k2;
     hipLaunchKernelGGL(k_bin_starts, dim3(grid_for(m, kBlock, G)), dim3(kBlock), 0, st, m, ks, flag);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(flag, pos, m, scratch_for(b, scan_scratch_bytes<uint64_t>(m)), st);
     const uint64_t nbin = read1(st, pos + m);
     int64_t* bval = buf<int64_t>(b->s_bval, nbin + 1);
     uint64_t* bst = buf<uint64_t>(b->s_bst, nbin + 1);
-    hipLaunchKernelGGL(k_bin_write, dim3(grid_for(m, kBlock, G)), dim3(kBlock), 0, st, m, ks, flag, pos, bval, bst);
+    hipLaunchKernelGGL(k_bin_write, dim3(grid_for(m, kBlock, G)), dim3(kBlock), 0, st, m, ks, lo, flag, pos, bval,
+                       bst);
     GX_HIP(hipGetLastError());
     std::vector<int64_t> hv(nbin);
     std::vector<uint64_t> hs(nbin);
