@@ -1589,8 +1589,11 @@ __global__ void k_replay_long(uint64_t nlong, const uint32_t* __restrict__ rlist
 
 __global__ void k_count_long(uint64_t n, const uint32_t* __restrict__ rkey, uint32_t lim,
                              unsigned long long* __restrict__ cnt) {
+  uint64_t c = 0;
   for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (uint64_t)gridDim.x * blockDim.x)
-    if (rkey[q] <= lim) atomicAdd(cnt, 1ull);
+    c += rkey[q] <= lim;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);  // one add per wave
 }
 
 // One thread per node of the replay list.  Only lists that end with >= 2
@@ -2177,17 +2180,30 @@ __global__ void k_pj_node_final(uint64_t nn, const uint4* __restrict__ R, uint32
   }
 }
 
+// Coverage (saturated at 2^24, as the reference's float sums) of the nodes
+// in unitig order; a prefix sum of it gives each unitig's total.
+__global__ void k_order_cov(uint64_t nn, const uint32_t* __restrict__ order, const uint32_t* __restrict__ cnt,
+                            uint32_t* __restrict__ c) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nn; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = cnt[order[j] & 0x7fffffffu];
+    c[j] = x > (1u << 24) ? (1u << 24) : x;
+  }
+}
+
+__global__ void k_unitig_cov(uint64_t nu, const uint64_t* __restrict__ ubase, const uint64_t* __restrict__ ulen,
+                             const uint64_t* __restrict__ cpre, unsigned long long* __restrict__ ucov) {
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x)
+    ucov[u] = cpre[ubase[u] + ulen[u]] - cpre[ubase[u]];
+}
+
 // Unitig u = rank of its representative; nodes placed in unitig order.
 __global__ void k_place(uint64_t nn, const uint32_t* __restrict__ urep, const uint32_t* __restrict__ upos,
                         const uint64_t* __restrict__ uid_of_rep, const uint64_t* __restrict__ ubase,
-                        const uint32_t* __restrict__ cnt, uint32_t* __restrict__ order,
-                        unsigned long long* __restrict__ ucov) {
+                        uint32_t* __restrict__ order) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t r = urep[i] & 0x7fffffffu;
     uint64_t u = uid_of_rep[r];
     order[ubase[u] + upos[i]] = (uint32_t)i | (urep[i] & 0x80000000u);
-    uint32_t c = cnt[i] > (1u << 24) ? (1u << 24) : cnt[i];
-    atomicAdd(&ucov[u], (unsigned long long)c);
   }
 }
 
@@ -3666,10 +3682,16 @@ static void run_unitigs(gx_build* b) {
   exclusive_scan<uint64_t, uint64_t>(ulen, ubase, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
   uint32_t* order = buf<uint32_t>(b->u_order, nn + 1);
   unsigned long long* ucov = buf<unsigned long long>(b->u_cov, nu + 1);
-  GX_HIP(hipMemsetAsync(ucov, 0, (nu + 1) * 8, st));
   if (nn) {
     hipLaunchKernelGGL(k_place, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, urep, upos, uid_of_rep, ubase,
-                       cnt, order, ucov);
+                       order);
+    GX_HIP(hipGetLastError());
+    uint32_t* ocov = buf<uint32_t>(b->u_isrep, nn + 1);  // isrep is read no more
+    uint64_t* cpre = buf<uint64_t>(b->u_lenrep, nn + 1);  // nor ulen_of_rep
+    hipLaunchKernelGGL(k_order_cov, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, order, cnt, ocov);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(ocov, cpre, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+    hipLaunchKernelGGL(k_unitig_cov, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, cpre, ucov);
     GX_HIP(hipGetLastError());
   }
   uint32_t* uedge = buf<uint32_t>(b->u_edge, kUEdges * (nu + 1));
