@@ -336,6 +336,13 @@ def main():
                      "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
                      "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
     unitigs = builder.unitig_count()
+    # the driver's coverage cutoff over the build's node graph: statistics,
+    # then the mixture fit (timed on the second call)
+    for rep in range(2):
+        devmem.synchronize()
+        t1 = time.perf_counter()
+        mix = builder.coverage_cutoff("nodes")
+        t_cut = time.perf_counter() - t1
     # graph cleaning after the step, as the reference pipeline runs it:
     # REMOVE_BAD_COVERAGE (min 3.0) then MERGE, on the graph in HBM (timed
     # on its second call, its buffers allocated)
@@ -360,7 +367,9 @@ def main():
         t_sn = time.perf_counter() - t1
     stats = {"unitig_stats_ms": round(1e3 * t_su, 3), "unitig_fasta_ms": round(1e3 * t_fa, 3),
              "fasta_bytes": len(fa), "node_stats_ms": round(1e3 * t_sn, 3),
-             "node_vertices": nst["totals"]["nodes"], "unitig_vertices": ust["totals"]["nodes"]}
+             "node_vertices": nst["totals"]["nodes"], "unitig_vertices": ust["totals"]["nodes"],
+             "coverage_cutoff_ms": round(1e3 * t_cut, 3), "coverage_cutoff": mix["cutoff"],
+             "coverage_normal_mean": round(mix["normal_mean"], 3)}
     clean = {"ms": round(1e3 * t_clean, 3), "min_coverage": 3.0,
              "unitigs_before": unitigs, "unitigs_after": builder.unitig_count(),
              "unitig_pass_ms": round(builder.phase_times().get("unitig", 0.0), 3)}

@@ -4587,6 +4587,76 @@ int gx_clean_tips(gx_build* b, int max_len) {
   });
 }
 
+// Densities of commons-math3 3.2 (the version genomix-driver/pom.xml:367-368
+// pins): ExponentialDistribution.density and NormalDistribution.density.
+static double mix_exp_density(double mean, double x) { return x < 0 ? 0.0 : std::exp(-x / mean) / mean; }
+static double mix_norm_density(double mu, double sd, double x) {
+  const double z = (x - mu) / sd;
+  return std::exp(-0.5 * z * z) / (sd * 2.5066282746310002);  // sd * sqrt(2 pi)
+}
+
+// FittingMixture.fittingMixture (FittingMixture.java:89-219) over binned data:
+// value x[i] stands for w[i] equal data points, so every sum over the data is
+// a w-weighted sum over the bins.  EM of an exponential plus a normal
+// component from mean 5, normal 20 / 5 and priors 1/2; an E step, then
+// `iters` rounds of M step (means, sd floored at 1e-9f) and E step, each E
+// step ending with the priors' update.  out: cutoff, exponential mean,
+// normal mean, normal sd, exponential prior, normal prior.
+static void fit_mixture(const double* x, const uint64_t* w, size_t n, double max, int iters, double* out) {
+  double total = 0;
+  for (size_t i = 0; i < n; i++) total += (double)w[i];
+  if (n == 0 || total == 0 || max == 0) throw GxError(GX_EINVAL, "No information for coverage!");
+  if (iters < 0) throw GxError(GX_EINVAL, "negative iteration count");
+  std::vector<double> me(n), mn(n);
+  double emean = 5, nmean = 20, nsd = 5, pe = 0.5, pn = 0.5, se = 0, sn = 0;
+  auto estep = [&] {
+    se = sn = 0;
+    for (size_t i = 0; i < n; i++) {
+      double a = mix_exp_density(emean, x[i]) * pe;
+      if (a == 0) a = 0.000000001;
+      double c = mix_norm_density(nmean, nsd, x[i]) * pn;
+      if (c == 0) c = 0.000000001;
+      const double s = a + c;
+      me[i] = a / s;
+      mn[i] = c / s;
+      se += (double)w[i] * me[i];
+      sn += (double)w[i] * mn[i];
+    }
+    pe = se / (se + sn);
+    pn = sn / (se + sn);
+  };
+  estep();
+  for (int it = 0; it < iters; it++) {
+    double a = 0, c = 0;
+    for (size_t i = 0; i < n; i++) {
+      a += (double)w[i] * me[i] * x[i];
+      c += (double)w[i] * mn[i] * x[i];
+    }
+    emean = a / se;
+    // ExponentialDistribution's constructor rejects a mean <= 0
+    // (NotStrictlyPositiveException).
+    if (emean <= 0) throw GxError(GX_EINVAL, "exponential mean must be positive");
+    nmean = c / sn;
+    double v = 0;
+    for (size_t i = 0; i < n; i++) v += (double)w[i] * mn[i] * (x[i] - nmean) * (x[i] - nmean);
+    nsd = std::sqrt(v / sn);
+    if (nsd == 0) nsd = (double)0.000000001f;
+    estep();
+  }
+  double cut = 0;
+  for (double cov = 1; cov < max; cov++)
+    if (pe * mix_exp_density(emean, cov) < pn * mix_norm_density(nmean, nsd, cov)) {
+      cut = cov;
+      break;
+    }
+  out[0] = cut;
+  out[1] = emean;
+  out[2] = nmean;
+  out[3] = nsd;
+  out[4] = pe;
+  out[5] = pn;
+}
+
 static void check_graph(gx_build* b, int graph) {
   if (graph != GX_GRAPH_NODES && graph != GX_GRAPH_UNITIGS) throw GxError(GX_EINVAL, "unknown graph");
   if (!b->finished) throw GxError(GX_ESTATE, "statistics before finish");
@@ -4636,6 +4706,44 @@ int gx_graph_stat_bins(gx_build* b, int stat, int64_t* values, uint64_t* counts,
       if (counts) counts[q] = bins[q].second;
     }
   });
+}
+
+int gx_coverage_cutoff(gx_build* b, int iterations, double* out) {
+  if (!b || !out) return GX_EINVAL;
+  return guarded(b, [&] {
+    if (b->st_graph < 0) throw GxError(GX_ESTATE, "gx_graph_stats first");
+    const auto& bins = b->st_bins[ST_COV];
+    // GraphStatistics.getCoverageStats / getMaxCoverage
+    // (GraphStatistics.java:165-199): the counters' values and the largest.
+    std::vector<double> x(bins.size());
+    std::vector<uint64_t> w(bins.size());
+    double mx = 0;
+    for (size_t q = 0; q < bins.size(); q++) {
+      x[q] = (double)bins[q].first;
+      w[q] = bins[q].second;
+      mx = std::max(mx, x[q]);
+    }
+    if (bins.empty() || mx == 0) throw GxError(GX_ESTATE, "No information for coverage!");
+    fit_mixture(x.data(), w.data(), x.size(), mx, iterations, out);
+  });
+}
+
+int gx_fit_mixture(const double* values, const uint64_t* counts, size_t n, double max, int iterations, double* out) {
+  g_create_err.clear();
+  if ((n && (!values || !counts)) || !out) {
+    g_create_err = "null argument";
+    return GX_EINVAL;
+  }
+  try {
+    fit_mixture(values, counts, n, max, iterations, out);
+    return GX_OK;
+  } catch (const GxError& e) {
+    g_create_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_create_err = "host allocation failed";
+    return GX_ENOMEM;
+  }
 }
 
 int gx_dump_fasta(gx_build* b, int graph, uint8_t* dst, size_t cap, size_t* written) {

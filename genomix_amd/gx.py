@@ -35,11 +35,13 @@ EXPORTED = [
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
+    "gx_coverage_cutoff", "gx_fit_mixture",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
               "kmerLength-with-FORWARD", "coverage-with-FORWARD", "scaffoldSeedScore-with-FORWARD",
               "kmerLength-with-REVERSE", "coverage-with-REVERSE", "scaffoldSeedScore-with-REVERSE"]
+MIXTURE_KEYS = ["cutoff", "exp_mean", "normal_mean", "normal_std", "prob_exp", "prob_normal"]
 TOTAL_NAMES = ["nodes", "pathNode", "tips-FORWARD", "tips-REVERSE", "tips-BOTH", "tips-ONE", "selfEdge-FF",
                "selfEdge-FR", "selfEdge-RF", "selfEdge-RR"]
 
@@ -116,6 +118,9 @@ def lib():
         L.gx_graph_stat.argtypes = [vp, i32, i64p, i64p, u64p]
         L.gx_graph_stat_bins.argtypes = [vp, i32, i64p, u64p, sz]
         L.gx_dump_fasta.argtypes = [vp, i32, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.gx_coverage_cutoff.argtypes = [vp, i32, dp]
+        L.gx_fit_mixture.argtypes = [dp, u64p, sz, ctypes.c_double, i32, dp]
         L.gx_batch_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
@@ -294,6 +299,17 @@ class GraphBuilder:
             out["maximum"][name] = mx.value
         return out
 
+    def coverage_cutoff(self, graph="nodes", iterations=10) -> dict:
+        """The driver's coverage cutoff (include/genomix_stats.h,
+        GenomixDriver.java:119-136): statistics of `graph` (the driver uses
+        the build's node graph), then the exponential + normal mixture fit of
+        their coverage counters.  Returns MIXTURE_KEYS -> value; a positive
+        "cutoff" is the min_cov the driver hands to REMOVE_BAD_COVERAGE."""
+        self._check(lib().gx_graph_stats(self._h, GRAPHS[graph]))
+        a = (ctypes.c_double * len(MIXTURE_KEYS))()
+        self._check(lib().gx_coverage_cutoff(self._h, iterations, a))
+        return dict(zip(MIXTURE_KEYS, list(a)))
+
     def fasta(self, graph="unitigs") -> bytes:
         """DriverUtils.dumpGraph of the node graph or of the last merge."""
         n = ctypes.c_size_t()
@@ -356,6 +372,23 @@ def build_graph(text: bytes, k: int, nparts: int = 1, edge_order: str = "jdk7", 
         b.add_reads(text)
         b.finish()
         return [b.part_bytes(p) for p in range(nparts)]
+
+
+def fit_mixture(values, counts, max_value=None, iterations=10) -> dict:
+    """FittingMixture.fittingMixture over binned data (gx_fit_mixture): value
+    values[i] occurring counts[i] times; max_value defaults to the largest
+    value.  Returns MIXTURE_KEYS -> value."""
+    n = len(values)
+    if len(counts) != n:
+        raise ValueError("values and counts differ in length")
+    v = (ctypes.c_double * max(n, 1))(*[float(x) for x in values])
+    c = (ctypes.c_uint64 * max(n, 1))(*[int(x) for x in counts])
+    mx = max((float(x) for x in values), default=0.0) if max_value is None else float(max_value)
+    a = (ctypes.c_double * len(MIXTURE_KEYS))()
+    rc = lib().gx_fit_mixture(v, c, n, mx, iterations, a)
+    if rc:
+        raise GxError(rc, lib().gx_last_error(None).decode())
+    return dict(zip(MIXTURE_KEYS, list(a)))
 
 
 STAT_KEYS = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",

@@ -1392,6 +1392,82 @@ static std::string dump_fasta(const uint8_t* recs, size_t n, int k) {
   return o;
 }
 
+// FittingMixture.fittingMixture (FittingMixture.java:89-219), one data point
+// at a time as the reference walks its expanded double[] (one entry per
+// counted node, GraphStatistics.getCoverageStats, GraphStatistics.java:182-199).
+// commons-math3 3.2 densities: ExponentialDistribution.density
+// exp(-x/mean)/mean for x >= 0, else 0; NormalDistribution.density
+// exp(-((x-mu)/sd)^2/2) / (sd sqrt(2 pi)).  out: cutoff, exponential mean,
+// normal mean, normal sd, exponential prior, normal prior.
+struct MixState {
+  double exp_mean, normal_mean, normal_std;
+};
+
+static double ora_exp_pdf(double mean, double x) { return x < 0 ? 0.0 : std::exp(-x / mean) / mean; }
+static double ora_normal_pdf(double mean, double sd, double x) {
+  const double x1 = (x - mean) / sd;
+  return std::exp(-0.5 * x1 * x1) / (sd * std::sqrt(2 * M_PI));
+}
+
+static void fitting_mixture(const double* data, size_t n, double max, int iterations, double* out) {
+  MixState cur{5, 20, 5};  // :94-98
+  double prob_exp = 0.5, prob_normal = 0.5;  // :102-103
+  std::vector<double> m_exp(n), m_normal(n);
+  double exp_sum = 0, normal_sum = 0;
+  // :112-135 and :162-185: memberships with 1e-9 floors, normalised, then
+  // the class priors from their sums.
+  auto expectation = [&] {
+    for (size_t i = 0; i < n; i++) {
+      m_exp[i] = ora_exp_pdf(cur.exp_mean, data[i]) * prob_exp;
+      if (m_exp[i] == 0) m_exp[i] = 0.000000001;
+      m_normal[i] = ora_normal_pdf(cur.normal_mean, cur.normal_std, data[i]) * prob_normal;
+      if (m_normal[i] == 0) m_normal[i] = 0.000000001;
+      const double sum = m_exp[i] + m_normal[i];
+      m_exp[i] = m_exp[i] / sum;
+      m_normal[i] = m_normal[i] / sum;
+    }
+    exp_sum = 0;
+    normal_sum = 0;
+    for (size_t i = 0; i < n; i++) {
+      exp_sum += m_exp[i];
+      normal_sum += m_normal[i];
+    }
+    const double all = exp_sum + normal_sum;
+    prob_exp = exp_sum / all;
+    prob_normal = normal_sum / all;
+  };
+  expectation();
+  for (int it = 0; it < iterations; it++) {  // :139-189
+    cur.exp_mean = 0;
+    cur.normal_mean = 0;
+    cur.normal_std = 0;
+    for (size_t i = 0; i < n; i++) {
+      cur.exp_mean += m_exp[i] * data[i];
+      cur.normal_mean += m_normal[i] * data[i];
+    }
+    cur.exp_mean = cur.exp_mean / exp_sum;
+    if (cur.exp_mean <= 0) throw OraError(E_INVAL, "exponential mean must be positive");
+    cur.normal_mean = cur.normal_mean / normal_sum;
+    for (size_t i = 0; i < n; i++) cur.normal_std += m_normal[i] * std::pow(data[i] - cur.normal_mean, 2);
+    cur.normal_std = std::sqrt(cur.normal_std / normal_sum);
+    if (cur.normal_std == 0) cur.normal_std = 0.000000001f;
+    expectation();
+  }
+  double cutoff = 0;  // :210-218
+  for (double cov = 1; cov < max; cov++) {
+    if (prob_exp * ora_exp_pdf(cur.exp_mean, cov) < prob_normal * ora_normal_pdf(cur.normal_mean, cur.normal_std, cov)) {
+      cutoff = cov;
+      break;
+    }
+  }
+  out[0] = cutoff;
+  out[1] = cur.exp_mean;
+  out[2] = cur.normal_mean;
+  out[3] = cur.normal_std;
+  out[4] = prob_exp;
+  out[5] = prob_normal;
+}
+
 }  // namespace ora
 
 // ---------------------------------------------------------------------------
@@ -1619,6 +1695,14 @@ int ora_unitigs(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* out
     auto o = unitigs(recs, n, k);
     *outlen = o.size();
     *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_fitting_mixture(const double* data, size_t n, double max, int iterations, double* out) {
+  return guard([&] {
+    // GenomixDriver.java:126-127
+    if (max == 0 || n == 0) throw OraError(E_INVAL, "No information for coverage!");
+    fitting_mixture(data, n, max, iterations, out);
   });
 }
 

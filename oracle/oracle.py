@@ -35,6 +35,17 @@ cannot be run; nothing was denied, it is simply unbuildable):
   the RandMergeOffsetTest invariant (one genome -> one unitig equal to it or
   its reverse complement, read-head offsets at read positions), and brute
   force on small graphs.
+* Graph cleaning (REMOVE_BAD_COVERAGE, TIP_SINGLE_NODE): identity bounds,
+  bound invariants and closed forms (tests/test_oracle_clean.py); parity
+  unpinned against the reference's one cleaning fixture (retired text
+  format, thresholds not in the tree).
+* GraphStatistics counters and the FASTA dump: hand-derived counters of a
+  single read's k-mer path and of a genome merged into one unitig
+  (tests/test_oracle_stats.py).
+* FittingMixture: scipy.stats densities for the initial E step, convergence
+  to generating parameters, stationarity of the mixture likelihood at the
+  fixed point, the cutoff rule, and the closed form of single-valued data
+  (tests/test_oracle_mixture.py).
 """
 
 import ctypes
@@ -94,6 +105,8 @@ def lib():
         if hasattr(L, "ora_clean_coverage_then_tips"):
             L.ora_clean_coverage_then_tips.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, i32,
                                                        ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        if hasattr(L, "ora_fitting_mixture"):
+            L.ora_fitting_mixture.argtypes = [vp, sz, ctypes.c_double, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_clean_tips"):
             L.ora_clean_tips.argtypes = [vp, sz, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
         if hasattr(L, "ora_clean_bad_coverage"):
@@ -279,6 +292,20 @@ def dump_fasta(records: bytes, k: int) -> bytes:
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_dump_fasta(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
+
+
+def fitting_mixture(data, max_value, iterations=10) -> dict:
+    """FittingMixture.fittingMixture (FittingMixture.java:89-219) over the
+    expanded coverage data (one value per node).  Returns the cutoff and the
+    fitted exponential mean, normal mean / sd and priors."""
+    import numpy as np
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    a = (ctypes.c_double * 6)()
+    _check(lib().ora_fitting_mixture(d.ctypes.data, d.size, float(max_value), iterations, a))
+    return dict(zip(MIXTURE_KEYS, list(a)))
+
+
+MIXTURE_KEYS = ["cutoff", "exp_mean", "normal_mean", "normal_std", "prob_exp", "prob_normal"]
 
 
 def unitig_summary(records: bytes, k: int):
