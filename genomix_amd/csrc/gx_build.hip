@@ -2291,13 +2291,16 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
   }
 }
 
-// Read heads of every node carried into its unitig: offset = k-mer position,
-// + k-1 for flipped heads; a reversed node swaps the two sets
-// (Node.mergeUnflippedAndFlippedReadIDs, Node.java:740-806).  Sort key:
+// Read heads of every node carried into its unitig: offset = k-mer position
+// + the head's letter in the k-mer (the build's heads: 0, or k - 1 for
+// flipped ones); a reversed node counts the letter from the other end and
+// swaps the two sets (Node.mergeUnflippedAndFlippedReadIDs,
+// Node.java:740-806).  Sort key:
 // hi = unitig, lo = flipped << 53 | offset << 30 | mate << 29 | readId.
 __global__ void k_unitig_heads(uint64_t nn, const uint64_t* __restrict__ hstart, const uint32_t* __restrict__ hcu,
                                const uint32_t* __restrict__ hcf, const uint32_t* __restrict__ uval,
-                               const uint32_t* __restrict__ urep, const uint32_t* __restrict__ upos,
+                               const uint8_t* __restrict__ hoff, const uint32_t* __restrict__ urep,
+                               const uint32_t* __restrict__ upos,
                                const uint64_t* __restrict__ uid_of_rep, const int64_t* __restrict__ rid, int k,
                                Key128* __restrict__ key, uint32_t* __restrict__ val) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -2310,7 +2313,9 @@ __global__ void k_unitig_heads(uint64_t nn, const uint64_t* __restrict__ hstart,
       uint32_t m = uval[h0 + j] & 0x7fffffffu;
       bool f = j >= hcu[i];
       bool nf = fwd ? f : !f;
-      uint64_t off = upos[i] + (nf ? (uint64_t)(k - 1) : 0);
+      // the head's letter in the node's k-mer: 0 / k - 1 from the build
+      const uint32_t a = hoff ? hoff[h0 + j] : (f ? (uint32_t)(k - 1) : 0u);
+      uint64_t off = upos[i] + (fwd ? a : (uint32_t)(k - 1) - a);
       Key128 kk;
       kk.hi = u;
       kk.lo = ((uint64_t)nf << 53) | (off << 30) | ((uint64_t)(m & 1) << 29) | (uint64_t)rid[m >> 1];
@@ -2672,6 +2677,11 @@ struct gx_build {
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   DevBuf hist2;               // key ranges: the next range's per-tile digit counts
   DevBuf c_keep, c_pos, c_klo, c_khi, c_cnt, c_msk, c_hcu, c_hcf, c_htot, c_hstart, c_uval;  // graph cleaning
+  DevBuf hoff, c_hoff;  // per node head: its letter in the node's k-mer (after a bubble merge)
+  DevBuf bb_min, bb_maj, bb_typ, bb_np, bb_pbase, bb_pairs, bb_ed, bb_into, bb_hn, bb_hb, bb_addu, bb_addf, bb_cur,
+      bb_tot, bb_astart, bb_hnode, bb_hmv, bb_hofs, bb_aval, bb_aoff;  // bubble merge
+  uint64_t bub_pairs = 0;
+  bool has_hoff = false;
   uint64_t range_occ = 0;     // occurrences per key-range pass (0: from the HBM size)
   gx_part_sink sink = nullptr;
   void* sink_user = nullptr;
@@ -2888,7 +2898,10 @@ static size_t handle_bytes(const gx_build* b) {
         &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
         &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
         &b->hregs, &b->sfeat, &b->spred, &b->c_keep, &b->c_pos, &b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk,
-        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
+        &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->hoff, &b->c_hoff, &b->bb_min, &b->bb_maj,
+        &b->bb_typ, &b->bb_np, &b->bb_pbase, &b->bb_pairs, &b->bb_ed, &b->bb_into, &b->bb_hn, &b->bb_hb,
+        &b->bb_addu, &b->bb_addf, &b->bb_cur, &b->bb_tot, &b->bb_astart, &b->bb_hnode, &b->bb_hmv, &b->bb_hofs,
+        &b->bb_aval, &b->bb_aoff, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
         &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out})
     s += d->cap;
   return s;
@@ -3065,6 +3078,7 @@ static void run_build(gx_build* b) {
   std::vector<uint64_t> seg_src, seg_len;  // several ranges, no sink: range r's part p bytes in outacc
   std::vector<uint64_t> prec(P + 1, 0), pbyt(P + 1, 0);  // per partition: records and bytes so far
   b->nuheads = 0;
+  b->has_hoff = false;
   b->stats[9] = 0;
   // record-sort bucket bytes of a pass of n records (LDS-sized buckets)
   auto bkb_of = [&](uint64_t n) { return nb < 2 ? 1 : (nb >= 4 && n > (1ull << 28)) ? 3 : 2; };
@@ -3710,7 +3724,8 @@ static void run_unitigs(gx_build* b) {
   if (nn && nh) {
     hipLaunchKernelGGL(k_unitig_heads, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
                        b->hstart.as<uint64_t>(), b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), b->uval.as<uint32_t>(),
-                       urep, upos, uid_of_rep, b->rid.as<int64_t>(), k, hk, hv);
+                       b->has_hoff ? b->hoff.as<uint8_t>() : nullptr, urep, upos, uid_of_rep, b->rid.as<int64_t>(), k,
+                       hk, hv);
     GX_HIP(hipGetLastError());
   }
   bool fl = radix_sort<Key128>(hk, hv, hk2, hv2, nh, 0, 12, scratch_for(b, radix_scratch_bytes<Key128>(nh)), st);
@@ -3928,13 +3943,23 @@ __global__ void k_unitig_keep(uint64_t nn, const uint32_t* __restrict__ urep, co
   }
 }
 
+// Read heads added to nodes by a bubble merge, grouped per node (unflipped
+// ones first): per old node the counts and the first slot, per slot the
+// mate index | flipped << 31 and the head's letter in the node's k-mer.
+struct AddedHeads {
+  const uint32_t *addu = nullptr, *addf = nullptr;
+  const uint64_t* astart = nullptr;
+  const uint32_t* aval = nullptr;
+  const uint8_t* aoff = nullptr;
+};
+
 // Surviving nodes, compacted in key order, with the edge bits towards
-// removed nodes cleared.
+// removed nodes cleared; added heads counted in.
 template <int W>
 __global__ void k_keep_nodes(uint64_t nn, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos,
                              const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
                              const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ msk, NodeIndex ix, int k,
-                             const uint32_t* __restrict__ hcu, const uint32_t* __restrict__ hcf,
+                             const uint32_t* __restrict__ hcu, const uint32_t* __restrict__ hcf, AddedHeads ah,
                              uint64_t* __restrict__ klo2, uint64_t* __restrict__ khi2, uint32_t* __restrict__ cnt2,
                              uint32_t* __restrict__ msk2, uint32_t* __restrict__ hcu2, uint32_t* __restrict__ hcf2,
                              uint64_t* __restrict__ htot2, unsigned long long* err) {
@@ -3954,20 +3979,40 @@ __global__ void k_keep_nodes(uint64_t nn, const uint32_t* __restrict__ keep, con
     if (W == 2) khi2[j] = khi[i];
     cnt2[j] = cnt[i];
     msk2[j] = m;
-    hcu2[j] = hcu[i];
-    hcf2[j] = hcf[i];
-    htot2[j] = (uint64_t)hcu[i] + hcf[i];
+    const uint32_t nu = hcu[i] + (ah.addu ? ah.addu[i] : 0u), nf = hcf[i] + (ah.addf ? ah.addf[i] : 0u);
+    hcu2[j] = nu;
+    hcf2[j] = nf;
+    htot2[j] = (uint64_t)nu + nf;
   }
 }
 
+// Heads of surviving nodes: unflipped (kept, then added), flipped (kept,
+// then added); with hoff2, each head's letter in its node's k-mer (hoff, or
+// the build's 0 / k - 1 where the graph has none yet).
 __global__ void k_keep_heads(uint64_t nn, const uint32_t* __restrict__ keep, const uint64_t* __restrict__ pos,
                              const uint64_t* __restrict__ hstart, const uint32_t* __restrict__ hcu,
                              const uint32_t* __restrict__ hcf, const uint32_t* __restrict__ uval,
-                             const uint64_t* __restrict__ hstart2, uint32_t* __restrict__ uval2) {
+                             const uint8_t* __restrict__ hoff, AddedHeads ah, int k,
+                             const uint64_t* __restrict__ hstart2, uint32_t* __restrict__ uval2,
+                             uint8_t* __restrict__ hoff2) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     if (!keep[i]) continue;
-    const uint64_t a = hstart[i], o = hstart2[pos[i]], n = (uint64_t)hcu[i] + hcf[i];
-    for (uint64_t q = 0; q < n; q++) uval2[o + q] = uval[a + q];
+    const uint64_t a = hstart[i];
+    uint64_t o = hstart2[pos[i]];
+    const uint32_t au = ah.addu ? ah.addu[i] : 0u, af = ah.addf ? ah.addf[i] : 0u;
+    const uint64_t a2 = ah.astart ? ah.astart[i] : 0;
+    for (int f = 0; f < 2; f++) {
+      const uint64_t s = a + (f ? hcu[i] : 0), n = f ? hcf[i] : hcu[i];
+      for (uint64_t q = 0; q < n; q++, o++) {
+        uval2[o] = uval[s + q];
+        if (hoff2) hoff2[o] = hoff ? hoff[s + q] : (uint8_t)(f ? k - 1 : 0);
+      }
+      const uint64_t s2 = a2 + (f ? au : 0), n2 = f ? af : au;
+      for (uint64_t q = 0; q < n2; q++, o++) {
+        uval2[o] = ah.aval[s2 + q];
+        hoff2[o] = ah.aoff[s2 + q];
+      }
+    }
   }
 }
 
@@ -3976,22 +4021,18 @@ static void swap_buf(DevBuf& x, DevBuf& y) {
   std::swap(x.cap, y.cap);
 }
 
+// The nodes with keep[i] set, with their heads and those of ah, become the
+// handle's node graph (the build's own records are unchanged); then the
+// node index and the unitig pass over it.
 template <int W>
-static void clean_graph(gx_build* b, const CleanArgs& ca) {
+static void compact_graph(gx_build* b, uint32_t* keep, const AddedHeads& ah) {
   hipStream_t st = b->st;
   const uint64_t nn = b->nnodes;
   const unsigned G = b->grid;
   const NodeIndex nix{b->nix_dir.as<uint32_t>(), b->nix_slots.p, b->nb, 1u << (8 * b->nix_bkb), b->nix_bkb};
   unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
   GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
-  uint32_t* keep = buf<uint32_t>(b->c_keep, nn + 1);
   uint64_t* pos = buf<uint64_t>(b->c_pos, nn + 1);
-  if (nn) {
-    hipLaunchKernelGGL(k_unitig_keep, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
-                       b->u_rep.as<uint32_t>(), b->u_uid.as<uint64_t>(), b->u_cov.as<unsigned long long>(),
-                       b->u_len.as<uint64_t>(), b->u_edge.as<uint32_t>(), b->u_ecnt.as<uint8_t>(), ca, keep);
-    GX_HIP(hipGetLastError());
-  }
   exclusive_scan<uint32_t, uint64_t>(keep, pos, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   const uint64_t nn2 = read1(st, pos + nn);
   uint64_t* klo2 = buf<uint64_t>(b->c_klo, nn2 + 1);
@@ -4005,22 +4046,22 @@ static void clean_graph(gx_build* b, const CleanArgs& ca) {
   if (nn) {
     hipLaunchKernelGGL(k_keep_nodes<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, keep, pos,
                        b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, b->cnt.as<uint32_t>(),
-                       b->msk.as<uint32_t>(), nix, b->cfg.k, b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), klo2, khi2,
-                       cnt2, msk2, hcu2, hcf2, htot2, errw);
+                       b->msk.as<uint32_t>(), nix, b->cfg.k, b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), ah, klo2,
+                       khi2, cnt2, msk2, hcu2, hcf2, htot2, errw);
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint64_t, uint64_t>(htot2, hstart2, nn2, scratch_for(b, scan_scratch_bytes<uint64_t>(nn2)), st);
   const uint64_t nh2 = read1(st, hstart2 + nn2);
   uint32_t* uval2 = buf<uint32_t>(b->c_uval, nh2 + 1);
+  const bool offs = b->has_hoff || ah.aval;  // heads at letters other than the build's
+  uint8_t* hoff2 = offs ? buf<uint8_t>(b->c_hoff, nh2 + 1) : nullptr;
   if (nn) {
     hipLaunchKernelGGL(k_keep_heads, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, keep, pos,
                        b->hstart.as<uint64_t>(), b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), b->uval.as<uint32_t>(),
-                       hstart2, uval2);
+                       b->has_hoff ? b->hoff.as<uint8_t>() : nullptr, ah, b->cfg.k, hstart2, uval2, hoff2);
     GX_HIP(hipGetLastError());
   }
   if (read1(st, errw) != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the graph");
-  // the surviving graph becomes the handle's node graph (the build's own
-  // records are unchanged)
   swap_buf(b->klo, b->c_klo);
   if (W == 2) swap_buf(b->khi, b->c_khi);
   swap_buf(b->cnt, b->c_cnt);
@@ -4029,10 +4070,422 @@ static void clean_graph(gx_build* b, const CleanArgs& ca) {
   swap_buf(b->hcf, b->c_hcf);
   swap_buf(b->hstart, b->c_hstart);
   swap_buf(b->uval, b->c_uval);
+  if (offs) swap_buf(b->hoff, b->c_hoff);
+  b->has_hoff = offs;
   b->nnodes = nn2;
   b->nuheads = nh2;
   build_node_index<W>(b, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, 0, 256);
   run_unitigs<W>(b);
+}
+
+template <int W>
+static void clean_graph(gx_build* b, const CleanArgs& ca) {
+  hipStream_t st = b->st;
+  const uint64_t nn = b->nnodes;
+  uint32_t* keep = buf<uint32_t>(b->c_keep, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_unitig_keep, dim3(grid_for(nn, kBlock, b->grid)), dim3(kBlock), 0, st, nn,
+                       b->u_rep.as<uint32_t>(), b->u_uid.as<uint64_t>(), b->u_cov.as<unsigned long long>(),
+                       b->u_len.as<uint64_t>(), b->u_edge.as<uint32_t>(), b->u_ecnt.as<uint8_t>(), ca, keep);
+    GX_HIP(hipGetLastError());
+  }
+  compact_graph<W>(b, keep, AddedHeads{});
+}
+
+// ===========================================================================
+// BUBBLE (SimpleBubbleMergeVertex.java:51-320) over the merged graph of the
+// last unitig pass, applied to the node graph: removed bubbles drop their
+// nodes, the kept one (the top) takes their k-mer counts on its first node
+// and their read heads, scaled to its length, as letters of its nodes.
+// ===========================================================================
+
+constexpr uint32_t kBubMaxLetters = 4096;  // longest bubble sequence (edit-distance row in LDS)
+
+__device__ inline int et_side_d(uint32_t t) { return (t == ET_FF || t == ET_FR) ? 0 : 1; }
+__device__ inline uint32_t et_mirror_d(uint32_t t) { return t == ET_FF ? ET_RR : t == ET_RR ? ET_FF : t; }
+__device__ inline bool et_flips_d(uint32_t t) { return t == ET_FR || t == ET_RF; }  // EDGETYPE.causesFlip
+
+// Bubbles (detectBubble :245-252, sendBubbleAndMajorVertexMsgToMinorVertex
+// :51-79): one outer edge on each side, to two different merged vertices,
+// and fewer than maxlen letters.  The neighbour with the larger key (node
+// order is key order) is the major vertex.  bmin/bmaj: minor / major unitig
+// (kEnd: not a bubble); btyp: majorToBubble | minorToBubble << 2.
+__global__ void k_bub_cand(uint64_t nu, const uint64_t* __restrict__ ulen, const uint32_t* __restrict__ uedge,
+                           const uint8_t* __restrict__ uecnt, const uint64_t* __restrict__ uid_of_rep, int k,
+                           uint64_t maxlen, uint32_t* __restrict__ bmin, uint32_t* __restrict__ bmaj,
+                           uint8_t* __restrict__ btyp) {
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t ent[2] = {kEnd, kEnd};
+    int deg[2] = {0, 0};
+    for (int e = 0; e < uecnt[u]; e++) {
+      const uint32_t x = uedge[kUEdges * u + e];
+      const int sd = et_side_d(x >> 30);
+      if (deg[sd]++ == 0) ent[sd] = x;
+    }
+    uint32_t mn = kEnd, mj = kEnd;
+    uint8_t ty = 0;
+    if (deg[0] == 1 && deg[1] == 1 && ulen[u] + (uint64_t)k - 1 < maxlen) {
+      const uint32_t rf = ent[0] & 0x3fffffffu, rr = ent[1] & 0x3fffffffu;
+      if (rf != rr) {
+        const bool fwd_major = rf >= rr;  // VKmer.compareTo of the two neighbours
+        const uint32_t emaj = fwd_major ? ent[0] : ent[1], emin = fwd_major ? ent[1] : ent[0];
+        mj = (uint32_t)uid_of_rep[emaj & 0x3fffffffu];
+        mn = (uint32_t)uid_of_rep[emin & 0x3fffffffu];
+        ty = (uint8_t)(et_mirror_d(emaj >> 30) | (et_mirror_d(emin >> 30) << 2));
+      }
+    }
+    bmin[u] = mn;
+    bmaj[u] = mj;
+    btyp[u] = ty;
+  }
+}
+
+struct BubCtx {
+  const uint64_t *ulen, *uid_of_rep;
+  const unsigned long long* ucov;
+  const uint32_t *uedge, *urep_node, *bmin, *bmaj;
+  const uint8_t *uecnt, *btyp;
+};
+
+__device__ inline float bub_cov(const BubCtx& c, uint32_t w) {
+  return (float)((double)c.ucov[w] / (double)c.ulen[w]);
+}
+// isValidMajorAndMinor (:98-105): same side of the major and of the minor.
+__device__ inline uint32_t bub_class(const BubCtx& c, uint32_t w) {
+  return (uint32_t)et_side_d(c.btyp[w] & 3u) | ((uint32_t)et_side_d((c.btyp[w] >> 2) & 3u) << 1);
+}
+
+// The bubbles whose minor vertex is v, in processing order: by major
+// vertex, then by class (bubbles of different classes never meet in
+// processSimilarSet), then by decreasing coverage (SortByCoverage, stable;
+// before it the minor's message order, taken as increasing key).
+__device__ inline uint32_t bub_set(const BubCtx& c, uint64_t v, uint32_t* w) {
+  uint32_t n = 0;
+  for (int e = 0; e < c.uecnt[v]; e++) {
+    const uint32_t x = (uint32_t)c.uid_of_rep[c.uedge[kUEdges * v + e] & 0x3fffffffu];
+    if (c.bmin[x] != (uint32_t)v) continue;
+    bool dup = false;
+    for (uint32_t a = 0; a < n; a++) dup |= w[a] == x;
+    if (dup) continue;
+    const uint32_t cx = bub_class(c, x);
+    const float fx = bub_cov(c, x);
+    uint32_t p = n++;
+    while (p > 0) {
+      const uint32_t y = w[p - 1];
+      const uint32_t cy = bub_class(c, y);
+      const float fy = bub_cov(c, y);
+      const bool before = c.bmaj[y] != c.bmaj[x] ? c.bmaj[y] < c.bmaj[x]
+                          : cy != cx             ? cy < cx
+                          : fy != fx             ? fy > fx
+                                                 : c.urep_node[y] < c.urep_node[x];
+      if (before) break;
+      w[p] = y;
+      p--;
+    }
+    w[p] = x;
+  }
+  return n;
+}
+
+__device__ inline bool bub_same_group(const BubCtx& c, uint32_t a, uint32_t b) {
+  return c.bmaj[a] == c.bmaj[b] && bub_class(c, a) == bub_class(c, b);
+}
+
+// Pairs to compare, per minor vertex: every pair within a group of the same
+// major and class.  pbase null: count them; else write (first, second,
+// opposite orientations).
+__global__ void k_bub_pairs(uint64_t nu, BubCtx c, uint32_t* __restrict__ npairs,
+                            const uint64_t* __restrict__ pbase, uint4* __restrict__ pairs) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nu; v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t w[kUEdges];
+    const uint32_t n = bub_set(c, v, w);
+    uint64_t q = pbase ? pbase[v] : 0;
+    uint32_t cntp = 0;
+    for (uint32_t i = 0; i < n; i++)
+      for (uint32_t j = i + 1; j < n && bub_same_group(c, w[i], w[j]); j++) {
+        if (pbase) {
+          const bool flip = et_flips_d(c.btyp[w[i]] & 3u) != et_flips_d(c.btyp[w[j]] & 3u);
+          pairs[q++] = make_uint4(w[i], w[j], flip ? 1u : 0u, 0u);
+        }
+        cntp++;
+      }
+    if (!pbase) npairs[v] = cntp;
+  }
+}
+
+// Letter j of unitig u's sequence (its first k-mer, then the last letter of
+// each next one).
+template <int W>
+__device__ inline uint32_t unitig_letter(const uint32_t* order, const uint64_t* ubase, const uint64_t* klo,
+                                         const uint64_t* khi, int k, uint32_t u, uint32_t j) {
+  const uint32_t p = j < (uint32_t)k ? 0u : j - (uint32_t)k + 1;
+  const uint32_t e = order[ubase[u] + p];
+  return oriented_letter<W>(klo, khi, e & 0x7fffffffu, !(e & 0x80000000u), k, (int)(j - p));
+}
+
+// VKmer.editDistance (VKmer.java:711-731) of each pair, against the
+// reverse complement of the second when the two are stored in opposite
+// orientations (VKmer.fracDissimilar :919-928).  A wave per pair, a row of
+// the distance matrix at a time: D[r][c] = min over c' <= c of
+// T[c'] + (c - c'), T[c] = min(D[r-1][c] + 1, D[r-1][c-1] + mismatch),
+// i.e. a prefix minimum of T[c] - c across the lanes.
+template <int W>
+__global__ void __launch_bounds__(64) k_bub_dist(uint64_t np, const uint4* __restrict__ pairs,
+                                                 const uint32_t* __restrict__ order, const uint64_t* __restrict__ ubase,
+                                                 const uint64_t* __restrict__ ulen, const uint64_t* __restrict__ klo,
+                                                 const uint64_t* __restrict__ khi, int k, uint32_t* __restrict__ ed) {
+  __shared__ uint16_t row[kBubMaxLetters + 1];
+  __shared__ uint8_t sa[kBubMaxLetters], sb[kBubMaxLetters];
+  const int lane = threadIdx.x;
+  constexpr int kBig = 1 << 28;
+  for (uint64_t p = blockIdx.x; p < np; p += gridDim.x) {
+    const uint4 pr = pairs[p];
+    const uint32_t La = (uint32_t)(ulen[pr.x] + k - 1), Lb = (uint32_t)(ulen[pr.y] + k - 1);
+    for (uint32_t j = lane; j < La; j += 64) sa[j] = (uint8_t)unitig_letter<W>(order, ubase, klo, khi, k, pr.x, j);
+    for (uint32_t j = lane; j < Lb; j += 64)
+      sb[j] = (uint8_t)(pr.z ? 3u - unitig_letter<W>(order, ubase, klo, khi, k, pr.y, Lb - 1 - j)
+                             : unitig_letter<W>(order, ubase, klo, khi, k, pr.y, j));
+    for (uint32_t c = lane; c <= Lb; c += 64) row[c] = (uint16_t)c;
+    __syncthreads();
+    for (uint32_t r = 1; r <= La; r++) {
+      const uint32_t ar = sa[r - 1];
+      int carry_old = 0, carry_min = kBig;
+      for (uint32_t c0 = 0; c0 <= Lb; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        const bool valid = c <= Lb;
+        const int old = valid ? (int)row[c] : kBig;
+        int oldm1 = __shfl_up(old, 1);
+        if (lane == 0) oldm1 = carry_old;
+        int v = kBig;
+        if (valid) {
+          const int t = c == 0 ? (int)r : min(old + 1, oldm1 + (ar != sb[c - 1] ? 1 : 0));
+          v = t - (int)c;
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+          const int o = __shfl_up(v, d);
+          if (lane >= d) v = min(v, o);
+        }
+        v = min(v, carry_min);
+        if (valid) row[c] = (uint16_t)(v + (int)c);
+        carry_old = __shfl(old, 63);
+        carry_min = __shfl(v, 63);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) ed[p] = row[Lb];
+    __syncthreads();
+  }
+}
+
+// processSimilarSet (:163-240) per minor vertex: in each group the first
+// bubble left is the top; every later one whose edit distance over the
+// shorter length is at most maxd is removed into it (into = top | opposite
+// orientations << 31); the top leaves the group.
+__global__ void k_bub_decide(uint64_t nu, BubCtx c, const uint64_t* __restrict__ pbase,
+                             const uint32_t* __restrict__ ed, int k, float maxd, uint32_t* __restrict__ into) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nu; v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t w[kUEdges];
+    const uint32_t n = bub_set(c, v, w);
+    uint64_t q = pbase[v];
+    uint32_t alive = (1u << n) - 1u;
+    for (uint32_t i = 0; i < n; i++) {
+      uint64_t qi = q;  // pair (i, i+1) of this group
+      uint32_t j = i + 1;
+      for (; j < n && bub_same_group(c, w[i], w[j]); j++, qi++) {
+        if (!((alive >> i) & 1u) || !((alive >> j) & 1u)) continue;
+        const uint32_t li = (uint32_t)(c.ulen[w[i]] + k - 1), lj = (uint32_t)(c.ulen[w[j]] + k - 1);
+        const float frac = __fdiv_rn((float)ed[qi], (float)min(li, lj));
+        if (frac <= maxd) {
+          const bool flip = et_flips_d(c.btyp[w[i]] & 3u) != et_flips_d(c.btyp[w[j]] & 3u);
+          into[w[j]] = w[i] | (flip ? 0x80000000u : 0u);
+          alive &= ~(1u << j);
+        }
+      }
+      q = qi;
+      alive &= ~(1u << i);
+    }
+  }
+}
+
+// Nodes of removed bubbles are dropped; the top's first node takes their
+// k-mer counts (Node.addCoverage :290-294 in counts); read heads per
+// removed bubble counted.
+__global__ void k_bub_apply(uint64_t nn, uint64_t nu, const uint32_t* __restrict__ urep,
+                            const uint64_t* __restrict__ uid_of_rep, const uint32_t* __restrict__ into,
+                            const uint32_t* __restrict__ order, const uint64_t* __restrict__ ubase,
+                            const unsigned long long* __restrict__ ucov, const uint32_t* __restrict__ hu,
+                            const uint32_t* __restrict__ hf, uint32_t* __restrict__ keep, uint32_t* __restrict__ cnt,
+                            uint32_t* __restrict__ hn) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn || i < nu;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < nn) keep[i] = into[uid_of_rep[urep[i] & 0x7fffffffu]] == kEnd ? 1u : 0u;
+    if (i < nu) {
+      const uint32_t t = into[i];
+      hn[i] = t == kEnd ? 0u : hu[i] + hf[i];
+      if (t != kEnd) atomicAdd(&cnt[order[ubase[t & 0x7fffffffu]] & 0x7fffffffu], (uint32_t)ucov[i]);
+    }
+  }
+}
+
+// ReadHeadSet.unionUpdate(other, lengthFactor, flip, otherLength)
+// (:80-101) for the heads of each removed bubble: offset o (counted from
+// the other end when the two are stored in opposite orientations, the set
+// swapped) becomes (int)((o + 1) * f - f) in float, f = top letters /
+// bubble letters; the head goes to the top's node covering that letter
+// (the last node past its end), as the letter's place in the node's k-mer.
+__global__ void k_bub_heads(uint64_t nu, const uint32_t* __restrict__ into, const uint64_t* __restrict__ hb,
+                            const uint64_t* __restrict__ hstart, const uint32_t* __restrict__ hu,
+                            const uint32_t* __restrict__ hf, const uint64_t* __restrict__ hkey,
+                            const uint32_t* __restrict__ hval, const uint64_t* __restrict__ ulen,
+                            const uint32_t* __restrict__ order, const uint64_t* __restrict__ ubase, int k,
+                            uint32_t* __restrict__ hnode, uint32_t* __restrict__ hmv, uint8_t* __restrict__ hofs,
+                            uint32_t* __restrict__ addu, uint32_t* __restrict__ addf) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nu; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = into[w];
+    if (x == kEnd) continue;
+    const uint32_t t = x & 0x7fffffffu;
+    const bool flip = (x >> 31) != 0;
+    const uint32_t Lo = (uint32_t)(ulen[w] + k - 1), Lt = (uint32_t)(ulen[t] + k - 1);
+    const uint64_t mt = ulen[t];
+    const float fac = __fdiv_rn((float)Lt, (float)Lo);
+    const uint32_t n = hu[w] + hf[w];
+    for (uint32_t j = 0; j < n; j++) {
+      const uint64_t key = hkey[hstart[w] + j];
+      const uint32_t off = (uint32_t)((key >> 30) & ((1ull << 23) - 1));
+      const bool nf = ((key >> 53) & 1) != 0;
+      const uint32_t o = flip ? Lo - 1 - off : off;
+      const float y = __fsub_rn(__fmul_rn((float)(o + 1), fac), fac);
+      const uint64_t a = (uint64_t)(int)y;  // a letter of the top's sequence
+      const bool nf2 = flip ? !nf : nf;
+      const uint64_t q = a < mt - 1 ? a : mt - 1;
+      const uint32_t e = order[ubase[t] + q];
+      const bool fwd = !(e & 0x80000000u);
+      const uint32_t rel = (uint32_t)(a - q);
+      const bool fn = fwd ? nf2 : !nf2;
+      const uint64_t s = hb[w] + j;
+      hnode[s] = e & 0x7fffffffu;
+      hmv[s] = (hval[hstart[w] + j] & 0x7fffffffu) | (fn ? 0x80000000u : 0u);
+      hofs[s] = (uint8_t)(fwd ? rel : (uint32_t)(k - 1) - rel);
+      atomicAdd(fn ? &addf[e & 0x7fffffffu] : &addu[e & 0x7fffffffu], 1u);
+    }
+  }
+}
+
+__global__ void k_bub_total(uint64_t nn, const uint32_t* __restrict__ addu, const uint32_t* __restrict__ addf,
+                            uint32_t* __restrict__ tot) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x)
+    tot[i] = addu[i] + addf[i];
+}
+
+// Added heads into their nodes' slots (unflipped first).
+__global__ void k_bub_place(uint64_t na, const uint32_t* __restrict__ hnode, const uint32_t* __restrict__ hmv,
+                            const uint8_t* __restrict__ hofs, const uint64_t* __restrict__ astart,
+                            const uint32_t* __restrict__ addu, uint32_t* __restrict__ cur,
+                            uint32_t* __restrict__ aval, uint8_t* __restrict__ aoff) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < na; h += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = hnode[h];
+    const bool f = (hmv[h] >> 31) != 0;
+    const uint64_t s = astart[i] + (f ? addu[i] : 0u) + atomicAdd(&cur[2 * (uint64_t)i + (f ? 1 : 0)], 1u);
+    aval[s] = hmv[h];
+    aoff[s] = hofs[h];
+  }
+}
+
+template <int W>
+static void bubble_graph(gx_build* b, float maxd, uint64_t maxlen) {
+  hipStream_t st = b->st;
+  const uint64_t nn = b->nnodes, nu = b->u_count;
+  const unsigned G = b->grid;
+  const int k = b->cfg.k;
+  uint32_t* bmin = buf<uint32_t>(b->bb_min, nu + 1);
+  uint32_t* bmaj = buf<uint32_t>(b->bb_maj, nu + 1);
+  uint8_t* btyp = buf<uint8_t>(b->bb_typ, nu + 1);
+  uint32_t* np = buf<uint32_t>(b->bb_np, nu + 1);
+  uint64_t* pbase = buf<uint64_t>(b->bb_pbase, nu + 1);
+  uint32_t* into = buf<uint32_t>(b->bb_into, nu + 1);
+  BubCtx c;
+  c.ulen = b->u_len.as<uint64_t>();
+  c.uid_of_rep = b->u_uid.as<uint64_t>();
+  c.ucov = b->u_cov.as<unsigned long long>();
+  c.uedge = b->u_edge.as<uint32_t>();
+  c.urep_node = b->u_repnode.as<uint32_t>();
+  c.uecnt = b->u_ecnt.as<uint8_t>();
+  c.bmin = bmin;
+  c.bmaj = bmaj;
+  c.btyp = btyp;
+  const uint32_t* order = b->u_order.as<uint32_t>();
+  const uint64_t* ubase = b->u_base.as<uint64_t>();
+  const uint64_t* klo = b->klo.as<uint64_t>();
+  const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
+  uint64_t npairs = 0;
+  if (nu) {
+    hipLaunchKernelGGL(k_bub_cand, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c.ulen, c.uedge, c.uecnt,
+                       c.uid_of_rep, k, maxlen, bmin, bmaj, btyp);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bub_pairs, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c, np, nullptr, nullptr);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, into, 0, nu, kEnd);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(np, pbase, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
+  npairs = read1(st, pbase + nu);
+  b->bub_pairs = npairs;
+  if (npairs) {
+    uint4* pairs = buf<uint4>(b->bb_pairs, npairs);
+    uint32_t* ed = buf<uint32_t>(b->bb_ed, npairs);
+    hipLaunchKernelGGL(k_bub_pairs, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c, np, pbase, pairs);
+    GX_HIP(hipGetLastError());
+    const unsigned gd = (unsigned)std::min<uint64_t>(npairs, 1u << 16);
+    hipLaunchKernelGGL(k_bub_dist<W>, dim3(gd), dim3(64), 0, st, npairs, pairs, order, ubase, c.ulen, klo, khi, k, ed);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bub_decide, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c, pbase, ed, k, maxd,
+                       into);
+    GX_HIP(hipGetLastError());
+  }
+  uint32_t* keep = buf<uint32_t>(b->c_keep, nn + 1);
+  uint32_t* hn = buf<uint32_t>(b->bb_hn, nu + 1);
+  uint64_t* hb = buf<uint64_t>(b->bb_hb, nu + 1);
+  if (nn || nu) {
+    hipLaunchKernelGGL(k_bub_apply, dim3(grid_for(std::max(nn, nu), kBlock, G)), dim3(kBlock), 0, st, nn, nu,
+                       b->u_rep.as<uint32_t>(), c.uid_of_rep, into, order, ubase, c.ucov, b->u_hu.as<uint32_t>(),
+                       b->u_hf.as<uint32_t>(), keep, b->cnt.as<uint32_t>(), hn);
+    GX_HIP(hipGetLastError());
+  }
+  exclusive_scan<uint32_t, uint64_t>(hn, hb, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
+  const uint64_t na = read1(st, hb + nu);
+  AddedHeads ah;
+  if (na) {
+    uint32_t* addu = buf<uint32_t>(b->bb_addu, nn + 1);
+    uint32_t* addf = buf<uint32_t>(b->bb_addf, nn + 1);
+    uint32_t* cur = buf<uint32_t>(b->bb_cur, 2 * nn + 2);
+    uint32_t* tot = buf<uint32_t>(b->bb_tot, nn + 1);
+    uint64_t* astart = buf<uint64_t>(b->bb_astart, nn + 1);
+    uint32_t* hnode = buf<uint32_t>(b->bb_hnode, na);
+    uint32_t* hmv = buf<uint32_t>(b->bb_hmv, na);
+    uint8_t* hofs = buf<uint8_t>(b->bb_hofs, na);
+    uint32_t* aval = buf<uint32_t>(b->bb_aval, na);
+    uint8_t* aoff = buf<uint8_t>(b->bb_aoff, na);
+    GX_HIP(hipMemsetAsync(addu, 0, (nn + 1) * 4, st));
+    GX_HIP(hipMemsetAsync(addf, 0, (nn + 1) * 4, st));
+    GX_HIP(hipMemsetAsync(cur, 0, (2 * nn + 2) * 4, st));
+    hipLaunchKernelGGL(k_bub_heads, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, into, hb,
+                       b->u_hstart.as<uint64_t>(), b->u_hu.as<uint32_t>(), b->u_hf.as<uint32_t>(),
+                       b->u_hklo.as<uint64_t>(), b->u_hvc.as<uint32_t>(), c.ulen, order, ubase, k, hnode, hmv, hofs,
+                       addu, addf);
+    GX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bub_total, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, addu, addf, tot);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(tot, astart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+    hipLaunchKernelGGL(k_bub_place, dim3(grid_for(na, kBlock, G)), dim3(kBlock), 0, st, na, hnode, hmv, hofs, astart,
+                       addu, cur, aval, aoff);
+    GX_HIP(hipGetLastError());
+    ah.addu = addu;
+    ah.addf = addf;
+    ah.astart = astart;
+    ah.aval = aval;
+    ah.aoff = aoff;
+  }
+  compact_graph<W>(b, keep, ah);
 }
 
 // ===========================================================================
@@ -4583,6 +5036,25 @@ int gx_clean_tips(gx_build* b, int max_len) {
     const CleanArgs ca{CLEAN_TIP, 0.f, 0.f, (uint64_t)(max_len < 0 ? b->cfg.k : max_len), b->cfg.k};
     if (b->W == 1) clean_graph<1>(b, ca);
     else clean_graph<2>(b, ca);
+    b->unitigs_done = true;
+  });
+}
+
+int gx_clean_bubbles(gx_build* b, float max_dissimilarity, int max_len) {
+  if (!b) return GX_EINVAL;
+  return guarded(b, [&] {
+    // GenomixJobConf.java:429-432 (NaN passes, as there, and merges nothing)
+    if (max_dissimilarity < 0.f || max_dissimilarity > 1.f)
+      throw GxError(GX_EINVAL, "bubbleMerge_maxDissimilarity must lie in [0, 1]");
+    const uint64_t maxlen = max_len < 0 ? 5ull * (uint64_t)b->cfg.k : (uint64_t)max_len;  // :481-482
+    if (maxlen > kBubMaxLetters + 1)
+      throw GxError(GX_ENOTSUP, "bubbles longer than " + std::to_string(kBubMaxLetters) + " letters");
+    if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_bubbles before unitig_run");
+    GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
+    b->unitigs_done = false;
+    if (b->W == 1) bubble_graph<1>(b, max_dissimilarity, maxlen);
+    else bubble_graph<2>(b, max_dissimilarity, maxlen);
     b->unitigs_done = true;
   });
 }

@@ -35,7 +35,7 @@ EXPORTED = [
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
-    "gx_coverage_cutoff", "gx_fit_mixture",
+    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
@@ -112,6 +112,7 @@ def lib():
         L.gx_build_set_stream.argtypes = [vp, ctypes.c_uint64, PART_SINK, vp]
         L.gx_clean_bad_coverage.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.gx_clean_tips.argtypes = [vp, i32]
+        L.gx_clean_bubbles.argtypes = [vp, ctypes.c_float, i32]
         i64p = ctypes.POINTER(ctypes.c_int64)
         L.gx_graph_stats.argtypes = [vp, i32]
         L.gx_graph_stat_totals.argtypes = [vp, u64p, i32]
@@ -275,6 +276,12 @@ class GraphBuilder:
         """TIP_SINGLE_NODE then MERGE (include/genomix_clean.h); max_len < 0:
         k.  Returns the merged records."""
         self._check(lib().gx_clean_tips(self._h, max_len))
+        return self.unitig_records()
+
+    def clean_bubbles(self, max_dissimilarity=0.05, max_len=-1) -> bytes:
+        """BUBBLE then MERGE (include/genomix_clean.h); max_len < 0: 5k.
+        Returns the merged records."""
+        self._check(lib().gx_clean_bubbles(self._h, ctypes.c_float(max_dissimilarity), max_len))
         return self.unitig_records()
 
     def graph_stats(self, graph="unitigs") -> dict:

@@ -1,8 +1,9 @@
 /* genomix_clean.h — graph cleaning on the built graph.
  *
- * Genomix's default pipeline after BUILD and MERGE runs REMOVE_BAD_COVERAGE
- * then MERGE again (GenomixJobConf.java:511-517).  The entry point here
- * performs both on the graph a handle holds.
+ * Genomix's default pipeline after BUILD and MERGE runs REMOVE_BAD_COVERAGE,
+ * TIP_SINGLE_NODE and BUBBLE, each followed by MERGE
+ * (GenomixJobConf.java:511-517).  Each entry point here performs one step
+ * and its merge on the graph a handle holds.
  *
  * All functions return GX_OK (0) or a negative GX_* code (genomix_gx.h).
  */
@@ -37,6 +38,25 @@ int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov);
  * GenomixJobConf.java:332,504-505).  Same calling rules as
  * gx_clean_bad_coverage. */
 int gx_clean_tips(gx_build* b, int max_len);
+
+/* BUBBLE, then MERGE.  Replaces SimpleBubbleMergeVertex
+ * (SimpleBubbleMergeVertex.java:51-320): a merged vertex with one edge on
+ * each side, to two different vertices, and fewer than max_len letters is
+ * a bubble; bubbles between the same two vertices (attached to the same
+ * sides of both) are taken in decreasing coverage, and each later one whose
+ * edit distance to the first, over the shorter length, is at most
+ * max_dissimilarity (against its reverse complement when stored the other
+ * way round) is removed and merged into it: its coverage added as k-mer
+ * counts, its read heads at offsets scaled to the kept one's length
+ * (Node.addFromNode, Node.java:582-614; ReadHeadSet.unionUpdate,
+ * ReadHeadSet.java:80-101).  Then the P4 merge.  Defaults
+ * (genomix.bubbleMerge.maxDissimilarity / maxLength,
+ * GenomixJobConf.java:478-482): 0.05 and 5k; max_len < 0 means 5k.
+ * max_dissimilarity outside [0, 1] fails with GX_EINVAL (:429-432);
+ * max_len above 4097 with GX_ENOTSUP.  Ties in coverage are broken by
+ * increasing key (the reference's order there is its message arrival
+ * order).  Same calling rules as gx_clean_bad_coverage. */
+int gx_clean_bubbles(gx_build* b, float max_dissimilarity, int max_len);
 
 #ifdef __cplusplus
 }

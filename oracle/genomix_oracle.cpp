@@ -1187,8 +1187,10 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     // is this weighted average, computed pairwise in float by the reference)
     u.has_cov = true;
     u.cov = path_coverage(g, path);
-    // read heads: offset = k-mer position in the unitig; flipped heads sit
-    // at position + k - 1; a reversed node swaps the two sets
+    // read heads: a head's offset is a letter of its node's k-mer (0 for
+    // the build's unflipped heads, k - 1 for its flipped ones,
+    // ReadsKeyValueParserFactory resetOffset(K-1)); in the unitig it is that
+    // letter's position, and a reversed node swaps the two sets
     // (Node.mergeUnflippedAndFlippedReadIDs, Node.java:740-806)
     for (size_t q = 0; q < m; q++) {
       const Node& x = g[path[q].node].node;
@@ -1196,7 +1198,7 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
         const auto& set = f ? x.flipped : x.unflipped;
         for (auto h : set) {
           bool nf = path[q].fwd ? f : !f;
-          int o = (int)q + (nf ? k - 1 : 0);
+          int o = (int)q + (path[q].fwd ? h.offset() : k - 1 - h.offset());
           h.uuid = make_uuid(h.mate_id(), h.lib(), h.read_id(), o);
           if (nf) { u.has_flipped = true; treeset_add(u.flipped, h); }
           else { u.has_unflipped = true; treeset_add(u.unflipped, h); }
@@ -1315,6 +1317,182 @@ static std::vector<uint8_t> clean_coverage_then_tips(const uint8_t* recs, size_t
                                                      int maxlen) {
   const std::vector<uint8_t> g1 = clean_nodes(recs, n, k, [&](const GNode& v) { return bad_coverage(v, minc, maxc); });
   return clean_tips(g1.data(), g1.size(), k, maxlen);
+}
+
+// VKmer.editDistance (VKmer.java:711-731): Levenshtein distance of two
+// letter strings.
+static int edit_distance(const std::string& a, const std::string& b) {
+  std::vector<int> prev(b.size() + 1), cur(b.size() + 1);
+  for (size_t c = 0; c <= b.size(); c++) prev[c] = (int)c;
+  for (size_t r = 1; r <= a.size(); r++) {
+    cur[0] = (int)r;
+    for (size_t c = 1; c <= b.size(); c++) {
+      const int match = a[r - 1] == b[c - 1] ? 0 : 1;
+      cur[c] = std::min(std::min(prev[c] + 1, cur[c - 1] + 1), prev[c - 1] + match);
+    }
+    std::swap(prev, cur);
+  }
+  return prev[b.size()];
+}
+
+// EDGETYPE.causesFlip (EDGETYPE.java:190-204).
+static inline bool et_flips(int t) { return t == FR || t == RF; }
+
+// BUBBLE then MERGE, restated on the node graph.  SimpleBubbleMergeVertex
+// (SimpleBubbleMergeVertex.java:51-320) on the merged graph: a merged
+// vertex with one edge on each side, to two different vertices, and a
+// sequence shorter than maxLength letters is a bubble; the neighbour with
+// the larger key is its major vertex, the other its minor.  Bubbles with the
+// same minor and major form a set, sorted by decreasing average coverage
+// (SimpleBubbleMergeMessage.SortByCoverage; Collections.sort is stable, and
+// the set's order before it -- the minor's message arrival order -- is
+// taken here as increasing bubble key).  processSimilarSet: the first
+// bubble is the top; every later one attached to the same sides of major
+// and minor (isValidMajorAndMinor) whose fraction of dissimilarity to it
+// (edit distance, against the reverse complement when the two are stored
+// in opposite orientations, over the shorter length, VKmer.fracDissimilar
+// :915-928) is at most maxDissimilarity is removed and added to the top
+// (Node.addFromNode :582-614: coverage as counts, read heads at offsets
+// scaled by the length ratio, ReadHeadSet.unionUpdate :80-101); the top
+// leaves the set and the rest repeats.  On the node graph: the nodes of
+// removed bubbles are dropped (their neighbours lose the edges to them),
+// the top's first node takes their k-mer counts, and each scaled read head
+// goes to the top's node covering its offset, as that node's letter.
+static std::vector<uint8_t> bubble_nodes(const uint8_t* recs, size_t n, int k, float maxd, int maxlen) {
+  std::vector<GNode> g = parse_graph(recs, n, k);
+  const Paths P = merge_paths(g);
+  const std::vector<uint8_t> merged = unitigs(recs, n, k);
+  const std::vector<GNode> mv = parse_graph(merged.data(), merged.size(), k);
+  std::map<std::vector<uint8_t>, size_t> comp_by_key;  // merged vertex key -> its path
+  for (size_t c = 0; c < P.comps.size(); c++) {
+    size_t mn = P.comps[c][0].node;
+    for (auto& e : P.comps[c]) mn = std::min(mn, e.node);
+    comp_by_key[g[mn].key.b] = c;
+  }
+  auto letters = [&](const GNode& v) { return v.node.has_internal ? v.node.internal.k : k; };
+  auto seq_of = [&](const GNode& v) { return v.node.has_internal ? v.node.internal.str() : v.key.str(); };
+  struct Msg {
+    size_t v;      // merged vertex
+    int mtb, ntb;  // major / minor to bubble edge types
+    float cov;
+  };
+  std::map<std::pair<std::vector<uint8_t>, std::vector<uint8_t>>, std::vector<Msg>> sets;  // (minor, major)
+  for (size_t v = 0; v < mv.size(); v++) {  // detectBubble :245-252 (mv is in increasing key order)
+    const Node& x = mv[v].node;
+    int deg[2] = {0, 0};
+    for (int t = 0; t < 4; t++) deg[et_side(t)] += (int)x.edges[t].size();
+    if (deg[0] != 1 || deg[1] != 1 || letters(mv[v]) >= maxlen) continue;
+    int tf = x.edges[FF].empty() ? FR : FF, tr = x.edges[RF].empty() ? RR : RF;  // getSingleNeighbor :213-223
+    const VKmer& kf = x.edges[tf][0];
+    const VKmer& kr = x.edges[tr][0];
+    if (kf == kr) continue;  // :58-60
+    const bool fwd_major = compare_bytes(kf.b.data(), (int)kf.b.size(), kr.b.data(), (int)kr.b.size()) >= 0;
+    const VKmer& major = fwd_major ? kf : kr;
+    const VKmer& minor = fwd_major ? kr : kf;
+    const int mtb = et_mirror(fwd_major ? tf : tr), ntb = et_mirror(fwd_major ? tr : tf);
+    sets[{minor.b, major.b}].push_back({v, mtb, ntb, x.cov});
+  }
+  std::vector<char> dead(g.size(), 0);
+  std::vector<float> add_cov(g.size(), 0.f);
+  struct NewHead { size_t node; bool flipped; ReadHead h; };
+  std::vector<NewHead> heads;
+  for (auto& kv : sets) {
+    std::vector<Msg> list = kv.second;
+    if (list.size() < 2) continue;
+    std::stable_sort(list.begin(), list.end(), [](const Msg& a, const Msg& b) { return a.cov > b.cov; });
+    while (!list.empty()) {  // processSimilarSet :163-240
+      const Msg top = list.front();
+      list.erase(list.begin());
+      const GNode& tv = mv[top.v];
+      const auto& tpath = P.comps[comp_by_key.at(tv.key.b)];
+      const int Lt = letters(tv);
+      for (size_t j = 0; j < list.size();) {
+        const Msg cur = list[j];
+        if (et_side(top.mtb) != et_side(cur.mtb) || et_side(top.ntb) != et_side(cur.ntb)) {
+          j++;
+          continue;
+        }
+        const bool same = et_flips(top.mtb) == et_flips(cur.mtb);
+        const GNode& cv = mv[cur.v];
+        std::string cs = seq_of(cv);
+        if (!same) cs = rc_string(cs);
+        const int Lo = letters(cv);
+        const float frac = (float)edit_distance(seq_of(tv), cs) / (float)std::min(Lt, Lo);
+        if (!(frac <= maxd)) {
+          j++;
+          continue;
+        }
+        // Node.addEdges: the bubble's edges (to the same major and minor)
+        // must already be the top's, as a node graph cannot hold others
+        for (int t = 0; t < 4; t++) {
+          const int into = same ? t : (t == FF ? RF : t == FR ? RR : t == RF ? FF : FR);
+          for (auto& z : cv.node.edges[t])
+            if (std::find(tv.node.edges[into].begin(), tv.node.edges[into].end(), z) == tv.node.edges[into].end())
+              throw OraError(E_INPUT, "bubble merge would add an edge the top vertex lacks");
+        }
+        const auto& cpath = P.comps[comp_by_key.at(cv.key.b)];
+        for (auto& e : cpath) {
+          dead[e.node] = 1;
+          add_cov[tpath[0].node] += g[e.node].node.cov;
+        }
+        // ReadHeadSet.unionUpdate(other, lengthFactor, flip, otherLength)
+        const float factor = (float)Lt / (float)Lo;
+        for (int f = 0; f < 2; f++) {
+          for (const ReadHead& h : f ? cv.node.flipped : cv.node.unflipped) {
+            const int o = same ? h.offset() : Lo - 1 - h.offset();
+            float t = (float)(o + 1) * factor;
+            t = t - factor;
+            const int a = (int)t;  // letter of the top's sequence
+            const bool nf = same ? f : !f;
+            const size_t q = std::min((size_t)a, tpath.size() - 1);
+            const bool fwd = tpath[q].fwd;
+            const int rel = a - (int)q;
+            NewHead nh{tpath[q].node, fwd ? nf : !nf, h};
+            nh.h.uuid = make_uuid(h.mate_id(), h.lib(), h.read_id(), fwd ? rel : k - 1 - rel);
+            heads.push_back(nh);
+          }
+        }
+        list.erase(list.begin() + j);
+      }
+    }
+  }
+  for (size_t i = 0; i < g.size(); i++) {
+    g[i].node.cov += add_cov[i];
+  }
+  for (auto& nh : heads) {
+    Node& x = g[nh.node].node;
+    if (nh.flipped) { x.has_flipped = true; treeset_add(x.flipped, nh.h); }
+    else { x.has_unflipped = true; treeset_add(x.unflipped, nh.h); }
+  }
+  std::vector<GNode> keep;
+  for (size_t i = 0; i < g.size(); i++) {
+    if (dead[i]) continue;
+    GNode x = g[i];
+    for (int t = 0; t < 4; t++) {
+      std::vector<VKmer> l;
+      for (auto& z : x.node.edges[t])
+        if (!dead[P.idx.at(z.b)]) l.push_back(z);
+      x.node.edges[t] = l;
+    }
+    keep.push_back(std::move(x));
+  }
+  return graph_records(keep);
+}
+
+// The reference pipeline's cleaning (GenomixJobConf.java:511-517):
+// REMOVE_BAD_COVERAGE, MERGE, TIP_SINGLE_NODE, MERGE, BUBBLE, MERGE.
+static std::vector<uint8_t> bubble_nodes(const uint8_t* recs, size_t n, int k, float maxd, int maxlen);
+static std::vector<uint8_t> clean_pipeline(const uint8_t* recs, size_t n, int k, float minc, float maxc, int tiplen,
+                                           float maxd, int maxlen) {
+  const std::vector<uint8_t> g1 = clean_nodes(recs, n, k, [&](const GNode& v) { return bad_coverage(v, minc, maxc); });
+  const std::vector<uint8_t> g2 = clean_nodes(g1.data(), g1.size(), k, [&](const GNode& v) { return short_tip(v, k, tiplen); });
+  const std::vector<uint8_t> g3 = bubble_nodes(g2.data(), g2.size(), k, maxd, maxlen);
+  return unitigs(g3.data(), g3.size(), k);
+}
+
+static std::vector<uint8_t> bubble_merge(const uint8_t* recs, size_t n, int k, float maxd, int maxlen) {
+  const std::vector<uint8_t> g1 = bubble_nodes(recs, n, k, maxd, maxlen);
+  return unitigs(g1.data(), g1.size(), k);
 }
 
 // GraphStatistics.map (GraphStatistics.java:79-146) over a record stream:
@@ -1693,6 +1871,31 @@ int ora_dump_fasta(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* 
 int ora_unitigs(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
   return guard([&] {
     auto o = unitigs(recs, n, k);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_bubble_merge(const uint8_t* recs, size_t n, int k, float maxd, int maxlen, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = bubble_merge(recs, n, k, maxd, maxlen < 0 ? 5 * k : maxlen);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_clean_pipeline(const uint8_t* recs, size_t n, int k, float minc, float maxc, int tiplen, float maxd,
+                       int maxlen, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = clean_pipeline(recs, n, k, minc, maxc, tiplen < 0 ? k : tiplen, maxd, maxlen < 0 ? 5 * k : maxlen);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_bubble_nodes(const uint8_t* recs, size_t n, int k, float maxd, int maxlen, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = bubble_nodes(recs, n, k, maxd, maxlen < 0 ? 5 * k : maxlen);
     *outlen = o.size();
     *out = dup_bytes(o.data(), o.size());
   });

@@ -42,6 +42,13 @@ cannot be run; nothing was denied, it is simply unbuildable):
 * GraphStatistics counters and the FASTA dump: hand-derived counters of a
   single read's k-mer path and of a genome merged into one unitig
   (tests/test_oracle_stats.py).
+* BUBBLE: closed forms (tests/test_oracle_bubble.py): a heterozygous SNP or
+  indel collapses into the majority allele, giving exactly the merged graph
+  of the reads with the minority allele corrected (sequence, coverage, edges
+  and read-head offsets; both stored orientations of the two alleles), the
+  dissimilarity and length thresholds at their boundaries.  Parity unpinned
+  against a reference run: the bubble fixtures (genomix-hadoop
+  graphs/bubblemerge) are inputs in the retired text format without outputs.
 * FittingMixture: scipy.stats densities for the initial E step, convergence
   to generating parameters, stationarity of the mixture likelihood at the
   fixed point, the cutoff rule, and the closed form of single-valued data
@@ -105,6 +112,12 @@ def lib():
         if hasattr(L, "ora_clean_coverage_then_tips"):
             L.ora_clean_coverage_then_tips.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, i32,
                                                        ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        for name in ("ora_bubble_merge", "ora_bubble_nodes"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [vp, sz, i32, ctypes.c_float, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        if hasattr(L, "ora_clean_pipeline"):
+            L.ora_clean_pipeline.argtypes = [vp, sz, i32, ctypes.c_float, ctypes.c_float, i32, ctypes.c_float, i32,
+                                             ctypes.POINTER(vp), ctypes.POINTER(sz)]
         if hasattr(L, "ora_fitting_mixture"):
             L.ora_fitting_mixture.argtypes = [vp, sz, ctypes.c_double, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_clean_tips"):
@@ -291,6 +304,35 @@ def dump_fasta(records: bytes, k: int) -> bytes:
     """DriverUtils.dumpGraph (DriverUtils.java:109-114) of a record stream."""
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_dump_fasta(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def bubble_merge(records: bytes, k: int, max_dissimilarity=0.05, max_length=None) -> bytes:
+    """BUBBLE then MERGE (SimpleBubbleMergeVertex.java:51-320; defaults
+    GenomixJobConf.java:478-482: 0.05 and 5k) on a node-graph record stream;
+    returns the merged records."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_bubble_merge(records, len(records), k, ctypes.c_float(max_dissimilarity),
+                                  -1 if max_length is None else max_length, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def clean_pipeline(records: bytes, k: int, min_cov=3.0, max_cov=float("inf"), tip_length=None,
+                   max_dissimilarity=0.05, max_length=None) -> bytes:
+    """The reference pipeline's cleaning after BUILD (GenomixJobConf.java:511-517):
+    REMOVE_BAD_COVERAGE, MERGE, TIP_SINGLE_NODE, MERGE, BUBBLE, MERGE."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_clean_pipeline(records, len(records), k, ctypes.c_float(min_cov), ctypes.c_float(max_cov),
+                                    -1 if tip_length is None else tip_length, ctypes.c_float(max_dissimilarity),
+                                    -1 if max_length is None else max_length, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def bubble_nodes(records: bytes, k: int, max_dissimilarity=0.05, max_length=None) -> bytes:
+    """The node graph after BUBBLE (before the merge that follows)."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_bubble_nodes(records, len(records), k, ctypes.c_float(max_dissimilarity),
+                                  -1 if max_length is None else max_length, ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
 
 

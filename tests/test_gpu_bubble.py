@@ -1,0 +1,133 @@
+"""BUBBLE then MERGE on the device (include/genomix_clean.h
+gx_clean_bubbles) against the oracle's restatement, byte for byte."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gx():
+    import genomix_amd.gx as g
+    return g
+
+
+def _sim(seed, G=40000, L=100, cov=20, paired=True, err=0.01):
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=G, read_len=L, coverage=cov, error_rate=err, paired=paired,
+                           insert_mean=3 * L, insert_sd=L // 4, repeat_frac=0.05, repeat_len=(300, 900),
+                           repeat_div=0.01, seed=seed)
+    return sim.simulate(spec)
+
+
+def _readids(seqs):
+    return b"".join(b"%d\t%s\n" % (4 * i + 2, s.encode()) for i, s in enumerate(seqs))
+
+
+def _rc(s):
+    return s[::-1].translate(str.maketrans("ACGT", "TGCA"))
+
+
+def _alleles(seed, G, L, alt_of, frac=0.3):
+    rng = np.random.default_rng(seed)
+    genome = "".join("ACGT"[x] for x in rng.integers(0, 4, G))
+    alt = alt_of(genome)
+    reads = []
+    for i, s in enumerate(range(0, G - L + 1, 2)):
+        r = (alt if (i % 10) < int(frac * 10) else genome)[s:s + L]
+        reads.append(_rc(r) if rng.random() < 0.5 else r)
+    return _readids(reads)
+
+
+def _gpu(gx, text, k, steps, method=None):
+    with gx.GraphBuilder(k, method=method) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        out = None
+        for name, args in steps:
+            out = getattr(b, name)(*args)
+        return out
+
+
+def _sub(s, j):
+    return s[:j] + "ACGT"[("ACGT".index(s[j]) + 1) % 4] + s[j + 1:]
+
+
+@pytest.mark.parametrize("k", [21, 31])
+def test_alleles_match_oracle(gx, k):
+    cases = [(s, lambda g: _sub(g, 300)) for s in (1, 9, 12, 17, 29)]
+    cases += [(s, lambda g: g[:300] + g[301:]) for s in (1, 2, 3)]                 # deletion
+    cases += [(s, lambda g: _sub(_sub(g, 300), 305)) for s in (3, 4)]              # two substitutions
+    for seed, alt in cases:
+        text = _alleles(seed, 600, 70, alt)
+        recs = oracle.build_graph(text, k)[0]
+        for maxd, maxlen in ((0.05, -1), (0.02, -1), (0.05, 2 * k - 1), (1.0, 4097)):
+            got = _gpu(gx, text, k, [("clean_bubbles", (maxd, maxlen))])
+            want = oracle.bubble_merge(recs, k, maxd, None if maxlen < 0 else maxlen)
+            assert got == want, (seed, maxd, maxlen)
+
+
+@pytest.mark.parametrize("k,paired", [(21, False), (31, True), (55, True), (24, True)])
+def test_raw_graph_matches_oracle(gx, k, paired):
+    """Sequencing errors make many bubbles in the graph before cleaning."""
+    text = _sim(5100 + k, paired=paired)
+    recs = oracle.build_graph(text, k)[0]
+    for maxd, maxlen in ((0.05, -1), (0.2, 300)):
+        got = _gpu(gx, text, k, [("clean_bubbles", (maxd, maxlen))])
+        assert got == oracle.bubble_merge(recs, k, maxd, None if maxlen < 0 else maxlen), (maxd, maxlen)
+
+
+@pytest.mark.parametrize("k", [31, 55])
+def test_pipeline_matches_oracle(gx, k):
+    """REMOVE_BAD_COVERAGE, TIP_SINGLE_NODE and BUBBLE, each with its merge
+    (GenomixJobConf.java:511-517), on a diploid-like read set."""
+    text = _sim(5200 + k, cov=30, err=0.005)
+    recs = oracle.build_graph(text, k)[0]
+    got = _gpu(gx, text, k, [("clean_bad_coverage", (3.0,)), ("clean_tips", (-1,)), ("clean_bubbles", (0.05, -1))])
+    assert got == oracle.clean_pipeline(recs, k)
+
+
+def test_bubbles_twice_then_coverage(gx):
+    """Heads moved by a bubble merge stay where they were put through a
+    second bubble merge and a coverage cleaning."""
+    k = 21
+    text = _sim(77, G=30000, cov=25, paired=False)
+    recs = oracle.build_graph(text, k)[0]
+    once = oracle.bubble_nodes(recs, k, 0.1)
+    assert _gpu(gx, text, k, [("clean_bubbles", (0.1, -1)), ("clean_bubbles", (0.2, -1))]) == \
+        oracle.bubble_merge(once, k, 0.2)
+    assert _gpu(gx, text, k, [("clean_bubbles", (0.1, -1)), ("clean_bad_coverage", (2.0,))]) == \
+        oracle.clean_bad_coverage(once, k, 2.0)
+
+
+def test_hash_method(gx):
+    k = 31
+    text = _sim(88, G=20000)
+    recs = oracle.build_graph(text, k)[0]
+    assert _gpu(gx, text, k, [("clean_bubbles", (0.05, -1))], method="hash") == oracle.bubble_merge(recs, k)
+
+
+def test_errors(gx):
+    text = _sim(3, G=5000, cov=8)
+    with gx.GraphBuilder(21) as b:
+        b.add_reads(text)
+        b.finish()
+        with pytest.raises(gx.GxError) as e:
+            b.clean_bubbles()
+        assert e.value.code == gx.GX_ESTATE
+        b.run_unitigs()
+        plain = b.unitigs()
+        for bad in (-0.1, 1.5):
+            with pytest.raises(gx.GxError) as e:
+                b.clean_bubbles(bad)
+            assert e.value.code == gx.GX_EINVAL
+        with pytest.raises(gx.GxError) as e:
+            b.clean_bubbles(0.05, 5000)
+        assert e.value.code == gx.GX_ENOTSUP
+        assert b.clean_bubbles(math.nan) == plain  # accepted, as the reference's check lets NaN through; merges nothing
