@@ -107,6 +107,7 @@ def batched_leg(steps, warmup, scale=1.0):
     with gx.GraphBatch(sets, sizer=net, tuning=tuning) as b:
         el = _timed(lambda: b.run(unitigs=True), steps, warmup, "batched")
         st = [b.set_stats(i) for i in range(len(sets))]
+        retried = b.retried()
     for d in dev:
         d.free()
     occ = sum(x["occurrences"] for x in st)
@@ -116,6 +117,7 @@ def batched_leg(steps, warmup, scale=1.0):
             "table_rebuilds": sum(x["table_rebuilds"] for x in st),
             "k_mix": {str(k): sum(1 for _, kk in specs if kk == k) for k in (21, 31, 55)},
             "workers": tuning["workers"] if tuning else 4, "learned_sizing": net is not None,
+            "sets_retried_alone": retried,
             "workload": "64 read sets, genomes log-uniform 100 kbp-10 Mbp" + (f" x{scale}" if scale != 1 else "") +
                         ", 100/150 bp SE/PE, 20-80x, 0.5-2% error, k 21/31/55; graph + unitigs per set"}
 
@@ -373,6 +375,30 @@ def main():
     clean = {"ms": round(1e3 * t_clean, 3), "min_coverage": 3.0,
              "unitigs_before": unitigs, "unitigs_after": builder.unitig_count(),
              "unitig_pass_ms": round(builder.phase_times().get("unitig", 0.0), 3)}
+    # the rest of the reference pipeline on the cleaned graph: TIP_SINGLE_NODE
+    # then BUBBLE, each with its merge
+    devmem.synchronize()
+    t1 = time.perf_counter()
+    builder.clean_tips(-1)
+    devmem.synchronize()
+    clean["tips_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
+    clean["unitigs_after_tips"] = builder.unitig_count()
+    t1 = time.perf_counter()
+    builder.clean_bubbles(0.05, -1)
+    devmem.synchronize()
+    clean["bubbles_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
+    clean["unitigs_after_bubbles"] = builder.unitig_count()
+    # BUBBLE + MERGE on the graph before any cleaning, where sequencing errors
+    # make most bubbles (timed on its second call)
+    for rep in range(2):
+        step()
+        devmem.synchronize()
+        t1 = time.perf_counter()
+        builder.clean_bubbles(0.05, -1)
+        devmem.synchronize()
+        t_rb = time.perf_counter() - t1
+    clean["raw_bubbles"] = {"ms": round(1e3 * t_rb, 3), "unitigs_before": unitigs,
+                            "unitigs_after": builder.unitig_count()}
     builder.close()
     dtext.free()
     batched = None

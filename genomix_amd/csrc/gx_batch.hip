@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/genomix_batch.h"
+#include "../../include/genomix_stream.h"
 
 namespace {
 
@@ -60,6 +61,7 @@ struct gx_batch {
   std::string err;
   std::mutex mu;
   std::vector<gx_build*> pool;  // one handle per worker, kept across runs
+  int retried = 0;              // sets of the last run taken again alone after running out of memory
 };
 
 extern "C" {
@@ -140,18 +142,71 @@ int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* use
   std::atomic<int> next{0};
   std::atomic<int> failed{0};
   int first_rc = GX_OK;
+  std::vector<int> retry;  // sets that ran out of device memory beside the other workers' caches
   const int nw = std::max(1, std::min(b->tune.workers, n));
   if ((int)b->pool.size() < nw) b->pool.resize((size_t)nw, nullptr);
+  auto fail = [&](int rc, const std::string& msg) {
+    std::lock_guard<std::mutex> g(b->mu);
+    if (first_rc == GX_OK) {
+      first_rc = rc;
+      b->err = msg;
+    }
+    failed = 1;
+  };
+  // One set on handle h (created on first use).  Returns GX_OK, or the
+  // set's error code with *msg set.
+  auto run_set = [&](gx_build*& h, int si, std::string* msg) -> int {
+    SetSpec& s = b->sets[(size_t)si];
+    const std::string where = "set " + std::to_string(si) + ": ";
+    int rc = GX_OK;
+    if (!h) {
+      std::lock_guard<std::mutex> g(b->mu);  // creation errors are reported through one global string
+      rc = gx_build_create(&s.cfg, &h);
+      if (rc) {
+        *msg = where + gx_last_error(nullptr);
+        h = nullptr;
+        return rc;
+      }
+    } else {
+      rc = gx_build_set_config(h, &s.cfg);
+    }
+    size_t c = 0;
+    uint64_t est = occ_estimate(s);
+    while (c < b->tune.max_occ.size() && est > b->tune.max_occ[c]) c++;
+    // one pass per set: the free-memory estimate behind automatic key
+    // ranges would count the other workers' buffers against this one
+    if (!rc && unitigs) rc = gx_build_set_stream(h, ~0ull, nullptr, nullptr);
+    if (!rc) rc = gx_build_set_tuning(h, b->tune.grid[c], b->tune.slack[c]);
+    if (!rc) rc = gx_build_set_sizer(h, b->sizer);
+    auto t0 = std::chrono::steady_clock::now();
+    if (!rc)
+      rc = s.on_device ? gx_build_add_reads_device(h, s.input, s.len, s.fmt)
+                       : gx_build_add_reads(h, (const char*)s.input, s.len, s.fmt);
+    if (!rc) rc = gx_build_finish(h);
+    if (!rc && unitigs) {
+      rc = gx_unitig_run(h);
+      // the build ran in key ranges for want of free device memory (held
+      // by the other workers): the set is taken again alone
+      if (rc == GX_ENOTSUP) rc = GX_ENOMEM;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (rc) {
+      *msg = where + gx_last_error(h);
+      return rc;
+    }
+    s.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    gx_build_stats(h, s.stats, 9);
+    if (consumer) {
+      int crc = consumer(user, si, h);
+      if (crc) {
+        *msg = where + "consumer returned " + std::to_string(crc);
+        return crc < 0 ? crc : GX_EINVAL;
+      }
+    }
+    return GX_OK;
+  };
   auto worker = [&](int w) {
     gx_build*& h = b->pool[(size_t)w];
-    auto fail = [&](int rc, const std::string& msg) {
-      std::lock_guard<std::mutex> g(b->mu);
-      if (first_rc == GX_OK) {
-        first_rc = rc;
-        b->err = msg;
-      }
-      failed = 1;
-    };
     if (hipSetDevice(b->device) != hipSuccess) {
       fail(GX_EDEVICE, "hipSetDevice failed");
       return;
@@ -160,55 +215,45 @@ int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* use
       if (failed) break;
       int j = next.fetch_add(1);
       if (j >= n) break;
-      int si = order[(size_t)j];
-      SetSpec& s = b->sets[(size_t)si];
-      std::string where = "set " + std::to_string(si) + ": ";
-      int rc = GX_OK;
-      if (!h) {
-        std::string msg;
-        {
-          std::lock_guard<std::mutex> g(b->mu);  // creation errors are reported through one global string
-          rc = gx_build_create(&s.cfg, &h);
-          if (rc) msg = gx_last_error(nullptr);
-        }
-        if (rc) {
-          h = nullptr;
-          fail(rc, where + msg);
-          break;
-        }
-      } else {
-        rc = gx_build_set_config(h, &s.cfg);
+      const int si = order[(size_t)j];
+      std::string msg;
+      const int rc = run_set(h, si, &msg);
+      if (rc == GX_ENOMEM) {
+        // the device memory is held by the workers' buffers: give this
+        // worker's back and run the set again alone at the end
+        if (h) gx_build_destroy(h);
+        h = nullptr;
+        std::lock_guard<std::mutex> g(b->mu);
+        retry.push_back(si);
+        continue;
       }
-      size_t c = 0;
-      uint64_t est = occ_estimate(s);
-      while (c < b->tune.max_occ.size() && est > b->tune.max_occ[c]) c++;
-      if (!rc) rc = gx_build_set_tuning(h, b->tune.grid[c], b->tune.slack[c]);
-      if (!rc) rc = gx_build_set_sizer(h, b->sizer);
-      auto t0 = std::chrono::steady_clock::now();
-      if (!rc)
-        rc = s.on_device ? gx_build_add_reads_device(h, s.input, s.len, s.fmt)
-                         : gx_build_add_reads(h, (const char*)s.input, s.len, s.fmt);
-      if (!rc) rc = gx_build_finish(h);
-      if (!rc && unitigs) rc = gx_unitig_run(h);
-      auto t1 = std::chrono::steady_clock::now();
       if (rc) {
-        fail(rc, where + gx_last_error(h));
+        fail(rc, msg);
         break;
-      }
-      s.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-      gx_build_stats(h, s.stats, 9);
-      if (consumer) {
-        int crc = consumer(user, si, h);
-        if (crc) {
-          fail(crc < 0 ? crc : GX_EINVAL, where + "consumer returned " + std::to_string(crc));
-          break;
-        }
       }
     }
   };
   std::vector<std::thread> threads;
   for (int w = 0; w < nw; w++) threads.emplace_back(worker, w);
   for (auto& t : threads) t.join();
+  if (!failed && !retry.empty()) {
+    for (gx_build*& h : b->pool)
+      if (h) {
+        gx_build_destroy(h);
+        h = nullptr;
+      }
+    std::sort(retry.begin(), retry.end());
+    if (hipSetDevice(b->device) != hipSuccess) return GX_EDEVICE;
+    for (int si : retry) {
+      std::string msg;
+      const int rc = run_set(b->pool[0], si, &msg);
+      if (rc) {
+        fail(rc, msg);
+        break;
+      }
+    }
+  }
+  b->retried = (int)retry.size();
   return first_rc;
 }
 
@@ -220,6 +265,8 @@ int gx_batch_set_stats(gx_batch* b, int set, uint64_t* out, int n, double* ms) {
   if (ms) *ms = s.ms;
   return GX_OK;
 }
+
+int gx_batch_retried(const gx_batch* b) { return b ? b->retried : 0; }
 
 const char* gx_batch_last_error(const gx_batch* b) { return b ? b->err.c_str() : "null handle"; }
 

@@ -3816,7 +3816,15 @@ static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src,
   if (need > dst.cap) {
     size_t ncap = std::max(need, dst.cap * 3 / 2);
     void* np = nullptr;
-    GX_HIP(hipMalloc(&np, ncap));
+    hipError_t e = hipMalloc(&np, ncap);
+    if (e == hipErrorOutOfMemory && reclaim_dead(&dst)) {  // free the handle's dead buffers, then again
+      (void)hipGetLastError();
+      e = hipMalloc(&np, ncap);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw HipError("hipMalloc of " + std::to_string(ncap) + " bytes: " + hipGetErrorString(e), (int)e);
+    }
     if (dlen) GX_HIP(hipMemcpyAsync(np, dst.p, dlen, hipMemcpyDeviceToDevice, b->st));
     GX_HIP(hipStreamSynchronize(b->st));
     dst.release();
@@ -4899,6 +4907,7 @@ int gx_build_add_reads(gx_build* b, const char* buf_, size_t len, int fmt) {
   return guarded(b, [&] {
     if (len && !buf_) throw GxError(GX_EINVAL, "null buffer");
     GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
     append_text(b, buf_, len, hipMemcpyHostToDevice, fmt);
   });
 }
@@ -4908,6 +4917,7 @@ int gx_build_add_reads_device(gx_build* b, const void* dev_buf, size_t len, int 
   return guarded(b, [&] {
     if (len && !dev_buf) throw GxError(GX_EINVAL, "null buffer");
     GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
     append_text(b, dev_buf, len, hipMemcpyDeviceToDevice, fmt);
   });
 }
@@ -4917,6 +4927,7 @@ int gx_build_add_fastq_pair(gx_build* b, const char* mate1, size_t n1, const cha
   return guarded(b, [&] {
     if ((n1 && !mate1) || (n2 && !mate2)) throw GxError(GX_EINVAL, "null buffer");
     GX_HIP(hipSetDevice(b->cfg.device));
+    ReclaimScope rs(&b->reclaim, b->st);
     set_format(b, kFmtFastqPair);
     append_to(b, b->text, b->text_len, mate1, n1, hipMemcpyHostToDevice);
     append_to(b, b->text2, b->text2_len, mate2, n2, hipMemcpyHostToDevice);

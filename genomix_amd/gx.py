@@ -35,7 +35,7 @@ EXPORTED = [
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
-    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles",
+    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
@@ -129,6 +129,7 @@ def lib():
         L.gx_batch_run.argtypes = [vp, i32, BATCH_CONSUMER, vp]
         L.gx_batch_set_stats.argtypes = [vp, i32, u64p, i32, ctypes.POINTER(ctypes.c_double)]
         L.gx_batch_last_error.argtypes = [vp]
+        L.gx_batch_retried.argtypes = [vp]
         L.gx_batch_last_error.restype = ctypes.c_char_p
         L.gx_batch_destroy.argtypes = [vp]
         _lib = L
@@ -504,6 +505,10 @@ class GraphBatch:
             raise errors[0]
         self._check(rc)
         return out
+
+    def retried(self):
+        """Sets of the last run built again alone for want of device memory."""
+        return lib().gx_batch_retried(self._b)
 
     def set_stats(self, i):
         a = (ctypes.c_uint64 * 9)()

@@ -62,8 +62,13 @@ int gx_batch_set_sizer(gx_batch* b, gx_mlp* m);
 int gx_batch_set_tuning(gx_batch* b, int workers, int nclasses, const uint64_t* max_occ, const int* grid_blocks,
                         const float* table_slack);
 /* Builds every set (and runs unitig compression when unitigs != 0),
- * calling consumer (may be NULL) after each. */
+ * calling consumer (may be NULL) after each.  A set that runs out of
+ * device memory beside the other workers' buffers (or is split into key
+ * ranges for want of it, which the unitig pass cannot take) is built again
+ * alone after the others. */
 int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* user);
+/* Sets of the last run that were built again alone. */
+int gx_batch_retried(const gx_batch* b);
 /* Per-set statistics of the last run (gx_build_stats layout, n <= 9) and
  * its device time in milliseconds (finish + unitigs, host clock). */
 int gx_batch_set_stats(gx_batch* b, int set, uint64_t* out, int n, double* ms);
