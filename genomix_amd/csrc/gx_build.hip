@@ -2358,6 +2358,7 @@ struct USerCtx {
   const uint32_t *hu, *hf;
   const uint64_t* hkey;    // offset etc.
   const uint32_t* hval;    // mate index | flipped << 31
+  const uint64_t* hbytes;  // head-rich unitigs: bytes of the unflipped / flipped heads
   const uint32_t* mlen;
   const uint64_t* wbase;
   const uint64_t* words;
@@ -2403,11 +2404,35 @@ __device__ inline uint64_t unitig_record_size(const USerCtx& c, uint64_t u) {
     uint32_t n = f ? nf : nu_;
     if (!n) continue;
     val += 1 + 4;
+    if (nu_ + nf > kHeadsLong) {  // summed by k_uhead_bytes_long
+      val += c.hbytes[2 * u + f];
+      continue;
+    }
     uint64_t a = h0 + (f ? nu_ : 0);
     for (uint64_t j = a; j < a + n; j++) val += uhead_bytes(c, j);
   }
   if (c.ulen[u] > 1) val += 4 + (useq_len(c, u) + 3) / 4;
   return 8 + 4 + c.nb + val;
+}
+
+// Bytes of the heads of head-rich unitigs (more than kHeadsLong), a wave
+// per unitig.
+__global__ void __launch_bounds__(kBlock) k_uhead_bytes_long(USerCtx c, uint64_t* __restrict__ hbytes) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t u = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); u < c.nu; u += nw) {
+    const uint32_t nu_ = c.hu[u], nf = c.hf[u];
+    if (nu_ + nf <= kHeadsLong) continue;
+    for (int f = 0; f < 2; f++) {
+      const uint32_t n = f ? nf : nu_;
+      const uint64_t a = c.hstart[u] + (f ? nu_ : 0);
+      uint64_t sum = 0;
+      for (uint32_t q = lane; q < n; q += 64) sum += uhead_bytes(c, a + q);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (lane == 0) hbytes[2 * u + f] = sum;
+    }
+  }
 }
 
 __global__ void k_useq_size(USerCtx c, uint64_t* __restrict__ rsize) {
@@ -2484,7 +2509,7 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
       uint64_t a = h0 + (f ? nu_ : 0);
       if (nu_ + nf > kHeadsLong) {  // k_uheads_long writes these heads
         head_at[2 * u + f] = rec_at + (uint64_t)(p - rs);
-        for (uint64_t j = a; j < a + n; j++) p += uhead_bytes(c, j);
+        p += c.hbytes[2 * u + f];
       } else {
         for (uint64_t j = a; j < a + n; j++) write_uhead(c, j, p);
       }
@@ -2667,6 +2692,7 @@ struct gx_build {
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
+  DevBuf u_hbytes;            // unitig pass: head bytes of head-rich unitigs
   DevBuf u_headat;            // unitig pass: byte position of the heads of head-rich unitigs
   DevBuf s_acc, s_k1, s_k2, s_v1, s_v2, s_flag, s_pos, s_bval, s_bst, s_out;  // statistics, FASTA
   int st_graph = -1;                                                          // graph of the last statistics
@@ -2896,7 +2922,7 @@ static size_t handle_bytes(const gx_build* b) {
         &b->u_base, &b->u_order, &b->u_cov, &b->u_edge, &b->u_ecnt, &b->u_hk, &b->u_hk2, &b->u_hv, &b->u_hv2,
         &b->u_uflag, &b->u_uposh, &b->u_hu, &b->u_hf, &b->u_htot, &b->u_hstart, &b->u_hklo, &b->u_hvc, &b->u_rsize,
         &b->u_roff, &b->u_out, &b->ppid, &b->pidx, &b->ppid2, &b->pidx2, &b->pbound, &b->sva, &b->svb,
-        &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->tophist, &b->outacc, &b->blist, &b->hist2,
+        &b->nix_dir, &b->nix_slots, &b->rcode, &b->u_seqat, &b->u_hbytes, &b->tophist, &b->outacc, &b->blist, &b->hist2,
         &b->hregs, &b->sfeat, &b->spred, &b->c_keep, &b->c_pos, &b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk,
         &b->c_hcu, &b->c_hcf, &b->c_htot, &b->c_hstart, &b->c_uval, &b->hoff, &b->c_hoff, &b->bb_min, &b->bb_maj,
         &b->bb_typ, &b->bb_np, &b->bb_pbase, &b->bb_pairs, &b->bb_ed, &b->bb_into, &b->bb_hn, &b->bb_hb,
@@ -3761,9 +3787,14 @@ static void run_unitigs(gx_build* b) {
   c.urep_node = urep_node; c.ucov = ucov; c.uedge = uedge; c.uecnt = uecnt; c.hstart = hstart; c.hu = hu; c.hf = hf;
   c.hkey = hkey_lo; c.hval = hval_c; c.mlen = b->mlen.as<uint32_t>(); c.wbase = b->wbase.as<uint64_t>();
   c.words = b->words.as<uint64_t>(); c.rid = b->rid.as<int64_t>();
+  uint64_t* hbytes = buf<uint64_t>(b->u_hbytes, 2 * nu + 2);
+  c.hbytes = hbytes;
   uint64_t* rsize = buf<uint64_t>(b->u_rsize, nu + 1);
   uint64_t* roff = buf<uint64_t>(b->u_roff, nu + 1);
   if (nu) {
+    hipLaunchKernelGGL(k_uhead_bytes_long, dim3(grid_for(nu, kBlock / 64, 1u << 16)), dim3(kBlock), 0, st, c,
+                       hbytes);
+    GX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_useq_size, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, c, rsize);
     GX_HIP(hipGetLastError());
   }

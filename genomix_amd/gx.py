@@ -267,23 +267,24 @@ class GraphBuilder:
         self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
         return buf.raw[:w.value]
 
-    def clean_bad_coverage(self, min_cov=3.0, max_cov=float("inf")) -> bytes:
+    def clean_bad_coverage(self, min_cov=3.0, max_cov=float("inf"), emit=True):
         """REMOVE_BAD_COVERAGE then MERGE (include/genomix_clean.h) on the
-        graph after a unitig pass; returns the merged records."""
+        graph after a unitig pass; returns the merged records (copied to the
+        host; emit=False leaves them in HBM and returns None)."""
         self._check(lib().gx_clean_bad_coverage(self._h, ctypes.c_float(min_cov), ctypes.c_float(max_cov)))
-        return self.unitig_records()
+        return self.unitig_records() if emit else None
 
-    def clean_tips(self, max_len=-1) -> bytes:
+    def clean_tips(self, max_len=-1, emit=True):
         """TIP_SINGLE_NODE then MERGE (include/genomix_clean.h); max_len < 0:
-        k.  Returns the merged records."""
+        k.  Returns the merged records (emit=False: None)."""
         self._check(lib().gx_clean_tips(self._h, max_len))
-        return self.unitig_records()
+        return self.unitig_records() if emit else None
 
-    def clean_bubbles(self, max_dissimilarity=0.05, max_len=-1) -> bytes:
+    def clean_bubbles(self, max_dissimilarity=0.05, max_len=-1, emit=True):
         """BUBBLE then MERGE (include/genomix_clean.h); max_len < 0: 5k.
-        Returns the merged records."""
+        Returns the merged records (emit=False: None)."""
         self._check(lib().gx_clean_bubbles(self._h, ctypes.c_float(max_dissimilarity), max_len))
-        return self.unitig_records()
+        return self.unitig_records() if emit else None
 
     def graph_stats(self, graph="unitigs") -> dict:
         """GraphStatistics counters (include/genomix_stats.h) of the node graph
