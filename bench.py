@@ -21,6 +21,10 @@ same for the k-mer pass of the aggregation, and a CPU baseline: the
 CPU oracle (a single-threaded C++ restatement of the reference's Hyracks
 dataflow) timed on a bounded sample of the same reads.
 
+After the plain leg (unless --no-graph-steps): the coverage cutoff,
+REMOVE_BAD_COVERAGE, statistics and FASTA, TIP_SINGLE_NODE and BUBBLE on
+the graph in HBM, timed one by one.
+
 "batched" (unless --no-batched): the batched workload -- 64 independent
 read sets, genomes log-uniform 100 kbp..10 Mbp, 100/150 bp SE/PE reads at
 20-80x, 0.5-2% error, k mixed 21/31/55 -- built (graph + unitigs) by the
@@ -246,6 +250,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-graph-steps", action="store_true",
+                    help="skip the cleaning, statistics and BUBBLE timings after the plain leg")
     ap.add_argument("--batched-scale", type=float, default=1.0)
     ap.add_argument("--workload", choices=["plain", "learned", "batched", "streamed"], default="plain")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -338,74 +344,77 @@ def main():
                      "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
                      "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
     unitigs = builder.unitig_count()
-    # the driver's coverage cutoff over the build's node graph: statistics,
-    # then the mixture fit (timed on the second call)
-    for rep in range(2):
+    clean = stats = None
+    if not args.no_graph_steps:
+        # the steps after the build on the graph in HBM (untimed in value)
+        # the driver's coverage cutoff over the build's node graph: statistics,
+        # then the mixture fit (timed on the second call)
+        for rep in range(2):
+            devmem.synchronize()
+            t1 = time.perf_counter()
+            mix = builder.coverage_cutoff("nodes")
+            t_cut = time.perf_counter() - t1
+        # graph cleaning after the step, as the reference pipeline runs it:
+        # REMOVE_BAD_COVERAGE (min 3.0) then MERGE, on the graph in HBM (timed
+        # on its second call, its buffers allocated)
+        for rep in range(2):
+            step()
+            devmem.synchronize()
+            t1 = time.perf_counter()
+            builder.clean_bad_coverage(3.0, emit=False)
+            devmem.synchronize()
+            t_clean = time.perf_counter() - t1
+        # statistics and FASTA dump of the cleaned merge, and statistics of the
+        # node graph (the GraphStatistics / dumpGraph consumers)
+        for rep in range(2):  # timed on the second call, its buffers allocated
+            t1 = time.perf_counter()
+            ust = builder.graph_stats("unitigs")
+            t_su = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            fa = builder.fasta("unitigs")
+            t_fa = time.perf_counter() - t1
+        for rep in range(2):  # timed on the second call, its buffers allocated
+            t1 = time.perf_counter()
+            nst = builder.graph_stats("nodes")
+            t_sn = time.perf_counter() - t1
+        stats = {"unitig_stats_ms": round(1e3 * t_su, 3), "unitig_fasta_ms": round(1e3 * t_fa, 3),
+                 "fasta_bytes": len(fa), "node_stats_ms": round(1e3 * t_sn, 3),
+                 "node_vertices": nst["totals"]["nodes"], "unitig_vertices": ust["totals"]["nodes"],
+                 "coverage_cutoff_ms": round(1e3 * t_cut, 3), "coverage_cutoff": mix["cutoff"],
+                 "coverage_normal_mean": round(mix["normal_mean"], 3)}
+        clean = {"ms": round(1e3 * t_clean, 3), "min_coverage": 3.0,
+                 "unitigs_before": unitigs, "unitigs_after": builder.unitig_count(),
+                 "unitig_pass_ms": round(builder.phase_times().get("unitig", 0.0), 3)}
+        # the rest of the reference pipeline on the cleaned graph: TIP_SINGLE_NODE
+        # then BUBBLE, each with its merge
         devmem.synchronize()
         t1 = time.perf_counter()
-        mix = builder.coverage_cutoff("nodes")
-        t_cut = time.perf_counter() - t1
-    # graph cleaning after the step, as the reference pipeline runs it:
-    # REMOVE_BAD_COVERAGE (min 3.0) then MERGE, on the graph in HBM (timed
-    # on its second call, its buffers allocated)
-    for rep in range(2):
-        step()
+        builder.clean_tips(-1, emit=False)
         devmem.synchronize()
-        t1 = time.perf_counter()
-        builder.clean_bad_coverage(3.0, emit=False)
-        devmem.synchronize()
-        t_clean = time.perf_counter() - t1
-    # statistics and FASTA dump of the cleaned merge, and statistics of the
-    # node graph (the GraphStatistics / dumpGraph consumers)
-    for rep in range(2):  # timed on the second call, its buffers allocated
-        t1 = time.perf_counter()
-        ust = builder.graph_stats("unitigs")
-        t_su = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        fa = builder.fasta("unitigs")
-        t_fa = time.perf_counter() - t1
-    for rep in range(2):  # timed on the second call, its buffers allocated
-        t1 = time.perf_counter()
-        nst = builder.graph_stats("nodes")
-        t_sn = time.perf_counter() - t1
-    stats = {"unitig_stats_ms": round(1e3 * t_su, 3), "unitig_fasta_ms": round(1e3 * t_fa, 3),
-             "fasta_bytes": len(fa), "node_stats_ms": round(1e3 * t_sn, 3),
-             "node_vertices": nst["totals"]["nodes"], "unitig_vertices": ust["totals"]["nodes"],
-             "coverage_cutoff_ms": round(1e3 * t_cut, 3), "coverage_cutoff": mix["cutoff"],
-             "coverage_normal_mean": round(mix["normal_mean"], 3)}
-    clean = {"ms": round(1e3 * t_clean, 3), "min_coverage": 3.0,
-             "unitigs_before": unitigs, "unitigs_after": builder.unitig_count(),
-             "unitig_pass_ms": round(builder.phase_times().get("unitig", 0.0), 3)}
-    # the rest of the reference pipeline on the cleaned graph: TIP_SINGLE_NODE
-    # then BUBBLE, each with its merge
-    devmem.synchronize()
-    t1 = time.perf_counter()
-    builder.clean_tips(-1, emit=False)
-    devmem.synchronize()
-    clean["tips_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
-    clean["unitigs_after_tips"] = builder.unitig_count()
-    t1 = time.perf_counter()
-    builder.clean_bubbles(0.05, -1, emit=False)
-    devmem.synchronize()
-    clean["bubbles_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
-    clean["unitigs_after_bubbles"] = builder.unitig_count()
-    # the cleaned merge's records copied to host memory (a Python bytes)
-    t1 = time.perf_counter()
-    recs = builder.unitig_records()
-    clean["records_to_host_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
-    clean["records_bytes"] = len(recs)
-    del recs
-    # BUBBLE + MERGE on the graph before any cleaning, where sequencing errors
-    # make most bubbles (timed on its second call)
-    for rep in range(2):
-        step()
-        devmem.synchronize()
+        clean["tips_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
+        clean["unitigs_after_tips"] = builder.unitig_count()
         t1 = time.perf_counter()
         builder.clean_bubbles(0.05, -1, emit=False)
         devmem.synchronize()
-        t_rb = time.perf_counter() - t1
-    clean["raw_bubbles"] = {"ms": round(1e3 * t_rb, 3), "unitigs_before": unitigs,
-                            "unitigs_after": builder.unitig_count()}
+        clean["bubbles_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
+        clean["unitigs_after_bubbles"] = builder.unitig_count()
+        # the cleaned merge's records copied to host memory (a Python bytes)
+        t1 = time.perf_counter()
+        recs = builder.unitig_records()
+        clean["records_to_host_ms"] = round(1e3 * (time.perf_counter() - t1), 3)
+        clean["records_bytes"] = len(recs)
+        del recs
+        # BUBBLE + MERGE on the graph before any cleaning, where sequencing errors
+        # make most bubbles (timed on its second call)
+        for rep in range(2):
+            step()
+            devmem.synchronize()
+            t1 = time.perf_counter()
+            builder.clean_bubbles(0.05, -1, emit=False)
+            devmem.synchronize()
+            t_rb = time.perf_counter() - t1
+        clean["raw_bubbles"] = {"ms": round(1e3 * t_rb, 3), "unitigs_before": unitigs,
+                                "unitigs_after": builder.unitig_count()}
     builder.close()
     dtext.free()
     batched = None

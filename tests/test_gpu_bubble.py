@@ -73,6 +73,50 @@ def test_alleles_match_oracle(gx, k):
             assert got == want, (seed, maxd, maxlen)
 
 
+def _multi_alleles(seed, G, L, alts, fracs):
+    """Reads of a genome whose window carries one of several alleles, each
+    in its fraction of the reads (the rest carry the genome's)."""
+    rng = np.random.default_rng(seed)
+    genome = "".join("ACGT"[x] for x in rng.integers(0, 4, G))
+    seqs = [genome] + [alt(genome) for alt in alts]
+    cut = np.cumsum([1.0 - sum(fracs)] + list(fracs))
+    reads = []
+    for i, s in enumerate(range(0, G - L + 1, 2)):
+        which = int(np.searchsorted(cut, ((i * 7) % 100 + 0.5) / 100.0))
+        r = seqs[min(which, len(seqs) - 1)][s:s + L]
+        reads.append(_rc(r) if rng.random() < 0.5 else r)
+    return _readids(reads)
+
+
+def _sub2(s, j):
+    return s[:j] + "ACGT"[("ACGT".index(s[j]) + 2) % 4] + s[j + 1:]
+
+
+@pytest.mark.parametrize("k", [21, 63])
+def test_several_alleles_match_oracle(gx, k):
+    """Four alleles at one position -- the genome's letter, two other
+    letters and a deletion -- make one set of four bubbles between the same
+    two vertices.  At thresholds between 1/(2k-1) (two substitutions apart
+    by one letter) and 1/(2k-2) (the deletion) the first top takes the
+    substitutions and leaves the deletion to the next round of the set."""
+    a = 300
+    alts = [lambda g: _sub(g, a), lambda g: _sub2(g, a), lambda g: g[:a] + g[a + 1:]]
+    f1 = np.float32(1) / np.float32(2 * k - 1)
+    f2 = np.float32(1) / np.float32(2 * k - 2)
+    maxds = [float(np.nextafter(f1, np.float32(0))), float(f1), float((f1 + f2) / np.float32(2)), float(f2), 0.2]
+    second_round = False
+    for seed in (1, 2, 3):
+        text = _multi_alleles(seed, 700, 2 * k + 30, alts, [0.25, 0.2, 0.15])
+        recs = oracle.build_graph(text, k)[0]
+        outs = []
+        for maxd in maxds:
+            got = _gpu(gx, text, k, [("clean_bubbles", (maxd, -1))])
+            assert got == oracle.bubble_merge(recs, k, maxd), (seed, maxd)
+            outs.append(got)
+        second_round |= outs[2] != outs[3]  # the deletion merged only at 1/(2k-2)
+    assert second_round
+
+
 @pytest.mark.parametrize("k,paired", [(21, False), (31, True), (55, True), (24, True)])
 def test_raw_graph_matches_oracle(gx, k, paired):
     """Sequencing errors make many bubbles in the graph before cleaning."""
