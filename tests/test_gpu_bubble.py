@@ -175,3 +175,22 @@ def test_errors(gx):
             b.clean_bubbles(0.05, 5000)
         assert e.value.code == gx.GX_ENOTSUP
         assert b.clean_bubbles(math.nan) == plain  # accepted, as the reference's check lets NaN through; merges nothing
+
+
+def test_handle_reused_after_bubbles(gx):
+    """A build on the same handle after a bubble merge starts from the
+    build's read heads again (no letters left over from the moved ones)."""
+    k = 31
+    text = _sim(91, G=20000)
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        assert b.clean_bubbles(0.2, -1) == oracle.bubble_merge(recs, k, 0.2)
+        b.reset()
+        b.add_reads(text)
+        b.finish()
+        assert b.part_bytes(0) == recs
+        assert b.unitigs() == oracle.unitigs(recs, k)
+        assert b.clean_bad_coverage(2.0) == oracle.clean_bad_coverage(recs, k, 2.0)
