@@ -37,6 +37,7 @@ def test_batch_matches_oracle_per_set(gx, workers):
     with gx.GraphBatch(sets, tuning=tuning) as b:
         out = b.run(unitigs=True, collect=True)
         assert sorted(out) == list(range(len(sets)))
+        assert b.retried() == 0  # small sets: none runs out of memory beside the others
         for i, s in enumerate(sets):
             parts, u = out[i]
             want = oracle.build_graph(s["input"], s["k"], nparts=s["nparts"])
