@@ -136,6 +136,15 @@ def lib():
     return _lib
 
 
+def _host_bytes(n, fill) -> bytes:
+    """n bytes written by fill(char pointer) -> count written, as bytes.  The
+    buffer is not zero-filled first, and the result is its one copy."""
+    import numpy as np
+    arr = np.empty(max(int(n), 1), dtype=np.uint8)
+    w = fill(ctypes.cast(ctypes.c_void_p(arr.ctypes.data), ctypes.c_char_p))
+    return arr[:w].tobytes()
+
+
 class GraphBuilder:
     """One graph-build task on one GPU (a gx_build handle)."""
 
@@ -234,10 +243,12 @@ class GraphBuilder:
 
     def part_bytes(self, part=0) -> bytes:
         _, nb = self.part_info(part)
-        buf = ctypes.create_string_buffer(max(nb, 1))
-        w = ctypes.c_size_t()
-        self._check(lib().gx_build_emit_part(self._h, part, buf, nb, ctypes.byref(w)))
-        return buf.raw[:w.value]
+
+        def fill(buf):
+            w = ctypes.c_size_t()
+            self._check(lib().gx_build_emit_part(self._h, part, buf, nb, ctypes.byref(w)))
+            return w.value
+        return _host_bytes(nb, fill)
 
     def run_unitigs(self):
         """Unitig compression of the built graph; records stay on the device."""
@@ -251,21 +262,18 @@ class GraphBuilder:
     def unitigs(self) -> bytes:
         """Unitig compression of the built graph; returns the merged records."""
         self._check(lib().gx_unitig_run(self._h))
-        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
-        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
-        buf = ctypes.create_string_buffer(max(nb.value, 1))
-        w = ctypes.c_size_t()
-        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
-        return buf.raw[:w.value]
+        return self.unitig_records()
 
     def unitig_records(self) -> bytes:
         """The merged records of the last unitig pass or cleaning step."""
         n, nb = ctypes.c_uint64(), ctypes.c_uint64()
         self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
-        buf = ctypes.create_string_buffer(max(nb.value, 1))
-        w = ctypes.c_size_t()
-        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
-        return buf.raw[:w.value]
+
+        def fill(buf):
+            w = ctypes.c_size_t()
+            self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
+            return w.value
+        return _host_bytes(nb.value, fill)
 
     def clean_bad_coverage(self, min_cov=3.0, max_cov=float("inf"), emit=True):
         """REMOVE_BAD_COVERAGE then MERGE (include/genomix_clean.h) on the
@@ -415,14 +423,7 @@ class _View:
     part_info = GraphBuilder.part_info
     part_bytes = GraphBuilder.part_bytes
     stats = GraphBuilder.stats
-
-    def unitig_bytes(self):
-        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
-        self._check(lib().gx_unitig_info(self._h, ctypes.byref(n), ctypes.byref(nb)))
-        buf = ctypes.create_string_buffer(max(nb.value, 1))
-        w = ctypes.c_size_t()
-        self._check(lib().gx_unitig_emit(self._h, buf, nb.value, ctypes.byref(w)))
-        return buf.raw[:w.value]
+    unitig_bytes = GraphBuilder.unitig_records
 
 
 def load_tuning(path=os.path.join(HERE, "batch_tuning.json")):
