@@ -2241,7 +2241,6 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
                                uint32_t* __restrict__ uedge, uint8_t* __restrict__ uecnt) {
   for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t ent[kUEdges];
-    KV entk[kUEdges];
     int ne = 0;
     uint64_t m = ulen[u];
     for (int end = 0; end < 2; end++) {
@@ -2266,21 +2265,21 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
           uint32_t wletter = (z5 == zfwd) ? 0 : 1;
           uint32_t ut = uletter == 0 ? (wletter == 0 ? ET_FF : ET_FR) : (wletter == 0 ? ET_RF : ET_RR);
           uint32_t rep = zr & 0x7fffffffu;
-          KV rk = node_key<W>(klo, khi, rep);
           bool dup = false;
           for (int a = 0; a < ne; a++) dup |= (ent[a] >> 30) == ut && (ent[a] & 0x3fffffffu) == rep;
           if (!dup && ne < kUEdges) {
-            // insertion: by type, then by neighbour k-mer (VKmer byte order)
+            // insertion: by type, then by neighbour k-mer (VKmer byte order,
+            // not the node order); the keys are read again rather than kept
+            // in a per-thread array, which would live in scratch memory
+            const KV rk = node_key<W>(klo, khi, rep);
             int p = ne;
             while (p > 0) {
               uint32_t pt = ent[p - 1] >> 30;
-              if (pt < ut || (pt == ut && kle<W>(entk[p - 1], rk))) break;
+              if (pt < ut || (pt == ut && kle<W>(node_key<W>(klo, khi, ent[p - 1] & 0x3fffffffu), rk))) break;
               ent[p] = ent[p - 1];
-              entk[p] = entk[p - 1];
               p--;
             }
             ent[p] = (ut << 30) | rep;
-            entk[p] = rk;
             ne++;
           }
         }
@@ -4146,11 +4145,13 @@ __device__ inline bool et_flips_d(uint32_t t) { return t == ET_FR || t == ET_RF;
 
 // Bubbles (detectBubble :245-252, sendBubbleAndMajorVertexMsgToMinorVertex
 // :51-79): one outer edge on each side, to two different merged vertices,
-// and fewer than maxlen letters.  The neighbour with the larger key (node
-// order is key order) is the major vertex.  bmin/bmaj: minor / major unitig
-// (kEnd: not a bubble); btyp: majorToBubble | minorToBubble << 2.
+// and fewer than maxlen letters.  The neighbour whose key is not smaller
+// (VKmer.compareTo, byte order) is the major vertex.  bmin/bmaj: minor /
+// major unitig (kEnd: not a bubble); btyp: majorToBubble | minorToBubble << 2.
+template <int W>
 __global__ void k_bub_cand(uint64_t nu, const uint64_t* __restrict__ ulen, const uint32_t* __restrict__ uedge,
-                           const uint8_t* __restrict__ uecnt, const uint64_t* __restrict__ uid_of_rep, int k,
+                           const uint8_t* __restrict__ uecnt, const uint64_t* __restrict__ uid_of_rep,
+                           const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi, int k,
                            uint64_t maxlen, uint32_t* __restrict__ bmin, uint32_t* __restrict__ bmaj,
                            uint8_t* __restrict__ btyp) {
   for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nu; u += (uint64_t)gridDim.x * blockDim.x) {
@@ -4166,7 +4167,7 @@ __global__ void k_bub_cand(uint64_t nu, const uint64_t* __restrict__ ulen, const
     if (deg[0] == 1 && deg[1] == 1 && ulen[u] + (uint64_t)k - 1 < maxlen) {
       const uint32_t rf = ent[0] & 0x3fffffffu, rr = ent[1] & 0x3fffffffu;
       if (rf != rr) {
-        const bool fwd_major = rf >= rr;  // VKmer.compareTo of the two neighbours
+        const bool fwd_major = kle<W>(node_key<W>(klo, khi, rr), node_key<W>(klo, khi, rf));
         const uint32_t emaj = fwd_major ? ent[0] : ent[1], emin = fwd_major ? ent[1] : ent[0];
         mj = (uint32_t)uid_of_rep[emaj & 0x3fffffffu];
         mn = (uint32_t)uid_of_rep[emin & 0x3fffffffu];
@@ -4197,7 +4198,8 @@ __device__ inline uint32_t bub_class(const BubCtx& c, uint32_t w) {
 // The bubbles whose minor vertex is v, in processing order: by major
 // vertex, then by class (bubbles of different classes never meet in
 // processSimilarSet), then by decreasing coverage (SortByCoverage, stable;
-// before it the minor's message order, taken as increasing key).
+// before it the minor's message order, taken as the merged records' order,
+// i.e. node order).
 __device__ inline uint32_t bub_set(const BubCtx& c, uint64_t v, uint32_t* w) {
   uint32_t n = 0;
   for (int e = 0; e < c.uecnt[v]; e++) {
@@ -4458,8 +4460,8 @@ static void bubble_graph(gx_build* b, float maxd, uint64_t maxlen) {
   const uint64_t* khi = W == 2 ? b->khi.as<uint64_t>() : nullptr;
   uint64_t npairs = 0;
   if (nu) {
-    hipLaunchKernelGGL(k_bub_cand, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c.ulen, c.uedge, c.uecnt,
-                       c.uid_of_rep, k, maxlen, bmin, bmaj, btyp);
+    hipLaunchKernelGGL(k_bub_cand<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c.ulen, c.uedge,
+                       c.uecnt, c.uid_of_rep, klo, khi, k, maxlen, bmin, bmaj, btyp);
     GX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bub_pairs, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, c, np, nullptr, nullptr);
     GX_HIP(hipGetLastError());

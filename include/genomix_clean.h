@@ -54,8 +54,8 @@ int gx_clean_tips(gx_build* b, int max_len);
  * GenomixJobConf.java:478-482): 0.05 and 5k; max_len < 0 means 5k.
  * max_dissimilarity outside [0, 1] fails with GX_EINVAL (:429-432);
  * max_len above 4097 with GX_ENOTSUP.  Ties in coverage are broken by
- * increasing key (the reference's order there is its message arrival
- * order).  Same calling rules as gx_clean_bad_coverage. */
+ * the order of the merged records (KmerPointable order of their keys; the
+ * reference's order there is its message arrival order).  Same calling rules as gx_clean_bad_coverage. */
 int gx_clean_bubbles(gx_build* b, float max_dissimilarity, int max_len);
 
 #ifdef __cplusplus

@@ -1346,7 +1346,7 @@ static inline bool et_flips(int t) { return t == FR || t == RF; }
 // same minor and major form a set, sorted by decreasing average coverage
 // (SimpleBubbleMergeMessage.SortByCoverage; Collections.sort is stable, and
 // the set's order before it -- the minor's message arrival order -- is
-// taken here as increasing bubble key).  processSimilarSet: the first
+// taken here as the merged records' order).  processSimilarSet: the first
 // bubble is the top; every later one attached to the same sides of major
 // and minor (isValidMajorAndMinor) whose fraction of dissimilarity to it
 // (edit distance, against the reverse complement when the two are stored
@@ -1377,7 +1377,7 @@ static std::vector<uint8_t> bubble_nodes(const uint8_t* recs, size_t n, int k, f
     float cov;
   };
   std::map<std::pair<std::vector<uint8_t>, std::vector<uint8_t>>, std::vector<Msg>> sets;  // (minor, major)
-  for (size_t v = 0; v < mv.size(); v++) {  // detectBubble :245-252 (mv is in increasing key order)
+  for (size_t v = 0; v < mv.size(); v++) {  // detectBubble :245-252 (mv in record order)
     const Node& x = mv[v].node;
     int deg[2] = {0, 0};
     for (int t = 0; t < 4; t++) deg[et_side(t)] += (int)x.edges[t].size();
