@@ -1972,7 +1972,7 @@ __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uin
 // thread writes its record into LDS, then the block stores the whole span
 // with coalesced 4-byte writes.  Spans larger than the staging buffer are
 // written directly.
-constexpr int kSerStage = 30720;
+constexpr int kSerStage = 16384;  // 256 node records average ~60 bytes; a smaller stage keeps 8 waves per SIMD
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* __restrict__ roff,
@@ -2443,6 +2443,7 @@ __global__ void k_useq_size(USerCtx c, uint64_t* __restrict__ rsize) {
 // per unitig, lanes over bytes); k_useq_write leaves their bytes to it and
 // records where they start.
 constexpr uint64_t kUseqLong = 256;
+constexpr int kUSerStage = 30720;  // LDS stage of 256 unitig records (longer than node records)
 
 template <int W>
 __device__ inline uint8_t useq_byte(const USerCtx& c, const uint32_t* ord, uint64_t len, uint64_t q) {
@@ -2571,13 +2572,13 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t* __restrict__ roff,
                                                        uint8_t* __restrict__ out, uint64_t* __restrict__ seq_at,
                                                        uint64_t* __restrict__ head_at) {
-  __shared__ __attribute__((aligned(16))) uint8_t stage[kSerStage + 8];
+  __shared__ __attribute__((aligned(16))) uint8_t stage[kUSerStage + 8];
   const uint64_t first = (uint64_t)blockIdx.x * kBlock;
   if (first >= c.nu) return;
   const uint64_t last = first + kBlock < c.nu ? first + kBlock : c.nu;
   const uint64_t s0 = roff[first], s1 = roff[last];
   const uint64_t base = s0 & ~3ull;
-  const bool staged = (s1 - base) <= kSerStage;
+  const bool staged = (s1 - base) <= kUSerStage;
   const uint64_t u = first + threadIdx.x;
   if (u < last)
     write_unitig_record<W>(c, u, staged ? stage + (roff[u] - base) : out + roff[u], roff[u + 1] - roff[u], roff[u],
