@@ -2727,7 +2727,7 @@ struct gx_build {
   std::vector<uint64_t> part_rec, part_off;  // per partition: first record, first byte (P + 1 entries)
 };
 
-static std::string g_create_err;
+static thread_local std::string g_create_err;  // errors of calls without a handle (per thread)
 
 enum Phase {
   PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX,

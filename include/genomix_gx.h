@@ -148,7 +148,9 @@ const char* gx_build_phase_name(int i);
 /* Drops the reads and results but keeps device allocations for reuse. */
 int gx_build_reset(gx_build* b);
 
-/* Message of the last failing call on this handle (or of gx_build_create when b is NULL). */
+/* Message of the last failing call on this handle; with b NULL, of the
+ * calling thread's last failing call that takes no handle (gx_build_create,
+ * gx_fit_mixture). */
 const char* gx_last_error(const gx_build* b);
 
 void gx_build_destroy(gx_build* b);
