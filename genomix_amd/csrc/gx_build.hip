@@ -2132,14 +2132,18 @@ __global__ void k_pj_init(uint64_t nn, const uint32_t* __restrict__ lnk, const u
 __global__ void __launch_bounds__(kBlock) k_pj_round(uint64_t ne, const uint4* __restrict__ Rc,
                                                      uint4* __restrict__ Rn, uint8_t* __restrict__ bflag) {
   int any = 0;
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4* __restrict__ Rcv = reinterpret_cast<const u32x4*>(Rc);
+  u32x4* __restrict__ Rnv = reinterpret_cast<u32x4*>(Rn);
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (uint64_t)gridDim.x * blockDim.x) {
-    uint4 r = Rc[e];
+    // own records stream through (nontemporal), successors are random reads
+    u32x4 r = __builtin_nontemporal_load(Rcv + e);
     if (r.x != kEnd) {
-      uint4 q = Rc[r.x];
-      r = make_uint4(q.x, r.y + q.y, min(r.z, q.z), q.w);
+      const uint4 q = Rc[r.x];
+      r = u32x4{q.x, r.y + q.y, min(r.z, q.z), q.w};
       any |= q.x != kEnd;
     }
-    Rn[e] = r;
+    __builtin_nontemporal_store(r, Rnv + e);
   }
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) bflag[blockIdx.x] = any ? 1 : 0;
