@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -257,6 +259,14 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const KeyT* __restrict__ 
 // 256 consecutive elements; inside a round the rank of an element among equal
 // digits comes from a wave-level match (8 ballots) plus the counts of the
 // lower waves, so ties keep their input order.
+// A load of data read once, in order: nontemporal for integer words, so
+// the stream does not evict lines other accesses reuse.
+template <class T>
+__device__ inline T load_stream(const T* p) {
+  if constexpr (std::is_integral<T>::value) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 template <class KeyT, bool kVals, class ValT = uint32_t>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict__ keys, const ValT* __restrict__ vals,
                                                           size_t n, int byte, const uint64_t* __restrict__ offs,
@@ -277,8 +287,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict
     ValT val = 0;
     uint32_t d = 0;
     if (valid) {
-      key = keys[j];
-      if (kVals) val = vals[j];
+      key = load_stream(keys + j);
+      if (kVals) val = load_stream(vals + j);
       d = key_byte(key, byte);
     }
     uint64_t act = __ballot(valid);
