@@ -314,7 +314,7 @@ class GraphBuilder:
             out[name + "-bins"] = {str(v): c for v, c in zip(vals, cnts)}
             out["totals"][name] = total.value
             out["maximum"][name] = mx.value
-        return out
+        return {g: c for g, c in out.items() if c}  # a group with no counter does not exist
 
     def coverage_cutoff(self, graph="nodes", iterations=10) -> dict:
         """The driver's coverage cutoff (include/genomix_stats.h,

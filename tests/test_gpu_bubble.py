@@ -194,3 +194,24 @@ def test_handle_reused_after_bubbles(gx):
         assert b.part_bytes(0) == recs
         assert b.unitigs() == oracle.unitigs(recs, k)
         assert b.clean_bad_coverage(2.0) == oracle.clean_bad_coverage(recs, k, 2.0)
+
+
+@pytest.mark.parametrize("text", [b"", b"1\tACGTACGTACGTTGCAACGTAG\n2\tTTGACCATGACGTAGGCAATCA\n",
+                                  b"1\tACGTACGTACGTTGCAACGTACGGT\n"])
+def test_graph_steps_on_empty_and_tiny_graphs(gx, text):
+    """The steps after the build on a graph with no node, with two short
+    reads, and with one read: the same records, statistics and FASTA as the
+    oracle."""
+    k = 21
+    recs = oracle.build_graph(text, k)[0]
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        assert b.part_bytes(0) == recs
+        assert b.unitigs() == oracle.unitigs(recs, k)
+        assert b.clean_bad_coverage(1.0) == oracle.clean_bad_coverage(recs, k, 1.0)
+        assert b.clean_tips(-1) == oracle.clean_coverage_then_tips(recs, k, 1.0)
+        final = b.clean_bubbles(0.05, -1)
+        assert final == oracle.clean_pipeline(recs, k, min_cov=1.0)
+        assert b.graph_stats("unitigs") == oracle.graph_stats(final, k)
+        assert b.fasta("unitigs") == oracle.dump_fasta(final, k)
