@@ -215,3 +215,20 @@ def test_graph_steps_on_empty_and_tiny_graphs(gx, text):
         assert final == oracle.clean_pipeline(recs, k, min_cov=1.0)
         assert b.graph_stats("unitigs") == oracle.graph_stats(final, k)
         assert b.fasta("unitigs") == oracle.dump_fasta(final, k)
+
+
+def test_node_graph_after_bubbles(gx):
+    """The node graph a bubble merge leaves (the kept bubble's first node
+    holding the removed ones' counts, moved heads on the nodes covering
+    them) has the oracle's statistics and FASTA."""
+    k = 31
+    text = _sim(93, G=30000, cov=25)
+    recs = oracle.build_graph(text, k)[0]
+    nodes = oracle.bubble_nodes(recs, k, 0.1)
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        b.clean_bubbles(0.1, -1, emit=False)
+        assert b.graph_stats("nodes") == oracle.graph_stats(nodes, k)
+        assert b.fasta("nodes") == oracle.dump_fasta(nodes, k)
