@@ -258,8 +258,8 @@ def main():
                     help="also run one 1 GiB device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; the sharded design is described in "
-                                   "DESIGN.md and no multi-GPU figure is produced", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: the build would shard by partition of the k-mer hash across "
+                                   "GPUs, but no multi-GPU path is built or measured", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from genomix_amd import devmem, gx, sim
