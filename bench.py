@@ -347,7 +347,7 @@ def main():
     clean = stats = None
     if not args.no_graph_steps:
         # the steps after the build on the graph in HBM (untimed in value)
-        # the driver's coverage cutoff over the build's node graph: statistics,
+        # GenomixDriver's coverage cutoff over the build's node graph: statistics,
         # then the mixture fit (timed on the second call)
         for rep in range(2):
             devmem.synchronize()

@@ -4,7 +4,7 @@ The hot path runs in hand-written HIP kernels for gfx950 behind a C ABI
 (include/genomix_gx.h, libgenomix_gx.so); this package is the host-side
 mirror of the reference's graph-build entry (GenomixHyracksDriver.runJob
 with Plan.BUILD_DEBRUIJN_GRAPH) plus the SequenceFile writer around it, and
-of the driver steps on the built graph: cleaning (REMOVE_BAD_COVERAGE,
+of the GenomixDriver steps on the built graph: cleaning (REMOVE_BAD_COVERAGE,
 TIP_SINGLE_NODE, BUBBLE), statistics, FASTA and the coverage cutoff
 (GraphBuilder methods; fit_mixture for binned data).
 """

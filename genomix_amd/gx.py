@@ -318,10 +318,10 @@ class GraphBuilder:
 
     def coverage_cutoff(self, graph="nodes", iterations=10) -> dict:
         """The driver's coverage cutoff (include/genomix_stats.h,
-        GenomixDriver.java:119-136): statistics of `graph` (the driver uses
+        GenomixDriver.java:119-136): statistics of `graph` (GenomixDriver uses
         the build's node graph), then the exponential + normal mixture fit of
         their coverage counters.  Returns MIXTURE_KEYS -> value; a positive
-        "cutoff" is the min_cov the driver hands to REMOVE_BAD_COVERAGE."""
+        "cutoff" is the min_cov GenomixDriver hands to REMOVE_BAD_COVERAGE."""
         self._check(lib().gx_graph_stats(self._h, GRAPHS[graph]))
         a = (ctypes.c_double * len(MIXTURE_KEYS))()
         self._check(lib().gx_coverage_cutoff(self._h, iterations, a))

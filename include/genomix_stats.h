@@ -6,7 +6,7 @@
  * rounded average coverage, read-head counts and seed score, binned,
  * summed and maximised, plus the node, path-node, tip and self-edge
  * totals.  The FASTA dump is DriverUtils.dumpGraph (DriverUtils.java:85-128).
- * The coverage cutoff is the driver's FittingMixture step over the coverage
+ * The coverage cutoff is GenomixDriver's FittingMixture step over the coverage
  * counters.
  *
  * All functions return GX_OK (0) or a negative GX_* code (genomix_gx.h).
@@ -67,7 +67,7 @@ int gx_graph_stat_bins(gx_build* b, int stat, int64_t* values, uint64_t* counts,
  * an exponential (erroneous k-mers) plus a normal (genomic k-mers) mixture
  * to the coverage values -- initial exponential mean 5, normal mean 20 and
  * sd 5, priors 1/2, memberships floored at 1e-9, `iterations` M+E rounds
- * (the driver uses 10) -- and the first integer coverage c in [1, max) at
+ * (GenomixDriver uses 10) -- and the first integer coverage c in [1, max) at
  * which prior * density of the normal exceeds that of the exponential, or 0
  * when there is none.  Densities are commons-math3 3.2's.
  * out[GX_MIXTURE_OUT]: [0] cutoff, [1] exponential mean, [2] normal mean,
@@ -78,7 +78,7 @@ int gx_graph_stat_bins(gx_build* b, int stat, int64_t* values, uint64_t* counts,
  * fits the build's node graph, GX_GRAPH_NODES), max being the largest
  * coverage value (GraphStatistics.getCoverageStats / getMaxCoverage,
  * GraphStatistics.java:165-199).  GX_ESTATE without gx_graph_stats or with
- * no coverage counted (the driver's "No information for coverage!"). */
+ * no coverage counted (GenomixDriver's "No information for coverage!"). */
 int gx_coverage_cutoff(gx_build* b, int iterations, double* out);
 
 /* The same fit over n distinct values with their counts (the reference

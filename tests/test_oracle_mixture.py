@@ -77,7 +77,7 @@ def test_fixed_point_is_stationary(seed, params):
 
 
 def test_likelihood_rises_over_the_reference_iteration_count():
-    """Ten rounds, as the driver runs, from the fixed initial guess: the
+    """Ten rounds, as GenomixDriver runs, from the fixed initial guess: the
     likelihood after ten is above that after one."""
     d = _sample(5, 8000, 2.5, 12000, 35.0, 6.0)
     r1 = oracle.fitting_mixture(d, d.max(), iterations=1)
