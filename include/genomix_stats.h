@@ -74,7 +74,7 @@ int gx_graph_stat_bins(gx_build* b, int stat, int64_t* values, uint64_t* counts,
  * [3] normal sd, [4] exponential prior, [5] normal prior. */
 #define GX_MIXTURE_OUT 6
 
-/* Over the "coverage-bins" counters of the last gx_graph_stats (the driver
+/* Over the "coverage-bins" counters of the last gx_graph_stats (GenomixDriver
  * fits the build's node graph, GX_GRAPH_NODES), max being the largest
  * coverage value (GraphStatistics.getCoverageStats / getMaxCoverage,
  * GraphStatistics.java:165-199).  GX_ESTATE without gx_graph_stats or with
