@@ -1999,7 +1999,8 @@ __global__ void __launch_bounds__(kBlock) k_ser_write(SerCtx c, const uint64_t* 
   for (uint64_t x = e4 + threadIdx.x; x < s1; x += kBlock) out[x] = stage[x - base];
   const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
   uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
-  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
+  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock)
+    __builtin_nontemporal_store(src[x], dst + x);  // written once, read by the caller later
 }
 
 // Output partition of each node: nonneg(hashBytes(key)) % P, the reference's
@@ -2595,7 +2596,8 @@ __global__ void __launch_bounds__(kBlock) k_useq_write(USerCtx c, const uint64_t
   for (uint64_t x = e4 + threadIdx.x; x < s1; x += kBlock) out[x] = stage[x - base];
   const uint32_t* src = reinterpret_cast<const uint32_t*>(stage + (a4 - base));
   uint32_t* dst = reinterpret_cast<uint32_t*>(out + a4);
-  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock) dst[x] = src[x];
+  for (uint64_t x = threadIdx.x; x < (e4 - a4) / 4; x += kBlock)
+    __builtin_nontemporal_store(src[x], dst + x);  // written once, read by the caller later
 }
 
 // Heads of unitigs with many of them: a wave per unitig; each pass places
