@@ -1142,6 +1142,12 @@ static float path_coverage(const std::vector<GNode>& g, const std::vector<Elem>&
   return (float)(csum / (double)path.size());
 }
 
+// Test hook: 0 = the restatement; 1 = FR and RF swapped in the walker's
+// outer-edge types; 2 = resolve_edge_through_path without the mirror of its
+// first argument; 3 = the fold's read-id merge without the offset flip.
+// Planted mistakes for negative controls only.
+static int g_unitig_plant = 0;
+
 static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
   const std::vector<GNode> g = parse_graph(recs, n, k);
   const size_t N = g.size();
@@ -1219,6 +1225,7 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
           bool z5 = (t == FF || t == RF);  // touches the neighbour's 5' end
           int wletter = (z5 == (bool)ori[zi]) ? 0 : 1;
           int ut = uletter == 0 ? (wletter == 0 ? FF : FR) : (wletter == 0 ? RF : RR);
+          if (g_unitig_plant == 1 && (ut == FR || ut == RF)) ut = ut == FR ? RF : FR;  // planted mistake
           edges.insert({ut, g[ukey[comp_of[zi]]].key.b});
         }
       }
@@ -1233,6 +1240,231 @@ static std::vector<uint8_t> unitigs(const uint8_t* recs, size_t n, int k) {
     VKmer key = g[ukey[c]].key;
     std::vector<uint8_t> kb, vb;
     key.write(kb);
+    u.write(vb);
+    put_be32(out, (uint32_t)(kb.size() + vb.size()));
+    put_be32(out, (uint32_t)kb.size());
+    out.insert(out.end(), kb.begin(), kb.end());
+    out.insert(out.end(), vb.begin(), vb.end());
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Second restatement of unitig compression: P4's pairwise merges replayed
+// one at a time.  The walker above writes each unitig's sequence, read
+// heads and outer edges in closed form.  This fold instead absorbs the
+// nodes of every path, one by one, into the path's survivor with the
+// reference's own merge operations, each restated from its source and
+// pinned by the reference's unit-test vectors:
+//   EDGETYPE.resolveEdgeThroughPath (EDGETYPE.java:130-188; vectors
+//     NodeFixedTest.java:128-145),
+//   Node.mergeEdges (Node.java:632-707; tables NodeRandomTest.java:274-340),
+//   VKmer.mergeWith{FF,FR,RF,RR}Kmer (VKmer.java:525-692; vectors
+//     VKmerFixedTest.java:131-452),
+//   Node.mergeUnflippedAndFlippedReadIDs (Node.java:727-806) with
+//     ReadHeadSet.prependOffsets / flipOffset (ReadHeadSet.java:109-130),
+//   and the neighbour update of BasicPathMergeVertex.updateNeighbors /
+//     receiveUpdates (BasicPathMergeVertex.java:104-160): when B merges into
+//     C, every neighbour A on B's far side replaces its edge to B by an edge
+//     to C of type resolveEdgeThroughPath(B->A, B->C).
+// The type of every link is read from the evolving edge lists, never
+// derived from orientations, so the fold checks the walker's closed form.
+// Path membership and the survivor rule are the walker's (merge_paths).
+// Coverage is the walker's f32(sum / k-mers): P4 averages pairwise in float
+// in a random merge order, which changes rounding only.
+// ---------------------------------------------------------------------------
+
+static inline int et_first(int t) { return t >> 1; }  // 0 = F, 1 = R
+static inline int et_second(int t) { return t & 1; }
+static inline int et_make(int a, int b) { return (a << 1) | b; }
+
+// EDGETYPE.resolveEdgeThroughPath: A <-(BtoA)- B -(BtoC)-> C gives A -> C.
+// The path is valid only when the inner letters of A->B and B->C agree; the
+// result takes the first letter of A->B and the second of B->C.
+static int resolve_edge_through_path(int BtoA, int BtoC) {
+  const int AtoB = g_unitig_plant == 2 ? BtoA : et_mirror(BtoA);
+  if (et_second(AtoB) != et_first(BtoC)) throw OraError(E_INVAL, "invalid link type through a path");
+  return et_make(et_first(AtoB), et_second(BtoC));
+}
+
+// VKmer.mergeWith{FF,FR,RF,RR}Kmer(initialKmerSize, other) on letters: the
+// two share initialKmerSize - 1 letters.  FF appends other, FR appends its
+// reverse complement, RR prepends other, RF prepends its reverse complement.
+static std::string vkmer_merge(int t, int initial, const std::string& self, const std::string& other) {
+  const size_t ov = (size_t)(initial - 1);
+  if (initial < 1 || self.size() < ov || other.size() < ov)
+    throw OraError(E_INVAL, "not enough letters in the kmers to perform a merge");
+  switch (t) {
+    case FF: return self + other.substr(ov);
+    case FR: return self + rc_string(other).substr(ov);
+    case RF: {
+      const std::string r = rc_string(other);
+      return r.substr(0, r.size() - ov) + self;
+    }
+    default: return other.substr(0, other.size() - ov) + self;
+  }
+}
+
+// Node.mergeEdges(edgeType, other): the side of `self` named by the first
+// letter of edgeType takes over other's lists on other's far side, each
+// list re-typed through the path (FF keeps FF/FR, FR maps RF->FF and
+// RR->FR, RF maps FF->RF and FR->RR, RR keeps RF/RR).
+static void node_merge_edges(int t, Node& self, const Node& other) {
+  const int myside = et_first(t);
+  const int far = 1 - et_first(et_mirror(t));  // other's side away from self
+  for (int u = 0; u < 4; u++) {
+    if (et_first(u) != myside) continue;
+    self.edges[u].clear();
+    self.edges_present[u] = false;
+  }
+  for (int u = 0; u < 4; u++) {
+    if (et_first(u) != far || !other.edges_present[u]) continue;
+    const int mine = resolve_edge_through_path(et_mirror(t), u);
+    self.edges[mine] = other.edges[u];
+    self.edges_present[mine] = true;
+  }
+}
+
+// Node.mergeUnflippedAndFlippedReadIDs(edgeType, other, otherLength) with
+// thisLength = the letters of self before the merge.
+static void node_merge_heads(int t, Node& self, const Node& other, int self_len, int other_len, int K) {
+  auto shifted = [](std::vector<ReadHead> s, int by, bool flip) {  // prependOffsets / flipOffset
+    for (auto& h : s) {
+      const int o = flip ? by - h.offset() : by + h.offset();
+      h.uuid = make_uuid(h.mate_id(), h.lib(), h.read_id(), o);
+    }
+    return s;
+  };
+  auto add_all = [](Node& n, bool flipped_set, const std::vector<ReadHead>& s) {
+    if (s.empty()) return;
+    auto& dst = flipped_set ? n.flipped : n.unflipped;
+    (flipped_set ? n.has_flipped : n.has_unflipped) = true;
+    for (auto& h : s) treeset_add(dst, h);
+  };
+  const bool keep = g_unitig_plant == 3;  // planted: no flip of the other's offsets
+  switch (t) {
+    case FF:
+      add_all(self, false, shifted(other.unflipped, self_len - K + 1, false));
+      add_all(self, true, shifted(other.flipped, self_len - K + 1, false));
+      break;
+    case FR:
+      add_all(self, !keep, shifted(other.unflipped, self_len - K + other_len, !keep));
+      add_all(self, keep, shifted(other.flipped, self_len - K + other_len, !keep));
+      break;
+    case RF:
+    case RR: {
+      self.unflipped = shifted(self.unflipped, other_len - K + 1, false);
+      self.flipped = shifted(self.flipped, other_len - K + 1, false);
+      if (t == RF) {
+        add_all(self, !keep, shifted(other.unflipped, other_len - 1, !keep));
+        add_all(self, keep, shifted(other.flipped, other_len - 1, !keep));
+      } else {
+        add_all(self, false, other.unflipped);
+        add_all(self, true, other.flipped);
+      }
+      break;
+    }
+  }
+}
+
+static bool list_remove(std::vector<VKmer>& l, const VKmer& x) {
+  for (size_t i = 0; i < l.size(); i++)
+    if (l[i] == x) { l.erase(l.begin() + (long)i); return true; }
+  return false;
+}
+static void list_add(Node& n, int t, const VKmer& x) {  // VKmerList.unionUpdate as a set
+  n.edges_present[t] = true;
+  for (auto& y : n.edges[t])
+    if (y == x) return;
+  n.edges[t].push_back(x);
+}
+
+static std::vector<uint8_t> unitigs_p4(const uint8_t* recs, size_t n, int k) {
+  const std::vector<GNode> g = parse_graph(recs, n, k);
+  const size_t N = g.size();
+  const Paths P = merge_paths(g);
+  std::vector<Node> cur(N);
+  std::vector<std::string> seq(N);
+  std::vector<double> covsum(N);
+  for (size_t i = 0; i < N; i++) {
+    cur[i] = g[i].node;
+    seq[i] = g[i].key.str();  // NodeToGenericVertexInputFormat.java:56: internal kmer := id
+    covsum[i] = g[i].node.cov;
+  }
+  auto find_link = [&](size_t s, int side, const VKmer& b) {
+    int found = -1;
+    for (int t = 0; t < 4; t++)
+      if (et_first(t) == side)
+        for (auto& z : cur[s].edges[t])
+          if (z == b) {
+            if (found >= 0) throw OraError(E_INPUT, "a merge link is listed twice");
+            found = t;
+          }
+    if (found < 0) throw OraError(E_INPUT, "a merge link is missing from the survivor's edges");
+    return found;
+  };
+  // B merges into the survivor S through S's link of type t to B.
+  auto absorb = [&](size_t s, size_t b, int t) {
+    const VKmer& sk = g[s].key;
+    const VKmer& bk = g[b].key;
+    // B's own type towards S (the message's mergeEdgetype.mirror()).
+    int btos = -1;
+    for (int u = 0; u < 4; u++)
+      for (auto& z : cur[b].edges[u])
+        if (z == sk && et_first(u) == et_first(et_mirror(t))) btos = u;
+    if (btos < 0) throw OraError(E_INPUT, "merge link not mirrored");
+    // updateNeighbors / receiveUpdates: B's far-side neighbours now point at S.
+    const int far = 1 - et_first(btos);
+    for (int upd = 0; upd < 4; upd++) {
+      if (et_first(upd) != far) continue;
+      for (const VKmer& a : std::vector<VKmer>(cur[b].edges[upd])) {
+        const size_t ai = P.idx.at(a.b);
+        list_remove(cur[ai].edges[et_mirror(upd)], bk);
+        list_add(cur[ai], resolve_edge_through_path(upd, btos), sk);
+      }
+    }
+    // Node.mergeWithNode(t, B): edges, read ids, coverage, internal kmer.
+    node_merge_edges(t, cur[s], cur[b]);
+    node_merge_heads(t, cur[s], cur[b], (int)seq[s].size(), (int)seq[b].size(), k);
+    covsum[s] += covsum[b];
+    seq[s] = vkmer_merge(t, k, seq[s], seq[b]);
+  };
+  std::vector<char> survivor(N, 0);
+  for (const auto& path : P.comps) {
+    size_t s = path[0].node, mpos = 0;
+    for (size_t q = 0; q < path.size(); q++)
+      if (path[q].node < s) { s = path[q].node; mpos = q; }
+    survivor[s] = 1;
+    if (!path[mpos].fwd) throw OraError(E_INPUT, "survivor does not read forward");
+    // extend the 3' end (F side), then the 5' end (R side)
+    for (size_t q = mpos + 1; q < path.size(); q++) {
+      const size_t b = path[q].node;
+      absorb(s, b, find_link(s, 0, g[b].key));
+    }
+    for (size_t q = mpos; q-- > 0;) {
+      const size_t b = path[q].node;
+      absorb(s, b, find_link(s, 1, g[b].key));
+    }
+  }
+  std::vector<uint8_t> out;
+  for (size_t s = 0; s < N; s++) {
+    if (!survivor[s]) continue;
+    Node u = cur[s];
+    const size_t m = seq[s].size() - (size_t)k + 1;
+    // GenericVertexToNodeOutputFormat.java:44-47: no internal kmer at length k
+    u.has_internal = m > 1;
+    u.internal = m > 1 ? VKmer::from_string((const uint8_t*)seq[s].data(), seq[s].size()) : VKmer();
+    u.has_cov = true;
+    u.cov = (float)(covsum[s] / (double)m);
+    for (int t = 0; t < 4; t++) {  // lists ascending, as the walker writes them
+      std::sort(u.edges[t].begin(), u.edges[t].end(),
+                [](const VKmer& a, const VKmer& b) { return a.b < b.b; });
+      u.edges_present[t] = !u.edges[t].empty();
+    }
+    u.has_unflipped = !u.unflipped.empty();
+    u.has_flipped = !u.flipped.empty();
+    std::vector<uint8_t> kb, vb;
+    g[s].key.write(kb);
     u.write(vb);
     put_be32(out, (uint32_t)(kb.size() + vb.size()));
     put_be32(out, (uint32_t)kb.size());
@@ -1308,6 +1540,8 @@ static bool short_tip(const GNode& v, int k, int maxlen) {
 }
 
 static std::vector<uint8_t> clean_tips(const uint8_t* recs, size_t n, int k, int maxlen) {
+  // GenomixJobConf.java:449-450
+  if (maxlen < k) throw OraError(E_INVAL, "tipRemove_maxLength must be at least as long as kmerLength");
   return clean_graph(recs, n, k, [&](const GNode& v) { return short_tip(v, k, maxlen); });
 }
 
@@ -1873,6 +2107,78 @@ int ora_unitigs(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* out
     auto o = unitigs(recs, n, k);
     *outlen = o.size();
     *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+int ora_unitigs_p4(const uint8_t* recs, size_t n, int k, uint8_t** out, size_t* outlen) {
+  return guard([&] {
+    auto o = unitigs_p4(recs, n, k);
+    *outlen = o.size();
+    *out = dup_bytes(o.data(), o.size());
+  });
+}
+
+// Test hook for planted mistakes in the unitig restatements (0 = none).
+void ora_set_unitig_plant(int m) { g_unitig_plant = m; }
+
+// EDGETYPE.resolveEdgeThroughPath on type codes FF=0 FR=1 RF=2 RR=3;
+// returns -1 (and sets the error) for an invalid path.
+int ora_resolve_edge(int BtoA, int BtoC) {
+  int r = -1;
+  if (guard([&] {
+        if (BtoA < 0 || BtoA > 3 || BtoC < 0 || BtoC > 3) throw OraError(E_INVAL, "bad edge type");
+        r = resolve_edge_through_path(BtoA, BtoC);
+      }) != 0)
+    return -1;
+  return r;
+}
+
+// VKmer.mergeWith{FF,FR,RF,RR}Kmer on letter strings.
+int ora_vkmer_merge(int t, int initial, const char* self, const char* other, char* out, size_t cap) {
+  return guard([&] {
+    if (t < 0 || t > 3) throw OraError(E_INVAL, "bad edge type");
+    auto s = vkmer_merge(t, initial, self, other);
+    if (s.size() + 1 > cap) throw OraError(E_INVAL, "buffer too small");
+    memcpy(out, s.c_str(), s.size() + 1);
+  });
+}
+
+// Node.mergeEdges(t, other) on edge lists written "FF|FR|RF|RR", each list
+// comma separated (an empty field = null list); writes self's lists back
+// in the same form.
+int ora_merge_edges(int t, const char* self, const char* other, char* out, size_t cap) {
+  return guard([&] {
+    if (t < 0 || t > 3) throw OraError(E_INVAL, "bad edge type");
+    auto parse = [](const char* s) {
+      Node nd;
+      int field = 0;
+      std::string cur;
+      for (const char* p = s;; p++) {
+        if (*p == ',' || *p == '|' || *p == 0) {
+          if (!cur.empty()) {
+            if (field > 3) throw OraError(E_INVAL, "more than four lists");
+            nd.edges[field].push_back(VKmer::from_string((const uint8_t*)cur.data(), cur.size()));
+            nd.edges_present[field] = true;
+          }
+          cur.clear();
+          if (*p == '|') field++;
+          if (!*p) break;
+        } else {
+          cur.push_back(*p);
+        }
+      }
+      return nd;
+    };
+    Node a = parse(self);
+    const Node b = parse(other);
+    node_merge_edges(t, a, b);
+    std::string o;
+    for (int e = 0; e < 4; e++) {
+      if (e) o += "|";
+      for (size_t i = 0; i < a.edges[e].size(); i++) o += (i ? "," : "") + a.edges[e][i].str();
+    }
+    if (o.size() + 1 > cap) throw OraError(E_INVAL, "buffer too small");
+    memcpy(out, o.c_str(), o.size() + 1);
   });
 }
 

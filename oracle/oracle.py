@@ -31,10 +31,24 @@ cannot be run; nothing was denied, it is simply unbuildable):
   fixtures (tests/golden/seqfile/**): header fields, sync cadence and CRC
   chunks are recomputed from their bytes.
 * FASTQ -> readids (readId = 4i+2): source only (GenomixDriver.java:665-714).
-* Unitig walker: VKmerFixedTest merge vectors for the overlap concatenation,
-  the RandMergeOffsetTest invariant (one genome -> one unitig equal to it or
-  its reverse complement, read-head offsets at read positions), and brute
-  force on small graphs.
+* Unitig walker (``unitigs``), pinned three ways
+  (tests/test_oracle_merge_pins.py, tests/test_oracle_unitigs.py):
+  - a second restatement, ``unitigs_p4``, replays P4's pairwise merges with
+    the reference's own primitives, each checked against the reference's
+    unit-test vectors: EDGETYPE.resolveEdgeThroughPath
+    (NodeFixedTest.java:128-145, all 16 type pairs), Node.mergeEdges
+    (NodeRandomTest.java:274-340) and VKmer.mergeWith{FF,FR,RF,RR}Kmer
+    (VKmerFixedTest.java:131-452); its bytes equal the walker's on random
+    read graphs and complete overlap graphs (k = 3..21, cycles, tandems,
+    palindromes);
+  - brute force from sequences alone: in graphs whose edges are every
+    (k-1)-overlap, each unitig's four edge lists equal the overlaps of the
+    oriented unitig sequences; k-mer membership equals a union-find of
+    mergeable links;
+  - the RandMergeOffsetTest invariant (one genome -> one unitig equal to it
+    or its reverse complement, read-head offsets at read positions).
+  Planted mistakes (FR/RF swapped in the outer edges, a missing mirror in
+  resolveEdgeThroughPath, a missing read-offset flip) each fail one check.
 * Graph cleaning (REMOVE_BAD_COVERAGE, TIP_SINGLE_NODE): identity bounds,
   bound invariants and closed forms (tests/test_oracle_clean.py); parity
   unpinned against the reference's one cleaning fixture (retired text
@@ -60,7 +74,9 @@ import os
 import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB names another build of the same source (the sanitizer build
+# of `make asan`); by default the plain one.
+_LIB = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
@@ -106,6 +122,11 @@ def lib():
         L.ora_unitig_summary.argtypes = [vp, sz, i32, ctypes.POINTER(ctypes.c_double)]
         if hasattr(L, "ora_unitigs"):
             L.ora_unitigs.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_unitigs_p4.argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
+        L.ora_set_unitig_plant.argtypes = [i32]
+        L.ora_resolve_edge.argtypes = [i32, i32]
+        L.ora_vkmer_merge.argtypes = [i32, i32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
+        L.ora_merge_edges.argtypes = [i32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
         for name in ("ora_graph_stats", "ora_dump_fasta"):
             if hasattr(L, name):
                 getattr(L, name).argtypes = [vp, sz, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]
@@ -255,6 +276,48 @@ def unitigs(records: bytes, k: int) -> bytes:
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
     _check(lib().ora_unitigs(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
     return _take(p, n)
+
+
+def unitigs_p4(records: bytes, k: int) -> bytes:
+    """The same unitigs as ``unitigs``, computed by replaying P4's pairwise
+    merges (Node.mergeWithNode, the neighbour updates and
+    resolveEdgeThroughPath) one node at a time into each path's survivor."""
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().ora_unitigs_p4(records, len(records), k, ctypes.byref(p), ctypes.byref(n)))
+    return _take(p, n)
+
+
+def set_unitig_plant(mode: int):
+    """Test hook: 0 = the restatement; 1 = FR/RF swapped in the walker's
+    outer edges; 2 = resolveEdgeThroughPath without its mirror; 3 = the fold's
+    read-id merge without the offset flip (negative controls only)."""
+    lib().ora_set_unitig_plant(mode)
+
+
+ETYPES = ("FF", "FR", "RF", "RR")
+
+
+def resolve_edge(b_to_a: str, b_to_c: str):
+    """EDGETYPE.resolveEdgeThroughPath; None for an invalid path."""
+    r = lib().ora_resolve_edge(ETYPES.index(b_to_a), ETYPES.index(b_to_c))
+    return None if r < 0 else ETYPES[r]
+
+
+def vkmer_merge(edge_type: str, initial: int, this: str, other: str) -> str:
+    """VKmer.mergeWith<edge_type>Kmer(initial, other) applied to ``this``."""
+    out = ctypes.create_string_buffer(len(this) + len(other) + 2)
+    _check(lib().ora_vkmer_merge(ETYPES.index(edge_type), initial, this.encode(), other.encode(), out, len(out)))
+    return out.value.decode()
+
+
+def merge_edges(edge_type: str, this: dict, other: dict) -> dict:
+    """Node.mergeEdges(edge_type, other) on {type: [kmer strings]} lists;
+    returns this node's lists afterwards."""
+    enc = lambda d: "|".join(",".join(d.get(t, [])) for t in ETYPES)  # noqa: E731
+    out = ctypes.create_string_buffer(4096)
+    _check(lib().ora_merge_edges(ETYPES.index(edge_type), enc(this).encode(), enc(other).encode(), out, 4096))
+    fields = out.value.decode().split("|")
+    return {t: (f.split(",") if f else []) for t, f in zip(ETYPES, fields)}
 
 
 def clean_bad_coverage(records: bytes, k: int, min_cov=3.0, max_cov=float("inf")) -> bytes:

@@ -219,3 +219,28 @@ def test_plain_gpu_workload_invariants(gx):
         nrec += 1
     assert nrec == st["distinct"]
     assert total_cov == st["occurrences"]
+
+
+def _digest_cases():
+    import json
+    path = os.path.join(GOLDEN, "digests", "plain_gpu_params.json")
+    return json.load(open(path))["cases"]
+
+
+@pytest.mark.parametrize("case", _digest_cases(), ids=lambda c: f"{c['genome_len'] // 1000}kbp")
+def test_plain_gpu_params_byte_parity(gx, case):
+    """The 5 Mbp workload's parameters (5% in 2 kb repeat copies at 1%, 150 bp
+    PE, insert 500/50, 50x, 1% errors, k = 31) at 0.5 and 1 Mbp: graph and
+    unitig records byte-identical to the oracle's, whose SHA-256 digests
+    tests/golden/make_c2_digests.py recorded (the oracle needs minutes here)."""
+    import dataclasses
+    import hashlib
+    from genomix_amd import sim
+    spec = dataclasses.replace(sim.PLAIN_GPU_5MBP, genome_len=case["genome_len"], seed=case["seed"])
+    text = sim.simulate(spec)
+    assert hashlib.sha256(text).hexdigest() == case["text_sha256"], "simulator output changed"
+    graph, merged = gx.build_and_merge(text, case["k"])
+    assert len(graph) == case["graph_bytes"]
+    assert hashlib.sha256(graph).hexdigest() == case["graph_sha256"]
+    assert len(merged) == case["unitigs_bytes"]
+    assert hashlib.sha256(merged).hexdigest() == case["unitigs_sha256"]

@@ -53,6 +53,17 @@ def test_random_unitigs(gx, k):
     check(gx, sim.simulate(spec), k)
 
 
+@pytest.mark.parametrize("k", [3, 4, 5, 6, 7, 8, 9, 11, 12])
+def test_overlap_graph_unitigs(gx, k):
+    """Graphs holding every (k-1)-overlap of their k-mers: all four outer
+    edge types, self edges of cycles, palindromic neighbours at even k; the
+    oracle's records here are pinned by sequence-only brute force."""
+    from .test_oracle_merge_pins import overlap_graph_text, random_kmer_set
+    for seed in range(6):
+        text, _ = overlap_graph_text(random_kmer_set(random.Random(1000 * k + seed), k), k)
+        check(gx, text, k)
+
+
 def test_cycles_and_tandems(gx):
     rng = random.Random(3)
     lines = []
@@ -102,3 +113,32 @@ def test_plain_gpu_workload_unitig_invariants(gx):
     assert nrec < st["distinct"]
     assert total_len == st["distinct"]
     assert abs(total_cov - st["occurrences"]) < 1e-5 * st["occurrences"]
+
+
+def test_head_beyond_offset_field_is_refused(gx):
+    """One error-free unitig longer than 2^23 letters: the heads of the reads
+    near its far end would need offsets beyond ReadHeadInfo's 23 bits, where
+    the reference's makeUUID throws (ReadHeadInfo.java:115-119); the unitig
+    pass fails with GX_ENOTSUP instead of writing truncated offsets."""
+    import numpy as np
+    from genomix_amd import sim
+    lim = gx.unitig_limits()["max_head_offset"]
+    G, L, step = lim + 200_000, 150, 100  # tiled reads: every (k+1)-mer is read
+    rng = np.random.default_rng(23)
+    g = rng.integers(0, 4, G, dtype=np.uint8)
+    starts = np.arange(0, G - L + 1, step)
+    starts[-1] = G - L
+    idx = starts[:, None] + np.arange(L)[None, :]
+    reads = g[idx]
+    flip = rng.random(len(starts)) < 0.5
+    reads[flip] = (3 - reads[flip])[:, ::-1]
+    text = sim.readids_text_np(reads)
+    with gx.GraphBuilder(31) as b:
+        b.add_reads(text)
+        b.finish()
+        with pytest.raises(gx.GxError) as e:
+            b.run_unitigs()
+        assert e.value.code == gx.GX_ENOTSUP
+        assert "23 bits" in str(e.value)
+    with pytest.raises(oracle.OracleError):
+        oracle.make_uuid(0, 0, 2, lim + 1)
