@@ -2062,6 +2062,22 @@ __global__ void k_valid_u64(uint64_t nmates, const uint8_t* __restrict__ valid, 
 // ===========================================================================
 
 constexpr int kUEdges = 16;
+// A unitig edge entry is (type << kRepBits) | the neighbour unitig's
+// representative node, so the unitig pass, the cleaning steps and BUBBLE
+// accept graphs of fewer than kMaxUnitigNodes nodes.
+constexpr int kRepBits = 30;
+constexpr uint32_t kRepMask = (1u << kRepBits) - 1;
+constexpr uint64_t kMaxUnitigNodes = 1ull << kRepBits;
+// A read head's letter offset in a unitig has ReadHeadInfo's 23 bits
+// (ReadHeadInfo.java:20,34; makeUUID throws beyond them, :115-119).
+constexpr uint64_t kMaxHeadOffset = (1ull << 23) - 1;
+// The largest representative and every type survive the packing; the sort
+// keys of unitig read heads (flipped << 53 | offset << 30 | mate << 29 |
+// readId) keep the offset field clear of the flag.
+static_assert((((3u << kRepBits) | (uint32_t)(kMaxUnitigNodes - 1)) >> kRepBits) == 3u, "edge type field");
+static_assert((((3u << kRepBits) | (uint32_t)(kMaxUnitigNodes - 1)) & kRepMask) == kMaxUnitigNodes - 1,
+              "edge representative field");
+static_assert(((kMaxHeadOffset << 30) >> 53) == 0, "head offset below the flipped bit");
 
 __device__ inline int side_deg(uint32_t mask, int d) {
   return d == 0 ? __popc(mask & 0xFFu) : __popc((mask >> 8) & 0xFFu);
@@ -2271,7 +2287,7 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
           uint32_t ut = uletter == 0 ? (wletter == 0 ? ET_FF : ET_FR) : (wletter == 0 ? ET_RF : ET_RR);
           uint32_t rep = zr & 0x7fffffffu;
           bool dup = false;
-          for (int a = 0; a < ne; a++) dup |= (ent[a] >> 30) == ut && (ent[a] & 0x3fffffffu) == rep;
+          for (int a = 0; a < ne; a++) dup |= (ent[a] >> kRepBits) == ut && (ent[a] & kRepMask) == rep;
           if (!dup && ne < kUEdges) {
             // insertion: by type, then by neighbour k-mer (VKmer byte order,
             // not the node order); the keys are read again rather than kept
@@ -2279,12 +2295,12 @@ __global__ void k_unitig_edges(uint64_t nu, const uint64_t* __restrict__ ubase, 
             const KV rk = node_key<W>(klo, khi, rep);
             int p = ne;
             while (p > 0) {
-              uint32_t pt = ent[p - 1] >> 30;
-              if (pt < ut || (pt == ut && kle<W>(node_key<W>(klo, khi, ent[p - 1] & 0x3fffffffu), rk))) break;
+              uint32_t pt = ent[p - 1] >> kRepBits;
+              if (pt < ut || (pt == ut && kle<W>(node_key<W>(klo, khi, ent[p - 1] & kRepMask), rk))) break;
               ent[p] = ent[p - 1];
               p--;
             }
-            ent[p] = (ut << 30) | rep;
+            ent[p] = (ut << kRepBits) | rep;
             ne++;
           }
         }
@@ -2306,7 +2322,8 @@ __global__ void k_unitig_heads(uint64_t nn, const uint64_t* __restrict__ hstart,
                                const uint8_t* __restrict__ hoff, const uint32_t* __restrict__ urep,
                                const uint32_t* __restrict__ upos,
                                const uint64_t* __restrict__ uid_of_rep, const int64_t* __restrict__ rid, int k,
-                               Key128* __restrict__ key, uint32_t* __restrict__ val) {
+                               Key128* __restrict__ key, uint32_t* __restrict__ val,
+                               unsigned long long* __restrict__ too_far) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t nh = hcu[i] + hcf[i];
     if (!nh) continue;
@@ -2320,6 +2337,10 @@ __global__ void k_unitig_heads(uint64_t nn, const uint64_t* __restrict__ hstart,
       // the head's letter in the node's k-mer: 0 / k - 1 from the build
       const uint32_t a = hoff ? hoff[h0 + j] : (f ? (uint32_t)(k - 1) : 0u);
       uint64_t off = upos[i] + (fwd ? a : (uint32_t)(k - 1) - a);
+      if (off > kMaxHeadOffset) {  // ReadHeadInfo.makeUUID throws here (ReadHeadInfo.java:115-119)
+        atomicMin(too_far, (unsigned long long)off);
+        off = 0;
+      }
       Key128 kk;
       kk.hi = u;
       kk.lo = ((uint64_t)nf << 53) | (off << 30) | ((uint64_t)(m & 1) << 29) | (uint64_t)rid[m >> 1];
@@ -2399,7 +2420,7 @@ __device__ inline uint64_t unitig_record_size(const USerCtx& c, uint64_t u) {
   uint64_t val = 1 + 4;  // flags + coverage
   int ne = c.uecnt[u];
   int per[4] = {0, 0, 0, 0};
-  for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> 30]++;
+  for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> kRepBits]++;
   for (int t = 0; t < 4; t++)
     if (per[t]) val += 4 + (uint64_t)per[t] * (4 + c.nb);
   uint64_t h0 = c.hstart[u];
@@ -2486,7 +2507,7 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
     put_key<W>(p, v, c.k, c.nb);
     int ne = c.uecnt[u];
     int per[4] = {0, 0, 0, 0};
-    for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> 30]++;
+    for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> kRepBits]++;
     uint32_t nu_ = c.hu[u], nf = c.hf[u];
     uint64_t m = c.ulen[u];
     uint8_t flags = 0x80;
@@ -2501,8 +2522,8 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
       put32(p, (uint32_t)per[t]);
       for (int a = 0; a < ne; a++) {
         uint32_t e = c.uedge[kUEdges * u + a];
-        if ((int)(e >> 30) != t) continue;
-        put_key<W>(p, node_key<W>(c.klo, c.khi, e & 0x3fffffffu), c.k, c.nb);
+        if ((int)(e >> kRepBits) != t) continue;
+        put_key<W>(p, node_key<W>(c.klo, c.khi, e & kRepMask), c.k, c.nb);
       }
     }
     uint64_t h0 = c.hstart[u];
@@ -3657,9 +3678,11 @@ static void run_unitigs(gx_build* b) {
   const uint32_t* cnt = b->cnt.as<uint32_t>();
   const uint32_t* msk = b->msk.as<uint32_t>();
   unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
-  GX_HIP(hipMemsetAsync(errw, 0xff, 8, st));
+  GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
+  if (nn >= kMaxUnitigNodes)
+    throw GxError(GX_ENOTSUP, "the unitig pass holds fewer than 2^30 distinct k-mers; this graph has " +
+                                  std::to_string(nn));
   uint64_t ne = 2 * nn;
-  if (ne >= 0xffffffffull) throw GxError(GX_ENOTSUP, "unitig pass limited to 2^31 nodes");
   ph_begin(b, PH_UNITIG);
   uint32_t* lnk = buf<uint32_t>(b->u_lnk, ne + 1);
   uint8_t* tand = buf<uint8_t>(b->u_tand, nn + 1);
@@ -3757,8 +3780,12 @@ static void run_unitigs(gx_build* b) {
     hipLaunchKernelGGL(k_unitig_heads, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
                        b->hstart.as<uint64_t>(), b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), b->uval.as<uint32_t>(),
                        b->has_hoff ? b->hoff.as<uint8_t>() : nullptr, urep, upos, uid_of_rep, b->rid.as<int64_t>(), k,
-                       hk, hv);
+                       hk, hv, errw + 1);
     GX_HIP(hipGetLastError());
+    const unsigned long long far = read1(st, errw + 1);
+    if (far != ~0ull)
+      throw GxError(GX_ENOTSUP, "a read head lies at letter " + std::to_string(far) +
+                                    " of its unitig; read-head offsets hold 23 bits");
   }
   bool fl = radix_sort<Key128>(hk, hv, hk2, hv2, nh, 0, 12, scratch_for(b, radix_scratch_bytes<Key128>(nh)), st);
   if (fl) { std::swap(hk, hk2); std::swap(hv, hv2); }
@@ -3978,7 +4005,7 @@ __global__ void k_unitig_keep(uint64_t nn, const uint32_t* __restrict__ urep, co
     } else {
       int deg[2] = {0, 0};
       for (int e = 0; e < uecnt[u]; e++) {
-        const uint32_t t = uedge[kUEdges * u + e] >> 30;
+        const uint32_t t = uedge[kUEdges * u + e] >> kRepBits;
         deg[(t == ET_FF || t == ET_FR) ? 0 : 1]++;
       }
       const bool tip = (deg[0] == 1 && deg[1] == 0) || (deg[1] == 1 && deg[0] == 0);
@@ -4166,19 +4193,19 @@ __global__ void k_bub_cand(uint64_t nu, const uint64_t* __restrict__ ulen, const
     int deg[2] = {0, 0};
     for (int e = 0; e < uecnt[u]; e++) {
       const uint32_t x = uedge[kUEdges * u + e];
-      const int sd = et_side_d(x >> 30);
+      const int sd = et_side_d(x >> kRepBits);
       if (deg[sd]++ == 0) ent[sd] = x;
     }
     uint32_t mn = kEnd, mj = kEnd;
     uint8_t ty = 0;
     if (deg[0] == 1 && deg[1] == 1 && ulen[u] + (uint64_t)k - 1 < maxlen) {
-      const uint32_t rf = ent[0] & 0x3fffffffu, rr = ent[1] & 0x3fffffffu;
+      const uint32_t rf = ent[0] & kRepMask, rr = ent[1] & kRepMask;
       if (rf != rr) {
         const bool fwd_major = kle<W>(node_key<W>(klo, khi, rr), node_key<W>(klo, khi, rf));
         const uint32_t emaj = fwd_major ? ent[0] : ent[1], emin = fwd_major ? ent[1] : ent[0];
-        mj = (uint32_t)uid_of_rep[emaj & 0x3fffffffu];
-        mn = (uint32_t)uid_of_rep[emin & 0x3fffffffu];
-        ty = (uint8_t)(et_mirror_d(emaj >> 30) | (et_mirror_d(emin >> 30) << 2));
+        mj = (uint32_t)uid_of_rep[emaj & kRepMask];
+        mn = (uint32_t)uid_of_rep[emin & kRepMask];
+        ty = (uint8_t)(et_mirror_d(emaj >> kRepBits) | (et_mirror_d(emin >> kRepBits) << 2));
       }
     }
     bmin[u] = mn;
@@ -4210,7 +4237,7 @@ __device__ inline uint32_t bub_class(const BubCtx& c, uint32_t w) {
 __device__ inline uint32_t bub_set(const BubCtx& c, uint64_t v, uint32_t* w) {
   uint32_t n = 0;
   for (int e = 0; e < c.uecnt[v]; e++) {
-    const uint32_t x = (uint32_t)c.uid_of_rep[c.uedge[kUEdges * v + e] & 0x3fffffffu];
+    const uint32_t x = (uint32_t)c.uid_of_rep[c.uedge[kUEdges * v + e] & kRepMask];
     if (c.bmin[x] != (uint32_t)v) continue;
     bool dup = false;
     for (uint32_t a = 0; a < n; a++) dup |= w[a] == x;
@@ -4589,9 +4616,9 @@ __device__ inline VProps vertex_props(const VertexCtx& c, uint64_t i) {
   } else {
     const uint32_t rep = c.urep_node[i];
     for (int e = 0; e < c.uecnt[i]; e++) {
-      const uint32_t x = c.uedge[kUEdges * i + e], t = x >> 30;
+      const uint32_t x = c.uedge[kUEdges * i + e], t = x >> kRepBits;
       p.deg[(t == ET_FF || t == ET_FR) ? 0 : 1]++;
-      if ((x & 0x3fffffffu) == rep) p.self[t]++;
+      if ((x & kRepMask) == rep) p.self[t]++;
     }
     p.len = c.ulen[i] + (uint64_t)c.k - 1;
     p.cov = (float)((double)c.ucov[i] / (double)c.ulen[i]);
@@ -5008,6 +5035,7 @@ int gx_build_finish(gx_build* b) {
       b->src_len = b->text_len;
     }
     b->unitigs_done = false;
+    b->st_graph = -1;
     b->reclaim.dead.clear();
     ReclaimScope rs(&b->reclaim, b->st);
     if (b->W == 1) run_build<1>(b);
@@ -5065,11 +5093,13 @@ int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_bad_coverage before unitig_run");
+    if (b->nnodes >= kMaxUnitigNodes) throw GxError(GX_ENOTSUP, "graph too large for the unitig pass");
     if (!(min_cov >= 0.f) || !(max_cov >= 0.f))  // GenomixJobConf.java:442-447
       throw GxError(GX_EINVAL, "coverage bounds must not be negative");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->unitigs_done = false;
+    b->st_graph = -1;
     const CleanArgs ca{CLEAN_COVERAGE, min_cov, max_cov, 0, b->cfg.k};
     if (b->W == 1) clean_graph<1>(b, ca);
     else clean_graph<2>(b, ca);
@@ -5080,10 +5110,14 @@ int gx_clean_bad_coverage(gx_build* b, float min_cov, float max_cov) {
 int gx_clean_tips(gx_build* b, int max_len) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
+    if (max_len >= 0 && max_len < b->cfg.k)  // GenomixJobConf.java:449-450
+      throw GxError(GX_EINVAL, "tipRemove_maxLength must be at least as long as kmerLength");
     if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_tips before unitig_run");
+    if (b->nnodes >= kMaxUnitigNodes) throw GxError(GX_ENOTSUP, "graph too large for the unitig pass");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->unitigs_done = false;
+    b->st_graph = -1;
     const CleanArgs ca{CLEAN_TIP, 0.f, 0.f, (uint64_t)(max_len < 0 ? b->cfg.k : max_len), b->cfg.k};
     if (b->W == 1) clean_graph<1>(b, ca);
     else clean_graph<2>(b, ca);
@@ -5101,9 +5135,11 @@ int gx_clean_bubbles(gx_build* b, float max_dissimilarity, int max_len) {
     if (maxlen > kBubMaxLetters + 1)
       throw GxError(GX_ENOTSUP, "bubbles longer than " + std::to_string(kBubMaxLetters) + " letters");
     if (!b->unitigs_done) throw GxError(GX_ESTATE, "clean_bubbles before unitig_run");
+    if (b->nnodes >= kMaxUnitigNodes) throw GxError(GX_ENOTSUP, "graph too large for the unitig pass");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->unitigs_done = false;
+    b->st_graph = -1;
     if (b->W == 1) bubble_graph<1>(b, max_dissimilarity, maxlen);
     else bubble_graph<2>(b, max_dissimilarity, maxlen);
     b->unitigs_done = true;
@@ -5289,10 +5325,17 @@ int gx_unitig_run(gx_build* b) {
                                     std::to_string(b->nranges) + " key ranges");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
+    b->st_graph = -1;
     if (b->W == 1) run_unitigs<1>(b);
     else run_unitigs<2>(b);
     b->unitigs_done = true;
   });
+}
+
+int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset) {
+  if (max_nodes) *max_nodes = kMaxUnitigNodes;
+  if (max_head_offset) *max_head_offset = kMaxHeadOffset;
+  return GX_OK;
 }
 
 int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes) {
@@ -5359,6 +5402,7 @@ int gx_build_set_config(gx_build* b, const gx_build_cfg* cfg) {
     b->unitigs_done = false;
     b->out_bytes = 0;
     b->nnodes = 0;
+    b->st_graph = -1;
   });
 }
 
@@ -5416,6 +5460,7 @@ int gx_build_reset(gx_build* b) {
   b->unitigs_done = false;
   b->out_bytes = 0;
   b->nnodes = 0;
+  b->st_graph = -1;  // statistics describe the graph that is gone
   return GX_OK;
 }
 

@@ -21,11 +21,7 @@ struct KV {
 __device__ __host__ inline uint64_t rev2_64(uint64_t x) {
   x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
   x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
-#if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_bswap64(x);
-#else
-  return __builtin_bswap64(x);
-#endif
 }
 
 // Mask of the 2k valid bits.

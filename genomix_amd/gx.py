@@ -35,7 +35,7 @@ EXPORTED = [
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
-    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried",
+    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried", "gx_unitig_limits",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
@@ -89,6 +89,7 @@ def lib():
         L.gx_build_phase_name.restype = ctypes.c_char_p
         L.gx_unitig_run.argtypes = [vp]
         L.gx_unitig_info.argtypes = [vp, u64p, u64p]
+        L.gx_unitig_limits.argtypes = [u64p, u64p]
         L.gx_unitig_emit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.gx_build_add_fastq_pair.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
         L.gx_build_readids.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
@@ -371,6 +372,15 @@ class GraphBuilder:
             self.close()
         except Exception:
             pass
+
+
+def unitig_limits() -> dict:
+    """The unitig pass's limits (no device needed): distinct k-mers per
+    graph and the last letter a read head may sit at (ReadHeadInfo's 23-bit
+    offset field)."""
+    n, o = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().gx_unitig_limits(ctypes.byref(n), ctypes.byref(o))
+    return {"max_nodes": n.value, "max_head_offset": o.value}
 
 
 def build_and_merge(text: bytes, k: int, edge_order: str = "jdk7", device: int = 0):

@@ -175,6 +175,10 @@ def simulate_chunks(spec: ReadSetSpec, chunk_reads=1 << 20, workers=1):
     with mp.get_context("spawn").Pool(workers) as pool:
         for t in pool.imap(_chunk_text, jobs):
             yield t
+        # let the workers exit on their own: the context manager's terminate()
+        # would signal them (a profiler's preload then reports an abort)
+        pool.close()
+        pool.join()
 
 
 def fastq_text(m, name="r") -> bytes:
@@ -243,4 +247,6 @@ def simulate_many(specs, workers=12):
     with mp.get_context("spawn").Pool(workers) as pool:
         for i, t in pool.imap_unordered(_simulate_indexed, list(enumerate(specs))):
             out[i] = t
+        pool.close()
+        pool.join()
     return out

@@ -198,6 +198,8 @@ def generate(runs, base_seed=20261205, workers=12, log=None):
             meta[i] = [k, P, N, D, skew]
             if log and (n_done + 1) % 500 == 0:
                 log(f"{n_done + 1}/{runs} read sets built ({time.time() - t0:.0f} s)")
+        pool.close()  # workers exit on their own rather than by terminate()
+        pool.join()
     for b in builders.values():
         b.close()
     return X, T, meta

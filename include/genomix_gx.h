@@ -125,6 +125,15 @@ int gx_unitig_run(gx_build* b);
 int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes);
 int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
 
+/* Limits of the unitig pass, for callers that plan around them (no device
+ * needed): max_nodes = the distinct k-mers one unitig pass, cleaning step or
+ * BUBBLE accepts (2^30; larger graphs fail with GX_ENOTSUP);
+ * max_head_offset = the last unitig letter a read head may sit at
+ * (2^23 - 1, the offset field of ReadHeadInfo.makeUUID,
+ * ReadHeadInfo.java:115-119, which throws beyond it; here the unitig pass
+ * fails with GX_ENOTSUP).  Either pointer may be NULL. */
+int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset);
+
 /* Statistics of the last finish: [0] occurrences, [1] distinct k-mers,
  * [2] valid mates, [3] table capacity, [4] replayed nodes, [5] replayed
  * occurrences, [6] largest coverage above 255 (0 if none), [7] distinct

@@ -81,6 +81,12 @@ def test_errors(gx):
         with pytest.raises(gx.GxError) as e:
             b.clean_bad_coverage(-1.0)
         assert e.value.code == gx.GX_EINVAL
+        # GenomixJobConf.java:449-450: tipRemove_maxLength >= kmerLength
+        for bad in (0, 20):
+            with pytest.raises(gx.GxError) as e:
+                b.clean_tips(bad)
+            assert e.value.code == gx.GX_EINVAL
+        b.clean_tips(21, emit=False)
 
 
 @pytest.mark.parametrize("k,paired", [(21, False), (31, True), (55, True)])

@@ -109,3 +109,27 @@ def test_cutoff_needs_statistics(gx):
         got = b.coverage_cutoff(graph="unitigs")
         bins = b.graph_stats("unitigs")["coverage-bins"]
     _same(got, oracle.fitting_mixture(_expand(bins), max(float(v) for v in bins), 10))
+
+
+def test_statistics_do_not_outlive_their_graph(gx):
+    """A new build, a reset, a re-configuration or a graph step drops the
+    previous statistics: the cutoff then needs gx_graph_stats again."""
+    import ctypes
+    out = (ctypes.c_double * 6)()
+    text = _reads(9, 5000, 100, 10, 0.01)
+    with gx.GraphBuilder(21) as b:
+        b.add_reads(text)
+        b.finish()
+        b.coverage_cutoff()
+        assert gx.lib().gx_coverage_cutoff(b._h, 10, out) == gx.GX_OK
+        b.reset()
+        assert gx.lib().gx_coverage_cutoff(b._h, 10, out) == gx.GX_ESTATE
+        b.add_reads(text)
+        b.finish()
+        assert gx.lib().gx_coverage_cutoff(b._h, 10, out) == gx.GX_ESTATE
+        b.coverage_cutoff()
+        b.run_unitigs()
+        assert gx.lib().gx_coverage_cutoff(b._h, 10, out) == gx.GX_ESTATE
+        b.coverage_cutoff(graph="unitigs")
+        b.clean_bad_coverage(2.0, emit=False)
+        assert gx.lib().gx_coverage_cutoff(b._h, 10, out) == gx.GX_ESTATE

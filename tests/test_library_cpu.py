@@ -51,3 +51,16 @@ def test_create_validates_config():
         with pytest.raises(gx.GxError) as e:
             gx.GraphBuilder(**kw)
         assert e.value.code == gx.GX_EINVAL
+
+
+def test_unitig_limits():
+    """The unitig pass refuses graphs its 30-bit edge entries cannot address
+    and heads beyond ReadHeadInfo's 23-bit offset (the oracle's makeUUID
+    throws there, as the reference's does)."""
+    from genomix_amd import gx
+    from oracle import oracle
+    lim = gx.unitig_limits()
+    assert lim == {"max_nodes": 1 << 30, "max_head_offset": (1 << 23) - 1}
+    oracle.make_uuid(0, 0, 1, lim["max_head_offset"])
+    with pytest.raises(oracle.OracleError):
+        oracle.make_uuid(0, 0, 1, lim["max_head_offset"] + 1)
