@@ -14,12 +14,15 @@ Workload (default): seeded 5 Mbp genome with 5% in 2 kb repeat copies at
 1% divergence; 150 bp paired-end reads, insert 500/50, 50x, 1% substitution
 errors; k = 31.  Synthetic data, generated at start-up.
 
-The JSON line also carries the roofline of the dominant kernel (the
-pointer-jumping round of the unitig pass: 48 algorithmic bytes per node end
-per launch, every launch timed with hipEvents on the library's stream), the
-same for the k-mer pass of the aggregation, and a CPU baseline: the
-CPU oracle (a single-threaded C++ restatement of the reference's Hyracks
-dataflow) timed on a bounded sample of the same reads.
+The JSON line also carries the roofline in the benchmark plan's unit,
+B = L/(L-k+1) + 2*(8*ceil(k/32) + 8) algorithmic bytes per k-mer occurrence,
+for the launch that takes every occurrence into the aggregation
+(k_occ_scatter, timed with hipEvents on the library's stream), for the whole
+aggregation phase and for the whole step; per-kernel lines for the k-mer pass
+and the pointer-jumping rounds on their own bytes; and a CPU baseline: the
+CPU oracle (a C++ restatement of the reference's Hyracks dataflow) on this
+host at P = cores partitions on cores threads and at one thread, on bounded
+samples of the same reads, plus the whole 50 kbp CPU workload.
 
 After the plain leg (unless --no-graph-steps): the coverage cutoff,
 REMOVE_BAD_COVERAGE, statistics and FASTA, TIP_SINGLE_NODE and BUBBLE on
@@ -54,22 +57,67 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(text, k, target_pairs=16000):
-    """Oracle (single thread) on the first target_pairs lines of the workload."""
-    from oracle import oracle
-    lines = text.split(b"\n", target_pairs)
-    sample = b"\n".join(lines[:target_pairs]) + b"\n"
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _sample_lines(text, n):
+    lines = text.split(b"\n", n)
+    return b"\n".join(lines[:n]) + b"\n", lines[:n]
+
+
+def _occurrences(lines, k):
     occ = 0
-    for ln in lines[:target_pairs]:
+    for ln in lines:
         for f in ln.split(b"\t")[1:]:
             if len(f) > k:
                 occ += len(f) - k + 1
+    return occ
+
+
+def cpu_baseline(text, k, pairs_per_thread=8000):
+    """The oracle (C++ restatement of the reference's Hyracks dataflow) on
+    this host, in the reference's two local settings: P = cores partitions
+    with the producer and consumer stages on `cores` threads (Hyracks with
+    threadsPerMachine = cores; `value`), and one thread with P = 1 (the CLI
+    default).  Bounded samples of the same reads; plus the whole 50 kbp CPU
+    workload (k = 21) on one thread."""
+    from genomix_amd import sim
+    from oracle import oracle
+    cores = max(1, min(os.cpu_count() or 1, 16))  # the GPU box's CPU share is 16
+    sample, lines = _sample_lines(text, pairs_per_thread * cores)
+    occ = _occurrences(lines, k)
     t0 = time.perf_counter()
-    oracle.build_graph(sample, k)
+    oracle.build_graph_threaded(sample, k, cores, cores)
     dt = time.perf_counter() - t0
-    return {"value": occ / dt, "unit": "k-mers/s", "cores": 1, "kind": "port",
-            "sample": f"first {target_pairs} read pairs of the same workload ({occ} k-mer occurrences), "
-                      f"parse -> stable sort -> local+cross aggregation -> Node records, {dt:.1f} s"}
+    s1, l1 = _sample_lines(text, 8000)
+    occ1 = _occurrences(l1, k)
+    t0 = time.perf_counter()
+    oracle.build_graph(s1, k)
+    dt1 = time.perf_counter() - t0
+    c1 = sim.simulate(sim.PLAIN_CPU_50KBP)
+    occ_c1 = _occurrences(c1.split(b"\n"), 21)
+    t0 = time.perf_counter()
+    oracle.build_graph(c1, 21)
+    dt_c1 = time.perf_counter() - t0
+    return {"value": round(occ / dt, 1), "unit": "k-mers/s", "cores": cores, "kind": "port",
+            "sample": f"first {len(lines)} read pairs of the same workload ({occ} k-mer occurrences): "
+                      f"P = {cores} partitions, {cores} producers + {cores} consumers on {cores} threads "
+                      f"(parse -> stable sort -> local aggregation -> partition -> cross aggregation -> "
+                      f"Node records), {dt:.1f} s",
+            "cpu_model": _cpu_model(),
+            "single_thread": {"value": round(occ1 / dt1, 1), "unit": "k-mers/s", "cores": 1, "partitions": 1,
+                              "sample": f"first {len(l1)} read pairs ({occ1} k-mer occurrences), {dt1:.1f} s"},
+            "plain_cpu_workload_full": {"workload": "50 kbp genome, 100 bp SE, 30x, 0.5% error, k = 21 (whole)",
+                                        "seconds": round(dt_c1, 3), "kmer_occurrences": occ_c1,
+                                        "value": round(occ_c1 / dt_c1, 1), "unit": "k-mers/s", "cores": 1}}
 
 
 def load_traffic(name):
@@ -321,28 +369,52 @@ def main():
     N = st["occurrences"]
     value = N / (elapsed / args.steps)
     phases = {kname: v / args.steps for kname, v in phases_sum.items()}
-    # dominant kernel: the pointer-jumping round of the unitig pass, 2 elements
-    # (node ends) per distinct k-mer, each round reads its own 16-byte record
-    # and its successor's and writes its new one: 48 algorithmic bytes per
-    # element per launch; hipEvents around every launch on the library stream
-    rounds = max(1, st["pj_rounds"])
-    t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
-    elems = 2 * st["distinct"]
-    achieved = 48.0 * elems / t_pj / 1e9 if t_pj > 0 else 0.0
-    traffic = load_traffic("k_pj_round_traffic_plain_gpu_5mbp.json")
+    # Roofline in the benchmark plan's unit: B = L/(L-k+1) + 2*(8*ceil(k/32) + 8)
+    # algorithmic bytes per k-mer occurrence (ASCII read in, one table slot
+    # read and written; 33.25 B here).  `roofline`: the launch that takes every
+    # occurrence into the aggregation (k_occ_scatter: canonical k-mer, neighbour
+    # code and the stable scatter of its 16-byte record), timed with hipEvents
+    # on the library's stream; the same B over the whole aggregation phase and
+    # over the whole step beside it.
+    B = L / (L - k + 1) + 2 * (8 * ((k + 31) // 32) + 8)
+    t_sc = phases.get("k_occ_scatter", 0.0) * 1e-3
+    achieved = B * N / t_sc / 1e9 if t_sc > 0 else 0.0
+    traffic = load_traffic("k_occ_scatter_traffic_plain_gpu_5mbp.json")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "kernel": "k_pj_round", "algorithmic_bytes_per_launch": 48 * elems,
-                "kernel_ms": round(t_pj * 1e3, 4), "launches_per_step": rounds}
-    # the k-mer pass of the aggregation (k-mer extraction fused with the first
-    # radix scatter): two reads of the 2-bit packed mates, 16-byte record out
+                "kernel": "k_occ_scatter", "algorithmic_bytes_per_kmer": round(B, 4),
+                "kmers_per_launch": N, "kernel_ms": round(t_sc * 1e3, 4), "launches_per_step": 1}
+    t_ins = phases.get("insert", 0.0) * 1e-3
+    roofline_phases = {
+        "aggregation": {"ms": round(t_ins * 1e3, 3), "frac": round(B * N / t_ins / 1e9 / HBM_PEAK_GBS, 4)
+                        if t_ins > 0 else None,
+                        "what": "k-mer pass + radix pass + bucket sorts + node runs (replaces the table insert)"},
+        "whole_step": {"ms": round(ms_step, 3), "frac": round(B * value / 1e9 / HBM_PEAK_GBS, 4),
+                       "what": "B x k-mers/s of the whole step (build + unitigs)"}}
+    # the k-mer pass (k_occ_hist + k_occ_scatter) on its own bytes: two reads
+    # of the 2-bit packed mates, one 16-byte record written per occurrence
     Bk = 2 * (L / (L - k + 1)) / 4 + 16
     t_k = phases.get("k_insert", 0.0) * 1e-3
     ach_k = Bk * N / t_k / 1e9 if t_k > 0 else 0.0
     roofline_kmer = {"bound": "hbm", "achieved": round(ach_k, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
                      "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
+    # pointer-jumping rounds of the unitig pass: every node end reads and
+    # writes its own 16-byte record (32 B); ends still following their chain
+    # also read their successor's (16 B), counted by the library
+    rounds = max(1, st["pj_rounds"])
+    t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
+    ends = 2 * st["distinct"]
+    pj_bytes = (32 * ends * rounds + 16 * st["pj_successor_reads"]) / rounds
+    ach_pj = pj_bytes / t_pj / 1e9 if t_pj > 0 else 0.0
+    tr_pj = load_traffic("k_pj_round_traffic_plain_gpu_5mbp.json")
+    roofline_pj = {"bound": "hbm", "achieved": round(ach_pj, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(ach_pj / HBM_PEAK_GBS, 4), "kernel": "k_pj_round",
+                   "traffic": tr_pj["bytes_per_launch"] if tr_pj else None,
+                   "algorithmic_bytes_per_launch": round(pj_bytes), "node_ends": ends,
+                   "successor_reads_per_step": st["pj_successor_reads"],
+                   "kernel_ms": round(t_pj * 1e3, 4), "launches_per_step": rounds}
     unitigs = builder.unitig_count()
     clean = stats = None
     if not args.no_graph_steps:
@@ -440,7 +512,9 @@ def main():
                    "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
+        "roofline_phases": roofline_phases,
         "roofline_kmer_pass": roofline_kmer,
+        "roofline_pointer_jumping": roofline_pj,
         "clean_bad_coverage": clean,
         "graph_stats": stats,
         "batched": batched,

@@ -2147,8 +2147,10 @@ __global__ void k_pj_init(uint64_t nn, const uint32_t* __restrict__ lnk, const u
 }
 
 __global__ void __launch_bounds__(kBlock) k_pj_round(uint64_t ne, const uint4* __restrict__ Rc,
-                                                     uint4* __restrict__ Rn, uint8_t* __restrict__ bflag) {
+                                                     uint4* __restrict__ Rn, uint8_t* __restrict__ bflag,
+                                                     unsigned long long* __restrict__ nread) {
   int any = 0;
+  uint32_t reads = 0;  // successor records read (ends still following their chain)
   typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
   const u32x4* __restrict__ Rcv = reinterpret_cast<const u32x4*>(Rc);
   u32x4* __restrict__ Rnv = reinterpret_cast<u32x4*>(Rn);
@@ -2159,9 +2161,12 @@ __global__ void __launch_bounds__(kBlock) k_pj_round(uint64_t ne, const uint4* _
       const uint4 q = Rc[r.x];
       r = u32x4{q.x, r.y + q.y, min(r.z, q.z), q.w};
       any |= q.x != kEnd;
+      reads++;
     }
     __builtin_nontemporal_store(r, Rnv + e);
   }
+  for (int o = 32; o > 0; o >>= 1) reads += __shfl_xor(reads, o);
+  if ((threadIdx.x & 63) == 0 && reads) atomicAdd(nread, (unsigned long long)reads);
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) bflag[blockIdx.x] = any ? 1 : 0;
 }
@@ -2699,7 +2704,7 @@ struct gx_build {
   DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
   DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[12] = {};
+  uint64_t stats[13] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t evpj[2 * kMaxTimedRounds] = {};  // per pointer-jumping round
@@ -2711,7 +2716,7 @@ struct gx_build {
   DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep;
   DevBuf u_uid, u_repnode, u_len, u_base, u_order, u_cov, u_edge, u_ecnt, u_hk, u_hk2, u_hv, u_hv2, u_uflag, u_uposh;
   DevBuf u_hu, u_hf, u_htot, u_hstart, u_hklo, u_hvc, u_rsize, u_roff, u_out;
-  uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0;
+  uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0, u_succ_reads = 0;
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
@@ -2758,10 +2763,10 @@ static thread_local std::string g_create_err;  // errors of calls without a hand
 
 enum Phase {
   PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX,
-  PH_KPJ, NPH
+  PH_KPJ, PH_KSCATTER, NPH
 };
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay", "unitig", "sizing", "node_index", "k_pj_round"};
+                                       "k_replay", "unitig", "sizing", "node_index", "k_pj_round", "k_occ_scatter"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -3174,9 +3179,11 @@ static void run_build(gx_build* b) {
       uint32_t* nh = next_hist ? buf<uint32_t>(*hnext, hn + 1) : nullptr;
       const uint32_t nlo = next_hist ? tb[ri + 1] : 0, nhi = next_hist ? tb[ri + 2] : 0;
       const int ndbyte = next_hist ? nb - bkb_of(rocc[ri + 1]) : 0;
+      ph_begin(b, PH_KSCATTER);
       hipLaunchKernelGGL(k_occ_scatter<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words,
                          k, nb, hoff, ntiles, ka, va, tlo, thi, dbyte, nlo, nhi, ndbyte, nh);
       GX_HIP(hipGetLastError());
+      ph_end(b, PH_KSCATTER);
       ph_end(b, PH_KINSERT);
     }
     hist_ready = next_hist;
@@ -3679,6 +3686,7 @@ static void run_unitigs(gx_build* b) {
   const uint32_t* msk = b->msk.as<uint32_t>();
   unsigned long long* errw = buf<unsigned long long>(b->errw, 4);
   GX_HIP(hipMemsetAsync(errw, 0xff, 16, st));
+  GX_HIP(hipMemsetAsync(errw + 2, 0, 8, st));  // successor reads of the pointer-jumping rounds
   if (nn >= kMaxUnitigNodes)
     throw GxError(GX_ENOTSUP, "the unitig pass holds fewer than 2^30 distinct k-mers; this graph has " +
                                   std::to_string(nn));
@@ -3711,7 +3719,7 @@ static void run_unitigs(gx_build* b) {
     for (int r = 0; r <= rmax && act; r++) {
       unsigned g = grid_for(ne, kBlock, jgrid);
       pj_begin(b);
-      hipLaunchKernelGGL(k_pj_round, dim3(g), dim3(kBlock), 0, st, ne, R[cur], R[cur ^ 1], bflag);
+      hipLaunchKernelGGL(k_pj_round, dim3(g), dim3(kBlock), 0, st, ne, R[cur], R[cur ^ 1], bflag, errw + 2);
       GX_HIP(hipGetLastError());
       pj_end(b);
       hipLaunchKernelGGL(k_flag_or, dim3(1), dim3(kBlock), 0, st, bflag, g, anyflag);
@@ -3727,6 +3735,7 @@ static void run_unitigs(gx_build* b) {
   }
   unsigned long long e = read1(st, errw);
   if (e != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the table");
+  b->u_succ_reads = read1(st, errw + 2);
 
   uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
   uint32_t* upos = buf<uint32_t>(b->u_pos, nn + 1);
@@ -5085,7 +5094,8 @@ int gx_build_part_device(gx_build* b, int part, const void** dev_ptr, uint64_t* 
 int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
   b->stats[10] = b->u_rounds;
-  for (int i = 0; i < n && i < 12; i++) out[i] = b->stats[i];
+  b->stats[12] = b->u_succ_reads;
+  for (int i = 0; i < n && i < 13; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 

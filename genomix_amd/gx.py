@@ -337,11 +337,11 @@ class GraphBuilder:
         return buf.raw[:n.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 12)()
-        self._check(lib().gx_build_stats(self._h, a, 12))
+        a = (ctypes.c_uint64 * 13)()
+        self._check(lib().gx_build_stats(self._h, a, 13))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
                 "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds",
-                "key_ranges"]
+                "key_ranges", "pj_successor_reads"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

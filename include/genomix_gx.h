@@ -141,15 +141,18 @@ int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset);
  * [8] table rebuilds after overflow, [9] records sorted by the global
  * fallback for oversized key buckets (sorted aggregation), [10] pointer-
  * jumping rounds of the last unitig run, [11] key-range passes of the build
- * (genomix_stream.h). */
+ * (genomix_stream.h), [12] successor records read by those pointer-jumping
+ * rounds (node ends still following their chain, summed over the rounds). */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on
  * the handle's stream (the C-ABI counterpart of GenomixJobConf.tick/tock,
  * GenomixJobConf.java:669-687).  set_profiling(1) enables recording;
  * phase_times fills ms[0..n) and returns the number of phases;
- * phase_name(i) names phase i (the last, "k_insert", is the insert kernel
- * alone). */
+ * phase_name(i) names phase i.  Phases named after a kernel ("k_insert":
+ * the k-mer pass, "k_occ_scatter": its scatter launch alone, "k_replay",
+ * "k_pj_round": the pointer-jumping launches) time those launches only;
+ * the others are stages of the build or of the unitig run. */
 int gx_build_set_profiling(gx_build* b, int on);
 int gx_build_phase_times(gx_build* b, double* ms, int n);
 const char* gx_build_phase_name(int i);

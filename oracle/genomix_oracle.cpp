@@ -21,7 +21,10 @@
 // oracle/oracle.py.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
+#include <thread>
 #include <functional>
 #include <cstdint>
 #include <cstdio>
@@ -844,6 +847,68 @@ static BuildResult build(const uint8_t* text, size_t len, int k, int P, const in
         if (partition_of(r.key, P) == part) merged.push_back(r);
     res.parts[part] = sort_and_group(merged);
   }
+  return res;
+}
+
+// The same dataflow with its stages on threads, as Hyracks runs it with
+// threadsPerMachine = P on one node (JobGen.java:60-72: P partitions; one
+// reader split per partition): P producers parse and group their byte range
+// of the input concurrently (local aggregation), then P consumers merge and
+// group their partition concurrently (cross aggregation), each taking the
+// producers' records in ascending producer order.  The result equals
+// build() with the same split starts.  Used for the CPU baseline only.
+static BuildResult build_threaded(const uint8_t* text, size_t len, int k, int P, int nthreads) {
+  if (k < 3) throw OraError(E_INVAL, "kmerLength must be at least 3!");
+  if (k > 63) throw OraError(E_INVAL, "k > 63 is not supported");
+  if (P < 1) throw OraError(E_INVAL, "nparts must be >= 1");
+  if (nthreads < 1) nthreads = 1;
+  // split starts: equal byte ranges, each moved to the next line start
+  std::vector<size_t> start(P + 1, len);
+  start[0] = 0;
+  for (int p = 1; p < P; p++) {
+    size_t s = std::max(start[p - 1], len * (size_t)p / (size_t)P);
+    // a split starts right after a line terminator ("\r\n" counts as one)
+    while (s > start[p - 1] && s < len &&
+           ((text[s - 1] != '\n' && text[s - 1] != '\r') || (text[s - 1] == '\r' && text[s] == '\n')))
+      s++;
+    start[p] = s;
+  }
+  auto run = [&](int ntasks, const std::function<void(int)>& task) {
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> err(ntasks);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(nthreads, ntasks); t++)
+      pool.emplace_back([&] {
+        for (int i; (i = next.fetch_add(1)) < ntasks;) {
+          try {
+            task(i);
+          } catch (...) {
+            err[i] = std::current_exception();
+          }
+        }
+      });
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
+  };
+  std::vector<std::vector<Record>> local(P);
+  run(P, [&](int p) {
+    std::vector<Record> prod;
+    ParseCtx c{k, &prod};
+    for_each_line(text + start[p], start[p + 1] - start[p],
+                  [&](size_t, const uint8_t* line, size_t n) { parse_line(c, line, n); });
+    local[p] = sort_and_group(prod);
+  });
+  BuildResult res;
+  res.k = k;
+  res.parts.resize(P);
+  run(P, [&](int part) {
+    std::vector<Record> merged;
+    for (int p = 0; p < P; p++)
+      for (auto& r : local[p])
+        if (partition_of(r.key, P) == part) merged.push_back(r);
+    res.parts[part] = sort_and_group(merged);
+  });
   return res;
 }
 
@@ -1927,6 +1992,22 @@ int ora_build(const char* text, size_t len, int k, int nparts, const int64_t* sp
     auto* r = new ora_result;
     try {
       r->r = build((const uint8_t*)text, len, k, nparts, split_starts, nsplits, local_only != 0);
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    for (size_t p = 0; p < r->r.parts.size(); p++) r->recs.push_back(part_records(r->r, (int)p));
+    *out = r;
+  });
+}
+
+// build_threaded: P partitions, P producers over equal byte ranges of the
+// input (line aligned), stages on `nthreads` threads.
+int ora_build_threaded(const char* text, size_t len, int k, int nparts, int nthreads, ora_result** out) {
+  return guard([&] {
+    auto* r = new ora_result;
+    try {
+      r->r = build_threaded((const uint8_t*)text, len, k, nparts, nthreads);
     } catch (...) {
       delete r;
       throw;

@@ -95,6 +95,7 @@ def lib():
         L.ora_last_error.restype = ctypes.c_char_p
         L.ora_build.argtypes = [ctypes.c_char_p, sz, i32, i32, ctypes.POINTER(i64), i32, i32,
                                 ctypes.POINTER(vp)]
+        L.ora_build_threaded.argtypes = [ctypes.c_char_p, sz, i32, i32, i32, ctypes.POINTER(vp)]
         L.ora_result_nparts.argtypes = [vp]
         L.ora_result_nrecords.argtypes = [vp, i32]
         L.ora_result_nrecords.restype = i64
@@ -178,6 +179,38 @@ def build_graph(text: bytes, k: int, nparts: int = 1, split_starts=None, local_o
         arr = (ctypes.c_int64 * len(split_starts))(*split_starts)
         sp, ns = arr, len(split_starts)
     _check(L.ora_build(text, len(text), k, nparts, sp, ns, 1 if local_only else 0, ctypes.byref(res)))
+    try:
+        parts = []
+        for p in range(L.ora_result_nparts(res)):
+            d = ctypes.c_void_p()
+            n = L.ora_result_part(res, p, ctypes.byref(d))
+            parts.append(ctypes.string_at(d, n))
+        return parts
+    finally:
+        L.ora_result_free(res)
+
+
+def split_starts(text: bytes, P: int):
+    """Line-aligned starts of P equal byte ranges (the producers of
+    build_graph_threaded)."""
+    n = len(text)
+    st = [0]
+    for p in range(1, P):
+        s = max(st[-1], n * p // P)
+        # a split starts right after a line terminator ("\r\n" counts as one)
+        while s > st[-1] and s < n and (text[s - 1] not in (10, 13) or (text[s - 1] == 13 and text[s] == 10)):
+            s += 1
+        st.append(s)
+    return st
+
+
+def build_graph_threaded(text: bytes, k: int, nparts: int, nthreads: int):
+    """The P-partition build with its producer and consumer stages on
+    threads (Hyracks with threadsPerMachine = P); equals
+    build_graph(text, k, P, split_starts(text, P))."""
+    L = lib()
+    res = ctypes.c_void_p()
+    _check(L.ora_build_threaded(text, len(text), k, nparts, nthreads, ctypes.byref(res)))
     try:
         parts = []
         for p in range(L.ora_result_nparts(res)):

@@ -138,3 +138,19 @@ def test_partitions_cover_everything():
             parts = oracle.build_graph(text, 3, nparts=P, split_starts=[0])
             allp = sorted(l for p in parts for l in oracle.dump_text(p).splitlines())
             assert allp == one
+
+
+@pytest.mark.parametrize("P,crlf", [(1, False), (3, False), (4, True), (8, False)])
+def test_threaded_build_equals_split_build(P, crlf):
+    """The threaded P-partition build (the multi-core CPU baseline) gives the
+    parts of the sequential build with the same producer splits."""
+    import random as _r
+    rng = _r.Random(P)
+    lines = []
+    for i in range(400):
+        s = "".join(rng.choice("ACGT") for _ in range(rng.randint(25, 60)))
+        m = "".join(rng.choice("ACGT") for _ in range(rng.randint(25, 60)))
+        lines.append(f"{4 * i + 2}\t{s}\t{m}" if i % 3 else f"{4 * i + 2}\t{s}")
+    text = (("\r\n" if crlf else "\n").join(lines) + "\n").encode()
+    want = oracle.build_graph(text, 21, P, oracle.split_starts(text, P))
+    assert oracle.build_graph_threaded(text, 21, P, 4) == want
