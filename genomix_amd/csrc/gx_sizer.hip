@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -217,30 +218,73 @@ void ensure_batch(gx_mlp* m, int B) {
   m->bcap = B;
 }
 
+// Device arrays of one gradient pass: inputs X [B][din], T [B][dout];
+// scratch A1, A2, D1, D2 [B][h], Y, D3 [B][dout], lossb [B] (every element
+// written before it is read); outputs G [params] and, when given, loss.
+struct GradBufs {
+  const float *X, *T;
+  float *A1, *A2, *Y, *D1, *D2, *D3, *lossb, *G, *loss;
+};
+
+// forward + deltas + gradient + loss sum, enqueued on st
+void mlp_gradient_on(const gx_mlp* m, const GradBufs& g, int B, hipStream_t st) {
+  const MlpDims& d = m->d;
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(B), dim3(kMaxHidden), 0, st, d, B, g.X, m->P.as<float>(), g.A1, g.A2, g.Y);
+  GX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_delta, dim3(B), dim3(kMaxHidden), 0, st, d, B, g.T, m->P.as<float>(), g.A1, g.A2, g.Y,
+                     g.D1, g.D2, g.D3, g.lossb);
+  GX_HIP(hipGetLastError());
+  size_t n = d.total();
+  hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, B, g.X, g.A1, g.A2, g.D1,
+                     g.D2, g.D3, g.G);
+  GX_HIP(hipGetLastError());
+  if (g.loss) {
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(64), 0, st, B, g.lossb, g.loss);
+    GX_HIP(hipGetLastError());
+  }
+}
+
 // forward + deltas + gradient into m->G; returns the loss
 float mlp_gradient(gx_mlp* m, const float* x, const float* t, int B) {
   const MlpDims& d = m->d;
   ensure_batch(m, B);
   GX_HIP(hipMemcpyAsync(m->X.p, x, (size_t)B * d.din * 4, hipMemcpyHostToDevice, m->st));
   GX_HIP(hipMemcpyAsync(m->T.p, t, (size_t)B * d.dout * 4, hipMemcpyHostToDevice, m->st));
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(B), dim3(kMaxHidden), 0, m->st, d, B, m->X.as<float>(), m->P.as<float>(),
-                     m->A1.as<float>(), m->A2.as<float>(), m->Y.as<float>());
-  GX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mlp_delta, dim3(B), dim3(kMaxHidden), 0, m->st, d, B, m->T.as<float>(), m->P.as<float>(),
-                     m->A1.as<float>(), m->A2.as<float>(), m->Y.as<float>(), m->D1.as<float>(), m->D2.as<float>(),
-                     m->D3.as<float>(), m->lossb.as<float>());
-  GX_HIP(hipGetLastError());
-  size_t n = d.total();
-  hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->st, d, B, m->X.as<float>(),
-                     m->A1.as<float>(), m->A2.as<float>(), m->D1.as<float>(), m->D2.as<float>(), m->D3.as<float>(),
-                     m->G.as<float>());
-  GX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(64), 0, m->st, B, m->lossb.as<float>(), m->loss.as<float>());
-  GX_HIP(hipGetLastError());
+  const GradBufs g{m->X.as<float>(), m->T.as<float>(), m->A1.as<float>(), m->A2.as<float>(), m->Y.as<float>(),
+                   m->D1.as<float>(), m->D2.as<float>(), m->D3.as<float>(), m->lossb.as<float>(), m->G.as<float>(),
+                   m->loss.as<float>()};
+  mlp_gradient_on(m, g, B, m->st);
   float l = 0.f;
   GX_HIP(hipMemcpyAsync(&l, m->loss.p, 4, hipMemcpyDeviceToHost, m->st));
   GX_HIP(hipStreamSynchronize(m->st));
   return l;
+}
+
+// Scratch of the allocation-free gradient (gx_mlp_gradient_noalloc): one
+// array per device, holding A1, A2, D1, D2, Y, D3 and the per-sample loss of
+// a batch of at most kNoAllocBatch samples at the largest network
+// dimensions (8.3 MiB).  In host code `scratch` only names the symbol; its
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr int kNoAllocBatch = 4096;
+constexpr size_t kNoAllocFloats =
+    (size_t)kNoAllocBatch * (4 * kMaxHidden + 2 * kMaxOut + 1);
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which is `device`).
+float* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) throw MlpArgError("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    GX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array", (int)hipErrorInvalidSymbol);
+    g_scratch_dev[device] = static_cast<float*>(p);
+  }
+  return g_scratch_dev[device];
 }
 
 }  // namespace
@@ -347,6 +391,25 @@ int gx_mlp_gradient(gx_mlp* m, const float* x, const float* t, int batch, float*
       GX_HIP(hipMemcpyAsync(grad, m->G.p, m->d.total() * 4, hipMemcpyDeviceToHost, m->st));
       GX_HIP(hipStreamSynchronize(m->st));
     }
+  });
+}
+
+int gx_mlp_gradient_noalloc(gx_mlp* m, const float* x, const float* t, int batch, float* grad, float* loss,
+                            void* stream) {
+  if (!m || !x || !t) return GX_EINVAL;
+  return mlp_guard(m, [&] {
+    if (batch <= 0) throw MlpArgError("batch must be positive");
+    if (batch > kNoAllocBatch)
+      throw MlpArgError("batch above " + std::to_string(kNoAllocBatch) + " samples, the scratch array's limit");
+    GX_HIP(hipSetDevice(m->device));
+    float* sp = scratch_on(m->device);
+    const MlpDims& d = m->d;
+    const size_t bh = (size_t)batch * d.h, bo = (size_t)batch * d.dout;
+    GradBufs g{x, t, sp, sp + bh, sp + 4 * bh, sp + 2 * bh, sp + 3 * bh, sp + 4 * bh + bo, sp + 4 * bh + 2 * bo,
+               grad ? grad : m->G.as<float>(), loss};
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : m->st;
+    mlp_gradient_on(m, g, batch, st);
+    if (!stream) GX_HIP(hipStreamSynchronize(st));
   });
 }
 

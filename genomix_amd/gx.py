@@ -36,6 +36,7 @@ EXPORTED = [
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
     "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried", "gx_unitig_limits",
+    "gx_mlp_gradient_noalloc",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
@@ -101,6 +102,7 @@ def lib():
         L.gx_mlp_get_params.argtypes = [vp, fp, sz]
         L.gx_mlp_forward.argtypes = [vp, fp, i32, fp]
         L.gx_mlp_gradient.argtypes = [vp, fp, fp, i32, fp, fp]
+        L.gx_mlp_gradient_noalloc.argtypes = [vp, vp, vp, i32, vp, vp, vp]
         L.gx_mlp_train_step.argtypes = [vp, fp, fp, i32, ctypes.c_float, fp]
         L.gx_mlp_last_error.argtypes = [vp]
         L.gx_mlp_last_error.restype = ctypes.c_char_p

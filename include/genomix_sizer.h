@@ -45,6 +45,20 @@ int gx_mlp_forward(gx_mlp* m, const float* x, int batch, float* y);
 /* Gradient of L = 1/(2B) sum (y - t)^2 with respect to the parameters
  * (grad may be NULL), and L. */
 int gx_mlp_gradient(gx_mlp* m, const float* x, const float* t, int batch, float* grad, float* loss);
+/* gx_mlp_gradient for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training or serving loops): no counterpart
+ * in the reference.  x [batch][din] and t [batch][dout] are DEVICE arrays;
+ * grad (params floats) and loss (one float) are device outputs, either may
+ * be NULL (the gradient then stays in the network's own buffer).  The
+ * activations and deltas live in a fixed device array of the library (one
+ * per GPU, 8.3 MiB, kept until the process ends), so batch must not exceed
+ * 4096 samples (GX_EINVAL otherwise) and the call makes no device
+ * allocation.  stream: a hipStream_t to enqueue on (the call then returns
+ * without waiting), or NULL for the network's own stream, synchronized
+ * before return.  Calls that overlap on one GPU share that array and race;
+ * order them on one stream. */
+int gx_mlp_gradient_noalloc(gx_mlp* m, const float* x, const float* t, int batch, float* grad, float* loss,
+                            void* stream);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch; loss before the step. */
 int gx_mlp_train_step(gx_mlp* m, const float* x, const float* t, int batch, float lr, float* loss);
 const char* gx_mlp_last_error(const gx_mlp* m);
