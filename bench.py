@@ -168,7 +168,15 @@ def batched_leg(steps, warmup, scale=1.0):
             "distinct_kmers": sum(x["distinct"] for x in st),
             "table_rebuilds": sum(x["table_rebuilds"] for x in st),
             "k_mix": {str(k): sum(1 for _, kk in specs if kk == k) for k in (21, 31, 55)},
-            "workers": tuning["workers"] if tuning else 4, "learned_sizing": net is not None,
+            "workers": tuning["workers"] if tuning else 4,
+            # the network runs on every set (its prediction is recorded), but the
+            # sorted aggregation sizes its buffers from exact counts between phases
+            # and every set here fits one pass, so the prediction sizes nothing
+            "learned_sizing": False,
+            "sizing_network": {"attached": net is not None,
+                               "sets_with_prediction": sum(1 for x in st if x["predicted_distinct"] > 0),
+                               "sizes": "the hash method's table and the key-range budget only; neither is used "
+                                        "by these one-pass sorted builds"},
             "sets_retried_alone": retried,
             "workload": "64 read sets, genomes log-uniform 100 kbp-10 Mbp" + (f" x{scale}" if scale != 1 else "") +
                         ", 100/150 bp SE/PE, 20-80x, 0.5-2% error, k 21/31/55; graph + unitigs per set"}
@@ -354,7 +362,6 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
-    st = builder.stats()
     devmem.synchronize()
     phases_sum = {}
     t0 = time.perf_counter()
@@ -365,6 +372,7 @@ def main():
     devmem.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
+    st = builder.stats()  # the same for every step
     ms_step = 1e3 * elapsed / max(args.steps, 1)
     N = st["occurrences"]
     value = N / (elapsed / args.steps)
@@ -403,15 +411,22 @@ def main():
     # pointer-jumping rounds of the unitig pass: every node end reads and
     # writes its own 16-byte record (32 B); ends still following their chain
     # also read their successor's (16 B), counted by the library
-    rounds = max(1, st["pj_rounds"])
-    t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
-    ends = 2 * st["distinct"]
-    pj_bytes = (32 * ends * rounds + 16 * st["pj_successor_reads"]) / rounds
-    ach_pj = pj_bytes / t_pj / 1e9 if t_pj > 0 else 0.0
-    tr_pj = load_traffic("k_pj_round_traffic_plain_gpu_5mbp.json")
-    roofline_pj = {"bound": "hbm", "achieved": round(ach_pj, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": round(ach_pj / HBM_PEAK_GBS, 4), "kernel": "k_pj_round",
-                   "traffic": tr_pj["bytes_per_launch"] if tr_pj else None,
+    # unitig ranking: walks from the path endpoints (odd k), or pointer-jumping
+    # rounds, in which every node end reads and writes its own 16-byte record
+    # (32 B) and ends still following their chain also read their
+    # successor's (16 B), counted by the library
+    if st["pj_rounds"] == 0:
+        ranking = {"method": "walks from path endpoints", "walkers": st["walkers"],
+                   "ms": round(phases.get("k_walk", 0.0), 4),
+                   "what": "k_walk_init + k_walk + k_walk_check: successor table, two walks per path"}
+    else:
+        rounds = st["pj_rounds"]
+        t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
+        ends = 2 * st["distinct"]
+        pj_bytes = (32 * ends * rounds + 16 * st["pj_successor_reads"]) / rounds
+        ach_pj = pj_bytes / t_pj / 1e9 if t_pj > 0 else 0.0
+        ranking = {"method": "pointer jumping", "bound": "hbm", "achieved": round(ach_pj, 1), "peak": HBM_PEAK_GBS,
+                   "unit": "GB/s", "frac": round(ach_pj / HBM_PEAK_GBS, 4), "kernel": "k_pj_round",
                    "algorithmic_bytes_per_launch": round(pj_bytes), "node_ends": ends,
                    "successor_reads_per_step": st["pj_successor_reads"],
                    "kernel_ms": round(t_pj * 1e3, 4), "launches_per_step": rounds}
@@ -514,7 +529,7 @@ def main():
         "roofline": roofline,
         "roofline_phases": roofline_phases,
         "roofline_kmer_pass": roofline_kmer,
-        "roofline_pointer_jumping": roofline_pj,
+        "unitig_ranking": ranking,
         "clean_bad_coverage": clean,
         "graph_stats": stats,
         "batched": batched,

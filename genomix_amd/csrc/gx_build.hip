@@ -515,17 +515,26 @@ __device__ inline uint32_t nibble(uint32_t t, uint32_t x) {
 
 // Canonical key of the k-mer at position p of a mate (L bases, packed
 // words w), its occurrence code and its neighbour bits (bit 4t + x).
-template <int W>
+// The neighbours' reverse complements come from the k-mer's own:
+// rc(w[1:]+c) = comp(c)+rc(w)[:-1] and rc(c+w[:-1]) = rc(w)[1:]+comp(c).
+// CODES = false: the key alone.
+template <int W, bool CODES = true>
 __device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32_t p, int k, uint32_t* code_out,
                               uint32_t* bits_out) {
   KV v = load_kmer<W>(w, p, k);
-  bool fwd;
-  KV key = canon<W>(v, k, &fwd);
+  const KV rv = revcomp<W>(v, k);
+  const bool fwd = kle<W>(v, rv);  // ties (palindromes) read forward
+  KV key = fwd ? v : rv;
   uint32_t code = 0, bits = 0;
+  if (!CODES) {
+    *code_out = 0;
+    *bits_out = 0;
+    return key;
+  }
   if (p + k < L) {  // the read's next k-mer w' = w[1:]+c
     uint32_t c = base_at(w, p + k);
     KV nx = succ_of<W>(v, k, c);
-    bool nfwd = kle<W>(nx, revcomp<W>(nx, k));
+    bool nfwd = kle<W>(nx, pred_of<W>(rv, k, 3 - c));
     // (cur F, next F) -> FF w'; (F, R) -> FR rc(w'); (R, F) -> RF w';
     // (R, R) -> RR rc(w')   (setEdgesForCurAndNext, cur side)
     uint32_t t = fwd ? (nfwd ? ET_FF : ET_FR) : (nfwd ? ET_RF : ET_RR);
@@ -536,7 +545,7 @@ __device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32
   if (p > 0) {  // the read's previous k-mer w'' = c+w[:-1]
     uint32_t c = base_at(w, p - 1);
     KV pv = pred_of<W>(v, k, c);
-    bool pfwd = kle<W>(pv, revcomp<W>(pv, k));
+    bool pfwd = kle<W>(pv, succ_of<W>(rv, k, 3 - c));
     // (prev F, cur F) -> RR w''; (F, R) -> FR w''; (R, F) -> RF rc(w'');
     // (R, R) -> FF rc(w'')   (setEdgesForCurAndNext, next side)
     uint32_t t = fwd ? (pfwd ? ET_RR : ET_RF) : (pfwd ? ET_FR : ET_FF);
@@ -937,15 +946,20 @@ __device__ inline uint64_t mate_search(const uint64_t* __restrict__ ob, uint64_t
   return lo;
 }
 
-// Occurrences [j0, j1) of a tile: finds their mates (a table of occbase in
-// LDS when the mate range is short) and calls f(index in tile, j, key, code)
-// for each, in rounds of kBlock consecutive occurrences.
-template <int W, class F>
+// Occurrences [j0, j1) of a tile: finds their mates and calls f(index in
+// tile, j, key, code) for each, in rounds of kBlock consecutive
+// occurrences.  When the tile's mates fit in LDS (always, for reads shorter
+// than kSortTile / kTileMateCap occurrences on average), a table gives each
+// occurrence's mate: every non-empty mate marks the slot of its first
+// occurrence and a running maximum over the tile fills the rest; otherwise
+// each occurrence searches occbase.
+template <int W, bool CODES, class F>
 __device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t nmates,
                                             const uint64_t* __restrict__ occbase, const uint32_t* __restrict__ mlen,
                                             const uint64_t* __restrict__ wbase, const uint64_t* __restrict__ words,
-                                            int k, uint64_t* lob, F&& f) {
+                                            int k, uint64_t* lob, uint16_t* mtab, F&& f) {
   __shared__ uint64_t mrange[2];
+  __shared__ uint16_t wmax[kBlock / 64];
   if (threadIdx.x == 0) {
     mrange[0] = mate_search(occbase, 0, nmates - 1, j0);
     mrange[1] = mate_search(occbase, mrange[0], nmates - 1, j1 - 1);
@@ -953,8 +967,39 @@ __device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t n
   __syncthreads();
   const uint64_t m0 = mrange[0], m1 = mrange[1];
   const bool local = m1 - m0 + 2 <= kTileMateCap;
-  if (local)
-    for (uint64_t i = threadIdx.x; i <= m1 - m0 + 1; i += kBlock) lob[i] = occbase[m0 + i];
+  if (local) {
+    for (uint64_t i = threadIdx.x; i <= m1 - m0 + 1; i += kBlock) lob[i] = i + m0 <= nmates ? occbase[m0 + i] : ~0ull;
+    for (uint32_t q = threadIdx.x; q < kSortTile; q += kBlock) mtab[q] = 0;
+    __syncthreads();
+    for (uint64_t i = threadIdx.x + 1; i <= m1 - m0; i += kBlock)
+      if (lob[i + 1] > lob[i] && lob[i] < j1) mtab[lob[i] - j0] = (uint16_t)i;
+    __syncthreads();
+    // running maximum: each thread over kSortTile / kBlock consecutive slots
+    constexpr int kPer = kSortTile / kBlock;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint16_t* my = mtab + threadIdx.x * kPer;
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) run = max(run, (uint32_t)my[q]);
+    uint32_t incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl = max(incl, y);
+    }
+    if (lane == 63) wmax[wid] = (uint16_t)incl;
+    __syncthreads();
+    uint32_t before = __shfl_up(incl, 1);
+    if (lane == 0) before = 0;
+    for (int v = 0; v < wid; v++) before = max(before, (uint32_t)wmax[v]);
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      before = max(before, (uint32_t)my[q]);
+      my[q] = (uint16_t)before;
+    }
+  } else {
+    __syncthreads();
+  }
   __syncthreads();
   for (int r = 0; r < kSortItems; r++) {
     uint64_t j = j0 + (uint64_t)r * kBlock + threadIdx.x;
@@ -962,10 +1007,10 @@ __device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t n
     KV key{0, 0};
     uint32_t code = 0;
     if (valid) {
-      uint64_t m = local ? m0 + mate_search(lob, 0, m1 - m0, j) : mate_search(occbase, m0, m1, j);
+      uint64_t m = local ? m0 + mtab[j - j0] : mate_search(occbase, m0, m1, j);
       uint64_t mb = local ? lob[m - m0] : occbase[m];
       uint32_t bits;
-      key = occ_info<W>(words + wbase[m], mlen[m], (uint32_t)(j - mb), k, &code, &bits);
+      key = occ_info<W, CODES>(words + wbase[m], mlen[m], (uint32_t)(j - mb), k, &code, &bits);
     }
     f(r, valid, j, key, code);
   }
@@ -981,17 +1026,18 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
   __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint16_t mtab[kSortTile];
   h[threadIdx.x] = 0;
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
-                          [&](int, bool valid, uint64_t, KV key, uint32_t) {
-                            if (valid) {
-                              SK sk;
-                              make_sort_key(key, nb, sk);
-                              const uint32_t t = key_byte(sk, nb - 1);
-                              if (t >= tlo && t < thi) atomicAdd(&h[key_byte(sk, dbyte)], 1u);
-                            }
-                          });
+  for_tile_occurrences<W, false>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+                                 [&](int, bool valid, uint64_t, KV key, uint32_t) {
+                                   if (valid) {
+                                     SK sk;
+                                     make_sort_key(key, nb, sk);
+                                     const uint32_t t = key_byte(sk, nb - 1);
+                                     if (t >= tlo && t < thi) atomicAdd(&h[key_byte(sk, dbyte)], 1u);
+                                   }
+                                 });
   __syncthreads();
   hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
@@ -1013,14 +1059,15 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
   __shared__ uint64_t run[256];
   __shared__ uint16_t wcnt[kBlock / 64][256];
   __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint16_t mtab[kSortTile];
   __shared__ uint32_t h2[256];  // the next key range's digit counts (as k_occ_hist would give)
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
   h2[threadIdx.x] = 0;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
-                          [&](int, bool valid, uint64_t j, KV key, uint32_t code) {
+  for_tile_occurrences<W, true>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+                                [&](int, bool valid, uint64_t j, KV key, uint32_t code) {
     for (int w = 0; w < kBlock / 64; w++) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
     SK sk{};
@@ -1068,16 +1115,17 @@ __global__ void __launch_bounds__(kBlock) k_top_hist(uint64_t N, uint64_t nmates
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
   __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint16_t mtab[kSortTile];
   h[threadIdx.x] = 0;
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob,
-                          [&](int, bool valid, uint64_t, KV key, uint32_t) {
-                            if (valid) {
-                              SK sk;
-                              make_sort_key(key, nb, sk);
-                              atomicAdd(&h[key_byte(sk, nb - 1)], 1u);
-                            }
-                          });
+  for_tile_occurrences<W, false>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+                                 [&](int, bool valid, uint64_t, KV key, uint32_t) {
+                                   if (valid) {
+                                     SK sk;
+                                     make_sort_key(key, nb, sk);
+                                     atomicAdd(&h[key_byte(sk, nb - 1)], 1u);
+                                   }
+                                 });
   __syncthreads();
   if (h[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
@@ -2087,28 +2135,85 @@ __device__ inline int side_deg(uint32_t mask, int d) {
 __device__ inline int back_side(uint32_t t) { return (t == ET_FF || t == ET_RF) ? 1 : 0; }
 
 // Per node: tandem flag, and for each side of degree 1 its partner
-// (node << 1 | partner side) or kEnd.
+// (node << 1 | partner side) or kEnd.  Neighbours come from the k-mer and its
+// reverse complement (FR: rc(w[1:]+x) = comp(x)+rc(w)[:-1]; RF: rc(x+w[:-1]) =
+// rc(w)[1:]+comp(x)); the two sides' lookups run interleaved, so their
+// loads overlap.
 template <int W>
 __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
                         const uint32_t* __restrict__ msk, NodeIndex ix, int k,
                         uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    KV v = node_key<W>(klo, khi, i);
-    uint32_t mask = msk[i] & 0xFFFFu;
+    const KV v = node_key<W>(klo, khi, i);
+    const KV rv = revcomp<W>(v, k);
+    const uint32_t mask = msk[i] & 0xFFFFu;
     bool tnd = false;
-    uint32_t l[2] = {kEnd, kEnd};
-    for (uint32_t t = 0; t < 4; t++) {
-      for (uint32_t x = 0; x < 4; x++) {
-        if (!((mask >> (4 * t + x)) & 1)) continue;
-        KV z = neighbour<W>(v, k, t, x);
-        if (keq<W>(z, v)) tnd = true;
-        int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
-        if (side_deg(mask, d) == 1) {
-          uint32_t j = node_lookup<W>(ix, z);
-          if (j == kEnd) atomicMin(err, (unsigned long long)i);
-          else l[d] = (j << 1) | (uint32_t)back_side(t);
+    KV z[2] = {v, v};
+    bool need[2] = {false, false};
+    uint32_t back[2] = {0, 0};
+    for (uint32_t m = mask; m; m &= m - 1) {
+      const uint32_t bit = (uint32_t)__builtin_ctz(m), t = bit >> 2, x = bit & 3;
+      const KV y = t == ET_FF ? succ_of<W>(v, k, x)
+                   : t == ET_FR ? pred_of<W>(rv, k, 3 - x)
+                   : t == ET_RF ? succ_of<W>(rv, k, 3 - x)
+                                : pred_of<W>(v, k, x);
+      if (keq<W>(y, v)) tnd = true;
+      const int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
+      if (side_deg(mask, d) == 1) {
+        z[d] = y;
+        need[d] = true;
+        back[d] = (uint32_t)back_side(t);
+      }
+    }
+    // both lookups: bucket bounds, then the probes side by side
+    uint64_t s[2] = {0, 0}, R[2] = {0, 0}, base[2] = {0, 0};
+    uint32_t found[2] = {kEnd, kEnd};
+    bool live[2];
+    for (int d = 0; d < 2; d++) {
+      live[d] = false;
+      if (!need[d]) continue;
+      const uint32_t b = node_bucket<W>(z[d], ix.nb, ix.bkb);
+      const uint32_t lo = ix.dir[b], hi = ix.dir[b + 1];
+      if (lo >= hi) continue;
+      R[d] = 2ull * (hi - lo);
+      base[d] = 2ull * lo;
+      s[d] = slot_home(khash<W>(z[d]), R[d]);
+      live[d] = true;
+    }
+    for (uint64_t probe = 0; live[0] || live[1]; probe++) {
+      for (int d = 0; d < 2; d++) {
+        if (!live[d]) continue;
+        if (probe >= R[d]) {
+          live[d] = false;
+          continue;
+        }
+        uint64_t key_lo, key_hi = 0;
+        uint32_t id;
+        if (W == 1) {
+          const Slot1 e = ((const Slot1*)ix.slots)[base[d] + s[d]];
+          key_lo = e.key;
+          id = e.count;
+        } else {
+          const Slot2 e = ((const Slot2*)ix.slots)[base[d] + s[d]];
+          key_lo = e.lo;
+          key_hi = e.hi;
+          id = e.count;
+        }
+        if (key_lo == z[d].lo && (W == 1 || key_hi == z[d].hi)) {
+          found[d] = id;
+          live[d] = false;
+        } else if ((W == 1 ? key_lo : key_hi) == kEmpty) {
+          live[d] = false;
+        } else {
+          s[d] = s[d] + 1 == R[d] ? 0 : s[d] + 1;
         }
       }
+    }
+    uint32_t l[2] = {kEnd, kEnd};
+    for (int d = 0; d < 2; d++) {
+      if (!need[d]) continue;
+      if (found[d] == kEnd) atomicMin(err, (unsigned long long)i);
+      else l[d] = (found[d] << 1) | back[d];
     }
     tand[i] = tnd ? 1 : 0;
     lnk[2 * i] = l[0];
@@ -2124,6 +2229,107 @@ __device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const u
   int sj = (int)(l & 1);
   if (j == i || tand[j]) return false;
   return side_deg(msk[j] & 0xFFFFu, sj) == 1;
+}
+
+// Ranking by walks (odd k).  A merge path is a chain of node ends, each
+// leaving its node through one side; next[e] is the end that follows e on
+// its path (kEnd at the path's end).  Each path endpoint walks its path
+// once, recording at every node end it passes (walker, position) and, at
+// the end, the path's length and smallest node with the side that node
+// leaves by; the two walkers of a path pass every node once each, leaving
+// it by opposite sides, so every end has one record.  A pass over the nodes
+// then turns the record of each node's F end into its unitig
+// representative (the smallest node; bit 31 when the node reads backwards
+// in the unitig, which reads its smallest node forwards) and position: the
+// values of the pointer-jumping ranking below.  Chains are short in read
+// graphs (errors end them every few letters: the 5 Mbp workload's longest
+// has ~130 nodes), so one dependent read per step replaces several full
+// passes over all ends.  Paths longer than the cap, cycles (no endpoint)
+// and even k (a palindromic k-mer links both its sides to one neighbour)
+// take the pointer-jumping ranking instead.
+constexpr uint64_t kEndUnvisited = ~0ull, kEndSingle = ~0ull - 1;
+constexpr uint32_t kWalkCap = 4096;
+
+// next[], the end records' initial values and the endpoint flags.
+__global__ void k_walk_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
+                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut,
+                            uint32_t* __restrict__ next, uint64_t* __restrict__ rec, uint32_t* __restrict__ endpoint) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    bool m[2];
+    uint32_t nx[2];
+    for (int d = 0; d < 2; d++) {
+      m[d] = mergeable(i, d, lnk, msk, tand, cut);
+      const uint32_t l = lnk[2 * i + d];
+      nx[d] = m[d] ? (((l >> 1) << 1) | (1u - (l & 1))) : kEnd;  // leave the partner by its other side
+    }
+    reinterpret_cast<uint2*>(next)[i] = make_uint2(nx[0], nx[1]);
+    const uint64_t r = (m[0] || m[1]) ? kEndUnvisited : kEndSingle;
+    reinterpret_cast<ulonglong2*>(rec)[i] = make_ulonglong2(r, r);
+    endpoint[i] = m[0] != m[1] ? 1u : 0u;
+  }
+}
+
+// The walkers' start ends, in node order.
+__global__ void k_walk_list(uint64_t nn, const uint32_t* __restrict__ endpoint, const uint64_t* __restrict__ at,
+                            const uint32_t* __restrict__ next, uint32_t* __restrict__ walkers) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x)
+    if (endpoint[i]) walkers[at[i]] = (uint32_t)(2 * i + (next[2 * i] != kEnd ? 0 : 1));
+}
+
+// One walk per path endpoint: rec[e] = walker << 32 | position of e's node
+// from the walker's start; sum[w] = length << 32 | (smallest node leaves by
+// its R side) << 31 | smallest node.
+__global__ void __launch_bounds__(kBlock) k_walk(uint32_t nwalk, const uint32_t* __restrict__ walkers,
+                                                 const uint32_t* __restrict__ next, uint64_t* __restrict__ rec,
+                                                 uint64_t* __restrict__ sum, uint32_t* __restrict__ capped) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwalk; w += gridDim.x * blockDim.x) {
+    uint32_t e = walkers[w], idx = 0, mn = e >> 1, mn_rev = e & 1;
+    for (;;) {
+      rec[e] = ((uint64_t)w << 32) | idx;
+      const uint32_t x = next[e];
+      if (x == kEnd) break;
+      if (idx + 1 >= kWalkCap) {
+        *capped = 1u;
+        break;
+      }
+      e = x;
+      idx++;
+      if ((e >> 1) < mn) {
+        mn = e >> 1;
+        mn_rev = e & 1;
+      }
+    }
+    sum[w] = ((uint64_t)(idx + 1) << 32) | ((uint64_t)mn_rev << 31) | mn;
+  }
+}
+
+// Per node from the record of its F end (the node reads forward along that
+// end's walker): representative, orientation, position; a node without a
+// record lies on a cycle or beyond the cap.
+__global__ void k_walk_final(uint64_t nn, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ sum,
+                             uint32_t* __restrict__ urep, uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
+                             uint64_t* __restrict__ ulen_of_rep, uint32_t* __restrict__ unranked) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = rec[2 * i];
+    if (r == kEndSingle) {
+      urep[i] = (uint32_t)i;
+      upos[i] = 0;
+      isrep[i] = 1;
+      ulen_of_rep[i] = 1;
+      continue;
+    }
+    if (r == kEndUnvisited) {
+      *unranked = 1u;
+      continue;
+    }
+    const uint64_t s = sum[r >> 32];
+    const uint32_t len = (uint32_t)(s >> 32), mn = (uint32_t)s & 0x7fffffffu, a = (uint32_t)r;
+    const bool against = ((s >> 31) & 1) != 0;  // the unitig reads against this walker
+    urep[i] = mn | (against ? 0x80000000u : 0u);
+    upos[i] = against ? len - 1 - a : a;
+    isrep[i] = mn == (uint32_t)i ? 1u : 0u;
+    ulen_of_rep[i] = mn == (uint32_t)i ? len : 0u;
+  }
 }
 
 // Packed list ranking: one 16-byte record per element (ptr, dist, min node,
@@ -2704,7 +2910,7 @@ struct gx_build {
   DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
   DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[13] = {};
+  uint64_t stats[14] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t evpj[2 * kMaxTimedRounds] = {};  // per pointer-jumping round
@@ -2713,10 +2919,11 @@ struct gx_build {
   bool ev_used[16] = {};
   double phase_ms[16] = {};
   // unitig pass
-  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep;
+  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep, u_next, u_walk, u_walk2;
   DevBuf u_uid, u_repnode, u_len, u_base, u_order, u_cov, u_edge, u_ecnt, u_hk, u_hk2, u_hv, u_hv2, u_uflag, u_uposh;
   DevBuf u_hu, u_hf, u_htot, u_hstart, u_hklo, u_hvc, u_rsize, u_roff, u_out;
-  uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0, u_succ_reads = 0;
+  uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0, u_succ_reads = 0, u_walkers = 0;
+  bool no_walk = false;  // rank by pointer jumping only (GX_UNITIG_RANK_POINTER_JUMPING)
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
@@ -2763,10 +2970,11 @@ static thread_local std::string g_create_err;  // errors of calls without a hand
 
 enum Phase {
   PH_PARSE, PH_INSERT, PH_NODES, PH_REPLAY, PH_HEADS, PH_SER, PH_KINSERT, PH_KREPLAY, PH_UNITIG, PH_SIZING, PH_NIX,
-  PH_KPJ, PH_KSCATTER, NPH
+  PH_KPJ, PH_KSCATTER, PH_KWALK, NPH
 };
 static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay", "heads", "serialize", "k_insert",
-                                       "k_replay", "unitig", "sizing", "node_index", "k_pj_round", "k_occ_scatter"};
+                                       "k_replay", "unitig", "sizing", "node_index", "k_pj_round", "k_occ_scatter",
+                                       "k_walk"};
 
 static void ph_begin(gx_build* b, int ph) {
   if (!b->prof) return;
@@ -2960,7 +3168,7 @@ static size_t handle_bytes(const gx_build* b) {
         &b->bb_typ, &b->bb_np, &b->bb_pbase, &b->bb_pairs, &b->bb_ed, &b->bb_into, &b->bb_hn, &b->bb_hb,
         &b->bb_addu, &b->bb_addf, &b->bb_cur, &b->bb_tot, &b->bb_astart, &b->bb_hnode, &b->bb_hmv, &b->bb_hofs,
         &b->bb_aval, &b->bb_aoff, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
-        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out})
+        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_next, &b->u_walk, &b->u_walk2})
     s += d->cap;
   return s;
 }
@@ -3701,13 +3909,57 @@ static void run_unitigs(gx_build* b) {
                        tand, errw);
     GX_HIP(hipGetLastError());
   }
-  uint4* R[2] = {buf<uint4>(b->u_ptr[0], ne + 1), buf<uint4>(b->u_ptr[1], ne + 1)};
+  uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
+  uint32_t* upos = buf<uint32_t>(b->u_pos, nn + 1);
+  uint32_t* isrep = buf<uint32_t>(b->u_isrep, nn + 1);
+  uint64_t* ulen_of_rep = buf<uint64_t>(b->u_lenrep, nn + 1);
+  // ranking by walks (odd k), pointer jumping when a walk cannot rank every node
+  bool walked = false;
+  b->u_walkers = 0;
+  if (nn && (k & 1) && !b->no_walk) {
+    ph_begin(b, PH_KWALK);
+    uint32_t* nxt = buf<uint32_t>(b->u_next, ne + 2);
+    uint64_t* rec = buf<uint64_t>(b->u_ptr[0], ne + 2);  // the pointer-jumping buffers, free until a fallback
+    uint32_t* endp = buf<uint32_t>(b->u_walk, nn + 1);
+    uint64_t* at = buf<uint64_t>(b->u_uid, nn + 1);
+    uint32_t* ws = buf<uint32_t>(b->u_any, 4);  // [0] capped, [1] unranked nodes
+    GX_HIP(hipMemsetAsync(ws, 0, 16, st));
+    const unsigned gn = grid_for(nn, kBlock, G * 4);
+    hipLaunchKernelGGL(k_walk_init, dim3(gn), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut, nxt, rec, endp);
+    GX_HIP(hipGetLastError());
+    exclusive_scan<uint32_t, uint64_t>(endp, at, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+    const uint64_t nw = read1(st, at + nn);
+    uint32_t* walkers = buf<uint32_t>(b->u_walk2, nw + 1);
+    uint64_t* sum = buf<uint64_t>(b->u_ptr[1], nw + 1);
+    if (nw) {
+      hipLaunchKernelGGL(k_walk_list, dim3(gn), dim3(kBlock), 0, st, nn, endp, at, nxt, walkers);
+      GX_HIP(hipGetLastError());
+      // one resident wave set, each lane walking ~a dozen paths in turn: a
+      // wave then waits for the longest of its lanes' sums, not of single paths
+      hipLaunchKernelGGL(k_walk, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, walkers, nxt, rec,
+                         sum, ws);
+      GX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_walk_final, dim3(gn), dim3(kBlock), 0, st, nn, rec, sum, urep, upos, isrep, ulen_of_rep,
+                       ws + 1);
+    GX_HIP(hipGetLastError());
+    ph_end(b, PH_KWALK);
+    uint32_t wr[4];
+    GX_HIP(hipMemcpyAsync(wr, ws, sizeof(wr), hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    walked = wr[0] == 0 && wr[1] == 0;
+    b->u_walkers = nw;
+  }
+  uint4* R[2] = {nullptr, nullptr};
   const unsigned jgrid = 8192;
   uint8_t* bflag = buf<uint8_t>(b->u_flag, jgrid);
   uint32_t* anyflag = buf<uint32_t>(b->u_any, 4);
   int rmax = 2;
   while ((1ull << rmax) < ne + 1) rmax++;
   int cur = 0, rounds = 0;
+  if (!walked) {
+  R[0] = buf<uint4>(b->u_ptr[0], ne + 1);
+  R[1] = buf<uint4>(b->u_ptr[1], ne + 1);
   for (int attempt = 0; attempt < 2; attempt++) {
     cur = 0;
     if (nn) {
@@ -3733,20 +3985,17 @@ static void run_unitigs(gx_build* b) {
     hipLaunchKernelGGL(k_pj_cut, dim3(grid_for(ne, kBlock, G)), dim3(kBlock), 0, st, ne, R[cur], lnk, cut);
     GX_HIP(hipGetLastError());
   }
-  unsigned long long e = read1(st, errw);
-  if (e != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the table");
-  b->u_succ_reads = read1(st, errw + 2);
-
-  uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
-  uint32_t* upos = buf<uint32_t>(b->u_pos, nn + 1);
-  uint32_t* isrep = buf<uint32_t>(b->u_isrep, nn + 1);
-  uint64_t* ulen_of_rep = buf<uint64_t>(b->u_lenrep, nn + 1);
-  uint64_t* uid_of_rep = buf<uint64_t>(b->u_uid, nn + 1);
   if (nn) {
     hipLaunchKernelGGL(k_pj_node_final, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, R[cur], urep, upos,
                        isrep, ulen_of_rep);
     GX_HIP(hipGetLastError());
   }
+  }  // pointer jumping
+  unsigned long long e = read1(st, errw);
+  if (e != ~0ull) throw GxError(GX_EDEVICE, "internal error: neighbour k-mer missing from the table");
+  b->u_succ_reads = read1(st, errw + 2);
+
+  uint64_t* uid_of_rep = buf<uint64_t>(b->u_uid, nn + 1);
   exclusive_scan<uint32_t, uint64_t>(isrep, uid_of_rep, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   uint64_t nu = read1(st, uid_of_rep + nn);
   uint32_t* urep_node = buf<uint32_t>(b->u_repnode, nu + 1);
@@ -5095,7 +5344,8 @@ int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   if (!b || !out) return GX_EINVAL;
   b->stats[10] = b->u_rounds;
   b->stats[12] = b->u_succ_reads;
-  for (int i = 0; i < n && i < 13; i++) out[i] = b->stats[i];
+  b->stats[13] = b->u_walkers;
+  for (int i = 0; i < n && i < 14; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 
@@ -5340,6 +5590,16 @@ int gx_unitig_run(gx_build* b) {
     else run_unitigs<2>(b);
     b->unitigs_done = true;
   });
+}
+
+int gx_unitig_set_ranking(gx_build* b, int mode) {
+  if (!b) return GX_EINVAL;
+  if (mode != GX_RANK_AUTO && mode != GX_RANK_POINTER_JUMPING) {
+    b->err = "unknown ranking mode";
+    return GX_EINVAL;
+  }
+  b->no_walk = mode == GX_RANK_POINTER_JUMPING;
+  return GX_OK;
 }
 
 int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset) {

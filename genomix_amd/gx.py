@@ -35,7 +35,7 @@ EXPORTED = [
     "gx_batch_set_sizer", "gx_batch_set_tuning", "gx_batch_run", "gx_batch_set_stats", "gx_batch_last_error",
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
-    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried", "gx_unitig_limits",
+    "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried", "gx_unitig_limits", "gx_unitig_set_ranking",
     "gx_mlp_gradient_noalloc",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
@@ -91,6 +91,7 @@ def lib():
         L.gx_unitig_run.argtypes = [vp]
         L.gx_unitig_info.argtypes = [vp, u64p, u64p]
         L.gx_unitig_limits.argtypes = [u64p, u64p]
+        L.gx_unitig_set_ranking.argtypes = [vp, i32]
         L.gx_unitig_emit.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.gx_build_add_fastq_pair.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
         L.gx_build_readids.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]
@@ -253,6 +254,11 @@ class GraphBuilder:
             return w.value
         return _host_bytes(nb, fill)
 
+    def set_ranking(self, mode):
+        """"auto" (walks for odd k, pointer jumping otherwise) or
+        "pointer_jumping"; the records are the same."""
+        self._check(lib().gx_unitig_set_ranking(self._h, {"auto": 0, "pointer_jumping": 1}[mode]))
+
     def run_unitigs(self):
         """Unitig compression of the built graph; records stay on the device."""
         self._check(lib().gx_unitig_run(self._h))
@@ -339,11 +345,11 @@ class GraphBuilder:
         return buf.raw[:n.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 13)()
-        self._check(lib().gx_build_stats(self._h, a, 13))
+        a = (ctypes.c_uint64 * 14)()
+        self._check(lib().gx_build_stats(self._h, a, 14))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
                 "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds",
-                "key_ranges", "pj_successor_reads"]
+                "key_ranges", "pj_successor_reads", "walkers"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):

@@ -125,6 +125,15 @@ int gx_unitig_run(gx_build* b);
 int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes);
 int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
 
+/* How the unitig pass ranks the nodes of its merge paths:
+ * GX_RANK_AUTO (default) walks each path from its endpoints when k is odd
+ * and falls back to pointer jumping for even k (palindromic k-mers), cycles
+ * and paths of more than 4096 nodes; GX_RANK_POINTER_JUMPING always uses
+ * pointer jumping.  The records are the same either way. */
+#define GX_RANK_AUTO 0
+#define GX_RANK_POINTER_JUMPING 1
+int gx_unitig_set_ranking(gx_build* b, int mode);
+
 /* Limits of the unitig pass, for callers that plan around them (no device
  * needed): max_nodes = the distinct k-mers one unitig pass, cleaning step or
  * BUBBLE accepts (2^30; larger graphs fail with GX_ENOTSUP);
@@ -142,7 +151,9 @@ int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset);
  * fallback for oversized key buckets (sorted aggregation), [10] pointer-
  * jumping rounds of the last unitig run, [11] key-range passes of the build
  * (genomix_stream.h), [12] successor records read by those pointer-jumping
- * rounds (node ends still following their chain, summed over the rounds). */
+ * rounds (node ends still following their chain, summed over the rounds),
+ * [13] path endpoints that walked their paths in the last unitig run (0 when
+ * it ranked by pointer jumping only). */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on

@@ -64,6 +64,52 @@ def test_overlap_graph_unitigs(gx, k):
         check(gx, text, k)
 
 
+@pytest.mark.parametrize("k,G,cov", [(21, 30000, 20), (31, 60000, 40), (33, 20000, 30), (55, 40000, 30)])
+def test_walk_ranking_equals_pointer_jumping(gx, k, G, cov):
+    """Odd k ranks by walks; forcing pointer jumping gives the same records,
+    and both equal the oracle's."""
+    from genomix_amd import sim
+    spec = sim.ReadSetSpec(genome_len=G, read_len=150, coverage=cov, error_rate=0.01, paired=True,
+                           insert_mean=500, insert_sd=50, repeat_frac=0.05, repeat_len=2000, repeat_div=0.01,
+                           seed=900 + k)
+    text = sim.simulate(spec)
+    outs, rounds = [], []
+    for mode in ("auto", "pointer_jumping"):
+        with gx.GraphBuilder(k) as b:
+            b.set_ranking(mode)
+            b.add_reads(text)
+            b.finish()
+            outs.append(b.unitigs())
+            rounds.append(b.stats()["pj_rounds"])
+    assert rounds[0] == 0 and rounds[1] > 0  # the walks ranked every node; the forced mode jumped
+    assert outs[0] == outs[1]
+    assert outs[0] == oracle.unitigs(oracle.build_graph(text, k)[0], k)
+
+
+def test_walk_falls_back_on_cycles_and_long_paths(gx):
+    """A circular genome (a cycle) and a path longer than the walk cap are
+    ranked by pointer jumping in the default mode, with the oracle's records."""
+    rng = random.Random(77)
+    k = 21
+    genome = "".join(rng.choice("ACGT") for _ in range(6000))
+    circ = genome + genome[:200]
+    reads = [circ[p:p + 120] for p in range(0, 6000, 40)]
+    text = "".join(f"{4 * i + 2}\t{s}\n" for i, s in enumerate(reads)).encode()
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        got = b.unitigs()
+        assert b.stats()["pj_rounds"] > 0
+    assert got == oracle.unitigs(oracle.build_graph(text, k)[0], k)
+    lin = "".join(f"{4 * i + 2}\t{genome[p:p + 120]}\n" for i, p in enumerate(range(0, 6000 - 120, 40))).encode()
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(lin)
+        b.finish()
+        got = b.unitigs()
+        assert b.stats()["pj_rounds"] > 0  # one path of ~5900 nodes, beyond the 4096-node cap
+    assert got == oracle.unitigs(oracle.build_graph(lin, k)[0], k)
+
+
 def test_cycles_and_tandems(gx):
     rng = random.Random(3)
     lines = []
