@@ -1043,7 +1043,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
 }
 
 // Stable scatter of the tile's records by the lower bucket byte (same
-// ranking as k_radix_scatter: wave match + lower waves' counts).
+// ranking as k_radix_scatter: wave match + lower waves' counts), staged in
+// LDS in digit order and written out as one contiguous run per digit.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nmates,
                                                         const uint64_t* __restrict__ occbase,
@@ -1057,26 +1058,30 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         uint32_t* __restrict__ nhist) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint64_t run[256];
+  __shared__ uint32_t lbase[256], lrun[256], wsum[kBlock / 64];
   __shared__ uint16_t wcnt[kBlock / 64][256];
   __shared__ uint64_t lob[kTileMateCap];
   __shared__ uint16_t mtab[kSortTile];
   __shared__ uint32_t h2[256];  // the next key range's digit counts (as k_occ_hist would give)
+  __shared__ SK sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];  // (occurrence - j0) << 8 | code
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
   h2[threadIdx.x] = 0;
+  tile_digit_bases(offs, ntiles, lbase, lrun, wsum);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
   for_tile_occurrences<W, true>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
                                 [&](int, bool valid, uint64_t j, KV key, uint32_t code) {
     for (int w = 0; w < kBlock / 64; w++) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
-    SK sk{};
+    SK skey{};
     uint32_t d = 0;
     if (valid) {
-      make_sort_key(key, nb, sk);
-      d = key_byte(sk, dbyte);
-      const uint32_t t = key_byte(sk, nb - 1);
-      if (nhist && t >= nlo && t < nhi) atomicAdd(&h2[key_byte(sk, ndbyte)], 1u);
+      make_sort_key(key, nb, skey);
+      d = key_byte(skey, dbyte);
+      const uint32_t t = key_byte(skey, nb - 1);
+      if (nhist && t >= nlo && t < nhi) atomicAdd(&h2[key_byte(skey, ndbyte)], 1u);
       valid = t >= tlo && t < thi;  // key-range pass: other records wait for their range
     }
     uint64_t act = __ballot(valid);
@@ -1090,17 +1095,25 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     if (valid && rank == 0) wcnt[wid][d] = (uint16_t)__popcll(match);
     __syncthreads();
     if (valid) {
-      uint64_t pos = run[d] + rank;
+      uint32_t pos = lrun[d] + rank;
       for (int w = 0; w < wid; w++) pos += wcnt[w][d];
-      okeys[pos] = sk;
-      ovals[pos] = (j << 8) | code;
+      sk[pos] = skey;
+      sv[pos] = ((uint32_t)(j - j0) << 8) | code;
     }
     __syncthreads();
     uint32_t tot = 0;
     for (int w = 0; w < kBlock / 64; w++) tot += wcnt[w][threadIdx.x];
-    run[threadIdx.x] += tot;
-    __syncthreads();
+    lrun[threadIdx.x] += tot;
   });
+  __syncthreads();
+  const uint32_t tn = lrun[255];  // records of this range in the tile
+  for (uint32_t i = threadIdx.x; i < tn; i += kBlock) {
+    const SK skey = sk[i];
+    const uint32_t d = key_byte(skey, dbyte);
+    const uint64_t pos = run[d] + (i - lbase[d]);
+    okeys[pos] = skey;
+    ovals[pos] = ((j0 + (sv[i] >> 8)) << 8) | (sv[i] & 0xffu);
+  }
   if (nhist) nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h2[threadIdx.x];
 }
 
@@ -2250,10 +2263,18 @@ __device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const u
 constexpr uint64_t kEndUnvisited = ~0ull, kEndSingle = ~0ull - 1;
 constexpr uint32_t kWalkCap = 4096;
 
-// next[], the end records' initial values and the endpoint flags.
+// One 16-byte entry per node end: .x = the next end on its path (kEnd at
+// the path's end), .z/.w = the walk record (low/high word).  A walk step
+// reads .x and writes the record of the same entry: one line per step.
+__device__ inline void end_record_store(uint4* E, uint32_t e, uint64_t r) {
+  reinterpret_cast<uint2*>(E + e)[1] = make_uint2((uint32_t)r, (uint32_t)(r >> 32));
+}
+__device__ inline uint64_t end_record(const uint4& x) { return ((uint64_t)x.w << 32) | x.z; }
+
+// Successors, the records' initial values and the endpoint flags.
 __global__ void k_walk_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
-                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut,
-                            uint32_t* __restrict__ next, uint64_t* __restrict__ rec, uint32_t* __restrict__ endpoint) {
+                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut, uint4* __restrict__ E,
+                            uint32_t* __restrict__ endpoint) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     bool m[2];
     uint32_t nx[2];
@@ -2262,31 +2283,31 @@ __global__ void k_walk_init(uint64_t nn, const uint32_t* __restrict__ lnk, const
       const uint32_t l = lnk[2 * i + d];
       nx[d] = m[d] ? (((l >> 1) << 1) | (1u - (l & 1))) : kEnd;  // leave the partner by its other side
     }
-    reinterpret_cast<uint2*>(next)[i] = make_uint2(nx[0], nx[1]);
     const uint64_t r = (m[0] || m[1]) ? kEndUnvisited : kEndSingle;
-    reinterpret_cast<ulonglong2*>(rec)[i] = make_ulonglong2(r, r);
+    E[2 * i] = make_uint4(nx[0], 0u, (uint32_t)r, (uint32_t)(r >> 32));
+    E[2 * i + 1] = make_uint4(nx[1], 0u, (uint32_t)r, (uint32_t)(r >> 32));
     endpoint[i] = m[0] != m[1] ? 1u : 0u;
   }
 }
 
 // The walkers' start ends, in node order.
 __global__ void k_walk_list(uint64_t nn, const uint32_t* __restrict__ endpoint, const uint64_t* __restrict__ at,
-                            const uint32_t* __restrict__ next, uint32_t* __restrict__ walkers) {
+                            const uint4* __restrict__ E, uint32_t* __restrict__ walkers) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x)
-    if (endpoint[i]) walkers[at[i]] = (uint32_t)(2 * i + (next[2 * i] != kEnd ? 0 : 1));
+    if (endpoint[i]) walkers[at[i]] = (uint32_t)(2 * i + (E[2 * i].x != kEnd ? 0 : 1));
 }
 
-// One walk per path endpoint: rec[e] = walker << 32 | position of e's node
-// from the walker's start; sum[w] = length << 32 | (smallest node leaves by
-// its R side) << 31 | smallest node.
+// One walk per path endpoint: record of end e = walker << 32 | position of
+// e's node from the walker's start; sum[w] = length << 32 | (smallest node
+// leaves by its R side) << 31 | smallest node.
 __global__ void __launch_bounds__(kBlock) k_walk(uint32_t nwalk, const uint32_t* __restrict__ walkers,
-                                                 const uint32_t* __restrict__ next, uint64_t* __restrict__ rec,
-                                                 uint64_t* __restrict__ sum, uint32_t* __restrict__ capped) {
+                                                 uint4* __restrict__ E, uint64_t* __restrict__ sum,
+                                                 uint32_t* __restrict__ capped) {
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwalk; w += gridDim.x * blockDim.x) {
     uint32_t e = walkers[w], idx = 0, mn = e >> 1, mn_rev = e & 1;
     for (;;) {
-      rec[e] = ((uint64_t)w << 32) | idx;
-      const uint32_t x = next[e];
+      const uint32_t x = E[e].x;
+      end_record_store(E, e, ((uint64_t)w << 32) | idx);
       if (x == kEnd) break;
       if (idx + 1 >= kWalkCap) {
         *capped = 1u;
@@ -2306,11 +2327,11 @@ __global__ void __launch_bounds__(kBlock) k_walk(uint32_t nwalk, const uint32_t*
 // Per node from the record of its F end (the node reads forward along that
 // end's walker): representative, orientation, position; a node without a
 // record lies on a cycle or beyond the cap.
-__global__ void k_walk_final(uint64_t nn, const uint64_t* __restrict__ rec, const uint64_t* __restrict__ sum,
+__global__ void k_walk_final(uint64_t nn, const uint4* __restrict__ E, const uint64_t* __restrict__ sum,
                              uint32_t* __restrict__ urep, uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
                              uint64_t* __restrict__ ulen_of_rep, uint32_t* __restrict__ unranked) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t r = rec[2 * i];
+    const uint64_t r = end_record(E[2 * i]);
     if (r == kEndSingle) {
       urep[i] = (uint32_t)i;
       upos[i] = 0;
@@ -2919,7 +2940,7 @@ struct gx_build {
   bool ev_used[16] = {};
   double phase_ms[16] = {};
   // unitig pass
-  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep, u_next, u_walk, u_walk2;
+  DevBuf u_lnk, u_tand, u_cut, u_ptr[2], u_flag, u_any, u_rep, u_pos, u_isrep, u_lenrep, u_walk, u_walk2;
   DevBuf u_uid, u_repnode, u_len, u_base, u_order, u_cov, u_edge, u_ecnt, u_hk, u_hk2, u_hv, u_hv2, u_uflag, u_uposh;
   DevBuf u_hu, u_hf, u_htot, u_hstart, u_hklo, u_hvc, u_rsize, u_roff, u_out;
   uint64_t nuheads = 0, u_bytes = 0, u_count = 0, u_rounds = 0, u_succ_reads = 0, u_walkers = 0;
@@ -3168,7 +3189,7 @@ static size_t handle_bytes(const gx_build* b) {
         &b->bb_typ, &b->bb_np, &b->bb_pbase, &b->bb_pairs, &b->bb_ed, &b->bb_into, &b->bb_hn, &b->bb_hb,
         &b->bb_addu, &b->bb_addf, &b->bb_cur, &b->bb_tot, &b->bb_astart, &b->bb_hnode, &b->bb_hmv, &b->bb_hofs,
         &b->bb_aval, &b->bb_aoff, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
-        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_next, &b->u_walk, &b->u_walk2})
+        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_walk, &b->u_walk2})
     s += d->cap;
   return s;
 }
@@ -3918,29 +3939,28 @@ static void run_unitigs(gx_build* b) {
   b->u_walkers = 0;
   if (nn && (k & 1) && !b->no_walk) {
     ph_begin(b, PH_KWALK);
-    uint32_t* nxt = buf<uint32_t>(b->u_next, ne + 2);
-    uint64_t* rec = buf<uint64_t>(b->u_ptr[0], ne + 2);  // the pointer-jumping buffers, free until a fallback
+    uint4* E = buf<uint4>(b->u_ptr[0], ne + 1);  // the pointer-jumping buffers, free until a fallback
     uint32_t* endp = buf<uint32_t>(b->u_walk, nn + 1);
     uint64_t* at = buf<uint64_t>(b->u_uid, nn + 1);
     uint32_t* ws = buf<uint32_t>(b->u_any, 4);  // [0] capped, [1] unranked nodes
     GX_HIP(hipMemsetAsync(ws, 0, 16, st));
     const unsigned gn = grid_for(nn, kBlock, G * 4);
-    hipLaunchKernelGGL(k_walk_init, dim3(gn), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut, nxt, rec, endp);
+    hipLaunchKernelGGL(k_walk_init, dim3(gn), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut, E, endp);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(endp, at, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
     const uint64_t nw = read1(st, at + nn);
     uint32_t* walkers = buf<uint32_t>(b->u_walk2, nw + 1);
     uint64_t* sum = buf<uint64_t>(b->u_ptr[1], nw + 1);
     if (nw) {
-      hipLaunchKernelGGL(k_walk_list, dim3(gn), dim3(kBlock), 0, st, nn, endp, at, nxt, walkers);
+      hipLaunchKernelGGL(k_walk_list, dim3(gn), dim3(kBlock), 0, st, nn, endp, at, E, walkers);
       GX_HIP(hipGetLastError());
       // one resident wave set, each lane walking ~a dozen paths in turn: a
       // wave then waits for the longest of its lanes' sums, not of single paths
-      hipLaunchKernelGGL(k_walk, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, walkers, nxt, rec,
-                         sum, ws);
+      hipLaunchKernelGGL(k_walk, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, walkers, E, sum,
+                         ws);
       GX_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_walk_final, dim3(gn), dim3(kBlock), 0, st, nn, rec, sum, urep, upos, isrep, ulen_of_rep,
+    hipLaunchKernelGGL(k_walk_final, dim3(gn), dim3(kBlock), 0, st, nn, E, sum, urep, upos, isrep, ulen_of_rep,
                        ws + 1);
     GX_HIP(hipGetLastError());
     ph_end(b, PH_KWALK);

@@ -236,8 +236,32 @@ __device__ __host__ inline uint32_t key_byte(const Key128& k, int b) {
   return b < 8 ? (uint32_t)(k.lo >> (8 * b)) & 0xff : (uint32_t)(k.hi >> (8 * (b - 8))) & 0xff;
 }
 
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr int kSortTile = kBlock * kSortItems;
+
+// Tile-local digit bases for a staged scatter: offs is the exclusive scan of
+// the digit-major per-tile counts (entry d * ntiles + tile, total last), so
+// this tile's count of digit d is the difference of two neighbours; lbase[d]
+// = elements of the tile with a smaller digit.  Block of kBlock = 256
+// threads, one digit each.
+__device__ inline void tile_digit_bases(const uint64_t* __restrict__ offs, unsigned ntiles, uint32_t* lbase,
+                                        uint32_t* lrun, uint32_t* wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const size_t at = (size_t)threadIdx.x * ntiles + blockIdx.x;
+  const uint32_t c = (uint32_t)(offs[at + 1] - offs[at]);
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t before = incl - c;
+  for (int w = 0; w < wid; w++) before += wsum[w];
+  lbase[threadIdx.x] = before;
+  lrun[threadIdx.x] = before;
+}
 
 template <class KeyT>
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const KeyT* __restrict__ keys, size_t n, int byte,
@@ -267,15 +291,23 @@ __device__ inline T load_stream(const T* p) {
   else return *p;
 }
 
+// The tile's elements are ranked into an LDS stage in digit order, then
+// written out as one contiguous run per digit (the tile's share of that
+// digit's output range): scattered element stores would leave lines partly
+// written across rounds, and the partial lines cost extra HBM writes.
 template <class KeyT, bool kVals, class ValT = uint32_t>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict__ keys, const ValT* __restrict__ vals,
                                                           size_t n, int byte, const uint64_t* __restrict__ offs,
                                                           unsigned ntiles, KeyT* __restrict__ okeys,
                                                           ValT* __restrict__ ovals) {
   __shared__ uint64_t run[256];
+  __shared__ uint32_t lbase[256], lrun[256], wsum[kBlock / 64];
   __shared__ uint16_t wcnt[kBlock / 64][256];
+  __shared__ KeyT sk[kSortTile];
+  __shared__ ValT sv[kVals ? kSortTile : 1];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
+  tile_digit_bases(offs, ntiles, lbase, lrun, wsum);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   size_t base = (size_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; r++) {
@@ -302,16 +334,24 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const KeyT* __restrict
     if (valid && rank == 0) wcnt[wid][d] = (uint16_t)__popcll(match);
     __syncthreads();
     if (valid) {
-      uint64_t pos = run[d] + rank;
+      uint32_t pos = lrun[d] + rank;
       for (int w = 0; w < wid; w++) pos += wcnt[w][d];
-      okeys[pos] = key;
-      if (kVals) ovals[pos] = val;
+      sk[pos] = key;
+      if (kVals) sv[pos] = val;
     }
     __syncthreads();
     uint32_t tot = 0;
     for (int w = 0; w < kBlock / 64; w++) tot += wcnt[w][threadIdx.x];
-    run[threadIdx.x] += tot;
-    __syncthreads();
+    lrun[threadIdx.x] += tot;
+  }
+  __syncthreads();
+  const uint32_t tn = (uint32_t)(n - base < (size_t)kSortTile ? n - base : (size_t)kSortTile);
+  for (uint32_t i = threadIdx.x; i < tn; i += kBlock) {
+    const KeyT key = sk[i];
+    const uint32_t d = key_byte(key, byte);
+    const uint64_t pos = run[d] + (i - lbase[d]);
+    okeys[pos] = key;
+    if (kVals) ovals[pos] = sv[i];
   }
 }
 
