@@ -188,22 +188,27 @@ def load_tuning():
 
 
 def learned_leg(steps, warmup):
-    """Learned-sizing workload: the hash-table aggregation sized by the trained
-    network, then sized from the occurrence count (the method without it)."""
+    """Learned-sizing workload (100 Mbp, 20% repeat families, 150 bp PE, 40x,
+    k = 31): the default sorted aggregation with and without the trained
+    network attached (the network's distinct-k-mer prediction sets the
+    key-range budget before the k-mer pass), and the hash-table aggregation
+    sized by the network and by the occurrence count (the method without
+    it).  Graph build only, input in HBM."""
     from genomix_amd import devmem, gx, sim, sizer
     spec = sim.LEARNED_100MBP
     t0 = time.perf_counter()
-    text = sim.simulate(spec)
-    log(f"learned: {len(text) / 1e9:.2f} GB generated in {time.perf_counter() - t0:.1f} s")
-    d = devmem.DeviceBytes(text)
-    n = len(text)
-    del text
+    chunks = list(sim.simulate_chunks(spec, chunk_reads=1 << 21, workers=12))
+    n = sum(len(c) for c in chunks)
+    log(f"learned: {n / 1e9:.2f} GB generated in {time.perf_counter() - t0:.1f} s")
+    d = devmem.DeviceBytes(b"".join(chunks))
+    del chunks
     res = {}
     occ = None
-    for name, net, method in (("learned_sizing", sizer.load(), "hash"), ("plain_sizing", None, "hash"),
-                              ("sorted_method", None, "sort")):
+    net = sizer.load()
+    for name, use_net, method in (("sorted_learned", True, "sort"), ("sorted_unsized", False, "sort"),
+                                  ("hash_learned", True, "hash"), ("hash_occurrence_sized", False, "hash")):
         try:
-            with gx.GraphBuilder(31, sizer=net, method=method) as b:
+            with gx.GraphBuilder(31, sizer=net if use_net else None, method=method) as b:
                 b.set_profiling(True)
 
                 def step():
@@ -212,21 +217,21 @@ def learned_leg(steps, warmup):
                     b.finish()
                 el = _timed(step, steps, warmup, name)
                 st = b.stats()
-                slot = 16
                 occ = st["occurrences"]
                 res[name] = {"ms_per_step": round(1e3 * el / steps, 3), "value": round(occ * steps / el, 1),
-                             "table_capacity": st["table_capacity"], "table_bytes": st["table_capacity"] * slot,
-                             "predicted_distinct": st["predicted_distinct"], "distinct": st["distinct"],
-                             "table_rebuilds": st["table_rebuilds"], "heads": st["heads"],
-                             "key_ranges": st["key_ranges"],
-                             "graph_bytes": sum(b.part_info(q)[1] for q in range(1)),
-                             "phases_ms": {kk: round(v, 3) for kk, v in b.phase_times().items()}}
-        except gx.GxError as e:  # the plain sizing may not fit in HBM at all
+                             "key_ranges": st["key_ranges"], "predicted_distinct": st["predicted_distinct"],
+                             "distinct": st["distinct"], "table_bytes": st["table_capacity"] * 16 if method == "hash"
+                             else 0, "table_rebuilds": st["table_rebuilds"]}
+        except gx.GxError as e:  # the occurrence-count table does not fit in HBM
             res[name] = {"error": str(e)}
             log(f"{name}: {e}")
     d.free()
-    return {"workload": "100 Mbp genome, 20% repeat families 1-10 kb at 2% divergence, 150 bp PE, 40x, "
-                        "1% error, k=31; graph build (hash-table aggregation with each sizing, then sorted)", "kmer_occurrences": occ, **res}
+    out = {"workload": "100 Mbp genome, 20% repeat families 1-10 kb at 2% divergence, 150 bp PE, 40x, 1% error, "
+                       "k=31; graph build", "kmer_occurrences": occ, **res}
+    a, b_ = res.get("sorted_learned", {}), res.get("sorted_unsized", {})
+    if "value" in a and "value" in b_:
+        out["sorted_learned_vs_unsized"] = round(a["value"] / b_["value"], 3)
+    return out
 
 
 def streamed_leg(steps, warmup, workers=12):
@@ -306,6 +311,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-sizing leg (100 Mbp)")
     ap.add_argument("--no-graph-steps", action="store_true",
                     help="skip the cleaning, statistics and BUBBLE timings after the plain leg")
     ap.add_argument("--batched-scale", type=float, default=1.0)
@@ -418,7 +424,7 @@ def main():
     if st["pj_rounds"] == 0:
         ranking = {"method": "walks from path endpoints", "walkers": st["walkers"],
                    "ms": round(phases.get("k_walk", 0.0), 4),
-                   "what": "k_walk_init + k_walk + k_walk_check: successor table, two walks per path"}
+                   "what": "k_walk_list + k_walk + k_walk_final (the walk entries come from k_links): one walk from each path endpoint"}
     else:
         rounds = st["pj_rounds"]
         t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
@@ -511,6 +517,12 @@ def main():
             batched["vs_plain"] = round(batched["value"] / value, 3)
         except Exception as e:  # the headline line is printed regardless
             batched = {"error": f"{type(e).__name__}: {e}"}
+    learned = None
+    if not args.no_learned:
+        try:
+            learned = learned_leg(2, 1)
+        except Exception as e:  # the headline line is printed regardless
+            learned = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(text, k)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps,
@@ -533,6 +545,7 @@ def main():
         "clean_bad_coverage": clean,
         "graph_stats": stats,
         "batched": batched,
+        "learned_sizing": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -867,8 +867,8 @@ constexpr int kNixLds = 4096;  // slots of one bucket region built in LDS (two-b
 constexpr int kNixLdsSmall = 1024;  // three-byte buckets: smaller regions, more blocks per CU
 template <int W, int LCAP>
 __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
-                                                      const uint32_t* __restrict__ dir, void* slots, uint32_t b0,
-                                                      uint32_t b1) {
+                                                      const uint32_t* __restrict__ msk, const uint32_t* __restrict__ dir,
+                                                      void* slots, uint32_t b0, uint32_t b1) {
   using S = typename std::conditional<W == 1, Slot1, Slot2>::type;
   __shared__ S lt[LCAP];
   for (uint32_t bk = b0 + blockIdx.x; bk < b1; bk += gridDim.x) {
@@ -893,6 +893,7 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
           if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmpty, (unsigned long long)key.lo) ==
               kEmpty) {
             e->count = (uint32_t)n;
+            e->mask = msk[n] & 0xFFFFu;
             break;
           }
         } else {  // distinct keys: a taken slot always holds another key
@@ -901,6 +902,7 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
               kEmpty) {
             e->lo = key.lo;
             e->count = (uint32_t)n;
+            e->mask = msk[n] & 0xFFFFu;
             break;
           }
         }
@@ -2147,15 +2149,43 @@ __device__ inline int side_deg(uint32_t mask, int d) {
 // facing back along an edge of type t.
 __device__ inline int back_side(uint32_t t) { return (t == ET_FF || t == ET_RF) ? 1 : 0; }
 
+// Initial walk records of a node end (ranking by walks, below): not yet
+// walked, or the end of a node that merges with nothing.
+constexpr uint64_t kEndUnvisited = ~0ull, kEndSingle = ~0ull - 1;
+constexpr uint32_t kWalkCap = 4096;
+
+// Neighbour y of k-mer v (reverse complement rv) for mask bit 4t + x:
+// FF w[1:]+x, FR rc(w[1:]+x) = comp(x)+rc(w)[:-1], RF rc(x+w[:-1]) =
+// rc(w)[1:]+comp(x), RR x+w[:-1].
+template <int W>
+__device__ inline KV mask_neighbour(KV v, KV rv, int k, uint32_t bit) {
+  const uint32_t t = bit >> 2, x = bit & 3;
+  return t == ET_FF ? succ_of<W>(v, k, x)
+         : t == ET_FR ? pred_of<W>(rv, k, 3 - x)
+         : t == ET_RF ? succ_of<W>(rv, k, 3 - x)
+                      : pred_of<W>(v, k, x);
+}
+// A node listing itself among its neighbours (a tandem repeat).
+template <int W>
+__device__ inline bool self_listed(KV v, KV rv, int k, uint32_t mask) {
+  for (uint32_t m = mask; m; m &= m - 1)
+    if (keq<W>(mask_neighbour<W>(v, rv, k, (uint32_t)__builtin_ctz(m)), v)) return true;
+  return false;
+}
+
 // Per node: tandem flag, and for each side of degree 1 its partner
-// (node << 1 | partner side) or kEnd.  Neighbours come from the k-mer and its
-// reverse complement (FR: rc(w[1:]+x) = comp(x)+rc(w)[:-1]; RF: rc(x+w[:-1]) =
-// rc(w)[1:]+comp(x)); the two sides' lookups run interleaved, so their
-// loads overlap.
+// (node << 1 | partner side) or kEnd.  The two sides' lookups run
+// interleaved, so their loads overlap.  With E given (ranking by walks),
+// also each node end's walk entry: the next end on its merge path when
+// the link is mergeable (neither node a tandem repeat, the partner's side
+// of degree 1; the index slot holds the partner's neighbour mask) and the
+// record's initial value, and whether the node ends a path (one mergeable
+// side).
 template <int W>
 __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
                         const uint32_t* __restrict__ msk, NodeIndex ix, int k,
-                        uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err) {
+                        uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err,
+                        uint4* __restrict__ E, uint32_t* __restrict__ endpoint) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
     const KV v = node_key<W>(klo, khi, i);
     const KV rv = revcomp<W>(v, k);
@@ -2165,12 +2195,10 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
     bool need[2] = {false, false};
     uint32_t back[2] = {0, 0};
     for (uint32_t m = mask; m; m &= m - 1) {
-      const uint32_t bit = (uint32_t)__builtin_ctz(m), t = bit >> 2, x = bit & 3;
-      const KV y = t == ET_FF ? succ_of<W>(v, k, x)
-                   : t == ET_FR ? pred_of<W>(rv, k, 3 - x)
-                   : t == ET_RF ? succ_of<W>(rv, k, 3 - x)
-                                : pred_of<W>(v, k, x);
+      const uint32_t bit = (uint32_t)__builtin_ctz(m);
+      const KV y = mask_neighbour<W>(v, rv, k, bit);
       if (keq<W>(y, v)) tnd = true;
+      const uint32_t t = bit >> 2;
       const int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
       if (side_deg(mask, d) == 1) {
         z[d] = y;
@@ -2180,7 +2208,7 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
     }
     // both lookups: bucket bounds, then the probes side by side
     uint64_t s[2] = {0, 0}, R[2] = {0, 0}, base[2] = {0, 0};
-    uint32_t found[2] = {kEnd, kEnd};
+    uint32_t found[2] = {kEnd, kEnd}, pmask[2] = {0, 0};
     bool live[2];
     for (int d = 0; d < 2; d++) {
       live[d] = false;
@@ -2201,19 +2229,22 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
           continue;
         }
         uint64_t key_lo, key_hi = 0;
-        uint32_t id;
+        uint32_t id, pm;
         if (W == 1) {
           const Slot1 e = ((const Slot1*)ix.slots)[base[d] + s[d]];
           key_lo = e.key;
           id = e.count;
+          pm = e.mask;
         } else {
           const Slot2 e = ((const Slot2*)ix.slots)[base[d] + s[d]];
           key_lo = e.lo;
           key_hi = e.hi;
           id = e.count;
+          pm = e.mask;
         }
         if (key_lo == z[d].lo && (W == 1 || key_hi == z[d].hi)) {
           found[d] = id;
+          pmask[d] = pm;
           live[d] = false;
         } else if ((W == 1 ? key_lo : key_hi) == kEmpty) {
           live[d] = false;
@@ -2231,6 +2262,20 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
     tand[i] = tnd ? 1 : 0;
     lnk[2 * i] = l[0];
     lnk[2 * i + 1] = l[1];
+    if (E) {
+      bool mg[2];
+      uint32_t nx[2];
+      for (int d = 0; d < 2; d++) {
+        mg[d] = l[d] != kEnd && !tnd && found[d] != (uint32_t)i &&
+                side_deg(pmask[d], (int)back[d]) == 1 &&
+                !self_listed<W>(z[d], revcomp<W>(z[d], k), k, pmask[d]);
+        nx[d] = mg[d] ? ((found[d] << 1) | (1u - back[d])) : kEnd;  // leave the partner by its other side
+      }
+      const uint64_t r = (mg[0] || mg[1]) ? kEndUnvisited : kEndSingle;
+      E[2 * i] = make_uint4(nx[0], 0u, (uint32_t)r, (uint32_t)(r >> 32));
+      E[2 * i + 1] = make_uint4(nx[1], 0u, (uint32_t)r, (uint32_t)(r >> 32));
+      endpoint[i] = mg[0] != mg[1] ? 1u : 0u;
+    }
   }
 }
 
@@ -2260,8 +2305,6 @@ __device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const u
 // passes over all ends.  Paths longer than the cap, cycles (no endpoint)
 // and even k (a palindromic k-mer links both its sides to one neighbour)
 // take the pointer-jumping ranking instead.
-constexpr uint64_t kEndUnvisited = ~0ull, kEndSingle = ~0ull - 1;
-constexpr uint32_t kWalkCap = 4096;
 
 // One 16-byte entry per node end: .x = the next end on its path (kEnd at
 // the path's end), .z/.w = the walk record (low/high word).  A walk step
@@ -2270,25 +2313,6 @@ __device__ inline void end_record_store(uint4* E, uint32_t e, uint64_t r) {
   reinterpret_cast<uint2*>(E + e)[1] = make_uint2((uint32_t)r, (uint32_t)(r >> 32));
 }
 __device__ inline uint64_t end_record(const uint4& x) { return ((uint64_t)x.w << 32) | x.z; }
-
-// Successors, the records' initial values and the endpoint flags.
-__global__ void k_walk_init(uint64_t nn, const uint32_t* __restrict__ lnk, const uint32_t* __restrict__ msk,
-                            const uint8_t* __restrict__ tand, const uint8_t* __restrict__ cut, uint4* __restrict__ E,
-                            uint32_t* __restrict__ endpoint) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
-    bool m[2];
-    uint32_t nx[2];
-    for (int d = 0; d < 2; d++) {
-      m[d] = mergeable(i, d, lnk, msk, tand, cut);
-      const uint32_t l = lnk[2 * i + d];
-      nx[d] = m[d] ? (((l >> 1) << 1) | (1u - (l & 1))) : kEnd;  // leave the partner by its other side
-    }
-    const uint64_t r = (m[0] || m[1]) ? kEndUnvisited : kEndSingle;
-    E[2 * i] = make_uint4(nx[0], 0u, (uint32_t)r, (uint32_t)(r >> 32));
-    E[2 * i + 1] = make_uint4(nx[1], 0u, (uint32_t)r, (uint32_t)(r >> 32));
-    endpoint[i] = m[0] != m[1] ? 1u : 0u;
-  }
-}
 
 // The walkers' start ends, in node order.
 __global__ void k_walk_list(uint64_t nn, const uint32_t* __restrict__ endpoint, const uint64_t* __restrict__ at,
@@ -3134,8 +3158,8 @@ static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src,
 // Key -> node index over nodes [0, nn) in key order whose keys' top byte
 // lies in [tlo, thi).
 template <int W>
-static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo, const uint64_t* khi, uint32_t tlo,
-                                  uint32_t thi) {
+static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo, const uint64_t* khi,
+                                  const uint32_t* msk, uint32_t tlo, uint32_t thi) {
   hipStream_t st = b->st;
   const int nb = b->nb;
   const unsigned G = b->grid;
@@ -3160,10 +3184,10 @@ static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo,
   GX_HIP(hipGetLastError());
   if (xbkb == 3)
     hipLaunchKernelGGL((k_nix_build<W, kNixLdsSmall>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo,
-                       khi, ndir, nslots, xb0, xb1);
+                       khi, msk, ndir, nslots, xb0, xb1);
   else
     hipLaunchKernelGGL((k_nix_build<W, kNixLds>), dim3(std::min(xb1 - xb0, 65536u)), dim3(kBlock), 0, st, klo, khi,
-                       ndir, nslots, xb0, xb1);
+                       msk, ndir, nslots, xb0, xb1);
   GX_HIP(hipGetLastError());
   return NodeIndex{ndir, nslots, nb, nbk, xbkb};
 }
@@ -3617,7 +3641,7 @@ static void run_build(gx_build* b) {
 
   // 5b. key -> node index (heads, unitig links)
   ph_begin(b, PH_NIX);
-  const NodeIndex nix = build_node_index<W>(b, nn, klo, khi, tlo, thi);
+  const NodeIndex nix = build_node_index<W>(b, nn, klo, khi, msk, tlo, thi);
   if (sorted) cap = 2 * nn;
   ph_end(b, PH_NIX);
 
@@ -3925,9 +3949,13 @@ static void run_unitigs(gx_build* b) {
   uint8_t* tand = buf<uint8_t>(b->u_tand, nn + 1);
   uint8_t* cut = buf<uint8_t>(b->u_cut, ne + 1);
   GX_HIP(hipMemsetAsync(cut, 0, ne + 1, st));
+  // ranking by walks (odd k): k_links writes the walk entries too
+  const bool walk = nn && (k & 1) && !b->no_walk;
+  uint4* E = walk ? buf<uint4>(b->u_ptr[0], ne + 1) : nullptr;  // the pointer-jumping buffers, free until a fallback
+  uint32_t* endp = walk ? buf<uint32_t>(b->u_walk, nn + 1) : nullptr;
   if (nn) {
     hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, nix, k, lnk,
-                       tand, errw);
+                       tand, errw, E, endp);
     GX_HIP(hipGetLastError());
   }
   uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
@@ -3937,16 +3965,12 @@ static void run_unitigs(gx_build* b) {
   // ranking by walks (odd k), pointer jumping when a walk cannot rank every node
   bool walked = false;
   b->u_walkers = 0;
-  if (nn && (k & 1) && !b->no_walk) {
+  if (walk) {
     ph_begin(b, PH_KWALK);
-    uint4* E = buf<uint4>(b->u_ptr[0], ne + 1);  // the pointer-jumping buffers, free until a fallback
-    uint32_t* endp = buf<uint32_t>(b->u_walk, nn + 1);
     uint64_t* at = buf<uint64_t>(b->u_uid, nn + 1);
     uint32_t* ws = buf<uint32_t>(b->u_any, 4);  // [0] capped, [1] unranked nodes
     GX_HIP(hipMemsetAsync(ws, 0, 16, st));
     const unsigned gn = grid_for(nn, kBlock, G * 4);
-    hipLaunchKernelGGL(k_walk_init, dim3(gn), dim3(kBlock), 0, st, nn, lnk, msk, tand, cut, E, endp);
-    GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(endp, at, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
     const uint64_t nw = read1(st, at + nn);
     uint32_t* walkers = buf<uint32_t>(b->u_walk2, nw + 1);
@@ -4424,7 +4448,8 @@ static void compact_graph(gx_build* b, uint32_t* keep, const AddedHeads& ah) {
   b->has_hoff = offs;
   b->nnodes = nn2;
   b->nuheads = nh2;
-  build_node_index<W>(b, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, 0, 256);
+  build_node_index<W>(b, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, b->msk.as<uint32_t>(),
+                      0, 256);
   run_unitigs<W>(b);
 }
 
