@@ -190,8 +190,9 @@ def load_tuning():
 def learned_leg(steps, warmup):
     """Learned-sizing workload (100 Mbp, 20% repeat families, 150 bp PE, 40x,
     k = 31): the default sorted aggregation with and without the trained
-    network attached (the network's distinct-k-mer prediction sets the
-    key-range budget before the k-mer pass), and the hash-table aggregation
+    network attached (a predicted distinct fraction above 0.6 would shrink
+    the key ranges; this read set predicts less, so both plan the same
+    ranges), and the hash-table aggregation
     sized by the network and by the occurrence count (the method without
     it).  Graph build only, input in HBM."""
     from genomix_amd import devmem, gx, sim, sizer
@@ -221,7 +222,8 @@ def learned_leg(steps, warmup):
                 res[name] = {"ms_per_step": round(1e3 * el / steps, 3), "value": round(occ * steps / el, 1),
                              "key_ranges": st["key_ranges"], "predicted_distinct": st["predicted_distinct"],
                              "distinct": st["distinct"], "table_bytes": st["table_capacity"] * 16 if method == "hash"
-                             else 0, "table_rebuilds": st["table_rebuilds"]}
+                             else 0, "table_rebuilds": st["table_rebuilds"],
+                             "phases_ms": {kk: round(v, 1) for kk, v in b.phase_times().items() if v > 0}}
         except gx.GxError as e:  # the occurrence-count table does not fit in HBM
             res[name] = {"error": str(e)}
             log(f"{name}: {e}")

@@ -3354,10 +3354,21 @@ static void run_build(gx_build* b) {
         live += d->cap;
       const double avail = (double)fr + (double)handle_bytes(b) - (double)live;
       const double tiles = (double)((N + kSortTile - 1) / kSortTile);
-      const double dfrac = b->pred[0] > 0 ? std::min(1.0, b->pred[0] / (double)N) : 0.6;
+      // distinct k-mers per occurrence: 0.6 covers resequencing depths; a
+      // predicted fraction above it (shallow or noisy read sets) shrinks the
+      // ranges, one below it never grows them (ranges planned to the edge of
+      // HBM cost more in allocation churn than a pass saves)
+      const double dfrac = std::max(0.6, b->pred[0] > 0 ? std::min(1.0, b->pred[0] / (double)N) : 0.0);
       const double per_occ = pass_bytes_per_occurrence(W, nb, dfrac);
       const double budget = avail - 0.1 * (double)tot - 48.0 * (double)nheads - 16.0 * 256.0 * tiles;
       target = budget > per_occ ? (uint64_t)(budget / per_occ) : 1;
+      // as few ranges as the budget allows, of even size: filling the first
+      // ranges up to the budget leaves the largest buffers at the edge of HBM
+      // (dead buffers freed and allocated again every build) for no fewer passes
+      if (target < N) {
+        const uint64_t R_min = (N + target - 1) / target;
+        target = std::min<uint64_t>(target, (uint64_t)(1.1 * (double)N / (double)R_min) + 1);
+      }
     }
     if (target < N) {
       unsigned long long* th = buf<unsigned long long>(b->tophist, 256);
