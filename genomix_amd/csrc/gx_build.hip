@@ -931,6 +931,10 @@ __host__ inline double pass_bytes_per_occurrence(int W, int nb, double dfrac) {
   const double out = 3.0 * (4 + nb) + 16;
   return rec + dfrac * (node + out);
 }
+// HBM one node of a key range holds until the build ends when the records
+// stay in HBM: its record (estimated as above) and its node graph (key
+// words, count, mask, two head counts, about one head).
+__host__ inline double kept_bytes_per_node(int W, int nb) { return 3.0 * (4 + nb) + 16 + 8.0 * W + 4 * 4 + 4; }
 constexpr int kIdxBits = 13;      // log2(kBucketCap)
 
 // The bucket bytes are sorted least significant first (LSD): the fused
@@ -1579,7 +1583,12 @@ __device__ inline LList j7_union(LList L, int sym, uint32_t hb) {
 // four lanes per node, one per edge type: the four lists are independent
 // state machines over the same records, so the serial chain per lane is a
 // quarter of the work.  Lanes 4q..4q+3 hold node q; the four 16-bit results
-// are merged with lane shuffles.
+// are merged with lane shuffles.  Each lane caches its last eight
+// transitions (n, order, symbol class) -> (n', order'): a list of fixed
+// letters moves between a handful of orders under the two reorderings (a
+// union with no letter, with a letter present), so after a few records every
+// lane hits its cache and the wave no longer runs the full union whenever
+// one of its 64 lanes misses.
 constexpr uint32_t kLongReplay = 64;
 
 __device__ inline int code_sym(uint32_t code, int t) {
@@ -1607,7 +1616,10 @@ __global__ void k_replay_long(uint64_t nlong, const uint32_t* __restrict__ rlist
       for (uint32_t x = 0; x < 4; x++)
         hb |= ((uint32_t)jdk7_spread(java_hash_vkmer<W>(neighbour<W>(v, k, (uint32_t)t, x), k, nb)) & 7u) << (3 * x);
       LList l{0, 0};
-      uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu;
+      uint32_t memo[8];  // key << 11 | (n' << 8 | order'); all ones = empty (keys have 14 bits)
+#pragma unroll
+      for (int m = 0; m < 8; m++) memo[m] = 0xffffffffu;
+      uint32_t rr = 0;  // slot the next miss fills
       const uint64_t a = seg[i], e = seg[i + 1];
       uint64_t g = a & ~15ull;
       uint4 cur = *reinterpret_cast<const uint4*>(codes + g);
@@ -1628,11 +1640,15 @@ __global__ void k_replay_long(uint64_t nlong, const uint32_t* __restrict__ rlist
               if ((int)ll_get(l, jj) == sy) cls = 4;
           }
           uint32_t key = ((uint32_t)l.n << 11) | (l.bits << 3) | cls;
-          if ((c0 >> 11) == key) { l.n = (c0 >> 8) & 7; l.bits = c0 & 0xff; continue; }
-          if ((c1 >> 11) == key) { l.n = (c1 >> 8) & 7; l.bits = c1 & 0xff; continue; }
+          uint32_t hit = 0xffffffffu;
+#pragma unroll
+          for (int m = 0; m < 8; m++) hit = (memo[m] >> 11) == key ? memo[m] : hit;
+          if (hit != 0xffffffffu) { l.n = (hit >> 8) & 7; l.bits = hit & 0xff; continue; }
           LList o = j7_union(l, sy, hb);
-          c1 = c0;
-          c0 = (key << 11) | ((uint32_t)o.n << 8) | o.bits;
+          const uint32_t nv = (key << 11) | ((uint32_t)o.n << 8) | o.bits;
+#pragma unroll
+          for (int m = 0; m < 8; m++) memo[m] = m == (int)rr ? nv : memo[m];
+          rr = (rr + 1) & 7;
           l = o;
         }
         cur = nxt;
@@ -2955,7 +2971,7 @@ struct gx_build {
   DevBuf selflag, selpos, selk, selv, selk2, selv2, lists, rpos, rkey, rval, rkey2, rval2;
   DevBuf hkey, hval, hkey2, hval2, uflag, upos, uval, hcu, hcf, htot, hstart, rsize, roff, out;
   DevBuf errw;
-  uint64_t stats[14] = {};
+  uint64_t stats[16] = {};
   // per-phase device timings (hipEvents on the handle's stream)
   bool prof = false;
   hipEvent_t evpj[2 * kMaxTimedRounds] = {};  // per pointer-jumping round
@@ -2986,7 +3002,8 @@ struct gx_build {
   DevBuf tophist, outacc;     // key ranges: top-byte histogram; records of earlier ranges
   DevBuf blist;               // record sort: buckets listed for the larger LDS tiers
   DevBuf hist2;               // key ranges: the next range's per-tile digit counts
-  DevBuf c_keep, c_pos, c_klo, c_khi, c_cnt, c_msk, c_hcu, c_hcf, c_htot, c_hstart, c_uval;  // graph cleaning
+  // graph cleaning: the compacted graph; key ranges: the node graph of the ranges so far
+  DevBuf c_keep, c_pos, c_klo, c_khi, c_cnt, c_msk, c_hcu, c_hcf, c_htot, c_hstart, c_uval;
   DevBuf hoff, c_hoff;  // per node head: its letter in the node's k-mer (after a bubble merge)
   DevBuf bb_min, bb_maj, bb_typ, bb_np, bb_pbase, bb_pairs, bb_ed, bb_into, bb_hn, bb_hb, bb_addu, bb_addf, bb_cur,
       bb_tot, bb_astart, bb_hnode, bb_hmv, bb_hofs, bb_aval, bb_aoff;  // bubble merge
@@ -2997,6 +3014,7 @@ struct gx_build {
   void* sink_user = nullptr;
   bool streamed = false;      // the last build's records went to the sink
   uint32_t nranges = 1;
+  bool graph_held = true;     // the node graph of the last build is in HBM (not after sink passes)
   int nix_bkb = 2;            // key bytes of the node index directory
   ReclaimList reclaim;        // buffers the last build reads no more
   // learned sizing
@@ -3153,7 +3171,14 @@ static void sizing_features(const uint8_t* regs, double occ_s, double valid_s, d
   f[71] = 1.f;
 }
 
-static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind);
+// Appends len bytes at dst + dlen, growing dst (to at least `hint` bytes
+// when that is larger) with its contents kept.
+static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind,
+                      size_t hint = 0);
+static void swap_buf(DevBuf& x, DevBuf& y) {
+  std::swap(x.p, y.p);
+  std::swap(x.cap, y.cap);
+}
 
 // Key -> node index over nodes [0, nn) in key order whose keys' top byte
 // lies in [tlo, thi).
@@ -3193,9 +3218,12 @@ static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo,
 }
 
 // Device bytes the handle holds (all of its buffers).
-static size_t handle_bytes(const gx_build* b) {
-  size_t s = b->u_ptr[0].cap + b->u_ptr[1].cap;
-  for (const DevBuf* d :
+// Every device buffer of a handle.
+template <class F>
+static void for_each_buf(gx_build* b, F&& f) {
+  f(&b->u_ptr[0]);
+  f(&b->u_ptr[1]);
+  for (DevBuf* d :
        {&b->text2, &b->rtext, &b->fterm1, &b->fterm2, &b->fsz, &b->foff, &b->text, &b->scratch, &b->blk, &b->blk2,
         &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase, &b->words, &b->valid, &b->nocc, &b->occbase,
         &b->hmask, &b->hidx, &b->table, &b->occ_slot, &b->occ_code, &b->claimed, &b->ckey, &b->cval, &b->ckey2,
@@ -3214,8 +3242,26 @@ static size_t handle_bytes(const gx_build* b) {
         &b->bb_addu, &b->bb_addf, &b->bb_cur, &b->bb_tot, &b->bb_astart, &b->bb_hnode, &b->bb_hmv, &b->bb_hofs,
         &b->bb_aval, &b->bb_aoff, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
         &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_walk, &b->u_walk2})
-    s += d->cap;
+    f(d);
+}
+
+static size_t handle_bytes(gx_build* b) {
+  size_t s = 0;
+  for_each_buf(b, [&](DevBuf* d) { s += d->cap; });
   return s;
+}
+
+// At the start of a build every buffer but the input is the last build's
+// and may be freed when an allocation would otherwise fail; a buffer the
+// build uses leaves the list on its first use (DevBuf::ensure).
+static void results_dead(gx_build* b) {
+  auto& dead = b->reclaim.dead;
+  dead.clear();
+  for_each_buf(b, [&](DevBuf* d) {
+    if (d != &b->text && d != &b->text2 && d != &b->rtext && d != &b->fterm1 && d != &b->fterm2 && d != &b->fsz &&
+        d != &b->foff && d->p)
+      dead.push_back(d);
+  });
 }
 
 template <int W>
@@ -3338,37 +3384,69 @@ static void run_build(gx_build* b) {
   //     ranges of the sort key's top byte when one pass would not fit in
   //     HBM (or when the caller bounds the occurrences per pass): every node
   //     lies in one range, the ranges follow KmerPointable order, and each
-  //     range's records are serialised and handed on before the next.
+  //     range's records are serialised and handed on (to the sink) or kept
+  //     in HBM with the range's node graph before the next.
   std::vector<uint32_t> tb{0u, 256u};
   std::vector<uint64_t> rocc{N};
+  std::vector<uint64_t> toph(256, 0);  // occurrences per top key byte (several ranges)
+  const bool keep = !b->sink;          // several ranges keep their records and node graph
+  DevBuf* const kept_bufs[8] = {&b->outacc, &b->c_klo, &b->c_khi, &b->c_cnt,
+                                &b->c_msk,  &b->c_hcu, &b->c_hcf, &b->c_uval};
+  const double tiles = (double)((N + kSortTile - 1) / kSortTile);
+  // HBM a pass may use: free now plus what this handle holds beyond the
+  // parsed reads (its buffers from an earlier build are reused) and beyond
+  // `kept` bytes this build holds for later, less a margin
+  auto pass_budget = [&](double kept) {
+    size_t fr = 0, tot = 0;
+    GX_HIP(hipMemGetInfo(&fr, &tot));
+    size_t live = 0;
+    for (DevBuf* d : {&b->text, &b->text2, &b->rtext, &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase,
+                      &b->words, &b->valid, &b->nocc, &b->occbase, &b->hmask, &b->hidx})
+      live += d->cap;
+    return (double)fr + (double)handle_bytes(b) - (double)live - kept - 0.1 * (double)tot - 48.0 * (double)nheads -
+           16.0 * 256.0 * tiles;
+  };
+  // as few ranges of n occurrences as `budget` allows, of even size: filling
+  // the first ranges up to the budget leaves the largest buffers at the edge
+  // of HBM (dead buffers freed and allocated again every build) for no
+  // fewer passes
+  auto range_target = [](uint64_t n, double budget, double per_occ) {
+    uint64_t target = budget > per_occ ? (uint64_t)(budget / per_occ) : 1;
+    if (target < n) {
+      const uint64_t r_min = (n + target - 1) / target;
+      target = std::min<uint64_t>(target, (uint64_t)(1.1 * (double)n / (double)r_min) + 1);
+    }
+    return target;
+  };
+  // ranges r, r+1, ... from top byte tb[r] on, each of at most `target`
+  // occurrences (a single byte value may hold more)
+  auto cut_ranges = [&](size_t r, uint64_t target) {
+    tb.resize(r + 1);
+    rocc.resize(r);
+    uint64_t cur = 0;
+    for (uint32_t t = tb[r]; t < 256; t++) {
+      if (cur && cur + toph[t] > target) {  // close the range before this byte value
+        tb.push_back(t);
+        rocc.push_back(cur);
+        cur = 0;
+      }
+      cur += toph[t];
+    }
+    tb.push_back(256u);
+    rocc.push_back(cur);
+  };
   if (sorted && N) {
     uint64_t target = b->range_occ;
     if (!target) {
-      // HBM free now plus what this handle holds beyond the parsed reads
-      // (its buffers from an earlier build are reused), less a margin
-      size_t fr = 0, tot = 0;
-      GX_HIP(hipMemGetInfo(&fr, &tot));
-      size_t live = 0;
-      for (DevBuf* d : {&b->text, &b->text2, &b->rtext, &b->term, &b->rid, &b->mstart, &b->mlen, &b->wcnt, &b->wbase,
-                        &b->words, &b->valid, &b->nocc, &b->occbase, &b->hmask, &b->hidx})
-        live += d->cap;
-      const double avail = (double)fr + (double)handle_bytes(b) - (double)live;
-      const double tiles = (double)((N + kSortTile - 1) / kSortTile);
-      // distinct k-mers per occurrence: 0.6 covers resequencing depths; a
-      // predicted fraction above it (shallow or noisy read sets) shrinks the
-      // ranges, one below it never grows them (ranges planned to the edge of
-      // HBM cost more in allocation churn than a pass saves)
-      const double dfrac = std::max(0.6, b->pred[0] > 0 ? std::min(1.0, b->pred[0] / (double)N) : 0.0);
+      // distinct k-mers per occurrence: the attached sizing network's
+      // prediction, else 0.6 (resequencing depths); ranges after the first
+      // are planned again from what the first ones measured
+      const double dfrac = b->pred[0] > 0 ? std::min(1.0, std::max(0.05, b->pred[0] / (double)N)) : 0.6;
       const double per_occ = pass_bytes_per_occurrence(W, nb, dfrac);
-      const double budget = avail - 0.1 * (double)tot - 48.0 * (double)nheads - 16.0 * 256.0 * tiles;
-      target = budget > per_occ ? (uint64_t)(budget / per_occ) : 1;
-      // as few ranges as the budget allows, of even size: filling the first
-      // ranges up to the budget leaves the largest buffers at the edge of HBM
-      // (dead buffers freed and allocated again every build) for no fewer passes
-      if (target < N) {
-        const uint64_t R_min = (N + target - 1) / target;
-        target = std::min<uint64_t>(target, (uint64_t)(1.1 * (double)N / (double)R_min) + 1);
-      }
+      const double budget = pass_budget(0);
+      target = range_target(N, budget, per_occ);
+      if (target < N && keep)  // several ranges: room for what they keep
+        target = range_target(N, budget - (double)N * dfrac * kept_bytes_per_node(W, nb), per_occ);
     }
     if (target < N) {
       unsigned long long* th = buf<unsigned long long>(b->tophist, 256);
@@ -3377,27 +3455,33 @@ static void run_build(gx_build* b) {
       hipLaunchKernelGGL(k_top_hist<W>, dim3(ntiles), dim3(kBlock), 0, st, N, nmates, occbase, mlen, wbase, words, k,
                          nb, th);
       GX_HIP(hipGetLastError());
-      uint64_t h[256];
-      GX_HIP(hipMemcpyAsync(h, th, sizeof(h), hipMemcpyDeviceToHost, st));
+      GX_HIP(hipMemcpyAsync(toph.data(), th, 256 * 8, hipMemcpyDeviceToHost, st));
       GX_HIP(hipStreamSynchronize(st));
-      tb.assign(1, 0u);
-      rocc.clear();
-      uint64_t cur = 0;
-      for (uint32_t t = 0; t < 256; t++) {
-        if (cur && cur + h[t] > target) {  // close the range before this byte value
-          tb.push_back(t);
-          rocc.push_back(cur);
-          cur = 0;
-        }
-        cur += h[t];
-      }
-      tb.push_back(256u);
-      rocc.push_back(cur);
+      cut_ranges(0, target);
     }
   }
-  const uint32_t R = (uint32_t)rocc.size();
+  uint32_t R = (uint32_t)rocc.size();
+  // Several ranges that keep their results: the stores that will hold the
+  // whole graph (the node arrays; the records too at P = 1) take the place
+  // of the accumulators for this build and the accumulators that of the
+  // per-range arrays, so a repeated build reuses one store of each size
+  // (swapped back after the last range) instead of growing both to the
+  // whole graph's size.
+  const bool acc_swap = R > 1 && keep;
+  auto swap_stores = [&] {
+    swap_buf(b->klo, b->c_klo);
+    if (W == 2) swap_buf(b->khi, b->c_khi);
+    swap_buf(b->cnt, b->c_cnt);
+    swap_buf(b->msk, b->c_msk);
+    swap_buf(b->hcu, b->c_hcu);
+    swap_buf(b->hcf, b->c_hcf);
+    swap_buf(b->uval, b->c_uval);
+    if (P == 1) swap_buf(b->out, b->outacc);
+  };
+  if (acc_swap) swap_stores();
   uint64_t cap = 0, rebuilds = 0, tot_nn = 0, tot_nrep = 0, tot_nsel = 0, tot_bytes = 0, acc_len = 0;
   std::vector<uint64_t> seg_src, seg_len;  // several ranges, no sink: range r's part p bytes in outacc
+  uint64_t glen[7] = {};  // several ranges, no sink: bytes of the node graph so far (klo, khi, cnt, msk, heads)
   std::vector<uint64_t> prec(P + 1, 0), pbyt(P + 1, 0);  // per partition: records and bytes so far
   b->nuheads = 0;
   b->has_hoff = false;
@@ -3407,6 +3491,23 @@ static void run_build(gx_build* b) {
   DevBuf *hcur = &b->selflag, *hnext = &b->hist2;  // per-tile digit counts: this range's, the next one's
   bool hist_ready = false;
   for (uint32_t ri = 0; ri < R; ri++) {
+  // the ranges after this one again, from what the ranges so far measured
+  // (nodes and kept bytes per occurrence) and the HBM free now; this range's
+  // own bounds stay (the last pass counted its digits)
+  if (!b->range_occ && ri >= 1 && ri + 1 < R) {
+    uint64_t done = 0, rest = 0;
+    for (uint32_t r = 0; r < ri; r++) done += rocc[r];
+    for (uint32_t r = ri + 1; r < R; r++) rest += rocc[r];
+    double kept_now = 0, kept_len = (double)acc_len;
+    if (keep)
+      for (DevBuf* d : kept_bufs) kept_now += (double)d->cap;
+    for (uint64_t g : glen) kept_len += (double)g;
+    const double dfrac = std::max(1e-3, (double)tot_nn / (double)std::max<uint64_t>(done, 1));
+    const double kpo = keep ? kept_len / (double)std::max<uint64_t>(done, 1) : 0.0;
+    const double more = std::max(0.0, 1.05 * kpo * (double)N - kept_now);  // kept bytes still to come
+    cut_ranges(ri + 1, range_target(rest, pass_budget(kept_now + more), pass_bytes_per_occurrence(W, nb, dfrac)));
+    R = (uint32_t)rocc.size();
+  }
   const uint32_t tlo = tb[ri], thi = tb[ri + 1];
   const uint64_t Nr = rocc[ri];
   uint64_t nn = 0;
@@ -3781,7 +3882,8 @@ static void run_build(gx_build* b) {
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint32_t, uint64_t>(uflag, upos, nh, scratch_for(b, scan_scratch_bytes<uint64_t>(nh)), st);
-  b->nuheads += read1(st, upos + nh);
+  const uint64_t nuh_r = read1(st, upos + nh);  // unique heads of this range
+  b->nuheads += nuh_r;
   if (nh) {
     hipLaunchKernelGGL(k_head_compact, dim3(grid_for(nh, kBlock, G)), dim3(kBlock), 0, st, nh, hk, hv, uflag, upos,
                        uval, hcu, hcf);
@@ -3793,8 +3895,8 @@ static void run_build(gx_build* b) {
   }
   exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
   ph_end(b, PH_HEADS);
-  // head sort arrays are dead; with several key ranges so is the node index
-  // (the unitig pass, which looks nodes up, needs a single-range build)
+  // head sort arrays are dead; with several key ranges so is the range's
+  // node index (rebuilt over the whole graph after the last range)
   b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->hkey, &b->hval, &b->hkey2, &b->hval2, &b->uflag, &b->upos});
   if (R > 1) b->reclaim.dead.push_back(&b->nix_slots);
 
@@ -3877,11 +3979,27 @@ static void run_build(gx_build* b) {
       if (rc) throw GxError(rc < 0 ? rc : GX_EINVAL, "sink returned " + std::to_string(rc));
     }
   } else if (R > 1) {  // kept in HBM; partitions are assembled after the last range
-    append_to(b, b->outacc, acc_len, out, total, hipMemcpyDeviceToDevice);
+    // the first range sizes each store for the whole build from its own
+    // bytes per occurrence (no re-allocation with both copies live later)
+    auto whole = [&](size_t len) -> size_t {
+      return ri == 0 && Nr * 256 >= N ? (size_t)(1.15 * (double)len * (double)N / (double)Nr) : 0;
+    };
+    append_to(b, b->outacc, acc_len, out, total, hipMemcpyDeviceToDevice, whole(total));
     for (uint32_t p = 0; p < P; p++) {
       seg_src.push_back(acc_len - total + b->part_off[p]);
       seg_len.push_back(b->part_off[p + 1] - b->part_off[p]);
     }
+    // and the range's node graph: ranges follow key order, so the nodes of
+    // range after range are the whole graph's nodes in key order; heads are
+    // stored per node and carry no node id
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+    append_to(b, b->c_klo, glen[0], klo, nn * 8, dd, whole(nn * 8));
+    if (W == 2) append_to(b, b->c_khi, glen[1], khi, nn * 8, dd, whole(nn * 8));
+    append_to(b, b->c_cnt, glen[2], cnt, nn * 4, dd, whole(nn * 4));
+    append_to(b, b->c_msk, glen[3], msk, nn * 4, dd, whole(nn * 4));
+    append_to(b, b->c_hcu, glen[4], hcu, nn * 4, dd, whole(nn * 4));
+    append_to(b, b->c_hcf, glen[5], hcf, nn * 4, dd, whole(nn * 4));
+    append_to(b, b->c_uval, glen[6], uval, nuh_r * 4, dd, whole(nuh_r * 4));
   }
   for (uint32_t p = 0; p < P; p++) {
     prec[p + 1] += b->part_rec[p + 1] - b->part_rec[p];
@@ -3902,8 +4020,7 @@ static void run_build(gx_build* b) {
   if (!b->sink && R > 1) {
     // part p = its bytes of range 0, then of range 1, ...
     if (P == 1) {
-      std::swap(b->out.p, b->outacc.p);
-      std::swap(b->out.cap, b->outacc.cap);
+      swap_buf(b->out, b->outacc);
       b->out.ensure(tot_bytes + 1);  // live again: off the reclaim list (no allocation)
     } else {
       uint8_t* out = buf<uint8_t>(b->out, tot_bytes + 1);
@@ -3919,7 +4036,34 @@ static void run_build(gx_build* b) {
       GX_HIP(hipStreamSynchronize(st));
     }
     b->reclaim.dead.push_back(&b->outacc);
+    // the whole node graph becomes the handle's, with its heads' starts and
+    // a node index over every node (the graph steps look nodes up)
+    DevBuf* held[7] = {&b->klo, &b->khi, &b->cnt, &b->msk, &b->hcu, &b->hcf, &b->uval};
+    DevBuf* acc[7] = {&b->c_klo, &b->c_khi, &b->c_cnt, &b->c_msk, &b->c_hcu, &b->c_hcf, &b->c_uval};
+    for (int i = 0; i < 7; i++) {
+      if (i == 1 && W != 2) continue;
+      swap_buf(*held[i], *acc[i]);
+      held[i]->ensure(glen[i] + 1);  // live again: off the reclaim list (no allocation)
+      b->reclaim.dead.push_back(acc[i]);
+    }
+    const uint64_t nn = tot_nn;
+    uint64_t* htot = buf<uint64_t>(b->htot, nn + 1);
+    uint64_t* hstart = buf<uint64_t>(b->hstart, nn + 1);
+    if (nn) {
+      hipLaunchKernelGGL(k_head_total, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, b->hcu.as<uint32_t>(),
+                         b->hcf.as<uint32_t>(), htot);
+      GX_HIP(hipGetLastError());
+    }
+    exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+    ph_begin(b, PH_NIX);
+    build_node_index<W>(b, nn, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr,
+                        b->msk.as<uint32_t>(), 0u, 256u);
+    ph_end(b, PH_NIX);
+    GX_HIP(hipStreamSynchronize(st));
+    ph_collect(b, true);
+    b->nnodes = nn;
   }
+  b->graph_held = R == 1 || !b->sink;
   b->part_rec = prec;
   b->part_off = pbyt;
   b->streamed = b->sink != nullptr;
@@ -4188,16 +4332,24 @@ static int guarded(gx_build* b, F&& f) {
   }
 }
 
-static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind) {
+static void append_to(gx_build* b, DevBuf& dst, uint64_t& dlen, const void* src, size_t len, hipMemcpyKind kind,
+                      size_t hint) {
+  reclaim_forget(&dst);  // in use (a dead listing from an earlier build must not free it)
   size_t need = dlen + len + 64;
   if (need > dst.cap) {
-    size_t ncap = std::max(need, dst.cap * 3 / 2);
+    size_t ncap = std::max({need, hint, dst.cap * 3 / 2});
     void* np = nullptr;
     hipError_t e = hipMalloc(&np, ncap);
-    if (e == hipErrorOutOfMemory && reclaim_dead(&dst)) {  // free the handle's dead buffers, then again
+    while (e == hipErrorOutOfMemory && reclaim_dead(&dst, ncap)) {  // free the handle's dead buffers one by one
       (void)hipGetLastError();
       e = hipMalloc(&np, ncap);
     }
+    if (e == hipErrorOutOfMemory && ncap > need) {  // the estimate or the growth step does not fit: what is needed
+      (void)hipGetLastError();
+      ncap = need;
+      e = hipMalloc(&np, ncap);
+    }
+    if (tl_reclaim) tl_reclaim->allocs++;
     if (e != hipSuccess) {
       (void)hipGetLastError();
       throw HipError("hipMalloc of " + std::to_string(ncap) + " bytes: " + hipGetErrorString(e), (int)e);
@@ -4399,11 +4551,6 @@ __global__ void k_keep_heads(uint64_t nn, const uint32_t* __restrict__ keep, con
       }
     }
   }
-}
-
-static void swap_buf(DevBuf& x, DevBuf& y) {
-  std::swap(x.p, y.p);
-  std::swap(x.cap, y.cap);
 }
 
 // The nodes with keep[i] set, with their heads and those of ah, become the
@@ -5350,10 +5497,13 @@ int gx_build_finish(gx_build* b) {
     }
     b->unitigs_done = false;
     b->st_graph = -1;
-    b->reclaim.dead.clear();
+    results_dead(b);
     ReclaimScope rs(&b->reclaim, b->st);
+    b->reclaim.allocs = b->reclaim.freed = 0;
     if (b->W == 1) run_build<1>(b);
     else run_build<2>(b);
+    b->stats[14] = b->reclaim.allocs;
+    b->stats[15] = b->reclaim.freed;
     b->finished = true;
   });
 }
@@ -5401,7 +5551,7 @@ int gx_build_stats(gx_build* b, uint64_t* out, int n) {
   b->stats[10] = b->u_rounds;
   b->stats[12] = b->u_succ_reads;
   b->stats[13] = b->u_walkers;
-  for (int i = 0; i < n && i < 14; i++) out[i] = b->stats[i];
+  for (int i = 0; i < n && i < 16; i++) out[i] = b->stats[i];
   return GX_OK;
 }
 
@@ -5535,7 +5685,7 @@ static void fit_mixture(const double* x, const uint64_t* w, size_t n, double max
 static void check_graph(gx_build* b, int graph) {
   if (graph != GX_GRAPH_NODES && graph != GX_GRAPH_UNITIGS) throw GxError(GX_EINVAL, "unknown graph");
   if (!b->finished) throw GxError(GX_ESTATE, "statistics before finish");
-  if (b->nranges > 1) throw GxError(GX_ENOTSUP, "the node graph of a multi-range build is not held in HBM");
+  if (!b->graph_held) throw GxError(GX_ENOTSUP, "the node graph of a build with a record sink is not held in HBM");
   if (graph == GX_GRAPH_UNITIGS && !b->unitigs_done) throw GxError(GX_ESTATE, "no unitig pass to report on");
 }
 
@@ -5636,9 +5786,9 @@ int gx_unitig_run(gx_build* b) {
   if (!b) return GX_EINVAL;
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "unitig_run before finish");
-    if (b->nranges > 1)
-      throw GxError(GX_ENOTSUP, "the unitig pass needs the whole graph in HBM; this build ran in " +
-                                    std::to_string(b->nranges) + " key ranges");
+    if (!b->graph_held)
+      throw GxError(GX_ENOTSUP, "the unitig pass needs the whole graph in HBM; this build handed its " +
+                                    std::to_string(b->nranges) + " key ranges to a record sink");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->st_graph = -1;

@@ -34,6 +34,8 @@ struct DevBuf;
 struct ReclaimList {
   hipStream_t st = nullptr;
   std::vector<DevBuf*> dead;
+  uint64_t allocs = 0;  // device allocations made while the list was installed
+  uint64_t freed = 0;   // listed buffers freed to make room
 };
 inline thread_local ReclaimList* tl_reclaim = nullptr;
 
@@ -55,40 +57,61 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
-// Frees the listed buffers (after the stream has drained: kernels in flight
-// may still read them); false if there was nothing to free.
-inline bool reclaim_dead(const DevBuf* self) {
+// Frees one listed buffer other than `self` (after the stream has drained:
+// kernels in flight may still read it) toward an allocation of `need`
+// bytes: the smallest that covers what free memory lacks, else the
+// largest; false if none is left.  Callers free one at a time until their
+// allocation fits, so a build short of memory gives up as little of the
+// next build's buffers as it can.
+inline bool reclaim_dead(const DevBuf* self, size_t need) {
   ReclaimList* r = tl_reclaim;
-  if (!r || r->dead.empty()) return false;
+  if (!r) return false;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+  const size_t lack = need > fr ? need - fr : 0;
+  const size_t none = r->dead.size();
+  size_t fit = none, big = none;
+  for (size_t i = 0; i < r->dead.size(); i++) {
+    const DevBuf* d = r->dead[i];
+    if (d == self || !d->p) continue;
+    if (d->cap >= lack && (fit == none || d->cap < r->dead[fit]->cap)) fit = i;
+    if (big == none || d->cap > r->dead[big]->cap) big = i;
+  }
+  const size_t pick = fit != none ? fit : big;
+  if (pick == none) return false;
   (void)hipStreamSynchronize(r->st);
-  bool any = false;
-  for (DevBuf* d : r->dead)
-    if (d != self && d->p) {
-      d->release();
-      any = true;
+  r->dead[pick]->release();
+  r->dead[pick] = r->dead.back();
+  r->dead.pop_back();
+  r->freed++;
+  return true;
+}
+
+// Takes a buffer in use again off the list (every listing: a buffer may be
+// listed by several phases).
+inline void reclaim_forget(const DevBuf* d) {
+  if (!tl_reclaim) return;
+  auto& v = tl_reclaim->dead;
+  for (size_t i = 0; i < v.size();)
+    if (v[i] == d) {
+      v[i] = v.back();
+      v.pop_back();
+    } else {
+      i++;
     }
-  r->dead.clear();
-  return any;
 }
 
 inline void* DevBuf::ensure(size_t bytes) {
-  if (tl_reclaim) {
-    auto& v = tl_reclaim->dead;
-    for (size_t i = 0; i < v.size(); i++)
-      if (v[i] == this) {
-        v[i] = v.back();
-        v.pop_back();
-        break;
-      }
-  }
+  reclaim_forget(this);
   if (bytes == 0) bytes = 256;
   if (bytes > cap) {
     release();
     hipError_t e = hipMalloc(&p, bytes);
-    if (e == hipErrorOutOfMemory && reclaim_dead(this)) {
+    while (e == hipErrorOutOfMemory && reclaim_dead(this, bytes)) {
       (void)hipGetLastError();
       e = hipMalloc(&p, bytes);
     }
+    if (tl_reclaim) tl_reclaim->allocs++;
     if (e != hipSuccess) {
       (void)hipGetLastError();
       p = nullptr;

@@ -153,7 +153,9 @@ int gx_unitig_limits(uint64_t* max_nodes, uint64_t* max_head_offset);
  * (genomix_stream.h), [12] successor records read by those pointer-jumping
  * rounds (node ends still following their chain, summed over the rounds),
  * [13] path endpoints that walked their paths in the last unitig run (0 when
- * it ranked by pointer jumping only). */
+ * it ranked by pointer jumping only), [14] device allocations the last
+ * finish made (0 when a repeated build reuses every buffer of the handle),
+ * [15] buffers of earlier builds or phases it freed to make room. */
 int gx_build_stats(gx_build* b, uint64_t* out, int n);
 
 /* Per-phase device timings of the last finish, measured with hipEvents on

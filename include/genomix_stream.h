@@ -37,17 +37,23 @@ typedef int (*gx_part_sink)(void* user, int part, const void* dev_bytes, uint64_
 /* Key-range passes and record delivery of a handle.
  *  range_occurrences: k-mer occurrences per pass; ranges are cut at values
  *    of the top key byte, so one may hold more when a single value does.
- *    0 (default): one pass when it fits, else as many as 70% of the GPU's
- *    HBM needs (the learned distinct-count prediction, when a sizing
- *    network is attached, refines the per-occurrence estimate).
+ *    0 (default): one pass when it fits in the GPU's free HBM, else as few
+ *    passes of even size as fit beside what the passes keep (records and
+ *    node graph, when there is no sink).  The first plan takes 0.6 distinct
+ *    k-mers per occurrence, or an attached sizing network's prediction
+ *    (genomix_sizer.h); the ranges after each pass are planned again from
+ *    the nodes and bytes the passes so far produced and the HBM free then.
  *  sink: NULL (default) keeps the records in HBM as usual (several ranges
  *    are assembled per partition after the last); otherwise each range's
  *    records go to the sink and none are kept: gx_build_part_info still
  *    reports totals, gx_build_emit_part / gx_build_part_device fail with
  *    GX_ESTATE.
- * The unitig pass (gx_unitig_run) needs the whole graph in HBM and fails
- * with GX_ENOTSUP after a build of more than one range.  The range count of
- * the last build is gx_build_stats entry 11. */
+ * Without a sink the ranges' nodes and read heads are also kept, in key
+ * order, and a node index over them is built after the last range, so the
+ * unitig pass, cleaning and statistics take a build of several ranges as
+ * they take a one-pass build.  After a build with a sink they fail with
+ * GX_ENOTSUP (its graph was never whole in HBM).  The range count of the
+ * last build is gx_build_stats entry 11. */
 int gx_build_set_stream(gx_build* b, uint64_t range_occurrences, gx_part_sink sink, void* user);
 
 #ifdef __cplusplus

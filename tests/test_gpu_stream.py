@@ -30,6 +30,9 @@ def _parts(b, P):
 
 @pytest.mark.parametrize("k,P", [(21, 1), (31, 3), (55, 2), (33, 1)])
 def test_ranges_match_single_pass_and_oracle(gx, k, P):
+    """Every partition's bytes, the statistics and, from the node graph the
+    ranges keep in HBM, the unitig records, cleaning and GraphStatistics
+    equal a single-pass build's (and the oracle's)."""
     text = _text(900 + k, paired=k != 33)
     with gx.GraphBuilder(k, nparts=P) as b:
         b.add_reads(text)
@@ -38,8 +41,15 @@ def test_ranges_match_single_pass_and_oracle(gx, k, P):
         info1 = [b.part_info(p) for p in range(P)]
         st1 = b.stats()
         assert st1["key_ranges"] == 1
-        u1 = b.unitigs() if P == 1 else None
-    assert want == oracle.build_graph(text, k, nparts=P)
+        nstats1 = b.graph_stats("nodes")
+        u1 = b.unitigs()
+        ustats1 = b.graph_stats("unitigs")
+        c1 = b.clean_bad_coverage(3.0)
+    want_all = oracle.build_graph(text, k, nparts=P)
+    assert want == want_all
+    one_part = want_all[0] if P == 1 else oracle.build_graph(text, k)[0]
+    assert u1 == oracle.unitigs(one_part, k)
+    assert c1 == oracle.clean_bad_coverage(one_part, k, 3.0)
     N = st1["occurrences"]
     for per in (N // 5, N // 40, 1):
         with gx.GraphBuilder(k, nparts=P) as b:
@@ -54,17 +64,38 @@ def test_ranges_match_single_pass_and_oracle(gx, k, P):
             for key in ("occurrences", "distinct", "heads", "replayed_nodes", "replayed_occurrences"):
                 assert st[key] == st1[key], key
             assert [b.part_info(p) for p in range(P)] == info1
-            if P == 1:
-                with pytest.raises(gx.GxError) as e:
-                    b.run_unitigs()
-                assert e.value.code == gx.GX_ENOTSUP
-    if u1 is not None:  # a one-range build keeps the unitig pass
+            assert b.graph_stats("nodes") == nstats1, f"node statistics, range_occurrences={per}"
+            assert b.unitigs() == u1, f"unitigs, range_occurrences={per}"
+            assert b.graph_stats("unitigs") == ustats1
+            assert b.clean_bad_coverage(3.0) == c1
+    with gx.GraphBuilder(k) as b:  # a one-range build keeps the unitig pass too
+        b.set_stream(range_occurrences=10 * N)
+        b.add_reads(text)
+        b.finish()
+        assert b.stats()["key_ranges"] == 1
+        assert b.unitigs() == u1
+
+
+def test_ranges_twice_on_one_handle(gx):
+    """A second key-range build on the same handle (its buffers, and the
+    node graph of the first, reused or reclaimed) gives the same unitigs;
+    a smaller read set after it too."""
+    k = 31
+    big, small = _text(4242), _text(4343, G=15000)
+    want = {}
+    for name, text in (("big", big), ("small", small)):
         with gx.GraphBuilder(k) as b:
-            b.set_stream(range_occurrences=10 * N)
             b.add_reads(text)
             b.finish()
-            assert b.stats()["key_ranges"] == 1
-            assert b.unitigs() == u1
+            want[name] = (b.part_bytes(0), b.unitigs())
+    with gx.GraphBuilder(k) as b:
+        for name, text in (("big", big), ("big", big), ("small", small), ("big", big)):
+            b.set_stream(range_occurrences=len(text) // 20)
+            b.reset()
+            b.add_reads(text)
+            b.finish()
+            assert b.stats()["key_ranges"] > 1
+            assert (b.part_bytes(0), b.unitigs()) == want[name], name
 
 
 @pytest.mark.parametrize("method", ["sort", "hash"])
@@ -93,6 +124,10 @@ def test_sink_receives_each_partition_in_order(gx, method):
         with pytest.raises(gx.GxError) as e:
             b.part_bytes(0)
         assert e.value.code == gx.GX_ESTATE
+        if method == "sort":  # the graph left range by range: no unitig pass over it
+            with pytest.raises(gx.GxError) as e:
+                b.run_unitigs()
+            assert e.value.code == gx.GX_ENOTSUP
         # back to records kept in HBM on the same handle
         b.set_stream()
         b.reset()
@@ -127,7 +162,9 @@ def test_sink_error_aborts_the_build(gx):
 def test_large_pass_three_byte_buckets(gx, k):
     """A pass of more than 2^28 records sorts in buckets of the top three key
     bytes; the same read set in smaller passes (two-byte buckets) and through
-    the hash-table aggregation gives the same bytes."""
+    the hash-table aggregation gives the same bytes, and the same unitigs
+    (after the passes: from the node graph they kept, with a node index of
+    more than 2^26 nodes built over all of them)."""
     from genomix_amd import sim
     spec = sim.ReadSetSpec(genome_len=6_000_000, read_len=150, coverage=70, error_rate=0.01, paired=True,
                            insert_mean=500, insert_sd=50, repeat_frac=0.05, repeat_len=2000, repeat_div=0.01,
@@ -144,12 +181,14 @@ def test_large_pass_three_byte_buckets(gx, k):
         b.run_unitigs()  # every neighbour lookup goes through the node index
         nu = b.unitig_count()
         assert 0 < nu < st["distinct"]
+        uone = b.unitig_records()
         b.set_stream(range_occurrences=st["occurrences"] // 3)
         b.reset()
         b.add_reads(text)
         b.finish()
         assert b.stats()["key_ranges"] >= 3
         assert b.part_bytes(0) == one
+        assert b.unitigs() == uone
     with gx.GraphBuilder(k, method="hash") as b:
         b.add_reads(text)
         b.finish()
