@@ -2,9 +2,11 @@
 
 Sweeps, on a sample of the batched workload, the knobs the runtime exposes:
 per size class of read set (k-mer occurrences <= 2^23, <= 2^27, larger) the
-grid of the grid-stride kernels and the learned-table slack (capacity /
-predicted distinct k-mers: a longer or shorter probe sequence against a
-smaller or larger table), then the number of worker streams.  Per-class
+aggregation method -- sorted, or the hash table sized by the learned
+distinct-k-mer prediction with a slack (capacity / predicted distinct
+k-mers: a longer or shorter probe sequence against a smaller or larger
+table) -- and the grid of the grid-stride kernels, then the number of
+worker streams.  Per-class
 times come from the scheduler's own per-set clock with one worker; the
 worker count from the wall time of whole-batch runs.  Writes
 genomix_amd/batch_tuning.json, which bench.py and GraphBatch users load.
@@ -13,7 +15,6 @@ genomix_amd/batch_tuning.json, which bench.py and GraphBatch users load.
 """
 
 import argparse
-import itertools
 import json
 import os
 import time
@@ -25,6 +26,8 @@ CLASSES = [1 << 23, 1 << 27]
 GRIDS = [512, 1024, 2048, 4096]
 SLACKS = [1.25, 1.5, 2.0]
 WORKERS = [1, 2, 3, 4, 6, 8]
+# (method, grid, slack); the slack only sizes the hash method's table
+CANDIDATES = [("sort", g, 1.5) for g in GRIDS] + [("hash", g, sl) for g in GRIDS for sl in SLACKS]
 
 
 def _class_of(occ):
@@ -55,27 +58,29 @@ def main():
     per_class = {c: {} for c in range(len(CLASSES) + 1)}
     with gx.GraphBatch(sets, sizer=net) as b:
         occ = None
-        for grid, slack in itertools.product(GRIDS, SLACKS):
-            b.set_tuning(1, CLASSES, [grid] * 3, [slack] * 3)
+        for cand in CANDIDATES:
+            method, grid, slack = cand
+            b.set_tuning(1, CLASSES, [grid] * 3, [slack] * 3, method=[method] * 3)
             b.run(unitigs=True)  # warm
             b.run(unitigs=True)
             st = [b.set_stats(i) for i in range(len(sets))]
             occ = [x["occurrences"] for x in st]
             for c in per_class:
-                per_class[c][(grid, slack)] = sum(x["ms"] for x, o in zip(st, occ) if _class_of(o) == c)
-            log(f"grid {grid} slack {slack}: " + ", ".join(f"class {c} {per_class[c][(grid, slack)]:.1f} ms"
-                                                            for c in per_class))
+                per_class[c][cand] = sum(x["ms"] for x, o in zip(st, occ) if _class_of(o) == c)
+            log(f"{method} grid {grid} slack {slack}: " +
+                ", ".join(f"class {c} {per_class[c][cand]:.1f} ms" for c in per_class))
         best = {}
         for c, d in per_class.items():
             if not any(v > 0 for v in d.values()):
-                best[c] = (2048, 1.5)  # no sample set in this class
+                best[c] = ("sort", 2048, 1.5)  # no sample set in this class
             else:
                 best[c] = min(d, key=d.get)
-        grid = [best[c][0] for c in sorted(best)]
-        slack = [best[c][1] for c in sorted(best)]
+        method = [best[c][0] for c in sorted(best)]
+        grid = [best[c][1] for c in sorted(best)]
+        slack = [best[c][2] for c in sorted(best)]
         wall = {}
         for w in WORKERS:
-            b.set_tuning(w, CLASSES, grid, slack)
+            b.set_tuning(w, CLASSES, grid, slack, method=method)
             b.run(unitigs=True)
             ts = []
             for _ in range(3):
@@ -88,8 +93,8 @@ def main():
     for d in dev:
         d.free()
     workers = min(wall, key=wall.get)
-    out = {"workers": workers, "max_occ": CLASSES, "grid": grid, "slack": slack,
-           "measured": {"per_class_ms": {str(c): {f"{g}/{s}": round(v, 3) for (g, s), v in d.items()}
+    out = {"workers": workers, "max_occ": CLASSES, "grid": grid, "slack": slack, "method": method,
+           "measured": {"per_class_ms": {str(c): {f"{m}/{g}/{sl}": round(v, 3) for (m, g, sl), v in d.items()}
                                          for c, d in per_class.items()},
                         "batch_ms_by_workers": {str(w): round(v * 1e3, 3) for w, v in wall.items()}},
            "sample": f"{len(sets)} sets of the batched workload (seed 777) at genome scale {args.scale}",

@@ -7,8 +7,8 @@
 // k with gx_build_set_config), so the host-side steps of one set (size
 // read-backs, launches) overlap the device work of the others and small
 // sets fill the GPU while a large one is between phases.  Per size class
-// (k-mer occurrences) the worker applies the tuned grid and learned-table
-// slack.  Outputs are handed to the caller's consumer while the handle
+// (k-mer occurrences) the worker applies the tuned aggregation method, grid
+// and learned-table slack.  Outputs are handed to the caller's consumer while the handle
 // still holds them.  No counterpart in the reference (one Hyracks job per
 // read set).
 
@@ -45,6 +45,7 @@ struct Tuning {
   std::vector<uint64_t> max_occ{1ull << 23, 1ull << 27};
   std::vector<int> grid{1024, 2048, 2048};
   std::vector<float> slack{1.5f, 1.5f, 1.5f};
+  std::vector<int> method{GX_AGG_SORT, GX_AGG_SORT, GX_AGG_SORT};
 };
 
 // rough k-mer occurrence count of a readids buffer (for the size class
@@ -123,7 +124,24 @@ int gx_batch_set_tuning(gx_batch* b, int workers, int nclasses, const uint64_t* 
   t.max_occ.assign(max_occ, max_occ + (nclasses - 1));
   t.grid.assign(grid_blocks, grid_blocks + nclasses);
   t.slack.assign(table_slack, table_slack + nclasses);
+  t.method.assign((size_t)nclasses, GX_AGG_SORT);
   b->tune = t;
+  return GX_OK;
+}
+
+int gx_batch_set_methods(gx_batch* b, int nclasses, const int* method) {
+  if (!b) return GX_EINVAL;
+  if (nclasses != (int)b->tune.grid.size() || !method) {
+    b->err = "methods for " + std::to_string(nclasses) + " classes; the tuning has " +
+             std::to_string(b->tune.grid.size());
+    return GX_EINVAL;
+  }
+  for (int c = 0; c < nclasses; c++)
+    if (method[c] != GX_AGG_SORT && method[c] != GX_AGG_HASH) {
+      b->err = "unknown aggregation method for class " + std::to_string(c);
+      return GX_EINVAL;
+    }
+  b->tune.method.assign(method, method + nclasses);
   return GX_OK;
 }
 
@@ -177,18 +195,14 @@ int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* use
     // ranges would count the other workers' buffers against this one
     if (!rc && unitigs) rc = gx_build_set_stream(h, ~0ull, nullptr, nullptr);
     if (!rc) rc = gx_build_set_tuning(h, b->tune.grid[c], b->tune.slack[c]);
+    if (!rc) rc = gx_build_set_method(h, b->tune.method[c]);
     if (!rc) rc = gx_build_set_sizer(h, b->sizer);
     auto t0 = std::chrono::steady_clock::now();
     if (!rc)
       rc = s.on_device ? gx_build_add_reads_device(h, s.input, s.len, s.fmt)
                        : gx_build_add_reads(h, (const char*)s.input, s.len, s.fmt);
     if (!rc) rc = gx_build_finish(h);
-    if (!rc && unitigs) {
-      rc = gx_unitig_run(h);
-      // the build ran in key ranges for want of free device memory (held
-      // by the other workers): the set is taken again alone
-      if (rc == GX_ENOTSUP) rc = GX_ENOMEM;
-    }
+    if (!rc && unitigs) rc = gx_unitig_run(h);
     auto t1 = std::chrono::steady_clock::now();
     if (rc) {
       *msg = where + gx_last_error(h);

@@ -22,6 +22,7 @@ GX_ENOTSUP = -95
 GX_FMT_READIDS = 0
 GX_FMT_FASTQ = 1
 EDGE_ORDER = {"canonical": 0, "jdk7": 1}
+METHOD = {"sort": 0, "hash": 1}  # GX_AGG_SORT, GX_AGG_HASH
 
 EXPORTED = [
     "gx_build_create", "gx_build_add_reads", "gx_build_add_reads_device", "gx_build_finish",
@@ -36,7 +37,7 @@ EXPORTED = [
     "gx_batch_destroy", "gx_build_set_method", "gx_build_set_stream", "gx_clean_bad_coverage", "gx_clean_tips",
     "gx_graph_stats", "gx_graph_stat_totals", "gx_graph_stat", "gx_graph_stat_bins", "gx_dump_fasta",
     "gx_coverage_cutoff", "gx_fit_mixture", "gx_clean_bubbles", "gx_batch_retried", "gx_unitig_limits", "gx_unitig_set_ranking",
-    "gx_mlp_gradient_noalloc",
+    "gx_mlp_gradient_noalloc", "gx_batch_set_methods",
 ]
 GRAPHS = {"nodes": 0, "unitigs": 1}
 STAT_NAMES = ["degree", "kmerLength", "coverage", "unflippedReadIds", "flippedReadIds", "scaffoldSeedScore",
@@ -130,6 +131,7 @@ def lib():
         L.gx_batch_set.argtypes = [vp, i32, ctypes.POINTER(_Cfg), vp, sz, i32, i32]
         L.gx_batch_set_sizer.argtypes = [vp, vp]
         L.gx_batch_set_tuning.argtypes = [vp, i32, i32, u64p, ctypes.POINTER(ctypes.c_int), fp]
+        L.gx_batch_set_methods.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
         L.gx_batch_run.argtypes = [vp, i32, BATCH_CONSUMER, vp]
         L.gx_batch_set_stats.argtypes = [vp, i32, u64p, i32, ctypes.POINTER(ctypes.c_double)]
         L.gx_batch_last_error.argtypes = [vp]
@@ -170,7 +172,7 @@ class GraphBuilder:
     def set_method(self, method):
         """'sort' (default: sort-partitioned aggregation) or 'hash' (atomic
         hash table, sized by the learned predictor when one is attached)."""
-        self._check(lib().gx_build_set_method(self._h, {"sort": 0, "hash": 1}[method]))
+        self._check(lib().gx_build_set_method(self._h, METHOD[method]))
 
     def set_stream(self, range_occurrences=0, sink=None):
         """Key-range passes (include/genomix_stream.h): at most about
@@ -345,11 +347,11 @@ class GraphBuilder:
         return buf.raw[:n.value]
 
     def stats(self):
-        a = (ctypes.c_uint64 * 14)()
-        self._check(lib().gx_build_stats(self._h, a, 14))
+        a = (ctypes.c_uint64 * 16)()
+        self._check(lib().gx_build_stats(self._h, a, 16))
         keys = ["occurrences", "distinct", "heads", "table_capacity", "replayed_nodes", "replayed_occurrences",
                 "max_coverage", "predicted_distinct", "table_rebuilds", "oversized_bucket_records", "pj_rounds",
-                "key_ranges", "pj_successor_reads", "walkers"]
+                "key_ranges", "pj_successor_reads", "walkers", "device_allocations", "buffers_reclaimed"]
         return dict(zip(keys, list(a)))
 
     def set_profiling(self, on=True):
@@ -451,7 +453,10 @@ def load_tuning(path=os.path.join(HERE, "batch_tuning.json")):
     import json
     with open(path) as f:
         t = json.load(f)
-    return {k: t[k] for k in ("workers", "max_occ", "grid", "slack")}
+    out = {k: t[k] for k in ("workers", "max_occ", "grid", "slack")}
+    if "method" in t:
+        out["method"] = t["method"]
+    return out
 
 
 class GraphBatch:
@@ -489,12 +494,17 @@ class GraphBatch:
         if rc != 0:
             raise GxError(rc, lib().gx_batch_last_error(self._b).decode(errors="replace"))
 
-    def set_tuning(self, workers, max_occ, grid, slack):
+    def set_tuning(self, workers, max_occ, grid, slack, method=None):
+        """Workers, size classes (max_occ bounds) and per class the grid,
+        table slack and aggregation method ("sort" or "hash"; default sort)."""
         n = len(grid)
         mo = (ctypes.c_uint64 * max(1, n - 1))(*max_occ)
         g = (ctypes.c_int * n)(*grid)
         sl = (ctypes.c_float * n)(*slack)
         self._check(lib().gx_batch_set_tuning(self._b, workers, n, mo, g, sl))
+        if method is not None:
+            m = (ctypes.c_int * len(method))(*[METHOD[x] for x in method])
+            self._check(lib().gx_batch_set_methods(self._b, len(method), m))
 
     def run(self, unitigs=True, collect=False, consumer=None):
         """Builds every set.  collect=True returns {set: (parts, unitig bytes)};

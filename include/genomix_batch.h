@@ -61,11 +61,16 @@ int gx_batch_set_sizer(gx_batch* b, gx_mlp* m);
  * its grid and table slack.  workers = worker threads / streams (1..8). */
 int gx_batch_set_tuning(gx_batch* b, int workers, int nclasses, const uint64_t* max_occ, const int* grid_blocks,
                         const float* table_slack);
+/* Aggregation method per size class (GX_AGG_SORT or GX_AGG_HASH; nclasses
+ * as in the last gx_batch_set_tuning, which resets every class to
+ * GX_AGG_SORT).  With GX_AGG_HASH and a sizing network attached, the
+ * network sizes the class's tables (the slack above). */
+int gx_batch_set_methods(gx_batch* b, int nclasses, const int* method);
 /* Builds every set (and runs unitig compression when unitigs != 0),
- * calling consumer (may be NULL) after each.  A set that runs out of
- * device memory beside the other workers' buffers (or is split into key
- * ranges for want of it, which the unitig pass cannot take) is built again
- * alone after the others. */
+ * calling consumer (may be NULL) after each.  With unitigs each set is
+ * built in one pass (the free-memory estimate behind key ranges would count
+ * the other workers' buffers); a set that runs out of device memory beside
+ * those buffers is built again alone after the others. */
 int gx_batch_run(gx_batch* b, int unitigs, gx_batch_consumer consumer, void* user);
 /* Sets of the last run that were built again alone. */
 int gx_batch_retried(const gx_batch* b);

@@ -29,11 +29,13 @@ def _sets():
     return out
 
 
-@pytest.mark.parametrize("workers", [1, 4])
-def test_batch_matches_oracle_per_set(gx, workers):
+@pytest.mark.parametrize("workers,method", [(1, ["sort"] * 3), (4, ["hash", "sort", "hash"]), (2, ["hash"] * 3)])
+def test_batch_matches_oracle_per_set(gx, workers, method):
+    """Every set equals the oracle whichever aggregation method its size
+    class is given (a worker's handle switches method between sets)."""
     sets = _sets()
     tuning = {"workers": workers, "max_occ": [20000, 200000], "grid": [256, 1024, 2048],
-              "slack": [1.5, 1.5, 2.0]}
+              "slack": [1.5, 1.5, 2.0], "method": method}
     with gx.GraphBatch(sets, tuning=tuning) as b:
         out = b.run(unitigs=True, collect=True)
         assert sorted(out) == list(range(len(sets)))
@@ -57,12 +59,18 @@ def test_batch_with_learned_sizing_and_device_inputs(gx):
     dev = [DeviceBytes(s["input"]) for s in sets]
     dsets = [dict(s, input=(d.address, d.n)) for s, d in zip(sets, dev)]
     net = sizer.load()
+    # every class on the hash method: the network sizes each set's table
+    tuning = {"workers": 3, "max_occ": [20000, 200000], "grid": [1024] * 3, "slack": [1.5] * 3,
+              "method": ["hash"] * 3}
     try:
-        with gx.GraphBatch(dsets, sizer=net) as b:
+        with gx.GraphBatch(dsets, sizer=net, tuning=tuning) as b:
             out = b.run(unitigs=False, collect=True)
             for i, s in enumerate(sets):
                 assert out[i][0] == oracle.build_graph(s["input"], s["k"], nparts=s["nparts"])
-                assert b.set_stats(i)["predicted_distinct"] > 0
+                st = b.set_stats(i)
+                assert st["predicted_distinct"] > 0
+                # ceil(1.5 x prediction) slots (at least 1024), or more after a rebuild
+                assert st["table_capacity"] >= max(1024, int(1.5 * st["predicted_distinct"]))
     finally:
         for d in dev:
             d.free()
@@ -71,6 +79,16 @@ def test_batch_with_learned_sizing_and_device_inputs(gx):
         a1 = b.run(unitigs=True, collect=True)
         a2 = b.run(unitigs=True, collect=True)
         assert a1 == a2
+
+
+def test_batch_method_per_class_is_checked(gx):
+    sets = _sets()[:2]
+    with gx.GraphBatch(sets, tuning={"workers": 1, "max_occ": [20000], "grid": [512, 512], "slack": [1.5, 1.5]}) as b:
+        with pytest.raises(gx.GxError) as e:  # three methods for two classes
+            b.set_tuning(1, [20000], [512, 512], [1.5, 1.5], method=["sort", "hash", "hash"])
+        assert e.value.code == gx.GX_EINVAL
+        m = (gx.ctypes.c_int * 2)(0, 7)
+        assert gx.lib().gx_batch_set_methods(b._b, 2, m) == gx.GX_EINVAL  # unknown method
 
 
 def test_batch_reports_the_failing_set(gx):
