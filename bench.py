@@ -160,6 +160,17 @@ def batched_leg(steps, warmup, scale=1.0):
         el = _timed(lambda: b.run(unitigs=True), steps, warmup, "batched")
         st = [b.set_stats(i) for i in range(len(sets))]
         retried = b.retried()
+    # the size class of each set (the scheduler's estimate: 0.8 occurrences
+    # per input byte) and the method the tuning gives it
+    methods = (tuning or {}).get("method") or ["sort"] * 3
+    bounds = (tuning or {}).get("max_occ", [1 << 23, 1 << 27])
+
+    def size_class(nbytes):
+        c = 0
+        while c < len(bounds) and int(0.8 * nbytes) > bounds[c]:
+            c += 1
+        return c
+    hash_sets = sum(1 for d in dev if methods[size_class(d.n)] == "hash")
     for d in dev:
         d.free()
     occ = sum(x["occurrences"] for x in st)
@@ -169,14 +180,14 @@ def batched_leg(steps, warmup, scale=1.0):
             "table_rebuilds": sum(x["table_rebuilds"] for x in st),
             "k_mix": {str(k): sum(1 for _, kk in specs if kk == k) for k in (21, 31, 55)},
             "workers": tuning["workers"] if tuning else 4,
-            # the network runs on every set (its prediction is recorded), but the
-            # sorted aggregation sizes its buffers from exact counts between phases
-            # and every set here fits one pass, so the prediction sizes nothing
-            "learned_sizing": False,
+            # the network predicts for every set; it sizes the tables of the sets
+            # whose size class the tuning gives the hash method (the sorted method
+            # sizes its buffers from exact counts, and these sets fit one pass)
+            "learned_sizing": net is not None and hash_sets > 0,
             "sizing_network": {"attached": net is not None,
                                "sets_with_prediction": sum(1 for x in st if x["predicted_distinct"] > 0),
-                               "sizes": "the hash method's table and the key-range budget only; neither is used "
-                                        "by these one-pass sorted builds"},
+                               "sets_with_learned_table": hash_sets if net is not None else 0,
+                               "method_per_class": methods},
             "sets_retried_alone": retried,
             "workload": "64 read sets, genomes log-uniform 100 kbp-10 Mbp" + (f" x{scale}" if scale != 1 else "") +
                         ", 100/150 bp SE/PE, 20-80x, 0.5-2% error, k 21/31/55; graph + unitigs per set"}
@@ -190,11 +201,13 @@ def load_tuning():
 def learned_leg(steps, warmup):
     """Learned-sizing workload (100 Mbp, 20% repeat families, 150 bp PE, 40x,
     k = 31): the default sorted aggregation with and without the trained
-    network attached (a predicted distinct fraction above 0.6 would shrink
-    the key ranges; this read set predicts less, so both plan the same
-    ranges), and the hash-table aggregation
+    network attached (its distinct-k-mer prediction plans the first key
+    range and the HBM kept for the records and node graph of all of them;
+    0.6 distinct k-mers per occurrence without it; later ranges are planned
+    again from what the first measured), and the hash-table aggregation
     sized by the network and by the occurrence count (the method without
-    it).  Graph build only, input in HBM."""
+    it).  Graph build, input in HBM; after the unsized sorted leg, one
+    unitig pass over the graph its key ranges kept (timed apart)."""
     from genomix_amd import devmem, gx, sim, sizer
     spec = sim.LEARNED_100MBP
     t0 = time.perf_counter()
@@ -223,7 +236,18 @@ def learned_leg(steps, warmup):
                              "key_ranges": st["key_ranges"], "predicted_distinct": st["predicted_distinct"],
                              "distinct": st["distinct"], "table_bytes": st["table_capacity"] * 16 if method == "hash"
                              else 0, "table_rebuilds": st["table_rebuilds"],
+                             "last_step_allocations": st["device_allocations"],
+                             "last_step_buffers_reclaimed": st["buffers_reclaimed"],
                              "phases_ms": {kk: round(v, 1) for kk, v in b.phase_times().items() if v > 0}}
+                if name == "sorted_unsized":
+                    # unitig compression of the graph the key ranges kept in HBM
+                    devmem.synchronize()
+                    t1 = time.perf_counter()
+                    b.run_unitigs()
+                    devmem.synchronize()
+                    res[name]["unitig_pass"] = {"ms": round(1e3 * (time.perf_counter() - t1), 1),
+                                                "nodes": st["distinct"], "unitigs": b.unitig_count(),
+                                                "ranking": "walks" if b.stats()["walkers"] else "pointer jumping"}
         except gx.GxError as e:  # the occurrence-count table does not fit in HBM
             res[name] = {"error": str(e)}
             log(f"{name}: {e}")
@@ -320,6 +344,10 @@ def main():
     ap.add_argument("--workload", choices=["plain", "learned", "batched", "streamed"], default="plain")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (reference kernel for PMC byte counters)")
+    ap.add_argument("--method", choices=["sort", "hash"], default="sort",
+                    help="aggregation method of the plain leg (profiling the alternatives; same records)")
+    ap.add_argument("--ranking", choices=["auto", "pointer_jumping"], default="auto",
+                    help="unitig ranking of the plain leg (profiling the alternatives; same records)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: the build would shard by partition of the k-mer hash across "
@@ -329,7 +357,7 @@ def main():
     from genomix_amd import devmem, gx, sim
 
     if args.workload == "learned":
-        print(json.dumps({"metric": METRIC, "learned": learned_leg(max(1, args.steps // 2), 1)}), flush=True)
+        print(json.dumps({"metric": METRIC, "learned": learned_leg(max(1, args.steps // 2), 2)}), flush=True)
         return
     if args.workload == "streamed":
         print(json.dumps({"metric": METRIC, "streamed": streamed_leg(max(1, args.steps // 2), 1)}), flush=True)
@@ -358,7 +386,8 @@ def main():
         h.hipFree(dst)
         log("copy calibration: 1 GiB device copy done")
 
-    builder = gx.GraphBuilder(k, edge_order="jdk7", device=0)
+    builder = gx.GraphBuilder(k, edge_order="jdk7", device=0, method=args.method)
+    builder.set_ranking(args.ranking)
     builder.set_profiling(True)
 
     def step():
@@ -416,9 +445,6 @@ def main():
     roofline_kmer = {"bound": "hbm", "achieved": round(ach_k, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_k / HBM_PEAK_GBS, 4), "kernel": "k_occ_hist + k_occ_scatter",
                      "algorithmic_bytes_per_kmer": round(Bk, 4), "kernel_ms": round(t_k * 1e3, 4)}
-    # pointer-jumping rounds of the unitig pass: every node end reads and
-    # writes its own 16-byte record (32 B); ends still following their chain
-    # also read their successor's (16 B), counted by the library
     # unitig ranking: walks from the path endpoints (odd k), or pointer-jumping
     # rounds, in which every node end reads and writes its own 16-byte record
     # (32 B) and ends still following their chain also read their
@@ -522,7 +548,8 @@ def main():
     learned = None
     if not args.no_learned:
         try:
-            learned = learned_leg(2, 1)
+            # two warmup builds: a key-range build settles its stores on the second
+            learned = learned_leg(2, 2)
         except Exception as e:  # the headline line is printed regardless
             learned = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(text, k)
@@ -537,8 +564,9 @@ def main():
                    "distinct_kmers": st["distinct"], "replayed_nodes": st["replayed_nodes"],
                    "replayed_occurrences": st["replayed_occurrences"], "table_capacity": st["table_capacity"],
                    "max_coverage": st["max_coverage"], "oversized_bucket_records": st["oversized_bucket_records"],
-                   "unitigs": unitigs, "edge_order": "jdk7",
-                   "parallelism": "single GPU"},
+                   "unitigs": unitigs, "edge_order": "jdk7", "aggregation": args.method,
+                   "device_allocations_per_step": st["device_allocations"],
+                   "ranking": args.ranking, "parallelism": "single GPU"},
         "phases_ms": {kname: round(v, 4) for kname, v in phases.items()},
         "roofline": roofline,
         "roofline_phases": roofline_phases,

@@ -932,9 +932,14 @@ __host__ inline double pass_bytes_per_occurrence(int W, int nb, double dfrac) {
   return rec + dfrac * (node + out);
 }
 // HBM one node of a key range holds until the build ends when the records
-// stay in HBM: its record (estimated as above) and its node graph (key
-// words, count, mask, two head counts, about one head).
-__host__ inline double kept_bytes_per_node(int W, int nb) { return 3.0 * (4 + nb) + 16 + 8.0 * W + 4 * 4 + 4; }
+// stay in HBM: its record (estimated as above), its node graph (key words,
+// count, mask, two head counts, about one head), and its share of what is
+// built over the whole graph after the last range (two node-index slots,
+// head total and start), which stays allocated beside the next build's
+// per-range buffers.
+__host__ inline double kept_bytes_per_node(int W, int nb) {
+  return 3.0 * (4 + nb) + 16 + 8.0 * W + 4 * 4 + 4 + 2.0 * (W == 1 ? 16 : 24) + 16;
+}
 constexpr int kIdxBits = 13;      // log2(kBucketCap)
 
 // The bucket bytes are sorted least significant first (LSD): the fused
@@ -952,38 +957,41 @@ __device__ inline uint64_t mate_search(const uint64_t* __restrict__ ob, uint64_t
   return lo;
 }
 
-// Occurrences [j0, j1) of a tile: finds their mates and calls f(index in
-// tile, j, key, code) for each, in rounds of kBlock consecutive
-// occurrences.  When the tile's mates fit in LDS (always, for reads shorter
-// than kSortTile / kTileMateCap occurrences on average), a table gives each
-// occurrence's mate: every non-empty mate marks the slot of its first
-// occurrence and a running maximum over the tile fills the rest; otherwise
-// each occurrence searches occbase.
-template <int W, bool CODES, class F>
-__device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t nmates,
-                                            const uint64_t* __restrict__ occbase, const uint32_t* __restrict__ mlen,
-                                            const uint64_t* __restrict__ wbase, const uint64_t* __restrict__ words,
-                                            int k, uint64_t* lob, uint16_t* mtab, F&& f) {
-  __shared__ uint64_t mrange[2];
+// The mates of a tile's occurrences [j0, j1), in LDS: the tile's first and
+// last mate (mr), and, when the mates fit (always, for reads shorter than
+// kSortTile / kTileMateCap occurrences on average), their first occurrences
+// relative to the first mate's (lob) and a table giving each occurrence's
+// mate (mtab): every non-empty mate marks the slot of its first occurrence
+// and a running maximum over the tile fills the rest; otherwise each
+// occurrence searches occbase.
+struct TileMates {
+  uint64_t* mr;    // [2] first and last mate; [2] = the first mate's first occurrence
+  uint32_t* lob;   // [kTileMateCap] occbase[m0 + i] - occbase[m0]
+  uint16_t* mtab;  // [kSortTile] mate of occurrence j0 + q, relative to m0
+};
+
+__device__ inline void tile_mates_init(const TileMates& t, uint64_t j0, uint64_t j1, uint64_t nmates,
+                                       const uint64_t* __restrict__ occbase) {
   __shared__ uint16_t wmax[kBlock / 64];
   if (threadIdx.x == 0) {
-    mrange[0] = mate_search(occbase, 0, nmates - 1, j0);
-    mrange[1] = mate_search(occbase, mrange[0], nmates - 1, j1 - 1);
+    t.mr[0] = mate_search(occbase, 0, nmates - 1, j0);
+    t.mr[1] = mate_search(occbase, t.mr[0], nmates - 1, j1 - 1);
+    t.mr[2] = occbase[t.mr[0]];
   }
   __syncthreads();
-  const uint64_t m0 = mrange[0], m1 = mrange[1];
-  const bool local = m1 - m0 + 2 <= kTileMateCap;
-  if (local) {
-    for (uint64_t i = threadIdx.x; i <= m1 - m0 + 1; i += kBlock) lob[i] = i + m0 <= nmates ? occbase[m0 + i] : ~0ull;
-    for (uint32_t q = threadIdx.x; q < kSortTile; q += kBlock) mtab[q] = 0;
+  const uint64_t m0 = t.mr[0], m1 = t.mr[1], b0 = t.mr[2];
+  if (m1 - m0 + 2 <= kTileMateCap) {
+    for (uint64_t i = threadIdx.x; i <= m1 - m0 + 1; i += kBlock)
+      t.lob[i] = i + m0 <= nmates ? (uint32_t)(occbase[m0 + i] - b0) : 0xffffffffu;
+    for (uint32_t q = threadIdx.x; q < kSortTile; q += kBlock) t.mtab[q] = 0;
     __syncthreads();
     for (uint64_t i = threadIdx.x + 1; i <= m1 - m0; i += kBlock)
-      if (lob[i + 1] > lob[i] && lob[i] < j1) mtab[lob[i] - j0] = (uint16_t)i;
+      if (t.lob[i + 1] > t.lob[i] && b0 + t.lob[i] < j1) t.mtab[b0 + t.lob[i] - j0] = (uint16_t)i;
     __syncthreads();
     // running maximum: each thread over kSortTile / kBlock consecutive slots
     constexpr int kPer = kSortTile / kBlock;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint16_t* my = mtab + threadIdx.x * kPer;
+    uint16_t* my = t.mtab + threadIdx.x * kPer;
     uint32_t run = 0;
 #pragma unroll
     for (int q = 0; q < kPer; q++) run = max(run, (uint32_t)my[q]);
@@ -1007,17 +1015,35 @@ __device__ inline void for_tile_occurrences(uint64_t j0, uint64_t j1, uint64_t n
     __syncthreads();
   }
   __syncthreads();
+}
+
+// Canonical key (and, with CODES, neighbour code) of occurrence j of the tile.
+template <int W, bool CODES>
+__device__ inline KV tile_occurrence(const TileMates& t, uint64_t j0, uint64_t j, const uint64_t* __restrict__ occbase,
+                                     const uint32_t* __restrict__ mlen, const uint64_t* __restrict__ wbase,
+                                     const uint64_t* __restrict__ words, int k, uint32_t* code) {
+  const uint64_t m0 = t.mr[0], m1 = t.mr[1];
+  const bool local = m1 - m0 + 2 <= kTileMateCap;
+  const uint64_t m = local ? m0 + t.mtab[j - j0] : mate_search(occbase, m0, m1, j);
+  const uint64_t mb = local ? t.mr[2] + t.lob[m - m0] : occbase[m];
+  uint32_t bits;
+  return occ_info<W, CODES>(words + wbase[m], mlen[m], (uint32_t)(j - mb), k, code, &bits);
+}
+
+// Occurrences [j0, j1) of a tile: calls f(round, valid, j, key, code) for
+// each, in rounds of kBlock consecutive occurrences.
+template <int W, bool CODES, class F>
+__device__ inline void for_tile_occurrences(const TileMates& t, uint64_t j0, uint64_t j1, uint64_t nmates,
+                                            const uint64_t* __restrict__ occbase, const uint32_t* __restrict__ mlen,
+                                            const uint64_t* __restrict__ wbase, const uint64_t* __restrict__ words,
+                                            int k, F&& f) {
+  tile_mates_init(t, j0, j1, nmates, occbase);
   for (int r = 0; r < kSortItems; r++) {
     uint64_t j = j0 + (uint64_t)r * kBlock + threadIdx.x;
     bool valid = j < j1;
     KV key{0, 0};
     uint32_t code = 0;
-    if (valid) {
-      uint64_t m = local ? m0 + mtab[j - j0] : mate_search(occbase, m0, m1, j);
-      uint64_t mb = local ? lob[m - m0] : occbase[m];
-      uint32_t bits;
-      key = occ_info<W, CODES>(words + wbase[m], mlen[m], (uint32_t)(j - mb), k, &code, &bits);
-    }
+    if (valid) key = tile_occurrence<W, CODES>(t, j0, j, occbase, mlen, wbase, words, k, &code);
     f(r, valid, j, key, code);
   }
 }
@@ -1031,11 +1057,12 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
                                                      uint32_t thi, int dbyte) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
-  __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint64_t mr[3];
+  __shared__ uint32_t lob[kTileMateCap];
   __shared__ uint16_t mtab[kSortTile];
   h[threadIdx.x] = 0;
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W, false>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+  for_tile_occurrences<W, false>(TileMates{mr, lob, mtab}, j0, j1, nmates, occbase, mlen, wbase, words, k,
                                  [&](int, bool valid, uint64_t, KV key, uint32_t) {
                                    if (valid) {
                                      SK sk;
@@ -1050,7 +1077,11 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
 
 // Stable scatter of the tile's records by the lower bucket byte (same
 // ranking as k_radix_scatter: wave match + lower waves' counts), staged in
-// LDS in digit order and written out as one contiguous run per digit.
+// LDS in digit order and written out as one contiguous run per digit.  The
+// stage holds 4 bytes per record (occurrence in the tile, neighbour code);
+// the write-out extracts each record's key again from the packed mates
+// (which the tile has just read, so they come from the caches), which
+// halves the block's LDS and doubles the blocks a CU holds.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nmates,
                                                         const uint64_t* __restrict__ occbase,
@@ -1066,10 +1097,10 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
   __shared__ uint64_t run[256];
   __shared__ uint32_t lbase[256], lrun[256], wsum[kBlock / 64];
   __shared__ uint16_t wcnt[kBlock / 64][256];
-  __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint64_t mr[3];
+  __shared__ uint32_t lob[kTileMateCap];
   __shared__ uint16_t mtab[kSortTile];
   __shared__ uint32_t h2[256];  // the next key range's digit counts (as k_occ_hist would give)
-  __shared__ SK sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];  // (occurrence - j0) << 8 | code
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
@@ -1077,7 +1108,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
   tile_digit_bases(offs, ntiles, lbase, lrun, wsum);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W, true>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+  const TileMates tm{mr, lob, mtab};
+  for_tile_occurrences<W, true>(tm, j0, j1, nmates, occbase, mlen, wbase, words, k,
                                 [&](int, bool valid, uint64_t j, KV key, uint32_t code) {
     for (int w = 0; w < kBlock / 64; w++) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
@@ -1103,7 +1135,6 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     if (valid) {
       uint32_t pos = lrun[d] + rank;
       for (int w = 0; w < wid; w++) pos += wcnt[w][d];
-      sk[pos] = skey;
       sv[pos] = ((uint32_t)(j - j0) << 8) | code;
     }
     __syncthreads();
@@ -1114,11 +1145,15 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
   __syncthreads();
   const uint32_t tn = lrun[255];  // records of this range in the tile
   for (uint32_t i = threadIdx.x; i < tn; i += kBlock) {
-    const SK skey = sk[i];
+    const uint32_t v = sv[i];
+    const uint64_t j = j0 + (v >> 8);
+    uint32_t code_unused;
+    SK skey;
+    make_sort_key(tile_occurrence<W, false>(tm, j0, j, occbase, mlen, wbase, words, k, &code_unused), nb, skey);
     const uint32_t d = key_byte(skey, dbyte);
     const uint64_t pos = run[d] + (i - lbase[d]);
     okeys[pos] = skey;
-    ovals[pos] = ((j0 + (sv[i] >> 8)) << 8) | (sv[i] & 0xffu);
+    ovals[pos] = (j << 8) | (v & 0xffu);
   }
   if (nhist) nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h2[threadIdx.x];
 }
@@ -1133,11 +1168,12 @@ __global__ void __launch_bounds__(kBlock) k_top_hist(uint64_t N, uint64_t nmates
                                                      unsigned long long* __restrict__ out) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint32_t h[256];
-  __shared__ uint64_t lob[kTileMateCap];
+  __shared__ uint64_t mr[3];
+  __shared__ uint32_t lob[kTileMateCap];
   __shared__ uint16_t mtab[kSortTile];
   h[threadIdx.x] = 0;
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
-  for_tile_occurrences<W, false>(j0, j1, nmates, occbase, mlen, wbase, words, k, lob, mtab,
+  for_tile_occurrences<W, false>(TileMates{mr, lob, mtab}, j0, j1, nmates, occbase, mlen, wbase, words, k,
                                  [&](int, bool valid, uint64_t, KV key, uint32_t) {
                                    if (valid) {
                                      SK sk;
