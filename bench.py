@@ -33,11 +33,13 @@ read sets, genomes log-uniform 100 kbp..10 Mbp, 100/150 bp SE/PE reads at
 20-80x, 0.5-2% error, k mixed 21/31/55 -- built (graph + unitigs) by the
 native batch scheduler with the tuned parameters and the learned table
 sizing, inputs resident in HBM; its k-mers/s beside the plain figure.
---workload learned runs the learned-sizing workload (100 Mbp, 20% repeat
-families, 150 bp PE, 40x, k = 31) with and without the sizing network.
---workload streamed runs the large streamed workload (500 Mbp, 30% repeat
-families, 150 bp PE, 30x, k = 55): readids text in ~1 GB chunks, key-range
-passes, each range's records handed to a sink.
+"learned_sizing" (unless --no-learned; alone: --workload learned): the
+learned-sizing workload (100 Mbp, 20% repeat families, 150 bp PE, 40x,
+k = 31), sorted and hash aggregation with and without the sizing network,
+and a unitig pass after the key-range build.  "streamed" (unless
+--no-streamed; alone: --workload streamed): the large streamed workload
+(500 Mbp, 30% repeat families, 150 bp PE, 30x, k = 55): readids text in
+~1 GB chunks, key-range passes, each range's records handed to a sink.
 """
 
 import argparse
@@ -338,6 +340,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-sizing leg (100 Mbp)")
+    ap.add_argument("--no-streamed", action="store_true", help="skip the large streamed leg (500 Mbp, k = 55)")
     ap.add_argument("--no-graph-steps", action="store_true",
                     help="skip the cleaning, statistics and BUBBLE timings after the plain leg")
     ap.add_argument("--batched-scale", type=float, default=1.0)
@@ -450,9 +453,10 @@ def main():
     # (32 B) and ends still following their chain also read their
     # successor's (16 B), counted by the library
     if st["pj_rounds"] == 0:
-        ranking = {"method": "walks from path endpoints", "walkers": st["walkers"],
+        ranking = {"method": "walks from both path endpoints", "walkers": st["walkers"],
                    "ms": round(phases.get("k_walk", 0.0), 4),
-                   "what": "k_walk_list + k_walk + k_walk_final (the walk entries come from k_links): one walk from each path endpoint"}
+                   "what": "k_walk_list + k_walk + k_walk_join + k_walk_final (the walk entries come from k_links): "
+                           "both endpoints of each path walk toward each other, each node claimed once"}
     else:
         rounds = st["pj_rounds"]
         t_pj = phases.get("k_pj_round", 0.0) / rounds * 1e-3
@@ -552,6 +556,12 @@ def main():
             learned = learned_leg(2, 2)
         except Exception as e:  # the headline line is printed regardless
             learned = {"error": f"{type(e).__name__}: {e}"}
+    streamed = None
+    if not args.no_streamed:
+        try:
+            streamed = streamed_leg(1, 1)
+        except Exception as e:  # the headline line is printed regardless
+            streamed = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(text, k)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps,
@@ -576,6 +586,7 @@ def main():
         "graph_stats": stats,
         "batched": batched,
         "learned_sizing": learned,
+        "streamed": streamed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

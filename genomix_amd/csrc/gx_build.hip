@@ -1342,39 +1342,65 @@ __global__ void k_big_sizes(uint32_t NB, const uint64_t* __restrict__ bstart, co
     sz[b] = (big[b] == 1 && bstart[b + 1] >= bstart[b]) ? bstart[b + 1] - bstart[b] : 0;
 }
 
+// Nodes are the runs of equal keys in the sorted records.  Run starts per
+// chunk of kBlock records (a record starts a run when its key differs from
+// the record before); their exclusive scan gives each chunk's first node,
+// and k_node_emit numbers the chunk's records from it.
 template <class SK>
-__global__ void k_run_flags(uint64_t N, const SK* __restrict__ keys, uint32_t* __restrict__ flag) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x)
-    flag[i] = (i == 0 || !sk_eq(keys[i], keys[i - 1])) ? 1u : 0u;
+__global__ void __launch_bounds__(kBlock) k_run_count(uint64_t N, const SK* __restrict__ keys,
+                                                      uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t ws[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (uint64_t c = blockIdx.x; c * kBlock < N; c += gridDim.x) {
+    const uint64_t i = c * kBlock + threadIdx.x;
+    const bool f = i < N && (i == 0 || !sk_eq(keys[i], keys[i - 1]));
+    const uint64_t b = __ballot(f);
+    if (lane == 0) ws[wid] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kBlock / 64; w++) t += ws[w];
+      cnt[c] = t;
+    }
+    __syncthreads();
+  }
 }
 
 // Run starts -> nodes (first record, key), and each node's neighbour mask:
-// the OR of its records' code bits, gathered per block in LDS (a block's
+// the OR of its records' code bits, gathered per block in LDS (a chunk's
 // 256 records touch at most 256 consecutive nodes); the first and last node
-// of a block may continue in a neighbouring block and are merged with a
-// global atomicOr, the others stored directly.  msk must start at zero.
+// of a chunk may continue in a neighbouring chunk and are merged with a
+// global atomicOr, the others stored directly.  Node of record i = the
+// chunk's first node (coff, the scan of k_run_count) + the run starts in
+// the chunk up to i - 1.  msk must start at zero.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename SortKeyOf<W>::T* __restrict__ keys,
                                                       const uint64_t* __restrict__ vals,
-                                                      const uint32_t* __restrict__ flag,
-                                                      const uint64_t* __restrict__ rpos, int nb,
+                                                      const uint64_t* __restrict__ coff, int nb,
                                                       uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
                                                       uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk,
                                                       uint8_t* __restrict__ rcode) {
   __shared__ uint32_t lm[kBlock];
-  __shared__ uint64_t nfirst, nlast;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < N; base += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t i = base + threadIdx.x;
-    const uint64_t last = base + kBlock < N ? base + kBlock - 1 : N - 1;
+  __shared__ uint32_t ws[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes up to and including this one
+  for (uint64_t c = blockIdx.x; c * kBlock < N; c += gridDim.x) {
+    const uint64_t base = c * kBlock, i = base + threadIdx.x;
+    const bool in = i < N;
+    const bool f = in && (i == 0 || !sk_eq(keys[i], keys[i - 1]));
+    const uint64_t b = __ballot(f);
     lm[threadIdx.x] = 0;
-    if (threadIdx.x == 0) {
-      nfirst = rpos[base + 1] - 1;
-      nlast = rpos[last + 1] - 1;
-    }
+    if (lane == 0) ws[wid] = (uint32_t)__popcll(b);
     __syncthreads();
-    if (i < N) {
-      uint64_t n = rpos[i + 1] - 1;
-      if (flag[i]) {
+    uint32_t incl = (uint32_t)__popcll(b & le);
+    for (int w = 0; w < wid; w++) incl += ws[w];
+    const uint64_t c0 = coff[c];
+    // the chunk's first node: its first record's run, which starts here or earlier
+    const uint64_t nfirst = (base == 0 || !sk_eq(keys[base], keys[base - 1])) ? c0 : c0 - 1;
+    const uint64_t nlast = coff[c + 1] - 1;
+    if (in) {
+      const uint64_t n = c0 + incl - 1;
+      if (f) {
         nstart[n] = i;
         KV v = key_of_sort(keys[i], nb);
         klo[n] = v.lo;
@@ -1385,7 +1411,7 @@ __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename
       atomicOr(&lm[n - nfirst], code_bits(code));
     }
     __syncthreads();
-    uint64_t n = nfirst + threadIdx.x;
+    const uint64_t n = nfirst + threadIdx.x;
     if (n <= nlast) {
       uint32_t bits = lm[threadIdx.x];
       if (n == nfirst || n == nlast) atomicOr(&msk[n], bits);
@@ -2343,27 +2369,28 @@ __device__ inline bool mergeable(uint64_t i, int d, const uint32_t* lnk, const u
 
 // Ranking by walks (odd k).  A merge path is a chain of node ends, each
 // leaving its node through one side; next[e] is the end that follows e on
-// its path (kEnd at the path's end).  Each path endpoint walks its path
-// once, recording at every node end it passes (walker, position) and, at
-// the end, the path's length and smallest node with the side that node
-// leaves by; the two walkers of a path pass every node once each, leaving
-// it by opposite sides, so every end has one record.  A pass over the nodes
-// then turns the record of each node's F end into its unitig
-// representative (the smallest node; bit 31 when the node reads backwards
-// in the unitig, which reads its smallest node forwards) and position: the
-// values of the pointer-jumping ranking below.  Chains are short in read
-// graphs (errors end them every few letters: the 5 Mbp workload's longest
-// has ~130 nodes), so one dependent read per step replaces several full
-// passes over all ends.  Paths longer than the cap, cycles (no endpoint)
-// and even k (a palindromic k-mer links both its sides to one neighbour)
-// take the pointer-jumping ranking instead.
+// its path (kEnd at the path's end).  Both endpoints of a path walk it
+// toward each other; a walker claims each node it reaches with a
+// compare-and-swap on the node's record (walker, side it leaves by,
+// position from its start) and stops at the first node the other walker
+// claimed, so the two walks split the path and every node is read once.
+// A pass over the walkers joins the two halves: the path's length (the sum
+// of the two claims) and its smallest node, with the side each walker
+// leaves that node by.  A pass over the nodes then turns each record into
+// the node's unitig representative (the smallest node; bit 31 when the
+// node reads backwards in the unitig, which reads its smallest node
+// forwards) and position: the values of the pointer-jumping ranking below.
+// Chains are short in read graphs (errors end them every few letters: the
+// 5 Mbp workload's longest has ~130 nodes), so one dependent read per node
+// replaces several full passes over all ends.  A walker longer than the
+// cap, cycles (no endpoint) and even k (a palindromic k-mer links both its
+// sides to one neighbour) take the pointer-jumping ranking instead.
 
 // One 16-byte entry per node end: .x = the next end on its path (kEnd at
-// the path's end), .z/.w = the walk record (low/high word).  A walk step
-// reads .x and writes the record of the same entry: one line per step.
-__device__ inline void end_record_store(uint4* E, uint32_t e, uint64_t r) {
-  reinterpret_cast<uint2*>(E + e)[1] = make_uint2((uint32_t)r, (uint32_t)(r >> 32));
-}
+// the path's end), .z/.w = a 64-bit record; node i's record is the one of
+// its F end (entry 2i), in the same line as the entry of its R end, so a
+// walk step reads one line.  Record = walker << 32 | side << 31 | position,
+// or kEndUnvisited / kEndSingle.
 __device__ inline uint64_t end_record(const uint4& x) { return ((uint64_t)x.w << 32) | x.z; }
 
 // The walkers' start ends, in node order.
@@ -2373,37 +2400,73 @@ __global__ void k_walk_list(uint64_t nn, const uint32_t* __restrict__ endpoint, 
     if (endpoint[i]) walkers[at[i]] = (uint32_t)(2 * i + (E[2 * i].x != kEnd ? 0 : 1));
 }
 
-// One walk per path endpoint: record of end e = walker << 32 | position of
-// e's node from the walker's start; sum[w] = length << 32 | (smallest node
-// leaves by its R side) << 31 | smallest node.
+// One walk per path endpoint, claiming nodes until the path's end or a
+// node the other walker of the path claimed.  claim[w] = nodes claimed <<
+// 32 | (the smallest of them is left by its R side) << 31 | that node
+// (0x7fffffff when none); met[w] = the record found on the first node
+// another walker claimed (~0 when the walk reached the path's end).
 __global__ void __launch_bounds__(kBlock) k_walk(uint32_t nwalk, const uint32_t* __restrict__ walkers,
-                                                 uint4* __restrict__ E, uint64_t* __restrict__ sum,
-                                                 uint32_t* __restrict__ capped) {
+                                                 uint4* __restrict__ E, uint64_t* __restrict__ claim,
+                                                 uint64_t* __restrict__ met, uint32_t* __restrict__ capped) {
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwalk; w += gridDim.x * blockDim.x) {
-    uint32_t e = walkers[w], idx = 0, mn = e >> 1, mn_rev = e & 1;
+    uint32_t e = walkers[w], idx = 0, mn = 0x7fffffffu, mn_rev = 0;
+    uint64_t other = ~0ull;
     for (;;) {
+      const uint32_t i = e >> 1;
       const uint32_t x = E[e].x;
-      end_record_store(E, e, ((uint64_t)w << 32) | idx);
+      unsigned long long* rec = reinterpret_cast<unsigned long long*>(E + 2 * (uint64_t)i) + 1;
+      const unsigned long long mine = ((unsigned long long)w << 32) | ((unsigned long long)(e & 1) << 31) | idx;
+      const unsigned long long old = atomicCAS(rec, (unsigned long long)kEndUnvisited, mine);
+      if (old != (unsigned long long)kEndUnvisited) {  // the other walker was here first
+        other = old;
+        break;
+      }
+      if (i < mn) {
+        mn = i;
+        mn_rev = e & 1;
+      }
+      idx++;
       if (x == kEnd) break;
-      if (idx + 1 >= kWalkCap) {
+      if (idx >= kWalkCap) {
         *capped = 1u;
         break;
       }
       e = x;
-      idx++;
-      if ((e >> 1) < mn) {
-        mn = e >> 1;
-        mn_rev = e & 1;
-      }
     }
-    sum[w] = ((uint64_t)(idx + 1) << 32) | ((uint64_t)mn_rev << 31) | mn;
+    claim[w] = ((uint64_t)idx << 32) | ((uint64_t)mn_rev << 31) | mn;
+    met[w] = other;
   }
 }
 
-// Per node from the record of its F end (the node reads forward along that
-// end's walker): representative, orientation, position; a node without a
-// record lies on a cycle or beyond the cap.
-__global__ void k_walk_final(uint64_t nn, const uint4* __restrict__ E, const uint64_t* __restrict__ sum,
+// Joins the two walks of each path: a walker that met the other writes
+// both walkers' summaries, sum = length << 32 | (the path's smallest node
+// is left by its R side, walking as this walker does) << 31 | that node.
+// When both met each other, both write the same values.
+__global__ void k_walk_join(uint32_t nwalk, const uint64_t* __restrict__ claim, const uint64_t* __restrict__ met,
+                            uint64_t* __restrict__ sum) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwalk; w += gridDim.x * blockDim.x) {
+    const uint64_t m = met[w];
+    if (m == ~0ull || (m >> 32) >= nwalk) continue;  // not another walker's record: no sum, so pointer jumping
+    const uint32_t p = (uint32_t)(m >> 32);
+    const uint64_t cw = claim[w], cp = claim[p];
+    const uint64_t len = (cw >> 32) + (cp >> 32);
+    const uint32_t mw = (uint32_t)cw & 0x7fffffffu, mp = (uint32_t)cp & 0x7fffffffu;
+    const uint32_t rw = (uint32_t)(cw >> 31) & 1u, rp = (uint32_t)(cp >> 31) & 1u;
+    // the other walker crosses every node the opposite way
+    const bool own = mw <= mp;
+    const uint32_t mn = own ? mw : mp;
+    const uint32_t rev_w = own ? rw : 1u - rp, rev_p = own ? 1u - rw : rp;
+    sum[w] = (len << 32) | ((uint64_t)rev_w << 31) | mn;
+    sum[p] = (len << 32) | ((uint64_t)rev_p << 31) | mn;
+  }
+}
+
+// Per node from its record: representative, orientation, position; a node
+// without a record lies on a cycle or beyond the cap.  The claiming walker
+// reads the unitig backwards when it leaves the smallest node by its R side
+// (bit 31 of its sum); the node reads backwards when that differs from the
+// side it leaves the node by.
+__global__ void k_walk_final(uint64_t nn, const uint4* __restrict__ E, uint64_t nwalk, const uint64_t* __restrict__ sum,
                              uint32_t* __restrict__ urep, uint32_t* __restrict__ upos, uint32_t* __restrict__ isrep,
                              uint64_t* __restrict__ ulen_of_rep, uint32_t* __restrict__ unranked) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -2415,14 +2478,19 @@ __global__ void k_walk_final(uint64_t nn, const uint4* __restrict__ E, const uin
       ulen_of_rep[i] = 1;
       continue;
     }
-    if (r == kEndUnvisited) {
+    if (r == kEndUnvisited || (r >> 32) >= nwalk) {
       *unranked = 1u;
       continue;
     }
     const uint64_t s = sum[r >> 32];
-    const uint32_t len = (uint32_t)(s >> 32), mn = (uint32_t)s & 0x7fffffffu, a = (uint32_t)r;
+    if (s == ~0ull) {  // no join reached this walker: rank again by pointer jumping
+      *unranked = 1u;
+      continue;
+    }
+    const uint32_t len = (uint32_t)(s >> 32), mn = (uint32_t)s & 0x7fffffffu, a = (uint32_t)r & 0x7fffffffu;
     const bool against = ((s >> 31) & 1) != 0;  // the unitig reads against this walker
-    urep[i] = mn | (against ? 0x80000000u : 0u);
+    const bool side = ((r >> 31) & 1) != 0;      // the walker leaves this node by its R side
+    urep[i] = mn | (against != side ? 0x80000000u : 0u);
     upos[i] = against ? len - 1 - a : a;
     isrep[i] = mn == (uint32_t)i ? 1u : 0u;
     ulen_of_rep[i] = mn == (uint32_t)i ? len : 0u;
@@ -3667,14 +3735,16 @@ static void run_build(gx_build* b) {
 
     // 5'. nodes = runs of equal keys
     ph_begin(b, PH_NODES);
-    uint32_t* rflag = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
-    uint64_t* rpos = vx;
+    // run starts per chunk of kBlock records, and each chunk's first node
+    const uint64_t nch = (Nr + kBlock - 1) / kBlock;
+    uint32_t* ccnt = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
+    uint64_t* coff = vx;
     if (Nr) {
-      hipLaunchKernelGGL(k_run_flags<SK>, dim3(grid_for(Nr, kBlock, G)), dim3(kBlock), 0, st, Nr, ky, rflag);
+      hipLaunchKernelGGL(k_run_count<SK>, dim3(grid_for(nch, 1, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, ccnt);
       GX_HIP(hipGetLastError());
     }
-    exclusive_scan<uint32_t, uint64_t>(rflag, rpos, Nr, scratch_for(b, scan_scratch_bytes<uint64_t>(Nr)), st);
-    nn = read1(st, rpos + Nr);
+    exclusive_scan<uint32_t, uint64_t>(ccnt, coff, nch, scratch_for(b, scan_scratch_bytes<uint64_t>(nch)), st);
+    nn = read1(st, coff + nch);
     if (nn >= 0xffffffffull) throw GxError(GX_ENOTSUP, "more than 2^32-1 distinct k-mers");
     klo = buf<uint64_t>(b->klo, nn + 1);
     khi = W == 2 ? buf<uint64_t>(b->khi, nn + 1) : nullptr;
@@ -3684,10 +3754,8 @@ static void run_build(gx_build* b) {
     uint8_t* rcodes = buf<uint8_t>(b->rcode, Nr + 32);
     GX_HIP(hipMemsetAsync(msk, 0, (nn + 1) * 4, st));
     if (Nr) {
-      // rpos[i + 1] - 1 is the node of record i (rpos = exclusive scan of
-      // the run-start flags, Nr + 1 entries)
-      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(Nr, kBlock, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, vy, rflag,
-                         rpos, nb, klo, khi, nst, msk, rcodes);
+      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(nch, 1, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, vy, coff, nb,
+                         klo, khi, nst, msk, rcodes);
       GX_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, Nr);
@@ -4165,17 +4233,23 @@ static void run_unitigs(gx_build* b) {
     exclusive_scan<uint32_t, uint64_t>(endp, at, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
     const uint64_t nw = read1(st, at + nn);
     uint32_t* walkers = buf<uint32_t>(b->u_walk2, nw + 1);
-    uint64_t* sum = buf<uint64_t>(b->u_ptr[1], nw + 1);
+    uint64_t* sum = buf<uint64_t>(b->u_ptr[1], 3 * (nw + 1));  // path sums, walker claims, meeting records
+    uint64_t* claim = sum + (nw + 1);
+    uint64_t* met = claim + (nw + 1);
+    GX_HIP(hipMemsetAsync(sum, 0xff, (nw + 1) * 8, st));  // a sum no join writes sends the pass to pointer jumping
     if (nw) {
       hipLaunchKernelGGL(k_walk_list, dim3(gn), dim3(kBlock), 0, st, nn, endp, at, E, walkers);
       GX_HIP(hipGetLastError());
       // one resident wave set, each lane walking ~a dozen paths in turn: a
       // wave then waits for the longest of its lanes' sums, not of single paths
-      hipLaunchKernelGGL(k_walk, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, walkers, E, sum,
-                         ws);
+      hipLaunchKernelGGL(k_walk, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, walkers, E, claim,
+                         met, ws);
+      GX_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_walk_join, dim3(grid_for(nw, kBlock, G)), dim3(kBlock), 0, st, (uint32_t)nw, claim, met,
+                         sum);
       GX_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_walk_final, dim3(gn), dim3(kBlock), 0, st, nn, E, sum, urep, upos, isrep, ulen_of_rep,
+    hipLaunchKernelGGL(k_walk_final, dim3(gn), dim3(kBlock), 0, st, nn, E, nw, sum, urep, upos, isrep, ulen_of_rep,
                        ws + 1);
     GX_HIP(hipGetLastError());
     ph_end(b, PH_KWALK);

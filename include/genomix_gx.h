@@ -126,10 +126,11 @@ int gx_unitig_info(gx_build* b, uint64_t* nrec, uint64_t* nbytes);
 int gx_unitig_emit(gx_build* b, uint8_t* dst, size_t cap, size_t* written);
 
 /* How the unitig pass ranks the nodes of its merge paths:
- * GX_RANK_AUTO (default) walks each path from its endpoints when k is odd
- * and falls back to pointer jumping for even k (palindromic k-mers), cycles
- * and paths of more than 4096 nodes; GX_RANK_POINTER_JUMPING always uses
- * pointer jumping.  The records are the same either way. */
+ * GX_RANK_AUTO (default) walks each path from both its endpoints toward
+ * each other when k is odd and falls back to pointer jumping for even k
+ * (palindromic k-mers), cycles and walks of more than 4096 nodes (every
+ * path of more than 8192); GX_RANK_POINTER_JUMPING always uses pointer
+ * jumping.  The records are the same either way. */
 #define GX_RANK_AUTO 0
 #define GX_RANK_POINTER_JUMPING 1
 int gx_unitig_set_ranking(gx_build* b, int mode);

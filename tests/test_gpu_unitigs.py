@@ -87,8 +87,9 @@ def test_walk_ranking_equals_pointer_jumping(gx, k, G, cov):
 
 
 def test_walk_falls_back_on_cycles_and_long_paths(gx):
-    """A circular genome (a cycle) and a path longer than the walk cap are
-    ranked by pointer jumping in the default mode, with the oracle's records."""
+    """A circular genome (a cycle) and a path longer than the two walkers'
+    caps together are ranked by pointer jumping in the default mode, with the
+    oracle's records."""
     rng = random.Random(77)
     k = 21
     genome = "".join(rng.choice("ACGT") for _ in range(6000))
@@ -101,12 +102,16 @@ def test_walk_falls_back_on_cycles_and_long_paths(gx):
         got = b.unitigs()
         assert b.stats()["pj_rounds"] > 0
     assert got == oracle.unitigs(oracle.build_graph(text, k)[0], k)
-    lin = "".join(f"{4 * i + 2}\t{genome[p:p + 120]}\n" for i, p in enumerate(range(0, 6000 - 120, 40))).encode()
+    # the two walkers of a path each claim at most 4096 nodes: a path of
+    # ~9900 nodes leaves nodes unclaimed, whichever walker runs ahead
+    long_genome = genome + "".join(rng.choice("ACGT") for _ in range(4000))
+    lin = "".join(f"{4 * i + 2}\t{long_genome[p:p + 120]}\n"
+                  for i, p in enumerate(range(0, len(long_genome) - 120, 40))).encode()
     with gx.GraphBuilder(k) as b:
         b.add_reads(lin)
         b.finish()
         got = b.unitigs()
-        assert b.stats()["pj_rounds"] > 0  # one path of ~5900 nodes, beyond the 4096-node cap
+        assert b.stats()["pj_rounds"] > 0
     assert got == oracle.unitigs(oracle.build_graph(lin, k)[0], k)
 
 
