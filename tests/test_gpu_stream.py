@@ -98,6 +98,29 @@ def test_ranges_twice_on_one_handle(gx):
             assert (b.part_bytes(0), b.unitigs()) == want[name], name
 
 
+@pytest.mark.parametrize("per", [0, 1 << 15])
+def test_repeated_builds_reuse_every_buffer(gx, per):
+    """A build repeated on its handle allocates nothing once the handle's
+    buffers have settled (one pass, and key ranges that keep their graph),
+    and gives the same bytes and unitigs."""
+    text = _text(31337)
+    with gx.GraphBuilder(31) as b:
+        b.set_stream(range_occurrences=per)
+        out = []
+        for step in range(4):
+            b.reset()
+            b.add_reads(text)
+            b.finish()
+            st = b.stats()
+            out.append((b.part_bytes(0), b.unitigs()))
+            if per:
+                assert st["key_ranges"] > 1
+            if step >= 2:
+                assert st["device_allocations"] == 0, f"step {step}"
+                assert st["buffers_reclaimed"] == 0, f"step {step}"
+        assert all(o == out[0] for o in out)
+
+
 @pytest.mark.parametrize("method", ["sort", "hash"])
 def test_sink_receives_each_partition_in_order(gx, method):
     from genomix_amd import devmem
