@@ -2259,9 +2259,15 @@ __device__ inline bool self_listed(KV v, KV rv, int k, uint32_t mask) {
 // of degree 1; the index slot holds the partner's neighbour mask) and the
 // record's initial value, and whether the node ends a path (one mergeable
 // side).
+// With half (odd k, where no k-mer is its own reverse complement, so each
+// node end has one partner end), a link is looked up only by the node of
+// the smaller key, which writes the partner entries of both its ends when
+// the link is mergeable and nothing otherwise: lnk must start at kEnd, it
+// then holds mergeable links only (all its readers want), and k_link_ends
+// makes the walk entries from it.
 template <int W>
 __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi,
-                        const uint32_t* __restrict__ msk, NodeIndex ix, int k,
+                        const uint32_t* __restrict__ msk, NodeIndex ix, int k, bool half,
                         uint32_t* __restrict__ lnk, uint8_t* __restrict__ tand, unsigned long long* err,
                         uint4* __restrict__ E, uint32_t* __restrict__ endpoint) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -2280,7 +2286,8 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
       const int d = (t == ET_FF || t == ET_FR) ? 0 : 1;
       if (side_deg(mask, d) == 1) {
         z[d] = y;
-        need[d] = true;
+        // half: the node of the smaller key looks the link up
+        need[d] = !half || (W == 1 ? v.lo < y.lo : (v.hi < y.hi || (v.hi == y.hi && v.lo < y.lo)));
         back[d] = (uint32_t)back_side(t);
       }
     }
@@ -2338,6 +2345,18 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
       else l[d] = (found[d] << 1) | back[d];
     }
     tand[i] = tnd ? 1 : 0;
+    if (half) {
+      // both sides on one partner end would give that end two partners
+      const bool same = l[0] != kEnd && l[0] == l[1];
+      for (int d = 0; d < 2; d++) {
+        if (l[d] == kEnd || same || tnd || found[d] == (uint32_t)i || side_deg(pmask[d], (int)back[d]) != 1 ||
+            self_listed<W>(z[d], revcomp<W>(z[d], k), k, pmask[d]))
+          continue;
+        lnk[2 * i + d] = l[d];
+        lnk[2 * (uint64_t)found[d] + back[d]] = ((uint32_t)i << 1) | (uint32_t)d;
+      }
+      continue;
+    }
     lnk[2 * i] = l[0];
     lnk[2 * i + 1] = l[1];
     if (E) {
@@ -2354,6 +2373,23 @@ __global__ void k_links(uint64_t nn, const uint64_t* __restrict__ klo, const uin
       E[2 * i + 1] = make_uint4(nx[1], 0u, (uint32_t)r, (uint32_t)(r >> 32));
       endpoint[i] = mg[0] != mg[1] ? 1u : 0u;
     }
+  }
+}
+
+// Walk entries from links that hold mergeable partners only (k_links with
+// half): the next end leaves the partner by its other side; the record
+// starts unvisited on a node with a mergeable side, single otherwise; a
+// node with one mergeable side ends a path.
+__global__ void k_link_ends(uint64_t nn, const uint32_t* __restrict__ lnk, uint4* __restrict__ E,
+                            uint32_t* __restrict__ endpoint) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 l = reinterpret_cast<const uint2*>(lnk)[i];
+    const bool m0 = l.x != kEnd, m1 = l.y != kEnd;
+    const uint32_t n0 = m0 ? (l.x ^ 1u) : kEnd, n1 = m1 ? (l.y ^ 1u) : kEnd;
+    const uint64_t r = (m0 || m1) ? kEndUnvisited : kEndSingle;
+    E[2 * i] = make_uint4(n0, 0u, (uint32_t)r, (uint32_t)(r >> 32));
+    E[2 * i + 1] = make_uint4(n1, 0u, (uint32_t)r, (uint32_t)(r >> 32));
+    endpoint[i] = m0 != m1 ? 1u : 0u;
   }
 }
 
@@ -4151,23 +4187,25 @@ static void run_build(gx_build* b) {
       b->reclaim.dead.push_back(acc[i]);
     }
     const uint64_t nn = tot_nn;
-    uint64_t* htot = buf<uint64_t>(b->htot, nn + 1);
-    uint64_t* hstart = buf<uint64_t>(b->hstart, nn + 1);
-    if (nn) {
-      hipLaunchKernelGGL(k_head_total, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, b->hcu.as<uint32_t>(),
-                         b->hcf.as<uint32_t>(), htot);
-      GX_HIP(hipGetLastError());
+    if (nn < 0xffffffffull) {  // the node index and the graph steps number nodes in 32 bits
+      uint64_t* htot = buf<uint64_t>(b->htot, nn + 1);
+      uint64_t* hstart = buf<uint64_t>(b->hstart, nn + 1);
+      if (nn) {
+        hipLaunchKernelGGL(k_head_total, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn,
+                           b->hcu.as<uint32_t>(), b->hcf.as<uint32_t>(), htot);
+        GX_HIP(hipGetLastError());
+      }
+      exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
+      ph_begin(b, PH_NIX);
+      build_node_index<W>(b, nn, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr,
+                          b->msk.as<uint32_t>(), 0u, 256u);
+      ph_end(b, PH_NIX);
+      GX_HIP(hipStreamSynchronize(st));
+      ph_collect(b, true);
     }
-    exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
-    ph_begin(b, PH_NIX);
-    build_node_index<W>(b, nn, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr,
-                        b->msk.as<uint32_t>(), 0u, 256u);
-    ph_end(b, PH_NIX);
-    GX_HIP(hipStreamSynchronize(st));
-    ph_collect(b, true);
     b->nnodes = nn;
   }
-  b->graph_held = R == 1 || !b->sink;
+  b->graph_held = R == 1 || (!b->sink && tot_nn < 0xffffffffull);
   b->part_rec = prec;
   b->part_off = pbyt;
   b->streamed = b->sink != nullptr;
@@ -4212,10 +4250,17 @@ static void run_unitigs(gx_build* b) {
   const bool walk = nn && (k & 1) && !b->no_walk;
   uint4* E = walk ? buf<uint4>(b->u_ptr[0], ne + 1) : nullptr;  // the pointer-jumping buffers, free until a fallback
   uint32_t* endp = walk ? buf<uint32_t>(b->u_walk, nn + 1) : nullptr;
+  // odd k: each link looked up once, by the node of the smaller key
+  const bool half = (k & 1) != 0;
+  if (half) GX_HIP(hipMemsetAsync(lnk, 0xff, (ne + 1) * 4, st));
   if (nn) {
-    hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, nix, k, lnk,
-                       tand, errw, E, endp);
+    hipLaunchKernelGGL(k_links<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, msk, nix, k, half,
+                       lnk, tand, errw, half ? nullptr : E, endp);
     GX_HIP(hipGetLastError());
+    if (half && E) {
+      hipLaunchKernelGGL(k_link_ends, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, lnk, E, endp);
+      GX_HIP(hipGetLastError());
+    }
   }
   uint32_t* urep = buf<uint32_t>(b->u_rep, nn + 1);
   uint32_t* upos = buf<uint32_t>(b->u_pos, nn + 1);
@@ -5795,7 +5840,9 @@ static void fit_mixture(const double* x, const uint64_t* w, size_t n, double max
 static void check_graph(gx_build* b, int graph) {
   if (graph != GX_GRAPH_NODES && graph != GX_GRAPH_UNITIGS) throw GxError(GX_EINVAL, "unknown graph");
   if (!b->finished) throw GxError(GX_ESTATE, "statistics before finish");
-  if (!b->graph_held) throw GxError(GX_ENOTSUP, "the node graph of a build with a record sink is not held in HBM");
+  if (!b->graph_held)
+    throw GxError(GX_ENOTSUP, "the node graph of this key-range build is not held in HBM (its records went to a sink, "
+                              "or it has 2^32 or more nodes)");
   if (graph == GX_GRAPH_UNITIGS && !b->unitigs_done) throw GxError(GX_ESTATE, "no unitig pass to report on");
 }
 
@@ -5897,8 +5944,9 @@ int gx_unitig_run(gx_build* b) {
   return guarded(b, [&] {
     if (!b->finished) throw GxError(GX_ESTATE, "unitig_run before finish");
     if (!b->graph_held)
-      throw GxError(GX_ENOTSUP, "the unitig pass needs the whole graph in HBM; this build handed its " +
-                                    std::to_string(b->nranges) + " key ranges to a record sink");
+      throw GxError(GX_ENOTSUP, "the unitig pass needs the whole graph in HBM; this build of " +
+                                    std::to_string(b->nranges) +
+                                    " key ranges handed them to a record sink or holds 2^32 or more nodes");
     GX_HIP(hipSetDevice(b->cfg.device));
     ReclaimScope rs(&b->reclaim, b->st);
     b->st_graph = -1;

@@ -52,8 +52,9 @@ typedef int (*gx_part_sink)(void* user, int part, const void* dev_bytes, uint64_
  * order, and a node index over them is built after the last range, so the
  * unitig pass, cleaning and statistics take a build of several ranges as
  * they take a one-pass build.  After a build with a sink they fail with
- * GX_ENOTSUP (its graph was never whole in HBM).  The range count of the
- * last build is gx_build_stats entry 11. */
+ * GX_ENOTSUP (its graph was never whole in HBM), as after a build of 2^32
+ * or more distinct k-mers (node numbers are 32-bit).  The range count of
+ * the last build is gx_build_stats entry 11. */
 int gx_build_set_stream(gx_build* b, uint64_t range_occurrences, gx_part_sink sink, void* user);
 
 #ifdef __cplusplus
