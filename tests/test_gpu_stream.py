@@ -76,6 +76,43 @@ def test_ranges_match_single_pass_and_oracle(gx, k, P):
         assert b.unitigs() == u1
 
 
+def test_bubbles_and_fastq_after_ranges(gx):
+    """BUBBLE on the graph a key-range build kept equals BUBBLE after a
+    one-pass build and the oracle's pipeline; paired FASTQ converted on the
+    device and built in ranges gives the one-pass bytes and unitigs."""
+    from genomix_amd import sim
+    k = 31
+    text = _text(2468, G=30000, cov=25)
+    with gx.GraphBuilder(k) as b:
+        b.add_reads(text)
+        b.finish()
+        b.run_unitigs()
+        want_bub = b.clean_bubbles(0.1)
+        N = b.stats()["occurrences"]
+    assert want_bub == oracle.bubble_merge(oracle.build_graph(text, k)[0], k, 0.1)
+    with gx.GraphBuilder(k) as b:
+        b.set_stream(range_occurrences=N // 7)
+        b.add_reads(text)
+        b.finish()
+        assert b.stats()["key_ranges"] > 1
+        b.run_unitigs()
+        assert b.clean_bubbles(0.1) == want_bub
+    spec = sim.ReadSetSpec(genome_len=20000, read_len=100, coverage=15, error_rate=0.01, paired=True,
+                           insert_mean=300, insert_sd=25, seed=1357)
+    m1, m2 = sim.reads(spec)
+    f1, f2 = sim.fastq_text(m1), sim.fastq_text(m2)
+    got = []
+    for per in (0, 20_000):
+        with gx.GraphBuilder(k) as b:
+            b.set_stream(range_occurrences=per)
+            b.add_fastq(f1, f2)
+            b.finish()
+            assert (b.stats()["key_ranges"] > 1) == (per > 0)
+            got.append((b.readids(), b.part_bytes(0), b.unitigs()))
+    assert got[0] == got[1]
+    assert got[0][1] == oracle.build_graph(oracle.fastq_to_readids(f1, f2), k)[0]
+
+
 def test_ranges_twice_on_one_handle(gx):
     """A second key-range build on the same handle (its buffers, and the
     node graph of the first, reused or reclaimed) gives the same unitigs;
