@@ -1223,18 +1223,21 @@ __device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
 __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 
 // One block per bucket: bitonic sort of (key below the bucket, index) in
-// LDS, then the records are written in that order.  Four tiers: one wave
-// per bucket (CAP = kSmallBucket) over every bucket of the key range; it
-// lists the buckets above that for the CAP = kBucketCap / 8 (256 threads),
-// kBucketCap / 2 (512) and kBucketCap (1024) blocks, which run over their
-// lists only (buckets marked 4, 3, 2).  Oversized buckets (above
-// kBucketCap) are copied unsorted and marked 1 for the global pass.
+// LDS, then the records are written in that order: the key, and of the
+// value only its neighbour code (the occurrence index in it only orders
+// equal keys, which the sort's index already does), one byte per record:
+// the replay's code array.  Four tiers: one wave per bucket (CAP =
+// kSmallBucket) over every bucket of the key range; it lists the buckets
+// above that for the CAP = kBucketCap / 8 (256 threads), kBucketCap / 2
+// (512) and kBucketCap (1024) blocks, which run over their lists only
+// (buckets marked 4, 3, 2).  Oversized buckets (above kBucketCap) are
+// marked 1 and left to the global pass.
 constexpr int kSmallBucket = 256;
 template <int W, int CAP, int NT>
 __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
                                        const typename SortKeyOf<W>::T* __restrict__ ik,
                                        const uint64_t* __restrict__ iv, typename SortKeyOf<W>::T* __restrict__ ok,
-                                       uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart, uint64_t N,
+                                       uint8_t* __restrict__ oc, const uint64_t* __restrict__ bstart, uint64_t N,
                                        int nb, int bkb, uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
                                        uint32_t* __restrict__ nlist, uint32_t NBK, unsigned long long* err) {
   using SK = typename SortKeyOf<W>::T;
@@ -1258,10 +1261,11 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
     }
     if (threadIdx.x == 0) big[bk] = 1;
   }
-  if (n > kBucketCap || n == 1 || nb < 2) {
+  if (n > kBucketCap) return;  // the global pass sorts and writes it
+  if (n == 1 || nb < 2) {
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
       ok[a + i] = ik[a + i];
-      ov[a + i] = iv[a + i];
+      oc[a + i] = (uint8_t)iv[a + i];
     }
     return;
   }
@@ -1291,7 +1295,7 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
   for (uint32_t i = threadIdx.x; i < n; i += NT) {
     uint32_t src = elem_idx(s[i]);
     ok[a + i] = ik[a + src];
-    ov[a + i] = iv[a + src];
+    oc[a + i] = (uint8_t)iv[a + src];
   }
 }
 
@@ -1301,7 +1305,7 @@ template <int W, int CAP, int NT>
 __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
                                                         const uint64_t* __restrict__ iv,
                                                         typename SortKeyOf<W>::T* __restrict__ ok,
-                                                        uint64_t* __restrict__ ov, const uint64_t* __restrict__ bstart,
+                                                        uint8_t* __restrict__ oc, const uint64_t* __restrict__ bstart,
                                                         uint64_t N, int nb, int bkb, uint32_t bucket0,
                                                         uint32_t count, const uint32_t* __restrict__ list,
                                                         uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
@@ -1310,27 +1314,28 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
   __shared__ typename SortKeyOf<W>::T s[CAP];
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t bk = list ? list[q] : bucket0 + q;
-    sort_one_bucket<W, CAP, NT>(bk, s, ik, iv, ok, ov, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
+    sort_one_bucket<W, CAP, NT>(bk, s, ik, iv, ok, oc, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
     __syncthreads();  // s is reused by the next bucket
   }
 }
 
-// Oversized buckets to and from a contiguous array (dir 0: gather, 1: back).
+// Oversized buckets to a contiguous array (dir 0: their records from the
+// bucket sort's input) and back (dir 1: keys and codes to its output).
 template <class SK>
 __global__ void k_big_move(const uint64_t* __restrict__ bstart, const uint32_t* __restrict__ big,
-                           const uint64_t* __restrict__ boff, SK* __restrict__ keys, uint64_t* __restrict__ vals,
-                           SK* __restrict__ tk, uint64_t* __restrict__ tv, int dir, uint32_t bucket0,
-                           uint32_t bucket1) {
+                           const uint64_t* __restrict__ boff, const SK* __restrict__ ik, const uint64_t* __restrict__ iv,
+                           SK* __restrict__ ok, uint8_t* __restrict__ oc, SK* __restrict__ tk,
+                           uint64_t* __restrict__ tv, int dir, uint32_t bucket0, uint32_t bucket1) {
   for (uint32_t bk = bucket0 + blockIdx.x; bk < bucket1; bk += gridDim.x) {
     if (big[bk] != 1) continue;
     const uint64_t a = bstart[bk], n = bstart[bk + 1] - a, o = boff[bk];
     for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
       if (dir == 0) {
-        tk[o + i] = keys[a + i];
-        tv[o + i] = vals[a + i];
+        tk[o + i] = ik[a + i];
+        tv[o + i] = iv[a + i];
       } else {
-        keys[a + i] = tk[o + i];
-        vals[a + i] = tv[o + i];
+        ok[a + i] = tk[o + i];
+        oc[a + i] = (uint8_t)tv[o + i];
       }
     }
   }
@@ -1375,11 +1380,10 @@ __global__ void __launch_bounds__(kBlock) k_run_count(uint64_t N, const SK* __re
 // the chunk up to i - 1.  msk must start at zero.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename SortKeyOf<W>::T* __restrict__ keys,
-                                                      const uint64_t* __restrict__ vals,
+                                                      const uint8_t* __restrict__ rcode,
                                                       const uint64_t* __restrict__ coff, int nb,
                                                       uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
-                                                      uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk,
-                                                      uint8_t* __restrict__ rcode) {
+                                                      uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk) {
   __shared__ uint32_t lm[kBlock];
   __shared__ uint32_t ws[kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1406,9 +1410,7 @@ __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename
         klo[n] = v.lo;
         if (W == 2) khi[n] = v.hi;
       }
-      uint32_t code = (uint32_t)(vals[i] & 0xff);
-      rcode[i] = (uint8_t)code;
-      atomicOr(&lm[n - nfirst], code_bits(code));
+      atomicOr(&lm[n - nfirst], code_bits(rcode[i]));
     }
     __syncthreads();
     const uint64_t n = nfirst + threadIdx.x;
@@ -3714,13 +3716,15 @@ static void run_build(gx_build* b) {
                        bk0, bk1, bstart);
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemsetAsync(errw + 2, 0xff, 8, st));
+    // sorted records: keys in ky, neighbour codes (the replay's code array) in rcodes
+    uint8_t* rcodes = buf<uint8_t>(b->rcode, Nr + 32);
     uint32_t* blists = buf<uint32_t>(b->blist, 3 * (size_t)NBK + 3);
     uint32_t* nlist = blists + 3 * (size_t)NBK;
     GX_HIP(hipMemsetAsync(nlist, 0, 12, st));
     const unsigned kMaxSortBlocks = 1u << 20;
     hipLaunchKernelGGL((k_bucket_sort<W, kSmallBucket, 64>), dim3(std::min(bk1 - bk0, kMaxSortBlocks)), dim3(64), 0,
-                       st, kx, vx, ky, vy, bstart, Nr, nb, bkb, bk0, bk1 - bk0, nullptr, big, blists, nlist, NBK,
-                       errw + 2);
+                       st, kx, vx, ky, rcodes, bstart, Nr, nb, bkb, bk0, bk1 - bk0, nullptr, big, blists, nlist,
+                       NBK, errw + 2);
     GX_HIP(hipGetLastError());
     uint32_t nl[3];
     GX_HIP(hipMemcpyAsync(nl, nlist, 12, hipMemcpyDeviceToHost, st));
@@ -3729,19 +3733,19 @@ static void run_build(gx_build* b) {
       throw GxError(GX_EDEVICE, "internal error: bucket list overflow");
     if (nl[0]) {
       hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 8, 256>), dim3(std::min(nl[0], 65536u)), dim3(256), 0, st, kx,
-                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[0], blists, big, nullptr, nullptr, NBK, errw + 2);
+                         vx, ky, rcodes, bstart, Nr, nb, bkb, bk0, nl[0], blists, big, nullptr, nullptr, NBK, errw + 2);
       GX_HIP(hipGetLastError());
     }
     if (nl[1]) {
       hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap / 2, 512>), dim3(std::min(nl[1], 65536u)), dim3(512), 0, st, kx,
-                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[1], blists + NBK, big, nullptr, nullptr, NBK,
+                         vx, ky, rcodes, bstart, Nr, nb, bkb, bk0, nl[1], blists + NBK, big, nullptr, nullptr, NBK,
                          errw + 2);
       GX_HIP(hipGetLastError());
     }
     if (nl[2]) {
       hipLaunchKernelGGL((k_bucket_sort<W, kBucketCap, 1024>), dim3(std::min(nl[2], 65536u)), dim3(1024), 0, st, kx,
-                         vx, ky, vy, bstart, Nr, nb, bkb, bk0, nl[2], blists + 2 * (size_t)NBK, big, nullptr, nullptr,
-                         NBK, errw + 2);
+                         vx, ky, rcodes, bstart, Nr, nb, bkb, bk0, nl[2], blists + 2 * (size_t)NBK, big, nullptr,
+                         nullptr, NBK, errw + 2);
       GX_HIP(hipGetLastError());
     }
     if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
@@ -3754,16 +3758,16 @@ static void run_build(gx_build* b) {
       SK* tk2 = buf<SK>(b->hkey2, nbig + 1);
       uint64_t* tv = buf<uint64_t>(b->hval, nbig + 1);
       uint64_t* tv2 = buf<uint64_t>(b->hval2, nbig + 1);
-      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, ky,
-                         vy, tk, tv, 0, bk0, bk1);
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, kx,
+                         vx, ky, rcodes, tk, tv, 0, bk0, bk1);
       GX_HIP(hipGetLastError());
       // all key bytes: several oversized buckets share the array, and the
       // stable sort keeps each bucket's records in occurrence order
       bool f = radix_sort<SK, uint64_t>(tk, tv, tk2, tv2, nbig, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nbig)),
                                         st);
       if (f) { std::swap(tk, tk2); std::swap(tv, tv2); }
-      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, ky,
-                         vy, tk, tv, 1, bk0, bk1);
+      hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, kx,
+                         vx, ky, rcodes, tk, tv, 1, bk0, bk1);
       GX_HIP(hipGetLastError());
     }
     b->stats[9] += nbig;
@@ -3787,11 +3791,10 @@ static void run_build(gx_build* b) {
     cnt = buf<uint32_t>(b->cnt, nn + 1);
     msk = buf<uint32_t>(b->msk, nn + 1);
     uint64_t* nst = buf<uint64_t>(b->nslot, nn + 2);
-    uint8_t* rcodes = buf<uint8_t>(b->rcode, Nr + 32);
     GX_HIP(hipMemsetAsync(msk, 0, (nn + 1) * 4, st));
     if (Nr) {
-      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(nch, 1, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, vy, coff, nb,
-                         klo, khi, nst, msk, rcodes);
+      hipLaunchKernelGGL(k_node_emit<W>, dim3(grid_for(nch, 1, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, rcodes, coff,
+                         nb, klo, khi, nst, msk);
       GX_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, nst + nn, Nr);
