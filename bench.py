@@ -421,7 +421,7 @@ def main():
     # algorithmic bytes per k-mer occurrence (ASCII read in, one table slot
     # read and written; 33.25 B here).  `roofline`: the launch that takes every
     # occurrence into the aggregation (k_occ_scatter: canonical k-mer, neighbour
-    # code and the stable scatter of its 16-byte record), timed with hipEvents
+    # code and the stable scatter of its record, key and code byte), timed with hipEvents
     # on the library's stream; the same B over the whole aggregation phase and
     # over the whole step beside it.
     B = L / (L - k + 1) + 2 * (8 * ((k + 31) // 32) + 8)
@@ -441,8 +441,9 @@ def main():
         "whole_step": {"ms": round(ms_step, 3), "frac": round(B * value / 1e9 / HBM_PEAK_GBS, 4),
                        "what": "B x k-mers/s of the whole step (build + unitigs)"}}
     # the k-mer pass (k_occ_hist + k_occ_scatter) on its own bytes: two reads
-    # of the 2-bit packed mates, one 16-byte record written per occurrence
-    Bk = 2 * (L / (L - k + 1)) / 4 + 16
+    # of the 2-bit packed mates, one record (sort key + neighbour code byte)
+    # written per occurrence
+    Bk = 2 * (L / (L - k + 1)) / 4 + 8 * ((k + 31) // 32) + 1
     t_k = phases.get("k_insert", 0.0) * 1e-3
     ach_k = Bk * N / t_k / 1e9 if t_k > 0 else 0.0
     roofline_kmer = {"bound": "hbm", "achieved": round(ach_k, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

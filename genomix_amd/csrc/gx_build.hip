@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
 
 constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
 // HBM of a key-range pass, per occurrence (sizes the ranges): the records
-// (two key and two value arrays, a replay code byte) and, per node (dfrac
+// (two key and two neighbour-code arrays, the sorted codes) and, per node (dfrac
 // nodes per occurrence), every node array of the pass -- key words, count,
 // mask, first record, two index slots, replay flags, counts, offsets,
 // lists and order, head counts and starts, record sizes and offsets -- and
@@ -926,7 +926,7 @@ constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 1
 // live together: freeing dead buffers on demand works, but re-allocating
 // 100 GB-scale buffers every pass costs more than an extra pass.
 __host__ inline double pass_bytes_per_occurrence(int W, int nb, double dfrac) {
-  const double rec = 2.0 * (W == 1 ? 8 : 16) + 16 + 1;
+  const double rec = 2.0 * (W == 1 ? 8 : 16) + 2 + 1;
   const double node = 8.0 * W + 4 + 4 + 8 + 2.0 * (W == 1 ? 16 : 24) + 4 + 8 + 8 + 8 + 8 + 16 + 8 + 16 + 16;
   const double out = 3.0 * (4 + nb) + 16;
   return rec + dfrac * (node + out);
@@ -1081,7 +1081,10 @@ __global__ void __launch_bounds__(kBlock) k_occ_hist(uint64_t N, uint64_t nmates
 // stage holds 4 bytes per record (occurrence in the tile, neighbour code);
 // the write-out extracts each record's key again from the packed mates
 // (which the tile has just read, so they come from the caches), which
-// halves the block's LDS and doubles the blocks a CU holds.
+// halves the block's LDS and doubles the blocks a CU holds.  A record is
+// its sort key and its neighbour code (one byte): the sorts after this are
+// stable or break ties by the record's index in its bucket, so occurrence
+// order among equal keys needs no stored occurrence number.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nmates,
                                                         const uint64_t* __restrict__ occbase,
@@ -1090,7 +1093,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         const uint64_t* __restrict__ words, int k, int nb,
                                                         const uint64_t* __restrict__ offs, unsigned ntiles,
                                                         typename SortKeyOf<W>::T* __restrict__ okeys,
-                                                        uint64_t* __restrict__ ovals, uint32_t tlo, uint32_t thi,
+                                                        uint8_t* __restrict__ ocodes, uint32_t tlo, uint32_t thi,
                                                         int dbyte, uint32_t nlo, uint32_t nhi, int ndbyte,
                                                         uint32_t* __restrict__ nhist) {
   using SK = typename SortKeyOf<W>::T;
@@ -1153,7 +1156,7 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     const uint32_t d = key_byte(skey, dbyte);
     const uint64_t pos = run[d] + (i - lbase[d]);
     okeys[pos] = skey;
-    ovals[pos] = (j << 8) | (v & 0xffu);
+    ocodes[pos] = (uint8_t)v;
   }
   if (nhist) nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h2[threadIdx.x];
 }
@@ -1223,20 +1226,18 @@ __device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
 __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 
 // One block per bucket: bitonic sort of (key below the bucket, index) in
-// LDS, then the records are written in that order: the key, and of the
-// value only its neighbour code (the occurrence index in it only orders
-// equal keys, which the sort's index already does), one byte per record:
-// the replay's code array.  Four tiers: one wave per bucket (CAP =
-// kSmallBucket) over every bucket of the key range; it lists the buckets
-// above that for the CAP = kBucketCap / 8 (256 threads), kBucketCap / 2
-// (512) and kBucketCap (1024) blocks, which run over their lists only
-// (buckets marked 4, 3, 2).  Oversized buckets (above kBucketCap) are
-// marked 1 and left to the global pass.
+// LDS, then the records are written in that order: the key and its
+// neighbour code (one byte per record: the replay's code array).  Four
+// tiers: one wave per bucket (CAP = kSmallBucket) over every bucket of the
+// key range; it lists the buckets above that for the CAP = kBucketCap / 8
+// (256 threads), kBucketCap / 2 (512) and kBucketCap (1024) blocks, which
+// run over their lists only (buckets marked 4, 3, 2).  Oversized buckets
+// (above kBucketCap) are marked 1 and left to the global pass.
 constexpr int kSmallBucket = 256;
 template <int W, int CAP, int NT>
 __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
                                        const typename SortKeyOf<W>::T* __restrict__ ik,
-                                       const uint64_t* __restrict__ iv, typename SortKeyOf<W>::T* __restrict__ ok,
+                                       const uint8_t* __restrict__ ic, typename SortKeyOf<W>::T* __restrict__ ok,
                                        uint8_t* __restrict__ oc, const uint64_t* __restrict__ bstart, uint64_t N,
                                        int nb, int bkb, uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
                                        uint32_t* __restrict__ nlist, uint32_t NBK, unsigned long long* err) {
@@ -1265,7 +1266,7 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
   if (n == 1 || nb < 2) {
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
       ok[a + i] = ik[a + i];
-      oc[a + i] = (uint8_t)iv[a + i];
+      oc[a + i] = ic[a + i];
     }
     return;
   }
@@ -1295,7 +1296,7 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
   for (uint32_t i = threadIdx.x; i < n; i += NT) {
     uint32_t src = elem_idx(s[i]);
     ok[a + i] = ik[a + src];
-    oc[a + i] = (uint8_t)iv[a + src];
+    oc[a + i] = ic[a + src];
   }
 }
 
@@ -1303,7 +1304,7 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
 // the first tier over buckets [bucket0, bucket1), the others over a list.
 template <int W, int CAP, int NT>
 __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>::T* __restrict__ ik,
-                                                        const uint64_t* __restrict__ iv,
+                                                        const uint8_t* __restrict__ ic,
                                                         typename SortKeyOf<W>::T* __restrict__ ok,
                                                         uint8_t* __restrict__ oc, const uint64_t* __restrict__ bstart,
                                                         uint64_t N, int nb, int bkb, uint32_t bucket0,
@@ -1314,7 +1315,7 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
   __shared__ typename SortKeyOf<W>::T s[CAP];
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t bk = list ? list[q] : bucket0 + q;
-    sort_one_bucket<W, CAP, NT>(bk, s, ik, iv, ok, oc, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
+    sort_one_bucket<W, CAP, NT>(bk, s, ik, ic, ok, oc, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
     __syncthreads();  // s is reused by the next bucket
   }
 }
@@ -1323,19 +1324,19 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
 // bucket sort's input) and back (dir 1: keys and codes to its output).
 template <class SK>
 __global__ void k_big_move(const uint64_t* __restrict__ bstart, const uint32_t* __restrict__ big,
-                           const uint64_t* __restrict__ boff, const SK* __restrict__ ik, const uint64_t* __restrict__ iv,
+                           const uint64_t* __restrict__ boff, const SK* __restrict__ ik, const uint8_t* __restrict__ ic,
                            SK* __restrict__ ok, uint8_t* __restrict__ oc, SK* __restrict__ tk,
-                           uint64_t* __restrict__ tv, int dir, uint32_t bucket0, uint32_t bucket1) {
+                           uint8_t* __restrict__ tc, int dir, uint32_t bucket0, uint32_t bucket1) {
   for (uint32_t bk = bucket0 + blockIdx.x; bk < bucket1; bk += gridDim.x) {
     if (big[bk] != 1) continue;
     const uint64_t a = bstart[bk], n = bstart[bk + 1] - a, o = boff[bk];
     for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
       if (dir == 0) {
         tk[o + i] = ik[a + i];
-        tv[o + i] = iv[a + i];
+        tc[o + i] = ic[a + i];
       } else {
         ok[a + i] = tk[o + i];
-        oc[a + i] = (uint8_t)tv[o + i];
+        oc[a + i] = tc[o + i];
       }
     }
   }
@@ -3663,8 +3664,12 @@ static void run_build(gx_build* b) {
     ph_begin(b, PH_INSERT);
     SK* ka = buf<SK>(b->ckey, Nr + 2);
     SK* kb = buf<SK>(b->ckey2, Nr + 2);
-    uint64_t* va = buf<uint64_t>(b->sva, Nr + 2);
-    uint64_t* vb = buf<uint64_t>(b->svb, Nr + 2);
+    // neighbour codes beside the keys; after the sorts the node pass keeps
+    // its chunk offsets (nch + 1 words) in the free one of the pair
+    const uint64_t nch = (Nr + kBlock - 1) / kBlock;
+    const size_t vbytes = std::max<size_t>(Nr + 2, 8 * (nch + 2));
+    uint8_t* va = buf<uint8_t>(b->sva, vbytes);
+    uint8_t* vb = buf<uint8_t>(b->svb, vbytes);
     const unsigned ntiles = (unsigned)std::max<uint64_t>(1, (N + kSortTile - 1) / kSortTile);
     // record-sort buckets: the top bkb key bytes (LDS-sized buckets)
     const int bkb = bkb_of(Nr);
@@ -3697,9 +3702,9 @@ static void run_build(gx_build* b) {
     std::swap(hcur, hnext);
     // the more significant bucket bytes
     SK *kx = ka, *ky = kb;
-    uint64_t *vx = va, *vy = vb;
+    uint8_t *vx = va, *vy = vb;
     if (bkb >= 2 && Nr > 1) {
-      bool f = radix_sort<SK, uint64_t>(ka, va, kb, vb, Nr, nb - bkb + 1, nb,
+      bool f = radix_sort<SK, uint8_t>(ka, va, kb, vb, Nr, nb - bkb + 1, nb,
                                         scratch_for(b, radix_scratch_bytes<SK>(Nr)), st);
       if (f) { std::swap(kx, ky); std::swap(vx, vy); }
     }
@@ -3756,14 +3761,14 @@ static void run_build(gx_build* b) {
     if (nbig) {
       SK* tk = buf<SK>(b->hkey, nbig + 1);
       SK* tk2 = buf<SK>(b->hkey2, nbig + 1);
-      uint64_t* tv = buf<uint64_t>(b->hval, nbig + 1);
-      uint64_t* tv2 = buf<uint64_t>(b->hval2, nbig + 1);
+      uint8_t* tv = buf<uint8_t>(b->hval, nbig + 1);
+      uint8_t* tv2 = buf<uint8_t>(b->hval2, nbig + 1);
       hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, kx,
                          vx, ky, rcodes, tk, tv, 0, bk0, bk1);
       GX_HIP(hipGetLastError());
       // all key bytes: several oversized buckets share the array, and the
       // stable sort keeps each bucket's records in occurrence order
-      bool f = radix_sort<SK, uint64_t>(tk, tv, tk2, tv2, nbig, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nbig)),
+      bool f = radix_sort<SK, uint8_t>(tk, tv, tk2, tv2, nbig, 0, nb, scratch_for(b, radix_scratch_bytes<SK>(nbig)),
                                         st);
       if (f) { std::swap(tk, tk2); std::swap(tv, tv2); }
       hipLaunchKernelGGL(k_big_move<SK>, dim3(std::min(bk1 - bk0, 65536u)), dim3(kBlock), 0, st, bstart, big, boff, kx,
@@ -3776,9 +3781,8 @@ static void run_build(gx_build* b) {
     // 5'. nodes = runs of equal keys
     ph_begin(b, PH_NODES);
     // run starts per chunk of kBlock records, and each chunk's first node
-    const uint64_t nch = (Nr + kBlock - 1) / kBlock;
     uint32_t* ccnt = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
-    uint64_t* coff = vx;
+    uint64_t* coff = reinterpret_cast<uint64_t*>(vx);
     if (Nr) {
       hipLaunchKernelGGL(k_run_count<SK>, dim3(grid_for(nch, 1, 1u << 16)), dim3(kBlock), 0, st, Nr, ky, ccnt);
       GX_HIP(hipGetLastError());
