@@ -916,7 +916,7 @@ __global__ void __launch_bounds__(kBlock) k_nix_build(const uint64_t* __restrict
   }
 }
 
-constexpr int kBucketCap = 8192;  // records sorted in LDS per bucket (64 KB / 128 KB of LDS)
+constexpr int kBucketCap = 8192;  // records sorted per bucket in one block (72 KB / 144 KB of LDS)
 // HBM of a key-range pass, per occurrence (sizes the ranges): the records
 // (two key and two neighbour-code arrays, the sorted codes) and, per node (dfrac
 // nodes per occurrence), every node array of the pass -- key words, count,
@@ -1225,14 +1225,47 @@ __device__ inline bool elem_gt(const Key128& a, const Key128& b) { return a.hi >
 __device__ inline void elem_max(uint64_t& e) { e = ~0ull; }
 __device__ inline void elem_max(Key128& e) { e.lo = e.hi = ~0ull; }
 
-// One block per bucket: bitonic sort of (key below the bucket, index) in
-// LDS, then the records are written in that order: the key and its
-// neighbour code (one byte per record: the replay's code array).  Four
-// tiers: one wave per bucket (CAP = kSmallBucket) over every bucket of the
-// key range; it lists the buckets above that for the CAP = kBucketCap / 8
-// (256 threads), kBucketCap / 2 (512) and kBucketCap (1024) blocks, which
-// run over their lists only (buckets marked 4, 3, 2).  Oversized buckets
-// (above kBucketCap) are marked 1 and left to the global pass.
+// Bitonic exchange partner in another lane of the wave (lane ^ m).
+__device__ inline uint64_t lane_xor(uint64_t x, int m) {
+  const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m);
+  const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline Key128 lane_xor(const Key128& x, int m) {
+  Key128 r;
+  r.lo = lane_xor(x.lo, m);
+  r.hi = lane_xor(x.hi, m);
+  return r;
+}
+// Stages whose partner is in the same thread (stride S < elements per
+// thread): compile-time indices keep the elements in registers.
+template <int S, int EPT, class E>
+__device__ inline void bitonic_in_thread(E* x, uint32_t i0, uint32_t size) {
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    if (e & S) continue;
+    const bool up = ((i0 + e) & size) == 0;
+    if (elem_gt(x[e], x[e | S]) == up) {
+      const E y = x[e];
+      x[e] = x[e | S];
+      x[e | S] = y;
+    }
+  }
+}
+
+// One block per bucket: bitonic sort of (key below the bucket, index),
+// then the records are written in that order: the key and its neighbour
+// code (one byte per record: the replay's code array).  Each thread holds
+// EPT = CAP / NT consecutive elements in registers: stages of stride < EPT
+// run inside the thread, strides < 64 * EPT across the wave's lanes
+// (shuffles, no barrier), only the larger ones through LDS.  Element i
+// lives at LDS slot i + i / EPT, so the threads' blocks of EPT fall on
+// different banks.  Four tiers: one wave per bucket (CAP =
+// kSmallBucket) over every bucket of the key range; it lists the buckets
+// above that for the CAP = kBucketCap / 8 (256 threads), kBucketCap / 2
+// (512) and kBucketCap (1024) blocks, which run over their lists only
+// (buckets marked 4, 3, 2).  Oversized buckets (above kBucketCap) are
+// marked 1 and left to the global pass.
 constexpr int kSmallBucket = 256;
 template <int W, int CAP, int NT>
 __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
@@ -1262,7 +1295,10 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
     }
     if (threadIdx.x == 0) big[bk] = 1;
   }
-  if (n > kBucketCap) return;  // the global pass sorts and writes it
+  if (n > (uint32_t)CAP) {  // the global pass sorts and writes it; in a listed tier, a bad list
+    if (!first && threadIdx.x == 0) atomicMin(err, (unsigned long long)bk);
+    return;
+  }
   if (n == 1 || nb < 2) {
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
       ok[a + i] = ik[a + i];
@@ -1270,31 +1306,68 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
     }
     return;
   }
-  uint32_t P = 2;
+  constexpr uint32_t EPT = CAP / NT, WSPAN = 64 * EPT;  // elements per thread, per wave
+  static_assert(EPT == 4 || EPT == 8, "four or eight elements per thread");
+  uint32_t P = EPT;  // network width: a power of two >= n, at least one thread's share
   while (P < n) P <<= 1;
   for (uint32_t i = threadIdx.x; i < P; i += NT) {
     E x;
     if (i < n) x = bucket_elem(ik[a + i], nb, bkb, i);
     else elem_max(x);
-    s[i] = x;
+    s[i + i / EPT] = x;
   }
   __syncthreads();
+  const uint32_t t = threadIdx.x, i0 = t * EPT, slot0 = i0 + t;  // slot of element i0 + e: slot0 + e
+  const bool act = i0 < P;  // threads past the network's width only take part in the barriers
+  E x[EPT];
+  if (act) {
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; e++) x[e] = s[slot0 + e];
+  }
   for (uint32_t size = 2; size <= P; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t t = threadIdx.x; t < P / 2; t += NT) {
-        uint32_t i = (t / stride) * stride * 2 + (t % stride), j = i + stride;
-        bool up = (i & size) == 0;
-        E x = s[i], y = s[j];
-        if (elem_gt(x, y) == up) {
-          s[i] = y;
-          s[j] = x;
+      if (stride >= WSPAN) {  // partner in another wave
+        __syncthreads();
+        if (act) {
+#pragma unroll
+          for (uint32_t e = 0; e < EPT; e++) s[slot0 + e] = x[e];
         }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+          for (uint32_t e = 0; e < EPT; e++) {
+            const uint32_t i = i0 + e, j = i ^ stride;
+            const E y = s[j + j / EPT];
+            const bool keep_min = ((i & stride) == 0) == ((i & size) == 0);
+            if (elem_gt(x[e], y) == keep_min) x[e] = y;
+          }
+        }
+      } else if (stride >= EPT) {  // partner in another lane (active: j < P)
+        if (act) {
+          const int m = (int)(stride / EPT);
+#pragma unroll
+          for (uint32_t e = 0; e < EPT; e++) {
+            const uint32_t i = i0 + e;
+            const E y = lane_xor(x[e], m);
+            const bool keep_min = ((i & stride) == 0) == ((i & size) == 0);
+            if (elem_gt(x[e], y) == keep_min) x[e] = y;
+          }
+        }
+      } else if (act) {
+        if (stride == 1) bitonic_in_thread<1, EPT>(x, i0, size);
+        else if (stride == 2) bitonic_in_thread<2, EPT>(x, i0, size);
+        else if constexpr (EPT > 4) bitonic_in_thread<4, EPT>(x, i0, size);
       }
-      __syncthreads();
     }
   }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (uint32_t e = 0; e < EPT; e++) s[slot0 + e] = x[e];
+  }
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i < n; i += NT) {
-    uint32_t src = elem_idx(s[i]);
+    const uint32_t src = elem_idx(s[i + i / EPT]);
     ok[a + i] = ik[a + src];
     oc[a + i] = ic[a + src];
   }
@@ -1312,7 +1385,7 @@ __global__ void __launch_bounds__(NT) k_bucket_sort(const typename SortKeyOf<W>:
                                                         uint32_t* __restrict__ big, uint32_t* __restrict__ lists,
                                                         uint32_t* __restrict__ nlist, uint32_t NBK,
                                                         unsigned long long* err) {
-  __shared__ typename SortKeyOf<W>::T s[CAP];
+  __shared__ typename SortKeyOf<W>::T s[CAP + NT];  // CAP elements, one padding slot per thread
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t bk = list ? list[q] : bucket0 + q;
     sort_one_bucket<W, CAP, NT>(bk, s, ik, ic, ok, oc, bstart, N, nb, bkb, big, lists, nlist, NBK, err);
@@ -3753,7 +3826,7 @@ static void run_build(gx_build* b) {
                          nullptr, NBK, errw + 2);
       GX_HIP(hipGetLastError());
     }
-    if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds");
+    if (read1(st, errw + 2) != ~0ull) throw GxError(GX_EDEVICE, "internal error: unordered bucket bounds or lists");
     hipLaunchKernelGGL(k_big_sizes, dim3(grid_for(NBK, kBlock, G)), dim3(kBlock), 0, st, NBK, bstart, big, bsz);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint64_t, uint64_t>(bsz, boff, NBK, scratch_for(b, scan_scratch_bytes<uint64_t>(NBK)), st);
