@@ -2876,7 +2876,29 @@ struct USerCtx {
   const uint64_t* wbase;
   const uint64_t* words;
   const int64_t* rid;
+  const uint8_t* ends;  // per node: letter k-1 | complement of letter 0 << 2
 };
+
+// The outer letters of every node, one byte per node (k_node_ends): a
+// unitig sequence takes one letter from each node after its first, and a
+// gather of these bytes touches far fewer lines than one of the keys.
+template <int W>
+__global__ void k_node_ends(uint64_t nn, const uint64_t* __restrict__ klo, const uint64_t* __restrict__ khi, int k,
+                            uint8_t* __restrict__ ends) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * blockDim.x) {
+    const int q = k - 1;
+    const uint32_t last = (uint32_t)((q < 32 ? klo[i] >> (2 * q) : (W == 2 ? khi[i] >> (2 * (q - 32)) : 0)) & 3);
+    const uint32_t first = (uint32_t)(klo[i] & 3);
+    ends[i] = (uint8_t)(last | ((3 - first) << 2));
+  }
+}
+
+// Letter k-1 of node o & 0x7fffffff read in the orientation o's top bit
+// gives (set: reverse complement), from the node's outer-letter byte.
+__device__ inline uint32_t next_letter(const USerCtx& c, uint32_t o) {
+  const uint32_t e = c.ends[o & 0x7fffffffu];
+  return (o & 0x80000000u) ? e >> 2 : e & 3u;
+}
 
 __device__ inline uint64_t useq_len(const USerCtx& c, uint64_t u) { return c.ulen[u] + c.k - 1; }
 
@@ -2971,8 +2993,7 @@ __device__ inline uint8_t useq_byte(const USerCtx& c, const uint32_t* ord, uint6
     if (j < (uint64_t)c.k) {
       x = oriented_letter<W>(c.klo, c.khi, o0 & 0x7fffffffu, !(o0 & 0x80000000u), c.k, (int)j);
     } else {
-      uint32_t o = ord[j - c.k + 1];
-      x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
+      x = next_letter(c, ord[j - c.k + 1]);
     }
     byte |= x << (2 * s2);
   }
@@ -3058,8 +3079,7 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
               x = (uint32_t)((qq < 32 ? v0.lo >> (2 * qq) : v0.hi >> (2 * (qq - 32))) & 3);
               if (!f0) x = 3 - x;
             } else {
-              const uint32_t o = ord[j - c.k + 1];
-              x = oriented_letter<W>(c.klo, c.khi, o & 0x7fffffffu, !(o & 0x80000000u), c.k, c.k - 1);
+              x = next_letter(c, ord[j - c.k + 1]);
             }
             byte |= x << (2 * s2);
           }
@@ -3204,12 +3224,13 @@ struct gx_build {
   bool unitigs_done = false;
   uint64_t out_bytes = 0, nnodes = 0;
   DevBuf ppid, pidx, ppid2, pidx2, pbound;
-  DevBuf sva, svb;  // sorted aggregation: record values
+  DevBuf sva, svb;  // sorted aggregation: record neighbour codes
   DevBuf nix_dir, nix_slots;  // key -> node index
   DevBuf rcode;               // replay codes, one byte per record in node order
   DevBuf u_seqat;             // unitig pass: byte position of long sequences
   DevBuf u_hbytes;            // unitig pass: head bytes of head-rich unitigs
   DevBuf u_headat;            // unitig pass: byte position of the heads of head-rich unitigs
+  DevBuf u_ends;              // unitig pass: each node's outer letters, one byte per node
   DevBuf s_acc, s_k1, s_k2, s_v1, s_v2, s_flag, s_pos, s_bval, s_bst, s_out;  // statistics, FASTA
   int st_graph = -1;                                                          // graph of the last statistics
   std::vector<uint64_t> st_tot;
@@ -3457,7 +3478,8 @@ static void for_each_buf(gx_build* b, F&& f) {
         &b->bb_typ, &b->bb_np, &b->bb_pbase, &b->bb_pairs, &b->bb_ed, &b->bb_into, &b->bb_hn, &b->bb_hb,
         &b->bb_addu, &b->bb_addf, &b->bb_cur, &b->bb_tot, &b->bb_astart, &b->bb_hnode, &b->bb_hmv, &b->bb_hofs,
         &b->bb_aval, &b->bb_aoff, &b->u_headat, &b->s_acc, &b->s_k1, &b->s_k2,
-        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_walk, &b->u_walk2})
+        &b->s_v1, &b->s_v2, &b->s_flag, &b->s_pos, &b->s_bval, &b->s_bst, &b->s_out, &b->u_walk, &b->u_walk2,
+        &b->u_ends})
     f(d);
 }
 
@@ -4512,6 +4534,12 @@ static void run_unitigs(gx_build* b) {
   c.urep_node = urep_node; c.ucov = ucov; c.uedge = uedge; c.uecnt = uecnt; c.hstart = hstart; c.hu = hu; c.hf = hf;
   c.hkey = hkey_lo; c.hval = hval_c; c.mlen = b->mlen.as<uint32_t>(); c.wbase = b->wbase.as<uint64_t>();
   c.words = b->words.as<uint64_t>(); c.rid = b->rid.as<int64_t>();
+  uint8_t* ends = buf<uint8_t>(b->u_ends, nn + 1);
+  if (nn) {
+    hipLaunchKernelGGL(k_node_ends<W>, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, klo, khi, k, ends);
+    GX_HIP(hipGetLastError());
+  }
+  c.ends = ends;
   uint64_t* hbytes = buf<uint64_t>(b->u_hbytes, 2 * nu + 2);
   c.hbytes = hbytes;
   uint64_t* rsize = buf<uint64_t>(b->u_rsize, nu + 1);

@@ -187,27 +187,46 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const Tin* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// The tile passes through LDS twice: loaded and stored with consecutive
+// threads on consecutive elements (whole lines per wave instruction), and
+// scanned with each thread on kScanItems consecutive elements.  Tile
+// position q sits at LDS slot q + q / kScanItems, so the threads' runs of
+// kScanItems fall on different banks.
 template <class Tin, class T>
 __global__ void __launch_bounds__(kBlock) k_scan_apply(const Tin* __restrict__ in, size_t n, const T* __restrict__ part,
                                                        T* __restrict__ out) {
-  size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  __shared__ T tile[kScanTile + kBlock];
+  const size_t base = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    const uint32_t q = (uint32_t)(i * kBlock + threadIdx.x);
+    const size_t j = base + q;
+    tile[q + q / kScanItems] = j < n ? (T)in[j] : (T)0;
+  }
+  __syncthreads();
+  const uint32_t run = threadIdx.x * (kScanItems + 1);  // slot of this thread's first element
   T v[kScanItems];
   T s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    size_t j = base + i;
-    v[i] = j < n ? (T)in[j] : (T)0;
+    v[i] = tile[run + i];
     s += v[i];
   }
   T tot;
   T ex = block_exclusive_scan<T>(s, &tot) + (part ? part[blockIdx.x] : (T)0);
 #pragma unroll
-  for (int i = 0; i < kScanItems; i++) {
-    size_t j = base + i;
-    if (j < n) out[j] = ex;
+  for (int i = 0; i < kScanItems; i++) {  // the thread's own slots: no other thread reads them
+    tile[run + i] = ex;
     ex += v[i];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    const uint32_t q = (uint32_t)(i * kBlock + threadIdx.x);
+    const size_t j = base + q;
+    if (j < n) out[j] = tile[q + q / kScanItems];
+  }
 }
 
 // Scratch needed by exclusive_scan for n elements (bytes), all levels.
