@@ -1422,24 +1422,33 @@ __global__ void k_big_sizes(uint32_t NB, const uint64_t* __restrict__ bstart, co
 }
 
 // Nodes are the runs of equal keys in the sorted records.  Run starts per
-// chunk of kBlock records (a record starts a run when its key differs from
-// the record before); their exclusive scan gives each chunk's first node,
-// and k_node_emit numbers the chunk's records from it.
+// chunk of kEmitChunk records (a record starts a run when its key differs
+// from the record before); their exclusive scan gives each chunk's first
+// node, and k_node_emit numbers the chunk's records from it.  A block takes
+// a chunk in kEmitRounds rounds of kBlock consecutive records, with every
+// round's loads issued before the first barrier.
+constexpr int kEmitRounds = 4;
+constexpr int kEmitChunk = kBlock * kEmitRounds;
+
 template <class SK>
 __global__ void __launch_bounds__(kBlock) k_run_count(uint64_t N, const SK* __restrict__ keys,
                                                       uint32_t* __restrict__ cnt) {
   __shared__ uint32_t ws[kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (uint64_t c = blockIdx.x; c * kBlock < N; c += gridDim.x) {
-    const uint64_t i = c * kBlock + threadIdx.x;
-    const bool f = i < N && (i == 0 || !sk_eq(keys[i], keys[i - 1]));
-    const uint64_t b = __ballot(f);
-    if (lane == 0) ws[wid] = (uint32_t)__popcll(b);
+  for (uint64_t c = blockIdx.x; c * kEmitChunk < N; c += gridDim.x) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int r = 0; r < kEmitRounds; r++) {
+      const uint64_t i = c * kEmitChunk + (uint64_t)r * kBlock + threadIdx.x;
+      const bool f = i < N && (i == 0 || !sk_eq(keys[i], keys[i - 1]));
+      t += (uint32_t)__popcll(__ballot(f));
+    }
+    if (lane == 0) ws[wid] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (int w = 0; w < kBlock / 64; w++) t += ws[w];
-      cnt[c] = t;
+      uint32_t a = 0;
+      for (int w = 0; w < kBlock / 64; w++) a += ws[w];
+      cnt[c] = a;
     }
     __syncthreads();
   }
@@ -1447,51 +1456,73 @@ __global__ void __launch_bounds__(kBlock) k_run_count(uint64_t N, const SK* __re
 
 // Run starts -> nodes (first record, key), and each node's neighbour mask:
 // the OR of its records' code bits, gathered per block in LDS (a chunk's
-// 256 records touch at most 256 consecutive nodes); the first and last node
-// of a chunk may continue in a neighbouring chunk and are merged with a
-// global atomicOr, the others stored directly.  Node of record i = the
-// chunk's first node (coff, the scan of k_run_count) + the run starts in
-// the chunk up to i - 1.  msk must start at zero.
+// kEmitChunk records touch at most kEmitChunk consecutive nodes); the
+// first and last node of a chunk may continue in a neighbouring chunk and
+// are merged with a global atomicOr, the others stored directly.  Node of
+// record i = the chunk's first node (coff, the scan of k_run_count) + the
+// run starts in the chunk up to i - 1.  msk must start at zero.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_node_emit(uint64_t N, const typename SortKeyOf<W>::T* __restrict__ keys,
                                                       const uint8_t* __restrict__ rcode,
                                                       const uint64_t* __restrict__ coff, int nb,
                                                       uint64_t* __restrict__ klo, uint64_t* __restrict__ khi,
                                                       uint64_t* __restrict__ nstart, uint32_t* __restrict__ msk) {
-  __shared__ uint32_t lm[kBlock];
-  __shared__ uint32_t ws[kBlock / 64];
+  using SK = typename SortKeyOf<W>::T;
+  __shared__ uint32_t lm[kEmitChunk];
+  __shared__ uint32_t ws[kEmitRounds][kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes up to and including this one
-  for (uint64_t c = blockIdx.x; c * kBlock < N; c += gridDim.x) {
-    const uint64_t base = c * kBlock, i = base + threadIdx.x;
-    const bool in = i < N;
-    const bool f = in && (i == 0 || !sk_eq(keys[i], keys[i - 1]));
-    const uint64_t b = __ballot(f);
-    lm[threadIdx.x] = 0;
-    if (lane == 0) ws[wid] = (uint32_t)__popcll(b);
-    __syncthreads();
-    uint32_t incl = (uint32_t)__popcll(b & le);
-    for (int w = 0; w < wid; w++) incl += ws[w];
-    const uint64_t c0 = coff[c];
+  for (uint64_t c = blockIdx.x; c * kEmitChunk < N; c += gridDim.x) {
+    const uint64_t base = c * kEmitChunk;
+    SK key[kEmitRounds];
+    uint32_t code[kEmitRounds];
+    uint64_t bal[kEmitRounds];
+#pragma unroll
+    for (int r = 0; r < kEmitRounds; r++) {
+      const uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
+      bool f = false;
+      code[r] = 0;
+      if (i < N) {
+        key[r] = keys[i];
+        code[r] = rcode[i];
+        f = i == 0 || !sk_eq(key[r], keys[i - 1]);
+      }
+      bal[r] = __ballot(f);
+      if (lane == 0) ws[r][wid] = (uint32_t)__popcll(bal[r]);
+    }
+    for (int q = threadIdx.x; q < kEmitChunk; q += kBlock) lm[q] = 0;
+    const uint64_t c0 = coff[c], c1 = coff[c + 1];
     // the chunk's first node: its first record's run, which starts here or earlier
     const uint64_t nfirst = (base == 0 || !sk_eq(keys[base], keys[base - 1])) ? c0 : c0 - 1;
-    const uint64_t nlast = coff[c + 1] - 1;
-    if (in) {
-      const uint64_t n = c0 + incl - 1;
-      if (f) {
-        nstart[n] = i;
-        KV v = key_of_sort(keys[i], nb);
-        klo[n] = v.lo;
-        if (W == 2) khi[n] = v.hi;
+    __syncthreads();
+    uint32_t before = 0;  // run starts of the earlier rounds
+#pragma unroll
+    for (int r = 0; r < kEmitRounds; r++) {
+      const uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
+      uint32_t incl = before + (uint32_t)__popcll(bal[r] & le);
+      for (int w = 0; w < kBlock / 64; w++) {
+        if (w < wid) incl += ws[r][w];
+        before += ws[r][w];
       }
-      atomicOr(&lm[n - nfirst], code_bits(rcode[i]));
+      if (i < N) {
+        const uint64_t n = c0 + incl - 1;
+        if ((bal[r] >> lane) & 1) {
+          nstart[n] = i;
+          const KV v = key_of_sort(key[r], nb);
+          klo[n] = v.lo;
+          if (W == 2) khi[n] = v.hi;
+        }
+        atomicOr(&lm[n - nfirst], code_bits(code[r]));
+      }
     }
     __syncthreads();
-    const uint64_t n = nfirst + threadIdx.x;
-    if (n <= nlast) {
-      uint32_t bits = lm[threadIdx.x];
-      if (n == nfirst || n == nlast) atomicOr(&msk[n], bits);
-      else msk[n] = bits;
+    for (int q = threadIdx.x; q < kEmitChunk; q += kBlock) {
+      const uint64_t n = nfirst + q;
+      if (n < c1) {
+        const uint32_t bits = lm[q];
+        if (n == nfirst || n == c1 - 1) atomicOr(&msk[n], bits);
+        else msk[n] = bits;
+      }
     }
     __syncthreads();
   }
@@ -3761,7 +3792,7 @@ static void run_build(gx_build* b) {
     SK* kb = buf<SK>(b->ckey2, Nr + 2);
     // neighbour codes beside the keys; after the sorts the node pass keeps
     // its chunk offsets (nch + 1 words) in the free one of the pair
-    const uint64_t nch = (Nr + kBlock - 1) / kBlock;
+    const uint64_t nch = (Nr + kEmitChunk - 1) / kEmitChunk;
     const size_t vbytes = std::max<size_t>(Nr + 2, 8 * (nch + 2));
     uint8_t* va = buf<uint8_t>(b->sva, vbytes);
     uint8_t* vb = buf<uint8_t>(b->svb, vbytes);
@@ -3875,7 +3906,7 @@ static void run_build(gx_build* b) {
 
     // 5'. nodes = runs of equal keys
     ph_begin(b, PH_NODES);
-    // run starts per chunk of kBlock records, and each chunk's first node
+    // run starts per chunk of kEmitChunk records, and each chunk's first node
     uint32_t* ccnt = reinterpret_cast<uint32_t*>(kx);  // the other buffers are free now
     uint64_t* coff = reinterpret_cast<uint64_t*>(vx);
     if (Nr) {
