@@ -945,7 +945,7 @@ constexpr int kIdxBits = 13;      // log2(kBucketCap)
 // The bucket bytes are sorted least significant first (LSD): the fused
 // extraction pass scatters by byte nb-bkb, radix passes after it by bytes
 // nb-bkb+1 .. nb-1; a one-byte key has a single pass on byte 0.
-constexpr int kTileMateCap = 1024;
+constexpr int kTileMateCap = 512;
 
 // Mate holding occurrence j: the largest m in [lo, hi] with occbase[m] <= j.
 __device__ inline uint64_t mate_search(const uint64_t* __restrict__ ob, uint64_t lo, uint64_t hi, uint64_t j) {
@@ -958,7 +958,7 @@ __device__ inline uint64_t mate_search(const uint64_t* __restrict__ ob, uint64_t
 }
 
 // The mates of a tile's occurrences [j0, j1), in LDS: the tile's first and
-// last mate (mr), and, when the mates fit (always, for reads shorter than
+// last mate (mr), and, when the mates fit (always, for mates of at least
 // kSortTile / kTileMateCap occurrences on average), their first occurrences
 // relative to the first mate's (lob) and a table giving each occurrence's
 // mate (mtab): every non-empty mate marks the slot of its first occurrence
@@ -1098,16 +1098,17 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
                                                         uint32_t* __restrict__ nhist) {
   using SK = typename SortKeyOf<W>::T;
   __shared__ uint64_t run[256];
-  __shared__ uint32_t lbase[256], lrun[256], wsum[kBlock / 64];
+  __shared__ uint16_t lbase[256], lrun[256];  // (LDS kept under 20 KB: 8 blocks per CU)
+  __shared__ uint32_t wsum[kBlock / 64];
   __shared__ uint16_t wcnt[kBlock / 64][256];
   __shared__ uint64_t mr[3];
   __shared__ uint32_t lob[kTileMateCap];
   __shared__ uint16_t mtab[kSortTile];
-  __shared__ uint32_t h2[256];  // the next key range's digit counts (as k_occ_hist would give)
+  __shared__ uint32_t h2[128];  // the next key range's digit counts (as k_occ_hist would give), two u16 per word
   __shared__ uint32_t sv[kSortTile];  // (occurrence - j0) << 8 | code
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   run[threadIdx.x] = offs[(size_t)threadIdx.x * ntiles + blockIdx.x];
-  h2[threadIdx.x] = 0;
+  if (threadIdx.x < 128) h2[threadIdx.x] = 0;
   tile_digit_bases(offs, ntiles, lbase, lrun, wsum);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t j0 = (uint64_t)blockIdx.x * kSortTile, j1 = j0 + kSortTile < N ? j0 + kSortTile : N;
@@ -1122,7 +1123,10 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
       make_sort_key(key, nb, skey);
       d = key_byte(skey, dbyte);
       const uint32_t t = key_byte(skey, nb - 1);
-      if (nhist && t >= nlo && t < nhi) atomicAdd(&h2[key_byte(skey, ndbyte)], 1u);
+      if (nhist && t >= nlo && t < nhi) {
+        const uint32_t nd = key_byte(skey, ndbyte);
+        atomicAdd(&h2[nd >> 1], 1u << (16 * (nd & 1)));  // at most kSortTile per digit
+      }
       valid = t >= tlo && t < thi;  // key-range pass: other records wait for their range
     }
     uint64_t act = __ballot(valid);
@@ -1158,7 +1162,8 @@ __global__ void __launch_bounds__(kBlock) k_occ_scatter(uint64_t N, uint64_t nma
     okeys[pos] = skey;
     ocodes[pos] = (uint8_t)v;
   }
-  if (nhist) nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h2[threadIdx.x];
+  if (nhist)
+    nhist[(size_t)threadIdx.x * ntiles + blockIdx.x] = (h2[threadIdx.x >> 1] >> (16 * (threadIdx.x & 1))) & 0xffffu;
 }
 
 // Occurrences per value of the sort key's top byte (the key ranges of a

@@ -284,10 +284,11 @@ constexpr int kSortTile = kBlock * kSortItems;
 // Tile-local digit bases for a staged scatter: offs is the exclusive scan of
 // the digit-major per-tile counts (entry d * ntiles + tile, total last), so
 // this tile's count of digit d is the difference of two neighbours; lbase[d]
-// = elements of the tile with a smaller digit.  Block of kBlock = 256
-// threads, one digit each.
-__device__ inline void tile_digit_bases(const uint64_t* __restrict__ offs, unsigned ntiles, uint32_t* lbase,
-                                        uint32_t* lrun, uint32_t* wsum) {
+// = elements of the tile with a smaller digit (IdxT: any type holding
+// kSortTile).  Block of kBlock = 256 threads, one digit each.
+template <class IdxT>
+__device__ inline void tile_digit_bases(const uint64_t* __restrict__ offs, unsigned ntiles, IdxT* lbase, IdxT* lrun,
+                                        uint32_t* wsum) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const size_t at = (size_t)threadIdx.x * ntiles + blockIdx.x;
   const uint32_t c = (uint32_t)(offs[at + 1] - offs[at]);
@@ -301,8 +302,8 @@ __device__ inline void tile_digit_bases(const uint64_t* __restrict__ offs, unsig
   __syncthreads();
   uint32_t before = incl - c;
   for (int w = 0; w < wid; w++) before += wsum[w];
-  lbase[threadIdx.x] = before;
-  lrun[threadIdx.x] = before;
+  lbase[threadIdx.x] = (IdxT)before;
+  lrun[threadIdx.x] = (IdxT)before;
 }
 
 template <class KeyT>
