@@ -3228,6 +3228,11 @@ struct gx_build {
   gx_build_cfg cfg{};
   int W = 1, nb = 0;
   hipStream_t st = nullptr;
+  // a second stream for a low-occupancy kernel that overlaps the main
+  // stream's work (the replay of the longest record runs): forked and
+  // joined by events, created on first use
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   uint64_t text_len = 0;
   int fmt = -1;                     // input format fixed by the first add call (GX_FMT_*)
@@ -3367,6 +3372,22 @@ static char* scratch_for(gx_build* b, size_t bytes) { return b->scratch.ensure(b
 template <class T>
 static T* buf(DevBuf& d, size_t n) {
   return static_cast<T*>(d.ensure(n * sizeof(T)));
+}
+
+// Work after this call on st2 starts after everything enqueued on st so far.
+static void fork_side(gx_build* b) {
+  if (!b->st2) {
+    GX_HIP(hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking));
+    GX_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    GX_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  }
+  GX_HIP(hipEventRecord(b->ev_fork, b->st));
+  GX_HIP(hipStreamWaitEvent(b->st2, b->ev_fork, 0));
+}
+// Work after this call on st starts after everything enqueued on st2 so far.
+static void join_side(gx_build* b) {
+  GX_HIP(hipEventRecord(b->ev_join, b->st2));
+  GX_HIP(hipStreamWaitEvent(b->st, b->ev_join, 0));
 }
 
 template <class T>
@@ -4036,6 +4057,7 @@ static void run_build(gx_build* b) {
   ph_end(b, PH_NIX);
 
   // 6. replay
+  bool side_open = false;  // k_replay_long runs on the side stream, not joined yet
   ph_begin(b, PH_REPLAY);
   const int jdk7 = b->cfg.edge_order == GX_EDGE_ORDER_JDK7;
   uint32_t* needs = buf<uint32_t>(b->needs, nn + 1);
@@ -4102,8 +4124,13 @@ static void run_build(gx_build* b) {
     const uint64_t nlong = read1(st, errw + 2);
     ph_begin(b, PH_KREPLAY);
     if (nlong) {
-      hipLaunchKernelGGL(k_replay_long<W>, dim3(grid_for(4 * nlong, kBlock, 1u << 16)), dim3(kBlock), 0, st, nlong,
-                         rlist, rseg, rbytes, klo, khi, msk, k, nb, lists);
+      // a few hundred waves, each as long as its longest run: on the side
+      // stream, overlapping the short runs, the heads and the record sizes;
+      // joined before the records are written
+      fork_side(b);
+      side_open = true;
+      hipLaunchKernelGGL(k_replay_long<W>, dim3(grid_for(4 * nlong, kBlock, 1u << 16)), dim3(kBlock), 0, b->st2,
+                         nlong, rlist, rseg, rbytes, klo, khi, msk, k, nb, lists);
       GX_HIP(hipGetLastError());
     }
     if (nrep > nlong) {
@@ -4114,9 +4141,13 @@ static void run_build(gx_build* b) {
     ph_end(b, PH_KREPLAY);
   }
   ph_end(b, PH_REPLAY);
-  // replay inputs and order are dead (lists, needs and the node arrays stay)
-  b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->nslot, &b->rcode, &b->selcnt, &b->seg, &b->rpos, &b->rkey,
-                                                 &b->rval, &b->rkey2, &b->rval2});
+  // replay inputs and order are dead (lists, needs and the node arrays
+  // stay) once the side stream has read them
+  auto replay_inputs_dead = [&] {
+    b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->nslot, &b->rcode, &b->selcnt, &b->seg, &b->rpos, &b->rkey,
+                                                   &b->rval, &b->rkey2, &b->rval2});
+  };
+  if (!side_open) replay_inputs_dead();
 
   // 7. heads
   ph_begin(b, PH_HEADS);
@@ -4232,6 +4263,11 @@ static void run_build(gx_build* b) {
   } else {
     b->part_rec[1] = nn;
     b->part_off[1] = total;
+  }
+  if (side_open) {  // the records need every replayed list
+    join_side(b);
+    side_open = false;
+    replay_inputs_dead();
   }
   uint8_t* out = buf<uint8_t>(b->out, total + 1);
   if (nn) {
@@ -4613,6 +4649,15 @@ static void run_unitigs(gx_build* b) {
 
 template <class F>
 static int guarded(gx_build* b, F&& f) {
+  // a call that fails between fork_side and join_side leaves work on the
+  // side stream: it ends before the call returns, so no later call reuses
+  // a buffer that it still reads
+  struct SideDrain {
+    gx_build* b;
+    ~SideDrain() {
+      if (b && b->st2) (void)hipStreamSynchronize(b->st2);
+    }
+  } drain{b};
   try {
     f();
     return GX_OK;
@@ -6248,7 +6293,11 @@ void gx_build_destroy(gx_build* b) {
   if (!b) return;
   (void)hipSetDevice(b->cfg.device);
   if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->st2) (void)hipStreamSynchronize(b->st2);
   if (b->st) (void)hipStreamDestroy(b->st);
+  if (b->st2) (void)hipStreamDestroy(b->st2);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : b->evpj)
