@@ -3228,9 +3228,10 @@ struct gx_build {
   gx_build_cfg cfg{};
   int W = 1, nb = 0;
   hipStream_t st = nullptr;
-  // a second stream for a low-occupancy kernel that overlaps the main
-  // stream's work (the replay of the longest record runs): forked and
-  // joined by events, created on first use
+  // a second stream for work that overlaps the main stream's (the replay
+  // of the longest record runs beside the heads, the unitigs' outer edges
+  // beside their coverage and heads): ordered by events, created on first
+  // use
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
@@ -3317,17 +3318,18 @@ static const char* kPhaseNames[NPH] = {"parse", "insert", "sort_nodes", "replay"
                                        "k_replay", "unitig", "sizing", "node_index", "k_pj_round", "k_occ_scatter",
                                        "k_walk"};
 
-static void ph_begin(gx_build* b, int ph) {
+// s: the stream the phase runs on (default: the main stream)
+static void ph_begin(gx_build* b, int ph, hipStream_t s = nullptr) {
   if (!b->prof) return;
   if (!b->ev[2 * ph]) {
     GX_HIP(hipEventCreate(&b->ev[2 * ph]));
     GX_HIP(hipEventCreate(&b->ev[2 * ph + 1]));
   }
-  GX_HIP(hipEventRecord(b->ev[2 * ph], b->st));
+  GX_HIP(hipEventRecord(b->ev[2 * ph], s ? s : b->st));
   b->ev_used[ph] = true;
 }
-static void ph_end(gx_build* b, int ph) {
-  if (b->prof) GX_HIP(hipEventRecord(b->ev[2 * ph + 1], b->st));
+static void ph_end(gx_build* b, int ph, hipStream_t s = nullptr) {
+  if (b->prof) GX_HIP(hipEventRecord(b->ev[2 * ph + 1], s ? s : b->st));
 }
 // add: sum into the phase totals (a build of several key ranges)
 static void ph_collect(gx_build* b, bool add = false) {
@@ -3374,20 +3376,18 @@ static T* buf(DevBuf& d, size_t n) {
   return static_cast<T*>(d.ensure(n * sizeof(T)));
 }
 
-// Work after this call on st2 starts after everything enqueued on st so far.
-static void fork_side(gx_build* b) {
-  if (!b->st2) {
-    GX_HIP(hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking));
-    GX_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-    GX_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-  }
-  GX_HIP(hipEventRecord(b->ev_fork, b->st));
-  GX_HIP(hipStreamWaitEvent(b->st2, b->ev_fork, 0));
+static void side_init(gx_build* b) {
+  if (b->st2) return;
+  GX_HIP(hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking));
+  b->reclaim.st2 = b->st2;
+  for (hipEvent_t* e : {&b->ev_fork, &b->ev_join})
+    GX_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
 }
-// Work after this call on st starts after everything enqueued on st2 so far.
-static void join_side(gx_build* b) {
-  GX_HIP(hipEventRecord(b->ev_join, b->st2));
-  GX_HIP(hipStreamWaitEvent(b->st, b->ev_join, 0));
+// Work enqueued on waiter after this call starts after everything enqueued
+// on producer so far.
+static void stream_after(hipStream_t waiter, hipStream_t producer, hipEvent_t ev) {
+  GX_HIP(hipEventRecord(ev, producer));
+  GX_HIP(hipStreamWaitEvent(waiter, ev, 0));
 }
 
 template <class T>
@@ -3477,9 +3477,8 @@ static void swap_buf(DevBuf& x, DevBuf& y) {
 // Key -> node index over nodes [0, nn) in key order whose keys' top byte
 // lies in [tlo, thi).
 template <int W>
-static NodeIndex build_node_index(gx_build* b, uint64_t nn, const uint64_t* klo, const uint64_t* khi,
+static NodeIndex build_node_index(gx_build* b, hipStream_t st, uint64_t nn, const uint64_t* klo, const uint64_t* khi,
                                   const uint32_t* msk, uint32_t tlo, uint32_t thi) {
-  hipStream_t st = b->st;
   const int nb = b->nb;
   const unsigned G = b->grid;
   const size_t slotsz = W == 1 ? sizeof(Slot1) : sizeof(Slot2);
@@ -4052,12 +4051,12 @@ static void run_build(gx_build* b) {
 
   // 5b. key -> node index (heads, unitig links)
   ph_begin(b, PH_NIX);
-  const NodeIndex nix = build_node_index<W>(b, nn, klo, khi, msk, tlo, thi);
+  const NodeIndex nix = build_node_index<W>(b, st, nn, klo, khi, msk, tlo, thi);
   if (sorted) cap = 2 * nn;
   ph_end(b, PH_NIX);
+  bool side_open = false;  // k_replay_long runs on the side stream, not joined yet
 
   // 6. replay
-  bool side_open = false;  // k_replay_long runs on the side stream, not joined yet
   ph_begin(b, PH_REPLAY);
   const int jdk7 = b->cfg.edge_order == GX_EDGE_ORDER_JDK7;
   uint32_t* needs = buf<uint32_t>(b->needs, nn + 1);
@@ -4127,7 +4126,8 @@ static void run_build(gx_build* b) {
       // a few hundred waves, each as long as its longest run: on the side
       // stream, overlapping the short runs, the heads and the record sizes;
       // joined before the records are written
-      fork_side(b);
+      side_init(b);
+      stream_after(b->st2, st, b->ev_fork);
       side_open = true;
       hipLaunchKernelGGL(k_replay_long<W>, dim3(grid_for(4 * nlong, kBlock, 1u << 16)), dim3(kBlock), 0, b->st2,
                          nlong, rlist, rseg, rbytes, klo, khi, msk, k, nb, lists);
@@ -4265,7 +4265,7 @@ static void run_build(gx_build* b) {
     b->part_off[1] = total;
   }
   if (side_open) {  // the records need every replayed list
-    join_side(b);
+    stream_after(st, b->st2, b->ev_join);
     side_open = false;
     replay_inputs_dead();
   }
@@ -4371,7 +4371,7 @@ static void run_build(gx_build* b) {
       }
       exclusive_scan<uint64_t, uint64_t>(htot, hstart, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
       ph_begin(b, PH_NIX);
-      build_node_index<W>(b, nn, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr,
+      build_node_index<W>(b, b->st, nn, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr,
                           b->msk.as<uint32_t>(), 0u, 256u);
       ph_end(b, PH_NIX);
       GX_HIP(hipStreamSynchronize(st));
@@ -4537,23 +4537,29 @@ static void run_unitigs(gx_build* b) {
   exclusive_scan<uint64_t, uint64_t>(ulen, ubase, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
   uint32_t* order = buf<uint32_t>(b->u_order, nn + 1);
   unsigned long long* ucov = buf<unsigned long long>(b->u_cov, nu + 1);
+  uint32_t* uedge = buf<uint32_t>(b->u_edge, kUEdges * (nu + 1));
+  uint8_t* uecnt = buf<uint8_t>(b->u_ecnt, nu + 1);
   if (nn) {
     hipLaunchKernelGGL(k_place, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, urep, upos, uid_of_rep, ubase,
                        order);
     GX_HIP(hipGetLastError());
+  }
+  // outer edges (latency-bound lookups) on the side stream, beside the
+  // coverage sums and the heads; joined before the records
+  side_init(b);
+  stream_after(b->st2, st, b->ev_fork);
+  if (nu) {
+    hipLaunchKernelGGL(k_unitig_edges<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, b->st2, nu, ubase, ulen,
+                       order, klo, khi, msk, nix, urep, k, uedge, uecnt);
+    GX_HIP(hipGetLastError());
+  }
+  if (nn) {
     uint32_t* ocov = buf<uint32_t>(b->u_isrep, nn + 1);  // isrep is read no more
     uint64_t* cpre = buf<uint64_t>(b->u_lenrep, nn + 1);  // nor ulen_of_rep
     hipLaunchKernelGGL(k_order_cov, dim3(grid_for(nn, kBlock, G)), dim3(kBlock), 0, st, nn, order, cnt, ocov);
     GX_HIP(hipGetLastError());
     exclusive_scan<uint32_t, uint64_t>(ocov, cpre, nn, scratch_for(b, scan_scratch_bytes<uint64_t>(nn)), st);
     hipLaunchKernelGGL(k_unitig_cov, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, cpre, ucov);
-    GX_HIP(hipGetLastError());
-  }
-  uint32_t* uedge = buf<uint32_t>(b->u_edge, kUEdges * (nu + 1));
-  uint8_t* uecnt = buf<uint8_t>(b->u_ecnt, nu + 1);
-  if (nu) {
-    hipLaunchKernelGGL(k_unitig_edges<W>, dim3(grid_for(nu, kBlock, G)), dim3(kBlock), 0, st, nu, ubase, ulen, order,
-                       klo, khi, msk, nix, urep, k, uedge, uecnt);
     GX_HIP(hipGetLastError());
   }
   // heads
@@ -4600,7 +4606,8 @@ static void run_unitigs(gx_build* b) {
     GX_HIP(hipGetLastError());
   }
   exclusive_scan<uint64_t, uint64_t>(htot, hstart, nu, scratch_for(b, scan_scratch_bytes<uint64_t>(nu)), st);
-  // records
+  // records (they list the outer edges)
+  stream_after(st, b->st2, b->ev_join);
   USerCtx c;
   c.k = k; c.nb = nb; c.nu = nu; c.klo = klo; c.khi = khi; c.ubase = ubase; c.ulen = ulen; c.order = order;
   c.urep_node = urep_node; c.ucov = ucov; c.uedge = uedge; c.uecnt = uecnt; c.hstart = hstart; c.hu = hu; c.hf = hf;
@@ -4649,9 +4656,9 @@ static void run_unitigs(gx_build* b) {
 
 template <class F>
 static int guarded(gx_build* b, F&& f) {
-  // a call that fails between fork_side and join_side leaves work on the
-  // side stream: it ends before the call returns, so no later call reuses
-  // a buffer that it still reads
+  // a call that fails between a fork to the side stream and its join
+  // leaves work on the side stream: it ends before the call returns, so no
+  // later call reuses a buffer that it still reads
   struct SideDrain {
     gx_build* b;
     ~SideDrain() {
@@ -4950,7 +4957,7 @@ static void compact_graph(gx_build* b, uint32_t* keep, const AddedHeads& ah) {
   b->has_hoff = offs;
   b->nnodes = nn2;
   b->nuheads = nh2;
-  build_node_index<W>(b, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, b->msk.as<uint32_t>(),
+  build_node_index<W>(b, b->st, nn2, b->klo.as<uint64_t>(), W == 2 ? b->khi.as<uint64_t>() : nullptr, b->msk.as<uint32_t>(),
                       0, 256);
   run_unitigs<W>(b);
 }
@@ -6296,8 +6303,8 @@ void gx_build_destroy(gx_build* b) {
   if (b->st2) (void)hipStreamSynchronize(b->st2);
   if (b->st) (void)hipStreamDestroy(b->st);
   if (b->st2) (void)hipStreamDestroy(b->st2);
-  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  for (hipEvent_t e : {b->ev_fork, b->ev_join})
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : b->evpj)

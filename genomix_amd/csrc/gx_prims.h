@@ -33,6 +33,7 @@ struct DevBuf;
 // otherwise fail; ensure() of a listed buffer takes it off the list.
 struct ReclaimList {
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  // a second stream of the same handle (drained too before a free)
   std::vector<DevBuf*> dead;
   uint64_t allocs = 0;  // device allocations made while the list was installed
   uint64_t freed = 0;   // listed buffers freed to make room
@@ -80,6 +81,7 @@ inline bool reclaim_dead(const DevBuf* self, size_t need) {
   const size_t pick = fit != none ? fit : big;
   if (pick == none) return false;
   (void)hipStreamSynchronize(r->st);
+  if (r->st2) (void)hipStreamSynchronize(r->st2);
   r->dead[pick]->release();
   r->dead[pick] = r->dead.back();
   r->dead.pop_back();
