@@ -490,10 +490,6 @@ __device__ inline KV load_kmer(const uint64_t* __restrict__ w, uint32_t p, int k
   return v;
 }
 
-__device__ inline uint32_t base_at(const uint64_t* __restrict__ w, uint32_t i) {
-  return (uint32_t)(w[i >> 5] >> (2 * (i & 31))) & 3;
-}
-
 // Edge types in EDGETYPE order (EDGETYPE.java:6-9).  The table's neighbour
 // mask has bit 4t+x set when list t holds the neighbour with letter x:
 // successor X[1:]+x for FF (stored as is) and FR (stored reverse-
@@ -517,11 +513,26 @@ __device__ inline uint32_t nibble(uint32_t t, uint32_t x) {
 // words w), its occurrence code and its neighbour bits (bit 4t + x).
 // The neighbours' reverse complements come from the k-mer's own:
 // rc(w[1:]+c) = comp(c)+rc(w)[:-1] and rc(c+w[:-1]) = rc(w)[1:]+comp(c).
-// CODES = false: the key alone.
+// The k-mer's words also hold its neighbour letters (the previous one in
+// the word before when the k-mer starts a word), so each word is loaded
+// once.  CODES = false: the key alone.
 template <int W, bool CODES = true>
 __device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32_t p, int k, uint32_t* code_out,
                               uint32_t* bits_out) {
-  KV v = load_kmer<W>(w, p, k);
+  // bits [sh, end) of words wi, wi + 1, wi + 2 hold the k-mer; the next
+  // letter is at bit end (read when p + k < L), the previous one at sh - 2
+  const uint32_t wi = p >> 5, sh = 2 * (p & 31), end = sh + 2 * (uint32_t)k;
+  const bool has_next = CODES && p + k < L, has_prev = CODES && p > 0;
+  const uint64_t wa = w[wi];
+  const uint64_t wb = (end > 64 || (has_next && end >= 64)) ? w[wi + 1] : 0;
+  const uint64_t wc = (W == 2 && (end > 128 || (has_next && end >= 128))) ? w[wi + 2] : 0;
+  const uint64_t wp = (has_prev && sh == 0) ? w[wi - 1] : 0;
+  KV v;
+  v.lo = sh ? (wa >> sh) | (wb << (64 - sh)) : wa;
+  v.hi = W == 2 ? (sh ? (wb >> sh) | (wc << (64 - sh)) : wb) : 0;
+  const KV km = kmask<W>(k);
+  v.lo &= km.lo;
+  v.hi &= km.hi;
   const KV rv = revcomp<W>(v, k);
   const bool fwd = kle<W>(v, rv);  // ties (palindromes) read forward
   KV key = fwd ? v : rv;
@@ -531,8 +542,8 @@ __device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32
     *bits_out = 0;
     return key;
   }
-  if (p + k < L) {  // the read's next k-mer w' = w[1:]+c
-    uint32_t c = base_at(w, p + k);
+  if (has_next) {  // the read's next k-mer w' = w[1:]+c
+    const uint32_t c = (uint32_t)((end < 64 ? wa >> end : end < 128 ? wb >> (end - 64) : wc >> (end - 128)) & 3);
     KV nx = succ_of<W>(v, k, c);
     bool nfwd = kle<W>(nx, pred_of<W>(rv, k, 3 - c));
     // (cur F, next F) -> FF w'; (F, R) -> FR rc(w'); (R, F) -> RF w';
@@ -542,8 +553,8 @@ __device__ inline KV occ_info(const uint64_t* __restrict__ w, uint32_t L, uint32
     code |= nibble(t, x) << (fwd ? 0 : 4);
     bits |= 1u << (4 * t + x);
   }
-  if (p > 0) {  // the read's previous k-mer w'' = c+w[:-1]
-    uint32_t c = base_at(w, p - 1);
+  if (has_prev) {  // the read's previous k-mer w'' = c+w[:-1]
+    const uint32_t c = (uint32_t)((sh ? wa >> (sh - 2) : wp >> 62) & 3);
     KV pv = pred_of<W>(v, k, c);
     bool pfwd = kle<W>(pv, succ_of<W>(rv, k, 3 - c));
     // (prev F, cur F) -> RR w''; (F, R) -> FR w''; (R, F) -> RF rc(w'');
@@ -4142,12 +4153,9 @@ static void run_build(gx_build* b) {
   }
   ph_end(b, PH_REPLAY);
   // replay inputs and order are dead (lists, needs and the node arrays
-  // stay) once the side stream has read them
-  auto replay_inputs_dead = [&] {
-    b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->nslot, &b->rcode, &b->selcnt, &b->seg, &b->rpos, &b->rkey,
-                                                   &b->rval, &b->rkey2, &b->rval2});
-  };
-  if (!side_open) replay_inputs_dead();
+  // stay); a free to make room drains the side stream first
+  b->reclaim.dead.insert(b->reclaim.dead.end(), {&b->nslot, &b->rcode, &b->selcnt, &b->seg, &b->rpos, &b->rkey,
+                                                 &b->rval, &b->rkey2, &b->rval2});
 
   // 7. heads
   ph_begin(b, PH_HEADS);
@@ -4267,7 +4275,6 @@ static void run_build(gx_build* b) {
   if (side_open) {  // the records need every replayed list
     stream_after(st, b->st2, b->ev_join);
     side_open = false;
-    replay_inputs_dead();
   }
   uint8_t* out = buf<uint8_t>(b->out, total + 1);
   if (nn) {
