@@ -195,16 +195,29 @@ __global__ void __launch_bounds__(kBlock) k_parse_lines(const uint8_t* __restric
     if (e < n && t[e] == '\n' && e > s && t[e - 1] == '\r') e--;
     uint64_t ntabs = 0;
     int64_t t1 = -1, t2 = -1, lastnt = (int64_t)s - 1;
-    for (uint64_t b = s; b < e; b += 64) {
-      uint64_t pos = b + lane;
-      bool in = pos < e;
-      uint8_t ch = in ? t[pos] : 0;
-      uint64_t mt = __ballot(in && ch == '\t');
-      uint64_t mn = __ballot(in && ch != '\t');
-      ntabs += __popcll(mt);
-      if (t1 < 0 && mt) { t1 = (int64_t)(b + __builtin_ctzll(mt)); mt &= mt - 1; }
-      if (t2 < 0 && mt) { t2 = (int64_t)(b + __builtin_ctzll(mt)); }
-      if (mn) lastnt = (int64_t)(b + 63 - __builtin_clzll(mn));
+    uint32_t head = 0;  // the line's first 64 bytes, one per lane (the readId is read from here)
+    // four 64-byte chunks per pass, their loads issued together
+    for (uint64_t b0 = s; b0 < e; b0 += 256) {
+      uint8_t chs[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint64_t pos = b0 + 64 * q + lane;
+        chs[q] = pos < e ? t[pos] : 0;
+      }
+      if (b0 == s) head = chs[0];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint64_t b = b0 + 64 * q;
+        if (b >= e) break;
+        const bool in = b + lane < e;
+        const uint8_t ch = chs[q];
+        uint64_t mt = __ballot(in && ch == '\t');
+        uint64_t mn = __ballot(in && ch != '\t');
+        ntabs += __popcll(mt);
+        if (t1 < 0 && mt) { t1 = (int64_t)(b + __builtin_ctzll(mt)); mt &= mt - 1; }
+        if (t2 < 0 && mt) { t2 = (int64_t)(b + __builtin_ctzll(mt)); }
+        if (mn) lastnt = (int64_t)(b + 63 - __builtin_clzll(mn));
+      }
     }
     uint64_t e2 = (uint64_t)(lastnt + 1);           // trailing tabs stripped
     uint64_t nt = ntabs - (e - e2);                  // tabs inside [s, e2)
@@ -217,12 +230,19 @@ __global__ void __launch_bounds__(kBlock) k_parse_lines(const uint8_t* __restric
       if (nt == 2) { f2s = (uint64_t)t2 + 1; f2e = e2; }
       // Long.parseLong of field 0 (lane-uniform loop; the field is short).
       uint64_t fs = s, fe = (uint64_t)t1;
+      // byte i of the line: from the lanes holding the first 64, else memory
+      auto byte_at = [&](uint64_t i) -> uint8_t {
+        return i - s < 64 ? (uint8_t)__shfl((int)head, (int)(i - s), 64) : t[i];
+      };
       bool neg = false, good = fe > fs;
       uint64_t i = fs;
-      if (good && (t[i] == '-' || t[i] == '+')) { neg = t[i] == '-'; i++; good = i < fe; }
+      if (good) {
+        const uint8_t c0 = byte_at(i);
+        if (c0 == '-' || c0 == '+') { neg = c0 == '-'; i++; good = i < fe; }
+      }
       unsigned __int128 v = 0;
       for (; good && i < fe; i++) {
-        uint8_t ch = t[i];
+        uint8_t ch = byte_at(i);
         if (ch < '0' || ch > '9') { good = false; break; }
         v = v * 10 + (ch - '0');
         if (v > ((unsigned __int128)1 << 63)) good = false;
@@ -1255,12 +1275,14 @@ __device__ inline Key128 lane_xor(const Key128& x, int m) {
 }
 // Stages whose partner is in the same thread (stride S < elements per
 // thread): compile-time indices keep the elements in registers.
+// The direction of element i0 + e is bit `size` of i0 when size >= EPT
+// (up_t, the same for all of the thread's elements), else bit `size` of e.
 template <int S, int EPT, class E>
-__device__ inline void bitonic_in_thread(E* x, uint32_t i0, uint32_t size) {
+__device__ inline void bitonic_in_thread(E* x, uint32_t size, bool up_t) {
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     if (e & S) continue;
-    const bool up = ((i0 + e) & size) == 0;
+    const bool up = size >= (uint32_t)EPT ? up_t : (((uint32_t)e & size) == 0);
     if (elem_gt(x[e], x[e | S]) == up) {
       const E y = x[e];
       x[e] = x[e | S];
@@ -1342,6 +1364,10 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
   }
   for (uint32_t size = 2; size <= P; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      // stride >= EPT: the stride and size bits of i0 + e are those of i0,
+      // so the direction is the thread's and the partner is element e of
+      // thread t ^ (stride / EPT)
+      const bool keep_min = ((i0 & stride) == 0) == ((i0 & size) == 0);
       if (stride >= WSPAN) {  // partner in another wave
         __syncthreads();
         if (act) {
@@ -1350,11 +1376,10 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
         }
         __syncthreads();
         if (act) {
+          const uint32_t pslot0 = (t ^ (stride / EPT)) * (EPT + 1);
 #pragma unroll
           for (uint32_t e = 0; e < EPT; e++) {
-            const uint32_t i = i0 + e, j = i ^ stride;
-            const E y = s[j + j / EPT];
-            const bool keep_min = ((i & stride) == 0) == ((i & size) == 0);
+            const E y = s[pslot0 + e];
             if (elem_gt(x[e], y) == keep_min) x[e] = y;
           }
         }
@@ -1363,16 +1388,15 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
           const int m = (int)(stride / EPT);
 #pragma unroll
           for (uint32_t e = 0; e < EPT; e++) {
-            const uint32_t i = i0 + e;
             const E y = lane_xor(x[e], m);
-            const bool keep_min = ((i & stride) == 0) == ((i & size) == 0);
             if (elem_gt(x[e], y) == keep_min) x[e] = y;
           }
         }
       } else if (act) {
-        if (stride == 1) bitonic_in_thread<1, EPT>(x, i0, size);
-        else if (stride == 2) bitonic_in_thread<2, EPT>(x, i0, size);
-        else if constexpr (EPT > 4) bitonic_in_thread<4, EPT>(x, i0, size);
+        const bool up_t = (i0 & size) == 0;
+        if (stride == 1) bitonic_in_thread<1, EPT>(x, size, up_t);
+        else if (stride == 2) bitonic_in_thread<2, EPT>(x, size, up_t);
+        else if constexpr (EPT > 4) bitonic_in_thread<4, EPT>(x, size, up_t);
       }
     }
   }
