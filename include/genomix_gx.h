@@ -14,8 +14,10 @@
  * appends.  All entry points return 0 on success and a negative GX_E* code
  * on failure (gx_last_error gives the message); nothing throws across the
  * ABI.  A handle is not thread-safe; distinct handles may run concurrently,
- * each on its own HIP stream.  The library owns all device memory it
- * allocates; callers own the buffers they pass in.
+ * each on its own HIP streams (a main stream, and a second one created on
+ * first use for work that overlaps it; every call has finished its work on
+ * both when it returns).  The library owns all device memory it allocates;
+ * callers own the buffers they pass in.
  */
 #ifndef GENOMIX_GX_H
 #define GENOMIX_GX_H
