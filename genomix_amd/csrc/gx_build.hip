@@ -1406,10 +1406,21 @@ __device__ inline void sort_one_bucket(uint32_t bk, typename SortKeyOf<W>::T* s,
     for (uint32_t e = 0; e < EPT; e++) s[slot0 + e] = x[e];
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += NT) {
-    const uint32_t src = elem_idx(s[i + i / EPT]);
-    ok[a + i] = ik[a + src];
-    oc[a + i] = ic[a + src];
+  if constexpr (W == 1) {
+    // the element holds the key bits below the bucket, and the bucket id
+    // is the bytes above them: the key needs no second read
+    const int bits = 8 * (nb - bkb);
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      const E el = s[i + i / EPT];
+      ok[a + i] = ((uint64_t)bk << bits) | (el >> kIdxBits);
+      oc[a + i] = ic[a + elem_idx(el)];
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      const uint32_t src = elem_idx(s[i + i / EPT]);
+      ok[a + i] = ik[a + src];
+      oc[a + i] = ic[a + src];
+    }
   }
 }
 
