@@ -2096,13 +2096,14 @@ __device__ inline uint64_t head_bytes(const SerCtx& c, uint32_t m) {
   return 1 + 8 + 4 + nbytes_of(L) + (Lm > 0 ? 4 + nbytes_of(Lm) : 0);
 }
 
-// Letters of edge type t in output order.
+// Letters of edge type t in output order, from the node's mask, replay
+// flag and replayed lists (read once per node by the caller).
 template <int W>
-__device__ inline LList edge_list(const SerCtx& c, uint64_t i, int t, KV v) {
-  uint32_t sel = (c.msk[i] >> (4 * t)) & 0xF;
+__device__ inline LList edge_list(const SerCtx& c, uint32_t mask, uint32_t needs, uint64_t lists, int t, KV v) {
+  uint32_t sel = (mask >> (4 * t)) & 0xF;
   LList l{0, 0};
-  if (c.jdk7 && c.needs[i] && __popc(sel) > 1) {
-    uint64_t w = c.lists[i] >> (16 * t);
+  if (c.jdk7 && needs && __popc(sel) > 1) {
+    uint64_t w = lists >> (16 * t);
     l.n = (int)(w & 7);
     l.bits = (uint32_t)(w >> 3) & 0xFF;
     return l;
@@ -2218,14 +2219,19 @@ __device__ inline void put_read(uint8_t*& p, const uint64_t* w, uint32_t L) {
 
 template <int W>
 __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uint64_t rec) {
-  KV v = node_key<W>(c.klo, c.khi, i);
+  // every per-node field is loaded here, before the first store: the
+  // stores through p could alias them, which would order each later load
+  // after them
+  const KV v = node_key<W>(c.klo, c.khi, i);
+  const uint32_t mask = c.msk[i], nu = c.hcu[i], nf = c.hcf[i], cov = c.cnt[i];
+  const uint32_t needs = c.jdk7 ? c.needs[i] : 0u;
+  const uint64_t lists = needs ? c.lists[i] : 0ull;
+  const uint64_t h0 = (nu | nf) ? c.hstart[i] : 0ull;
   put32(p, (uint32_t)(rec - 8));
   put32(p, 4 + c.nb);
   put_key<W>(p, v, c.k, c.nb);
   int n[4];
-  type_sizes(c.msk[i], n);
-  uint32_t nu = c.hcu[i], nf = c.hcf[i];
-  uint64_t h0 = c.hstart[i];
+  type_sizes(mask, n);
   uint8_t flags = 0x80;
   for (int t = 0; t < 4; t++)
     if (n[t]) flags |= 1 << t;
@@ -2234,7 +2240,7 @@ __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uin
   *p++ = flags;
   for (int t = 0; t < 4; t++) {
     if (!n[t]) continue;
-    LList l = edge_list<W>(c, i, t, v);
+    LList l = edge_list<W>(c, mask, needs, lists, t, v);
     if (l.n != n[t]) {  // cannot happen: the replayed set holds every neighbour once
       if (c.bad) atomicMin(c.bad, (unsigned long long)i);
       l.n = n[t];
@@ -2260,7 +2266,7 @@ __device__ inline void write_record(const SerCtx& c, uint64_t i, uint8_t* p, uin
     }
   }
   // coverage: Java float accumulation of 1.0f saturates at 2^24
-  uint32_t cv = c.cnt[i] > (1u << 24) ? (1u << 24) : c.cnt[i];
+  uint32_t cv = cov > (1u << 24) ? (1u << 24) : cov;
   float fc = (float)cv;
   uint32_t u;
   memcpy(&u, &fc, 4);
