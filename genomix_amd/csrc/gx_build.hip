@@ -3097,16 +3097,32 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
                                            uint64_t* __restrict__ head_at) {
   {
     uint8_t* p = rs;
-    uint32_t rep = c.urep_node[u];
-    KV v = node_key<W>(c.klo, c.khi, rep);
+    // the unitig's fields, its edge entries and the first edges' keys are
+    // loaded before the first store (stores through p could alias them and
+    // would order every later load after them)
+    const uint32_t rep = c.urep_node[u];
+    const int ne = c.uecnt[u];
+    const uint32_t nu_ = c.hu[u], nf = c.hf[u];
+    const uint64_t m = c.ulen[u];
+    const uint64_t h0 = (nu_ | nf) ? c.hstart[u] : 0ull;
+    const unsigned long long ucov = c.ucov[u];
+    const uint32_t* ord = c.order + (m > 1 ? c.ubase[u] : 0ull);  // unitig order of its nodes
+    const uint32_t o0 = m > 1 ? ord[0] : 0u;                      // first node (with orientation)
+    uint32_t ed[kUEdges];
+#pragma unroll
+    for (int a = 0; a < kUEdges; a++) ed[a] = a < ne ? c.uedge[kUEdges * u + a] : 0u;
+    const KV v = node_key<W>(c.klo, c.khi, rep);
+    constexpr int kPre = 4;  // edges whose keys are gathered up front
+    KV ek[kPre];
+#pragma unroll
+    for (int a = 0; a < kPre; a++) ek[a] = a < ne ? node_key<W>(c.klo, c.khi, ed[a] & kRepMask) : KV{0, 0};
     put32(p, (uint32_t)(rec - 8));
     put32(p, 4 + c.nb);
     put_key<W>(p, v, c.k, c.nb);
-    int ne = c.uecnt[u];
     int per[4] = {0, 0, 0, 0};
-    for (int a = 0; a < ne; a++) per[c.uedge[kUEdges * u + a] >> kRepBits]++;
-    uint32_t nu_ = c.hu[u], nf = c.hf[u];
-    uint64_t m = c.ulen[u];
+#pragma unroll
+    for (int a = 0; a < kUEdges; a++)
+      if (a < ne) per[ed[a] >> kRepBits]++;
     uint8_t flags = 0x80;
     for (int t = 0; t < 4; t++)
       if (per[t]) flags |= 1 << t;
@@ -3114,16 +3130,20 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
     if (nf) flags |= 1 << 5;
     if (m > 1) flags |= 1 << 6;
     *p++ = flags;
-    for (int t = 0; t < 4; t++) {
-      if (!per[t]) continue;
-      put32(p, (uint32_t)per[t]);
-      for (int a = 0; a < ne; a++) {
-        uint32_t e = c.uedge[kUEdges * u + a];
-        if ((int)(e >> kRepBits) != t) continue;
-        put_key<W>(p, node_key<W>(c.klo, c.khi, e & kRepMask), c.k, c.nb);
+    // the entries are sorted by type (k_unitig_edges), so each type's list
+    // is a run of them: its count before its first entry
+    int prev_t = -1;
+#pragma unroll
+    for (int a = 0; a < kUEdges; a++) {
+      if (a < ne) {
+        const int t = (int)(ed[a] >> kRepBits);
+        if (t != prev_t) {
+          put32(p, (uint32_t)per[t]);
+          prev_t = t;
+        }
+        put_key<W>(p, a < kPre ? ek[a] : node_key<W>(c.klo, c.khi, ed[a] & kRepMask), c.k, c.nb);
       }
     }
-    uint64_t h0 = c.hstart[u];
     for (int f = 0; f < 2; f++) {
       uint32_t n = f ? nf : nu_;
       if (!n) continue;
@@ -3140,10 +3160,9 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
     if (m > 1) {
       // internal k-mer: the unitig sequence (overlap-(k-1) concatenation,
       // VKmer.mergeWith*Kmer, VKmer.java:525-692) as VKmer bytes
-      uint64_t len = useq_len(c, u);
+      const uint64_t len = m + c.k - 1;  // useq_len
       put32(p, (uint32_t)len);
       uint64_t nbl = (len + 3) / 4;
-      const uint32_t* ord = c.order + c.ubase[u];
       if (len > kUseqLong) {
         seq_at[u] = rec_at + (uint64_t)(p - rs);
         p += nbl;
@@ -3151,7 +3170,6 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
         // bytes from the most significant end, eight per store; the first
         // node's k letters come from its key in registers, each later
         // letter is the last letter of the next node
-        const uint32_t o0 = ord[0];
         const KV v0 = node_key<W>(c.klo, c.khi, o0 & 0x7fffffffu);
         const bool f0 = !(o0 & 0x80000000u);
         uint64_t acc = 0;
@@ -3182,7 +3200,7 @@ __device__ inline void write_unitig_record(const USerCtx& c, uint64_t u, uint8_t
         put_le(p, acc, nacc);
       }
     }
-    double cov = (double)c.ucov[u] / (double)m;
+    double cov = (double)ucov / (double)m;
     float fc = (float)cov;
     uint32_t bits;
     memcpy(&bits, &fc, 4);
